@@ -1,0 +1,194 @@
+"""
+Benchmark of the tracer-advection step on one MI355X.
+
+Workload: the regional case, 1024 x 1024 x 50 curvilinear grid, ~20 % land, z* layers, ten tracers
+[T, S, 8 passive], plain FP64 WENO5-Z + Zalesak FCT + SSP-RK3 (``ocx_advect_fct_f64``), inputs resident in HBM.
+One step = one FCT step of the whole stack (C_{n+1} = step(C_n)); the metric counts every grid cell, land
+included: nt * nz * ny * nx tracer-cell updates per step.
+
+Prints ONE JSON line with the throughput, per-step p50/p99 latency, the roofline of the dominant kernel (per-kernel
+durations from hipEvents recorded by the library around each launch on its stream), the step-level roofline with
+the algorithmic bytes per update B = 72 + 99/nt, and a CPU baseline (the oracle's C loops on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12                 # MI355X HBM3E, bytes/s
+METRIC = "tracer-cell updates/s (1024²×50, 10 tracers); % HBM roofline; p50/p99"
+
+
+def step_bytes(nt: int) -> float:
+    """Algorithmic HBM bytes per tracer-cell update of one FCT step (three dependent RK3 sweeps, FCT fused into the
+    third): tracer streams 48 B read + 24 B written, shared Qx, Qy, V0, V1 (32 B) and ok (1 B) read once per sweep."""
+    return 72.0 + 99.0 / nt
+
+
+# compulsory bytes per tracer-cell of each kernel of the plain FP64 step (every array it touches moved once):
+# (bytes per tracer-cell, shared bytes per cell amortised over the tracers)
+KERNEL_BYTES = {
+    "fct_stage1": (40.0, 33.0),        # C -> c1, q1, F0 (x, y); Qx, Qy, V0, V1, ok
+    "fct_stage2": (64.0, 33.0),        # c1, q1, C, F0 -> c2, F0 + F1
+    "fct_stage3_split": (56.0, 33.0),  # c2, F0 + F1, C -> A (x, y), c_L
+    "fct_ratio": (48.0, 9.0),          # C, c_L, A -> R+, R-; V1, ok
+    "fct_final": (48.0, 32.0),         # R+-, A, C -> out; Qx, Qy, V0, V1
+}
+
+
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc summary, scaled by the copy-kernel
+    calibration written in the same file; None when no summary for this workload exists."""
+    path = ROOT / "profiles" / f"traffic_{workload}.txt"
+    if not path.exists():
+        return None
+    scale, fetch, write = 1.0, None, None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == "scale":
+            scale = float(parts[1])
+        if len(parts) >= 3 and parts[0] == kernel:
+            fetch, write = float(parts[1]), float(parts[2])
+    if fetch is None:
+        return None
+    return (fetch + write) * 1024.0 * scale
+
+
+def cpu_baseline(case, nt_sample: int, reps: int):
+    """The oracle's C loops (OpenMP over (tracer, layer) planes, like the reference's prange) on the first
+    ``nt_sample`` tracers of the same workload, one step per repetition, median."""
+    from oracle import cref
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    C = np.ascontiguousarray(case.C[:nt_sample])
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        cref.advect_fct(C, case.V0, case.V1, case.Qx, case.Qy, case.ok, case.dt)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    n = C.size
+    return {"value": n / t, "unit": "tracer-cell updates/s", "cores": threads, "kind": "port",
+            "sample": f"{nt_sample} of {case.C.shape[0]} tracers of the same {case.info['name']} inputs, one FCT "
+                      f"step, median of {reps} ({t:.2f} s each); C loops of the oracle, -O3 -ffp-contract=off, "
+                      f"OpenMP over (tracer, layer) planes"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--nz", type=int, default=50)
+    ap.add_argument("--nt", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-tracers", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU layout (tracer or domain sharding, halo "
+              "exchange) is described in DESIGN.md but not launched", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from pyocean_amd import _native, ops, synth
+
+    print(f"[bench] generating regional {args.nx}x{args.ny}x{args.nz} nt={args.nt}", file=sys.stderr, flush=True)
+    case = synth.regional(nx=args.nx, ny=args.ny, nz=args.nz, nt=args.nt, seed=0)
+    dev = torch.device("cuda", 0)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)     # noqa: E731
+    V0, V1, Qx, Qy = d(case.V0), d(case.V1), d(case.Qx), d(case.Qy)
+    ok = d(case.ok.astype(np.uint8))
+    C = d(case.C)
+    out = torch.empty_like(C)
+    ws = ops.Workspace()
+    nt, nz, ny, nx = C.shape
+    updates = nt * nz * ny * nx
+
+    def step(a, b):
+        return ops.advect_fct(a, V0, V1, Qx, Qy, ok, case.dt, out=b, workspace=ws)
+
+    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step(C, out)
+        C, out = out, C
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    _native.profile_enable(True)
+    _native.profile_collect()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record()
+        step(C, out)
+        ev[k][1].record()
+        C, out = out, C
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _native.profile_enable(False)
+    prof = _native.profile_collect()
+    step_ms = np.array([a.elapsed_time(b) for a, b in ev])
+    # conservation of the last step: sum V1 c^{n+1} against sum V0 c^n (closed edges)
+    m0 = ops.content(out, V0).cpu().numpy()
+    m1 = ops.content(C, V1).cpu().numpy()
+    print(f"[bench] {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+
+    value = updates * args.steps / wall
+    dom = max(prof, key=lambda k: prof[k][0])
+    dom_ms = prof[dom][0] / prof[dom][1]
+    per_tc, shared = KERNEL_BYTES.get(dom, (float("nan"), 0.0))
+    alg = updates * (per_tc + shared / nt)
+    achieved = alg / (dom_ms * 1e-3)
+    workload = f"regional{args.nx}x{args.ny}x{args.nz}nt{args.nt}"
+    traffic = read_traffic(dom, workload)
+    B = step_bytes(nt)
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tracer-cell updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "p50_ms": float(np.percentile(step_ms, 50)),
+        "p99_ms": float(np.percentile(step_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded: curvilinear metrics, land mask, z* layers, k^-3 stream function, 10 tracers)",
+        "config": {"workload": f"regional {nx}x{ny}x{nz}, {nt} tracers, plain FP64 WENO5-Z + Zalesak FCT + SSP-RK3",
+                   "nx": nx, "ny": ny, "nz": nz, "nt": nt, "dt": case.dt, "land_fraction": case.info["land"],
+                   "wet_cells": case.info["wet_cells"], "form": "plain_f64"},
+        "wet_updates_per_s": value * case.info["wet_cells"] / (nz * ny * nx),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic, "alg_bytes_per_launch": alg, "avg_launch_ms": dom_ms},
+        "step_roofline": {"bytes_per_update": B, "achieved": value * B / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": value * B / HBM_PEAK},
+        "kernels_ms": {k: v[0] / v[1] for k, v in prof.items()},
+        "last_step_content_rel_err": float(np.max(np.abs(m1 - m0) / np.abs(m0))),
+    }
+    if not args.no_cpu:
+        print(f"[bench] CPU baseline ({args.cpu_tracers} tracers x {args.cpu_reps})", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(case, args.cpu_tracers, args.cpu_reps)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

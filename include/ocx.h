@@ -1,0 +1,109 @@
+/*
+ * ocx - MI355X (gfx950) tracer advection of pyOCEAN: WENO5-Z/JS faces, Zalesak FCT, SSP-RK3, on the masked
+ * C-grid.  C ABI of libocx.so.
+ *
+ * Conventions
+ *   - All array pointers are DEVICE pointers (HBM), C-contiguous, FP64 unless stated, index order
+ *     (tracer t, layer k, row j, column i), k = 0 at the bed.
+ *     stacks (nt, nz, ny, nx); cells V0, V1, ok (nz, ny, nx), ok one byte 0/1; faces Qx (nz, ny, nx+1),
+ *     Qy (nz, ny+1, nx), Fz (nz+1, ny, nx); face stacks fx (nt, nz, ny, nx+1), fy (nt, nz, ny+1, nx).
+ *   - stream: a hipStream_t (NULL = the default stream).  Calls are asynchronous on it.
+ *   - Every call returns 0 on success or a negative OCX_E_* code; ocx_last_error() then describes it
+ *     (thread-local).  Shapes and pointers are checked on the host before any launch; kernels never read or write
+ *     outside the extents implied by the shapes.
+ *   - Results are bit-identical to the reference's FP64 arithmetic (no contraction, IEEE divisions) and do not
+ *     depend on launch configuration.
+ */
+#ifndef OCX_H
+#define OCX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* flags */
+#define OCX_OPEN_W 1u      /* west edge open (upstream values on its faces) */
+#define OCX_OPEN_E 2u
+#define OCX_OPEN_S 4u
+#define OCX_OPEN_N 8u
+#define OCX_WENO_JS 16u    /* WENO5-JS instead of WENO5-Z (tracer_scheme == "weno5js") */
+#define OCX_UNLIMITED 32u  /* FCT coefficients = 1: the unlimited RK3 result (advect_fct(unlimited=True)) */
+
+/* error codes */
+#define OCX_OK 0
+#define OCX_E_ARG (-1)        /* invalid shape, pointer or flag */
+#define OCX_E_HIP (-2)        /* a HIP runtime call failed */
+#define OCX_E_WORKSPACE (-3)  /* workspace missing or too small */
+
+const char* ocx_last_error(void);
+int ocx_version(void);
+
+/* Workspace (device bytes) of ocx_advect_fct_f64 / ocx_advect_rk3_f64 for one call of that shape. */
+size_t ocx_fct_workspace_bytes(int nt, int nz, int ny, int nx);
+size_t ocx_rk3_workspace_bytes(int nt, int nz, int ny, int nx);
+
+/* One SSP-RK3 + Zalesak FCT step of the stack C.
+ * Replaces TracerAdvection.advect_fct (core/components/tracers.py:208-254) and its kernels _advect_faces_h,
+ * _fct_stage, _fct_fluxes, _fct_coeff_kernel, _fct_final (core/dynamics/sigma_baroclinic.py:265-321, 407-530).
+ * inflow: exterior values (nt, nz, ny, nx) or NULL (stage 1 and the donor flux then use C, stages 2 and 3 their
+ * own values, tracers.py:231).  flux_face_i: -1, or a u-face column 0..nx whose effective flux F_L + C A is
+ * written to fx_i (nt, nz, ny).  open_flux: NULL or (nt) net open-edge inflow rate per tracer (tracers.py:65-78).
+ * C_out must not alias C. */
+int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                       const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx, double dt,
+                       unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* One SSP-RK3 step without limiter, x, y and (if vert) z sweeps.  Replaces TracerAdvection.advect
+ * (core/components/tracers.py:342-375) with _advect_div and _rk_stage (sigma_baroclinic.py:324-364, 856-875).
+ * Fz (nz+1, ny, nx) may be NULL when vert == 0; vert as the reference's any(Fz != 0).  fx_i as above
+ * ((F(c0) + F(c1))/6 + 2 F(c2)/3 of column flux_face_i). */
+int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                       const double* Fz, const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx,
+                       double dt, unsigned flags, int vert, int flux_face_i, double* C_out, double* fx_i,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* WENO horizontal face fluxes (fx, fy) of the stack c.  Replaces _advect_faces_h (sigma_baroclinic.py:265-321)
+ * as called by TracerAdvection.faces_h (tracers.py:166-176).  ext: exterior values or NULL (= c). */
+int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const uint8_t* ok, const double* ext,
+                    int nt, int nz, int ny, int nx, unsigned flags, double* fx, double* fy, void* stream);
+
+/* Donor-cell face fluxes.  Replaces TracerAdvection.upwind_faces_h (tracers.py:178-193). */
+int ocx_upwind_faces_h_f64(const double* c, const double* Qx, const double* Qy, const double* ext, int nt, int nz,
+                           int ny, int nx, unsigned flags, double* fx, double* fy, void* stream);
+
+/* Advection RHS -div(Q c) with x, y and (vert) z WENO faces.  Replaces _advect_div (sigma_baroclinic.py:324-364)
+ * as called by TracerAdvection.rhs (tracers.py:153-164). */
+int ocx_rhs_f64(const double* c, const double* Qx, const double* Qy, const double* Fz, const uint8_t* ok,
+                const double* ext, int nt, int nz, int ny, int nx, unsigned flags, int vert, double* out,
+                void* stream);
+
+/* Zalesak coefficients (Cx, Cy) of antidiffusive fluxes (Ax, Ay).  Replaces _fct_coeff_kernel
+ * (sigma_baroclinic.py:407-449).  workspace: 2 * nt*nz*ny*nx doubles. */
+int ocx_fct_coefficients_f64(const double* c0, const double* cL, const double* V1, const double* Ax,
+                             const double* Ay, const uint8_t* ok, double dt, int nt, int nz, int ny, int nx,
+                             double* Cx, double* Cy, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Periodic pad/crop (Periodic.pad / crop, core/model/periodic.py:71-95) of `lead` slabs of a (sy, sx) array.
+ * kind: 0 cells, 1 u-faces (x staggered), 2 v-faces (y staggered).  pad_x/pad_y: halo on that axis (0 = axis not
+ * periodic).  elem: 8 (FP64) or 1 (bytes).  crop != 0: the inverse (dst gets the interior of src). */
+int ocx_periodic_pad(const void* src, void* dst, size_t lead, int sy, int sx, int kind, int pad_y, int pad_x,
+                     int elem, int crop, void* stream);
+
+/* Per-tracer content sum_k,j,i V c (FP64, deterministic order): out (nt). */
+int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* stream);
+
+/* Per-kernel timing (no counterpart in the reference; for benchmarks): while enabled, every kernel launch of this
+ * library is bracketed by a hipEvent pair on its stream.  ocx_profile_collect waits for the recorded events,
+ * returns the number of distinct kernels and fills up to max_kernels entries (static name strings, total
+ * milliseconds, launch counts), then forgets the records. */
+int ocx_profile_enable(int on);
+int ocx_profile_collect(int max_kernels, const char** names, double* ms_total, long long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,120 @@
+"""
+Build recipe and ctypes wrapper of the C loop oracle ``loops.c`` (test infrastructure only; see ``oracle/__init__``).
+
+``build()`` compiles ``oracle/_build/liboracle.so`` with gcc (``-O3 -ffp-contract=off -fopenmp``).  The library
+is git-ignored but travels with the tree, so the CPU baseline can be timed where no compiler run is wanted.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+SRC = HERE / "loops.c"
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile the C oracle (always; a few seconds)."""
+    LIB.parent.mkdir(exist_ok=True)
+    tmp = LIB.with_suffix(f".so.tmp{os.getpid()}")
+    cmd = ["gcc", "-O3", "-march=x86-64-v2", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+           "-shared", "-std=c11", str(SRC), "-o", str(tmp), "-lm"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ctypes.CDLL(str(LIB))
+        P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
+        L.oc_advect_fct.argtypes = [P, P, P, P, P, P, P, I, I, I, I, D, I, I, I, I, P, P, P, P]
+        L.oc_advect_fct.restype = I
+        L.oc_advect_rk3.argtypes = [P, P, P, P, P, P, P, P, I, I, I, I, D, I, I, P]
+        L.oc_advect_rk3.restype = I
+        L.oc_faces_h.argtypes = [P, P, P, P, P, I, I, I, I, I, I, P, P]
+        L.oc_fct_coeff.argtypes = [P, P, P, P, P, P, D, I, I, I, I, P, P]
+        L.oc_advect_div.argtypes = [P, P, P, P, P, P, I, I, I, I, I, I, I, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u8(a, shape):
+    return np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.uint8)
+
+
+def advect_fct(C, V0, V1, Qx, Qy, ok, dt, inflow=None, open_mask=0, z_variant=True, unlimited=False,
+               flux_face_i=None, want_faces=False):
+    """One FCT step by the C loops (same meaning as ``numpy_ref.advect_fct``).  Returns out, or a dict with
+    ``out``, ``fx_i`` and the effective face fluxes ``Gx``/``Gy`` when asked for."""
+    C = _f64(C)
+    nt, n, ny, nx = C.shape
+    args = [_f64(V0), _f64(V1), _f64(Qx), _f64(Qy), _u8(ok, (n, ny, nx))]
+    ext = None if inflow is None else _f64(inflow)
+    out = np.empty_like(C)
+    fxi = np.empty((nt, n, ny)) if flux_face_i is not None else None
+    Gx = np.empty((nt, n, ny, nx + 1)) if want_faces else None
+    Gy = np.empty((nt, n, ny + 1, nx)) if want_faces else None
+    rc = lib().oc_advect_fct(_p(C), *map(_p, args), _p(ext), nt, n, ny, nx, float(dt), int(open_mask),
+                             int(bool(z_variant)), int(bool(unlimited)), -1 if flux_face_i is None else int(flux_face_i),
+                             _p(out), _p(fxi), _p(Gx), _p(Gy))
+    if rc != 0:
+        raise MemoryError("oracle: allocation failed")
+    if flux_face_i is None and not want_faces:
+        return out
+    return {"out": out, "fx_i": fxi, "Gx": Gx, "Gy": Gy}
+
+
+def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, open_mask=0, z_variant=True):
+    """One SSP-RK3 step by the C loops (same meaning as ``numpy_ref.advect_rk3``)."""
+    C = _f64(C)
+    nt, n, ny, nx = C.shape
+    args = [_f64(V0), _f64(V1), _f64(Qx), _f64(Qy), _f64(Fz), _u8(ok, (n, ny, nx))]
+    ext = None if inflow is None else _f64(inflow)
+    out = np.empty_like(C)
+    rc = lib().oc_advect_rk3(_p(C), *map(_p, args), _p(ext), nt, n, ny, nx, float(dt), int(open_mask),
+                             int(bool(z_variant)), _p(out))
+    if rc != 0:
+        raise MemoryError("oracle: allocation failed")
+    return out
+
+
+def faces_h(c, Qx, Qy, ok, ext, open_mask=0, z_variant=True):
+    c = _f64(c)
+    nt, n, ny, nx = c.shape
+    fx = np.zeros((nt, n, ny, nx + 1))
+    fy = np.zeros((nt, n, ny + 1, nx))
+    lib().oc_faces_h(_p(c), _p(_f64(Qx)), _p(_f64(Qy)), _p(_u8(ok, (n, ny, nx))), _p(_f64(ext)), nt, n, ny, nx,
+                     int(open_mask), int(bool(z_variant)), _p(fx), _p(fy))
+    return fx, fy
+
+
+def fct_coefficients(c0, cL, V1, Ax, Ay, ok, dt):
+    c0 = _f64(c0)
+    nt, n, ny, nx = c0.shape
+    Cx = np.zeros((nt, n, ny, nx + 1))
+    Cy = np.zeros((nt, n, ny + 1, nx))
+    lib().oc_fct_coeff(_p(c0), _p(_f64(cL)), _p(_f64(V1)), _p(_f64(Ax)), _p(_f64(Ay)), _p(_u8(ok, (n, ny, nx))),
+                       float(dt), nt, n, ny, nx, _p(Cx), _p(Cy))
+    return Cx, Cy

@@ -1,0 +1,352 @@
+"""
+Vectorised NumPy restatement of pyOCEAN's tracer advection (test infrastructure only; see ``oracle/__init__``).
+
+Index order (tracer t, layer k, row j, column i), k = 0 at the bed.  Face arrays: Qx (N, ny, nx+1),
+Qy (N, ny+1, nx), Fz (N+1, ny, nx); cells V0, V1, ok (N, ny, nx); stacks (nt, N, ny, nx).  All FP64.
+
+Every expression keeps the operand order of the reference so that IEEE rounding matches it; the reference
+file:line each function follows is given in its docstring.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+C1312 = 13 / 12          # the constant 13/12 of the smoothness indicators, as Python folds it
+EPS_Z = 1e-40            # WENO-Z epsilon (sigma_baroclinic.py:219)
+EPS_JS = 1e-6            # WENO-JS and WENO3 epsilon (sigma_baroclinic.py:224, 229-230)
+
+OPEN_W, OPEN_E, OPEN_S, OPEN_N = 1, 2, 4, 8
+
+
+def edges_mask(open_edges) -> int:
+    """Bit mask of the open edge names ('west', 'east', 'south', 'north')."""
+    bits = {"west": OPEN_W, "east": OPEN_E, "south": OPEN_S, "north": OPEN_N}
+    m = 0
+    for e in open_edges:
+        m |= bits[e]
+    return m
+
+
+# ------------------------------------------------------------------------------------------------ WENO values
+def weno_value(v1, v2, v3, v4, v5, m5, m3, z_variant=True):
+    """
+    Face value from the upwind-ordered stencil v1..v5 (v3 the upwind cell): WENO5-Z or -JS where ``m5`` (all five
+    cells valid), WENO3 where ``m3`` (v2..v4 valid), else v3.  Follows ``_wval`` (sigma_baroclinic.py:204-233)
+    and ``_weno5_faces_numpy.recon`` (:603-618), term for term.
+    """
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        q0 = (2 * v1 - 7 * v2 + 11 * v3) / 6
+        q1 = (-v2 + 5 * v3 + 2 * v4) / 6
+        q2 = (2 * v3 + 5 * v4 - v5) / 6
+        b0 = C1312 * (v1 - 2 * v2 + v3) ** 2 + 0.25 * (v1 - 4 * v2 + 3 * v3) ** 2
+        b1 = C1312 * (v2 - 2 * v3 + v4) ** 2 + 0.25 * (v2 - v4) ** 2
+        b2 = C1312 * (v3 - 2 * v4 + v5) ** 2 + 0.25 * (3 * v3 - 4 * v4 + v5) ** 2
+        if z_variant:
+            tau = np.abs(b2 - b0)
+            a0 = 0.1 * (1 + tau / (EPS_Z + b0))
+            a1 = 0.6 * (1 + tau / (EPS_Z + b1))
+            a2 = 0.3 * (1 + tau / (EPS_Z + b2))
+        else:
+            a0 = 0.1 / (EPS_JS + b0) ** 2
+            a1 = 0.6 / (EPS_JS + b1) ** 2
+            a2 = 0.3 / (EPS_JS + b2) ** 2
+        w5 = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2)
+        c0 = (-v2 + 3 * v3) / 2
+        c1 = (v3 + v4) / 2
+        e0 = (1 / 3) / (EPS_JS + (v3 - v2) ** 2) ** 2
+        e1 = (2 / 3) / (EPS_JS + (v4 - v3) ** 2) ** 2
+        w3 = (e0 * c0 + e1 * c1) / (e0 + e1)
+    return np.where(m5, w5, np.where(m3, w3, v3))
+
+
+def _shifted(c, ok, s):
+    """c[..., m + s] and its validity along the last axis; positions outside the row are 0 and invalid."""
+    n = c.shape[-1]
+    val = np.zeros_like(c)
+    good = np.zeros(ok.shape, bool)
+    lo, hi = max(0, -s), min(n, n - s)
+    if hi > lo:
+        val[..., lo:hi] = c[..., lo + s:hi + s]
+        good[..., lo:hi] = ok[..., lo + s:hi + s]
+    return val, good
+
+
+def weno5_faces(c, flux, ok, axis, z_variant=True):
+    """
+    WENO face values on the interior faces 1..n-1 along ``axis`` (face m between cells m-1 and m), upwind with
+    respect to the sign of ``flux`` (>= 0: from the low side); other faces 0.  ``flux`` has the face count of the
+    output along ``axis`` (n+1 for x/y, n for the vertical as ``rhs_numpy`` passes it).  Follows
+    ``_weno5_faces_numpy`` (sigma_baroclinic.py:581-629) and ``_weno5_rows`` (:236-262).
+    """
+    cm = np.moveaxis(np.asarray(c, float), axis, -1)
+    om = np.moveaxis(np.asarray(ok, bool), axis, -1)
+    fm = np.moveaxis(np.asarray(flux, float), axis, -1)
+    lead = np.broadcast_shapes(cm.shape[:-1], om.shape[:-1], fm.shape[:-1])
+    n = cm.shape[-1]
+    cm = np.broadcast_to(cm, lead + (n,))
+    om = np.broadcast_to(om, lead + (n,))
+    fm = np.broadcast_to(fm, lead + (fm.shape[-1],))
+    sh = {s: _shifted(cm, om, s) for s in (-3, -2, -1, 0, 1, 2)}
+    v = {s: sh[s][0] for s in sh}
+    k = {s: sh[s][1] for s in sh}
+    pos = weno_value(v[-3], v[-2], v[-1], v[0], v[1], k[-3] & k[-2] & k[-1] & k[0] & k[1],
+                     k[-2] & k[-1] & k[0], z_variant)
+    neg = weno_value(v[2], v[1], v[0], v[-1], v[-2], k[2] & k[1] & k[0] & k[-1] & k[-2],
+                     k[1] & k[0] & k[-1], z_variant)
+    face = np.zeros(fm.shape)
+    face[..., 1:n] = np.where(fm[..., 1:n] >= 0, pos[..., 1:n], neg[..., 1:n])
+    return np.moveaxis(face, -1, axis)
+
+
+# ------------------------------------------------------------------------------------------------ face fluxes
+def faces_h(c, Qx, Qy, ok, ext, open_mask=0, z_variant=True):
+    """
+    Horizontal face fluxes (fx, fy) = Q * c_face: WENO faces on interior faces with Q != 0 (Q == 0: exactly 0),
+    upstream values on open edges (``ext`` on inflow, the edge cell on outflow), 0 on closed edges.  Follows
+    ``_advect_faces_h`` (sigma_baroclinic.py:265-321).
+    """
+    c = np.asarray(c, float)
+    nt, n, ny, nx = c.shape
+    okb = np.broadcast_to(ok, c.shape)
+    Qxb = np.broadcast_to(Qx, (nt,) + Qx.shape)
+    Qyb = np.broadcast_to(Qy, (nt,) + Qy.shape)
+    fx = np.where(Qxb == 0.0, 0.0, Qxb * weno5_faces(c, Qxb, okb, 3, z_variant))
+    fy = np.where(Qyb == 0.0, 0.0, Qyb * weno5_faces(c, Qyb, okb, 2, z_variant))
+    fx[..., 0] = 0.0
+    fx[..., nx] = 0.0
+    fy[..., 0, :] = 0.0
+    fy[..., ny, :] = 0.0
+    _open_edges(fx, fy, c, Qx, Qy, ext, open_mask, donor=False)
+    return fx, fy
+
+
+def _open_edges(fx, fy, c, Qx, Qy, ext, open_mask, donor):
+    """Open-edge face fluxes q * (ext on inflow, else the edge cell) (sigma_baroclinic.py:298-303, 315-320)."""
+    if open_mask & OPEN_W:
+        q = Qx[None, ..., 0]
+        fx[..., 0] = q * np.where(q > 0.0, ext[..., 0], c[..., 0])
+    if open_mask & OPEN_E:
+        q = Qx[None, ..., -1]
+        fx[..., -1] = q * np.where(q < 0.0, ext[..., -1], c[..., -1])
+    if open_mask & OPEN_S:
+        q = Qy[None, :, 0, :]
+        fy[..., 0, :] = q * np.where(q > 0.0, ext[..., 0, :], c[..., 0, :])
+    if open_mask & OPEN_N:
+        q = Qy[None, :, -1, :]
+        fy[..., -1, :] = q * np.where(q < 0.0, ext[..., -1, :], c[..., -1, :])
+
+
+def upwind_faces_h(c, Qx, Qy, ext, open_mask=0):
+    """
+    Donor-cell face fluxes F_L = Q * c_up on all interior faces (Q >= 0: the low-side cell), open edges as
+    ``faces_h``, closed edges 0.  Follows ``TracerAdvection.upwind_faces_h`` (tracers.py:178-193) and
+    ``_fct_fluxes`` (sigma_baroclinic.py:482-505).
+    """
+    c = np.asarray(c, float)
+    nt, n, ny, nx = c.shape
+    fx = np.zeros((nt, n, ny, nx + 1))
+    fy = np.zeros((nt, n, ny + 1, nx))
+    qx = Qx[None, ..., 1:-1]
+    fx[..., 1:-1] = qx * np.where(qx >= 0, c[..., :-1], c[..., 1:])
+    qy = Qy[None, :, 1:-1, :]
+    fy[..., 1:-1, :] = qy * np.where(qy >= 0, c[..., :-1, :], c[..., 1:, :])
+    _open_edges(fx, fy, c, Qx, Qy, ext, open_mask, donor=True)
+    return fx, fy
+
+
+def faces_z(c, Fz, ok, z_variant=True):
+    """
+    Vertical face fluxes fz (nt, N+1, ny, nx) = Fz * c_face on the interfaces 1..N-1 (Fz == 0 skipped, bed and
+    surface 0), WENO along the layer index with each layer's own ``ok``.  Follows ``_advect_div``'s vertical pass
+    (sigma_baroclinic.py:335-353).
+    """
+    c = np.asarray(c, float)
+    nt, n, ny, nx = c.shape
+    okb = np.broadcast_to(ok, c.shape)
+    Fzb = np.broadcast_to(Fz[:-1], c.shape)
+    v = weno5_faces(c, Fzb, okb, 1, z_variant)
+    fz = np.zeros((nt, n + 1, ny, nx))
+    fz[:, :-1] = np.where(Fzb == 0.0, 0.0, Fzb * v)
+    fz[:, 0] = 0.0
+    return fz
+
+
+def div_h(fx, fy):
+    """((fx_e - fx_w) + fy_n) - fy_s, the grouping of _fct_stage (sigma_baroclinic.py:463) and the NumPy oracle."""
+    return fx[..., 1:] - fx[..., :-1] + fy[..., 1:, :] - fy[..., :-1, :]
+
+
+def rhs(c, Qx, Qy, Fz, ok, ext, open_mask=0, z_variant=True, vert=None):
+    """
+    Advection RHS -div(Q c) of the stack, x, y and (if ``vert``, default: any Fz != 0) z faces.  Follows
+    ``_advect_div`` (sigma_baroclinic.py:324-364) and ``TracerAdvection.rhs`` (tracers.py:153-164).
+    """
+    fx, fy = faces_h(c, Qx, Qy, ok, ext, open_mask, z_variant)
+    if vert is None:
+        vert = bool(np.any(np.asarray(Fz) != 0.0))
+    if vert:
+        fz = faces_z(c, Fz, ok, z_variant)
+        return -(fx[..., 1:] - fx[..., :-1] + fy[..., 1:, :] - fy[..., :-1, :] + fz[:, 1:] - fz[:, :-1])
+    return -(fx[..., 1:] - fx[..., :-1] + fy[..., 1:, :] - fy[..., :-1, :])
+
+
+# ------------------------------------------------------------------------------------------------ FCT
+def fct_coefficients(c0, cL, V1, Ax, Ay, ok, dt):
+    """
+    Zalesak (1979, eqs. 7-13) limiting coefficients (Cx, Cy) in [0, 1] of the antidiffusive fluxes A, with the
+    bounds taken over the cell, its x and y neighbours and the layers k +- 1 (``ok`` cells only); cells not ok get
+    R+- = 0; edge faces 0.  Follows ``fct_coefficients_numpy`` (sigma_baroclinic.py:367-404) and
+    ``_fct_coeff_kernel`` (:407-449).
+    """
+    okb = np.broadcast_to(ok, c0.shape)
+    hi = np.where(okb, np.maximum(c0, cL), -np.inf)
+    lo = np.where(okb, np.minimum(c0, cL), np.inf)
+
+    def around(a, f):
+        out = a.copy()
+        out[..., 1:] = f(out[..., 1:], a[..., :-1])
+        out[..., :-1] = f(out[..., :-1], a[..., 1:])
+        out[..., 1:, :] = f(out[..., 1:, :], a[..., :-1, :])
+        out[..., :-1, :] = f(out[..., :-1, :], a[..., 1:, :])
+        out[:, 1:] = f(out[:, 1:], a[:, :-1])
+        out[:, :-1] = f(out[:, :-1], a[:, 1:])
+        return out
+
+    cmax, cmin = around(hi, np.maximum), around(lo, np.minimum)
+    p_in = dt * (np.maximum(Ax[..., :-1], 0.0) - np.minimum(Ax[..., 1:], 0.0)
+                 + np.maximum(Ay[..., :-1, :], 0.0) - np.minimum(Ay[..., 1:, :], 0.0))
+    p_out = dt * (np.maximum(Ax[..., 1:], 0.0) - np.minimum(Ax[..., :-1], 0.0)
+                  + np.maximum(Ay[..., 1:, :], 0.0) - np.minimum(Ay[..., :-1, :], 0.0))
+    with np.errstate(invalid="ignore"):
+        q_in = np.maximum(V1[None] * (cmax - cL), 0.0)
+        q_out = np.maximum(V1[None] * (cL - cmin), 0.0)
+    r_in = np.where(okb & (p_in > 0), np.minimum(1.0, q_in / np.where(p_in > 0, p_in, 1.0)), 0.0)
+    r_out = np.where(okb & (p_out > 0), np.minimum(1.0, q_out / np.where(p_out > 0, p_out, 1.0)), 0.0)
+    Cx = np.zeros_like(Ax)
+    Cy = np.zeros_like(Ay)
+    Cx[..., 1:-1] = np.where(Ax[..., 1:-1] >= 0, np.minimum(r_in[..., 1:], r_out[..., :-1]),
+                             np.minimum(r_in[..., :-1], r_out[..., 1:]))
+    Cy[..., 1:-1, :] = np.where(Ay[..., 1:-1, :] >= 0, np.minimum(r_in[..., 1:, :], r_out[..., :-1, :]),
+                                np.minimum(r_in[..., :-1, :], r_out[..., 1:, :]))
+    return Cx, Cy
+
+
+def _safe(q, v, C):
+    """c = q / V where V > 0, else the step's initial C (tracers.py:275)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(v > 0, q / np.where(v > 0, v, 1.0), C)
+
+
+def advect_fct(C, V0, V1, Qx, Qy, ok, dt, inflow=None, open_mask=0, z_variant=True, unlimited=False,
+               flux_face_i=None, open_flux=False):
+    """
+    One SSP-RK3 + Zalesak FCT step of the stack C.  Follows ``advect_fct`` / ``advect_fct_numpy``
+    (tracers.py:208-301): stage faces F0, F1, F2 with the stage's own values as exterior when ``inflow`` is None
+    (every stage uses ``inflow`` when given), F_H = (F0 + F1)/6 + 2 F2/3, donor-cell F_L, A = F_H - F_L, c_L, the
+    coefficients and V1 c = V0 c0 - dt div(F_L + C A).  ``unlimited``: C = 1.  Returns ``out`` or a dict with
+    ``out`` and the optional side outputs ``fx_i`` (effective flux of u-face column i, (nt, N, ny)) and
+    ``open_flux`` (net open-edge inflow rate per tracer, (nt,)).
+    """
+    C = np.ascontiguousarray(C, dtype=float)
+    Qx, Qy = np.asarray(Qx, float), np.asarray(Qy, float)
+    V0, V1 = np.asarray(V0, float), np.asarray(V1, float)
+    okc = np.broadcast_to(ok, C.shape[1:])
+    ext0 = C if inflow is None else np.asarray(inflow, float)
+    dV = V1 - V0
+    Va, Vb = V0 + dV, V0 + 0.5 * dV
+    q0 = V0 * C
+    fx0, fy0 = faces_h(C, Qx, Qy, okc, ext0, open_mask, z_variant)
+    q1 = 0.0 * q0 + 1.0 * (q0 - dt * div_h(fx0, fy0))
+    c1 = _safe(q1, Va, C)
+    fx1, fy1 = faces_h(c1, Qx, Qy, okc, c1 if inflow is None else ext0, open_mask, z_variant)
+    q2 = 0.75 * q0 + 0.25 * (q1 - dt * div_h(fx1, fy1))
+    c2 = _safe(q2, Vb, C)
+    fx2, fy2 = faces_h(c2, Qx, Qy, okc, c2 if inflow is None else ext0, open_mask, z_variant)
+    FHx = (fx0 + fx1) / 6.0 + 2.0 * fx2 / 3.0
+    FHy = (fy0 + fy1) / 6.0 + 2.0 * fy2 / 3.0
+    FLx, FLy = upwind_faces_h(C, Qx, Qy, ext0, open_mask)
+    Ax, Ay = FHx - FLx, FHy - FLy
+    if unlimited:
+        Gx, Gy = FHx, FHy
+        out = _safe(q0 - dt * div_h(FHx, FHy), V1, C)
+    else:
+        cL = _safe(0.0 * q0 + 1.0 * (q0 - dt * div_h(FLx, FLy)), V1, C)
+        Cx, Cy = fct_coefficients(C, cL, V1, Ax, Ay, okc, dt)
+        Gx, Gy = FLx + Cx * Ax, FLy + Cy * Ay
+        out = _safe(q0 - dt * div_h(Gx, Gy), V1, C)
+    if flux_face_i is None and not open_flux:
+        return out
+    res = {"out": out}
+    if flux_face_i is not None:
+        res["fx_i"] = np.ascontiguousarray(Gx[..., flux_face_i])
+    if open_flux:
+        res["open_flux"] = open_inflow(Gx, Gy, open_mask)
+    return res
+
+
+def open_inflow(Fx, Fy, open_mask):
+    """Net inflow rate per stack member, sum_W F - sum_E F + sum_S F - sum_N F (tracers.py:65-78)."""
+    out = np.zeros(Fx.shape[0])
+    for bit, F, sl, sgn in ((OPEN_W, Fx, np.s_[..., 0], 1.0), (OPEN_E, Fx, np.s_[..., -1], -1.0),
+                            (OPEN_S, Fy, np.s_[..., 0, :], 1.0), (OPEN_N, Fy, np.s_[..., -1, :], -1.0)):
+        if open_mask & bit:
+            out += sgn * F[sl].reshape(F.shape[0], -1).sum(axis=1)
+    return out
+
+
+def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, open_mask=0, z_variant=True, flux_face_i=None):
+    """
+    One SSP-RK3 step without limiter, x, y and z sweeps.  Follows ``advect`` (tracers.py:342-375) with
+    ``_rk_stage`` (sigma_baroclinic.py:856-875): q1 = q0 + dt r0, q2 = 0.75 q0 + 0.25 (q1 + dt r1),
+    q3 = q0/3 + 2/3 (q2 + dt r2), each with the ``safe`` division by Va, Vb, V1.  The exterior values are
+    ``inflow`` in every stage, else each stage's own values.  Returns ``out`` or (out, fx_i).
+    """
+    C = np.ascontiguousarray(C, dtype=float)
+    Qx, Qy, Fz = np.asarray(Qx, float), np.asarray(Qy, float), np.asarray(Fz, float)
+    V0, V1 = np.asarray(V0, float), np.asarray(V1, float)
+    okc = np.broadcast_to(ok, C.shape[1:])
+    vert = bool(np.any(Fz != 0.0))
+    ext = lambda c: c if inflow is None else np.asarray(inflow, float)          # noqa: E731
+    dV = V1 - V0
+    Va, Vb = V0 + dV, V0 + 0.5 * dV
+    q0 = V0 * C
+    r0 = rhs(C, Qx, Qy, Fz, okc, ext(C), open_mask, z_variant, vert)
+    q1 = q0 + dt * r0
+    c1 = _safe(q1, Va, C)
+    r1 = rhs(c1, Qx, Qy, Fz, okc, ext(c1), open_mask, z_variant, vert)
+    q2 = 0.75 * q0 + 0.25 * (q1 + dt * r1)
+    c2 = _safe(q2, Vb, C)
+    r2 = rhs(c2, Qx, Qy, Fz, okc, ext(c2), open_mask, z_variant, vert)
+    q3 = q0 / 3.0 + 2.0 / 3.0 * (q2 + dt * r2)
+    out = _safe(q3, V1, C)
+    if flux_face_i is None:
+        return out
+    fxi = lambda c: faces_h(c, Qx, Qy, okc, ext(c), open_mask, z_variant)[0][..., flux_face_i]   # noqa: E731
+    return out, (fxi(C) + fxi(c1)) / 6.0 + 2.0 * fxi(c2) / 3.0
+
+
+# ------------------------------------------------------------------------------------------------ periodic
+TRACER_HALO = 13         # 3 stages x 3 cells + FCT 3 + 1 (core/model/periodic.py:31)
+
+
+def pad(a, kind, halo, px, py):
+    """Wrapped halo of ``halo`` cells/faces on the periodic axes (core/model/periodic.py:71-87)."""
+    a = np.asarray(a)
+    if px:
+        face = kind in ("u", "q")
+        n = a.shape[-1] - face
+        a = np.take(a, np.arange(-halo, n + halo + face) % n, axis=-1)
+    if py:
+        face = kind in ("v", "q")
+        n = a.shape[-2] - face
+        a = np.take(a, np.arange(-halo, n + halo + face) % n, axis=-2)
+    return a
+
+
+def crop(a, halo, px, py):
+    """Inverse of ``pad`` for cell arrays (core/model/periodic.py:89-95)."""
+    if px:
+        a = a[..., halo:a.shape[-1] - halo]
+    if py:
+        a = a[..., halo:a.shape[-2] - halo, :]
+    return np.ascontiguousarray(a)

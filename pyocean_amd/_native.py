@@ -1,0 +1,86 @@
+"""
+ctypes binding of ``libocx.so`` (the C ABI declared in ``include/ocx.h``).
+
+There is no fallback: if the library is missing or fails to load, ``lib()`` raises, so nothing in the package can
+silently run anything but the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libocx.so"
+
+OPEN_W, OPEN_E, OPEN_S, OPEN_N = 1, 2, 4, 8
+WENO_JS = 16
+UNLIMITED = 32
+SPLIT = 64
+
+OCX_E_ARG, OCX_E_HIP, OCX_E_WORKSPACE = -1, -2, -3
+
+# (name, restype, argtypes) of every exported symbol; the CPU tests check that each one is exported
+P, I, U, D, Z = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_double, ctypes.c_size_t
+SIGNATURES = {
+    "ocx_last_error": (ctypes.c_char_p, []),
+    "ocx_version": (I, []),
+    "ocx_fct_workspace_bytes": (Z, [I, I, I, I]),
+    "ocx_rk3_workspace_bytes": (Z, [I, I, I, I]),
+    "ocx_advect_fct_f64": (I, [P, P, P, P, P, P, P, I, I, I, I, D, U, I, P, P, P, P, Z, P]),
+    "ocx_advect_rk3_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, U, I, I, P, P, P, Z, P]),
+    "ocx_faces_h_f64": (I, [P, P, P, P, P, I, I, I, I, U, P, P, P]),
+    "ocx_upwind_faces_h_f64": (I, [P, P, P, P, I, I, I, I, U, P, P, P]),
+    "ocx_rhs_f64": (I, [P, P, P, P, P, P, I, I, I, I, U, I, P, P]),
+    "ocx_fct_coefficients_f64": (I, [P, P, P, P, P, P, D, I, I, I, I, P, P, P, Z, P]),
+    "ocx_periodic_pad": (I, [P, P, Z, I, I, I, I, I, I, I, P]),
+    "ocx_content_f64": (I, [P, P, I, I, I, I, P, P]),
+    "ocx_profile_enable": (I, [I]),
+    "ocx_profile_collect": (I, [I, P, P, P]),
+}
+
+
+class OcxError(RuntimeError):
+    """A libocx call returned an error code."""
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library with typed entry points (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libocx.so is not built ({LIB_PATH}); run __graft_entry__.build() or "
+                               "python -m pyocean_amd.build")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def profile_enable(on: bool) -> None:
+    lib().ocx_profile_enable(int(bool(on)))
+
+
+def profile_collect(max_kernels: int = 64) -> dict:
+    """{kernel label: (total ms, launches)} of the launches recorded since the last collect."""
+    names = (ctypes.c_char_p * max_kernels)()
+    ms = (ctypes.c_double * max_kernels)()
+    cnt = (ctypes.c_longlong * max_kernels)()
+    n = lib().ocx_profile_collect(max_kernels, names, ms, cnt)
+    if n < 0:
+        check(n, "ocx_profile_collect")
+    return {names[m].decode(): (ms[m], cnt[m]) for m in range(min(n, max_kernels))}
+
+
+def check(rc: int, what: str) -> None:
+    """Raise OcxError (ValueError for argument errors) when a call failed."""
+    if rc == 0:
+        return
+    msg = lib().ocx_last_error().decode(errors="replace")
+    if rc == OCX_E_ARG:
+        raise ValueError(f"{what}: {msg}")
+    raise OcxError(f"{what} failed ({rc}): {msg}")

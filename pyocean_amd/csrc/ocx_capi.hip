@@ -1,0 +1,507 @@
+// C ABI of the FP64 tracer-advection library (declarations and contracts: include/ocx.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ocx.h"
+#include "ocx_kernels_f64.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) return fail(OCX_E_HIP, "%s: %s", what, hipGetErrorString(e));
+    return OCX_OK;
+}
+
+#define OCX_TRY(expr)              \
+    do {                           \
+        int _rc = (expr);          \
+        if (_rc != OCX_OK) return _rc; \
+    } while (0)
+
+constexpr int TX = 64, TY = 4;
+
+int check_shape(int nt, int nz, int ny, int nx) {
+    if (nt < 1 || nz < 1 || ny < 1 || nx < 1)
+        return fail(OCX_E_ARG, "shape (nt=%d, nz=%d, ny=%d, nx=%d) must be positive", nt, nz, ny, nx);
+    if ((long long)nt * nz > (1LL << 30) || ny > (1 << 24) || nx > (1 << 24))
+        return fail(OCX_E_ARG, "shape (nt=%d, nz=%d, ny=%d, nx=%d) too large", nt, nz, ny, nx);
+    return OCX_OK;
+}
+
+int need(const void* p, const char* name) {
+    if (!p) return fail(OCX_E_ARG, "%s is NULL", name);
+    return OCX_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Carves a workspace into consecutive 256-byte-aligned pieces.
+struct Carver {
+    char* base;
+    size_t used = 0;
+    explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+    template <typename T>
+    T* take(size_t count) {
+        T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+        used += align256(count * sizeof(T));
+        return p;
+    }
+};
+
+struct Sizes {
+    size_t S, SX, SY, P;
+    Sizes(int nt, int nz, int ny, int nx) {
+        S = (size_t)nt * nz * ny * nx;
+        SX = (size_t)nt * nz * ny * (nx + 1);
+        SY = (size_t)nt * nz * (ny + 1) * nx;
+        P = (size_t)nt * nz;
+    }
+};
+
+struct FctWs {
+    double *c1, *q1, *c2, *fx, *fy, *ax, *ay, *gedge;
+};
+
+size_t fct_layout(void* base, int nt, int nz, int ny, int nx, FctWs* w) {
+    Sizes s(nt, nz, ny, nx);
+    Carver cv(base);
+    FctWs t;
+    t.c1 = cv.take<double>(s.S);
+    t.q1 = cv.take<double>(s.S);
+    t.c2 = cv.take<double>(s.S);
+    t.fx = cv.take<double>(s.SX);
+    t.fy = cv.take<double>(s.SY);
+    t.ax = cv.take<double>(s.SX);
+    t.ay = cv.take<double>(s.SY);
+    t.gedge = cv.take<double>(2 * s.P * ny + 2 * s.P * nx);
+    if (w) *w = t;
+    return cv.used;
+}
+
+struct Rk3Ws {
+    double *q, *c1, *c2, *fxs;
+};
+
+size_t rk3_layout(void* base, int nt, int nz, int ny, int nx, Rk3Ws* w) {
+    Sizes s(nt, nz, ny, nx);
+    Carver cv(base);
+    Rk3Ws t;
+    t.q = cv.take<double>(s.S);
+    t.c1 = cv.take<double>(s.S);
+    t.c2 = cv.take<double>(s.S);
+    t.fxs = cv.take<double>(3 * s.P * ny);
+    if (w) *w = t;
+    return cv.used;
+}
+
+dim3 tile_grid(int nt, int nz, int ny, int nx) {
+    const long long planes = (long long)nt * nz;
+    return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + TY - 1) / TY), (unsigned)std::min(planes, 65535LL));
+}
+
+int launch_check(const char* what) { return hip_check(hipGetLastError(), what); }
+
+// Optional per-kernel timing: a hipEvent pair on the launch stream around every kernel launch while enabled.
+struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+std::vector<ProfRec> g_prof;
+bool g_prof_on = false;
+
+struct ProfScope {
+    hipStream_t s;
+    bool on;
+    ProfRec rec{};
+    ProfScope(const char* name, hipStream_t st) : s(st), on(g_prof_on) {
+        if (!on) return;
+        rec.name = name;
+        if (hipEventCreate(&rec.a) != hipSuccess || hipEventCreate(&rec.b) != hipSuccess) {
+            on = false;
+            return;
+        }
+        (void)hipEventRecord(rec.a, s);
+    }
+    ~ProfScope() {
+        if (!on) return;
+        (void)hipEventRecord(rec.b, s);
+        std::lock_guard<std::mutex> g(g_prof_mu);
+        g_prof.push_back(rec);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* ocx_last_error(void) { return g_err.c_str(); }
+
+int ocx_version(void) { return 1; }
+
+size_t ocx_fct_workspace_bytes(int nt, int nz, int ny, int nx) {
+    if (check_shape(nt, nz, ny, nx) != OCX_OK) return 0;
+    return fct_layout(nullptr, nt, nz, ny, nx, nullptr);
+}
+
+size_t ocx_rk3_workspace_bytes(int nt, int nz, int ny, int nx) {
+    if (check_shape(nt, nz, ny, nx) != OCX_OK) return 0;
+    return rk3_layout(nullptr, nt, nz, ny, nx, nullptr);
+}
+
+int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                       const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx, double dt,
+                       unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(C, "C"));
+    OCX_TRY(need(V0, "V0"));
+    OCX_TRY(need(V1, "V1"));
+    OCX_TRY(need(Qx, "Qx"));
+    OCX_TRY(need(Qy, "Qy"));
+    OCX_TRY(need(ok, "ok"));
+    OCX_TRY(need(C_out, "C_out"));
+    if (C_out == C) return fail(OCX_E_ARG, "C_out must not alias C");
+    if (flags & ~63u) return fail(OCX_E_ARG, "unknown flag bits 0x%x", flags & ~63u);
+    if (flux_face_i != -1 && (flux_face_i < 0 || flux_face_i > nx))
+        return fail(OCX_E_ARG, "flux_face_i %d outside 0..%d", flux_face_i, nx);
+    if (flux_face_i >= 0 && !fx_i) return fail(OCX_E_ARG, "fx_i is NULL with flux_face_i = %d", flux_face_i);
+    const size_t wsb = fct_layout(nullptr, nt, nz, ny, nx, nullptr);
+    if (!workspace || workspace_bytes < wsb)
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed, %zu given", wsb, workspace ? workspace_bytes : 0);
+    FctWs w;
+    fct_layout(workspace, nt, nz, ny, nx, &w);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
+    const bool js = flags & WENO_JS, unlim = flags & UNLIMITED;
+    const int fi = fx_i ? flux_face_i : -1;
+    // stage 1: F0, q1, c1
+    {
+    ProfScope _ps("fct_stage1", s);
+    if (js)
+        k_fct_stage<1, false, TX, TY><<<grid, block, 0, s>>>(C, C, inflow, V0, V1, Qx, Qy, ok, nullptr, w.fx, w.fy,
+                                                             w.q1, w.c1, nt, nz, ny, nx, dt, flags);
+    else
+        k_fct_stage<1, true, TX, TY><<<grid, block, 0, s>>>(C, C, inflow, V0, V1, Qx, Qy, ok, nullptr, w.fx, w.fy,
+                                                            w.q1, w.c1, nt, nz, ny, nx, dt, flags);
+    }
+    OCX_TRY(launch_check("fct stage 1"));
+    // stage 2: F0 + F1, c2
+    {
+    ProfScope _ps("fct_stage2", s);
+    if (js)
+        k_fct_stage<2, false, TX, TY><<<grid, block, 0, s>>>(w.c1, C, inflow, V0, V1, Qx, Qy, ok, w.q1, w.fx, w.fy,
+                                                             nullptr, w.c2, nt, nz, ny, nx, dt, flags);
+    else
+        k_fct_stage<2, true, TX, TY><<<grid, block, 0, s>>>(w.c1, C, inflow, V0, V1, Qx, Qy, ok, w.q1, w.fx, w.fy,
+                                                            nullptr, w.c2, nt, nz, ny, nx, dt, flags);
+    }
+    OCX_TRY(launch_check("fct stage 2"));
+    if (unlim) {
+        {
+        ProfScope _ps("fct_unlimited", s);
+        if (js)
+            k_fct_high<false, true, TX, TY><<<grid, block, 0, s>>>(w.c2, C, inflow, V0, V1, Qx, Qy, ok, w.fx, w.fy,
+                                                                   nullptr, nullptr, C_out, fx_i, fi, nt, nz, ny, nx,
+                                                                   dt, flags);
+        else
+            k_fct_high<true, true, TX, TY><<<grid, block, 0, s>>>(w.c2, C, inflow, V0, V1, Qx, Qy, ok, w.fx, w.fy,
+                                                                  nullptr, nullptr, C_out, fx_i, fi, nt, nz, ny, nx,
+                                                                  dt, flags);
+        }
+        OCX_TRY(launch_check("fct unlimited"));
+        if (open_flux) {
+            // the unlimited step keeps no open-flux record (advect_fct_numpy returns before it, tracers.py:287-289)
+            k_fill<<<1, 256, 0, s>>>(open_flux, 0.0, (size_t)nt);
+            OCX_TRY(launch_check("open_flux clear"));
+        }
+        return OCX_OK;
+    }
+    // stage 3: F2, A = F_H - F_L, c_L (into q1)
+    {
+    ProfScope _ps("fct_stage3_split", s);
+    if (js)
+        k_fct_high<false, false, TX, TY><<<grid, block, 0, s>>>(w.c2, C, inflow, V0, V1, Qx, Qy, ok, w.fx, w.fy, w.ax,
+                                                                w.ay, w.q1, nullptr, -1, nt, nz, ny, nx, dt, flags);
+    else
+        k_fct_high<true, false, TX, TY><<<grid, block, 0, s>>>(w.c2, C, inflow, V0, V1, Qx, Qy, ok, w.fx, w.fy, w.ax,
+                                                               w.ay, w.q1, nullptr, -1, nt, nz, ny, nx, dt, flags);
+    }
+    OCX_TRY(launch_check("fct high/low split"));
+    // ratios R+- into c1 / c2
+    {
+        ProfScope _ps("fct_ratio", s);
+        k_fct_ratio<TX, TY><<<grid, block, 0, s>>>(C, w.q1, V1, w.ax, w.ay, ok, w.c1, w.c2, nt, nz, ny, nx, dt);
+    }
+    OCX_TRY(launch_check("fct ratios"));
+    {
+        ProfScope _ps("fct_final", s);
+        k_fct_final<TX, TY><<<grid, block, 0, s>>>(C, inflow, V0, V1, Qx, Qy, w.ax, w.ay, w.c1, w.c2, C_out, fx_i, fi,
+                                                   open_flux ? w.gedge : nullptr, nt, nz, ny, nx, dt, flags);
+    }
+    OCX_TRY(launch_check("fct final"));
+    if (open_flux) {
+        k_open_flux<<<nt, 256, 0, s>>>(w.gedge, open_flux, nt, nz, ny, nx, flags);
+        OCX_TRY(launch_check("open flux"));
+    }
+    return OCX_OK;
+}
+
+int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                       const double* Fz, const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx,
+                       double dt, unsigned flags, int vert, int flux_face_i, double* C_out, double* fx_i,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(C, "C"));
+    OCX_TRY(need(V0, "V0"));
+    OCX_TRY(need(V1, "V1"));
+    OCX_TRY(need(Qx, "Qx"));
+    OCX_TRY(need(Qy, "Qy"));
+    OCX_TRY(need(ok, "ok"));
+    OCX_TRY(need(C_out, "C_out"));
+    if (vert) OCX_TRY(need(Fz, "Fz"));
+    if (C_out == C) return fail(OCX_E_ARG, "C_out must not alias C");
+    if (flags & ~31u) return fail(OCX_E_ARG, "flags 0x%x not valid for the RK3 step", flags);
+    if (flux_face_i != -1 && (flux_face_i < 0 || flux_face_i > nx))
+        return fail(OCX_E_ARG, "flux_face_i %d outside 0..%d", flux_face_i, nx);
+    if (flux_face_i >= 0 && !fx_i) return fail(OCX_E_ARG, "fx_i is NULL with flux_face_i = %d", flux_face_i);
+    const size_t wsb = rk3_layout(nullptr, nt, nz, ny, nx, nullptr);
+    if (!workspace || workspace_bytes < wsb)
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed, %zu given", wsb, workspace ? workspace_bytes : 0);
+    Rk3Ws w;
+    rk3_layout(workspace, nt, nz, ny, nx, &w);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
+    const bool js = flags & WENO_JS;
+    const int fi = fx_i ? flux_face_i : -1;
+    const size_t col = (size_t)nt * nz * ny;
+    double* f0 = fx_i ? w.fxs : nullptr;
+    double* f1 = fx_i ? w.fxs + col : nullptr;
+    double* f2 = fx_i ? w.fxs + 2 * col : nullptr;
+#define OCX_RK(STAGE, CIN, COUT, FS)                                                                                  \
+    do {                                                                                                             \
+        ProfScope _ps("rk3_stage" #STAGE, s);                                                                        \
+        if (js && vert)                                                                                              \
+            k_rk_stage<STAGE, false, true, TX, TY><<<grid, block, 0, s>>>(CIN, C, inflow, V0, V1, Qx, Qy, Fz, ok, w.q, \
+                                                                          COUT, FS, fi, nt, nz, ny, nx, dt, flags);   \
+        else if (js)                                                                                                 \
+            k_rk_stage<STAGE, false, false, TX, TY><<<grid, block, 0, s>>>(CIN, C, inflow, V0, V1, Qx, Qy, Fz, ok,    \
+                                                                           w.q, COUT, FS, fi, nt, nz, ny, nx, dt,     \
+                                                                           flags);                                    \
+        else if (vert)                                                                                               \
+            k_rk_stage<STAGE, true, true, TX, TY><<<grid, block, 0, s>>>(CIN, C, inflow, V0, V1, Qx, Qy, Fz, ok, w.q,  \
+                                                                         COUT, FS, fi, nt, nz, ny, nx, dt, flags);    \
+        else                                                                                                         \
+            k_rk_stage<STAGE, true, false, TX, TY><<<grid, block, 0, s>>>(CIN, C, inflow, V0, V1, Qx, Qy, Fz, ok, w.q, \
+                                                                          COUT, FS, fi, nt, nz, ny, nx, dt, flags);   \
+        OCX_TRY(launch_check("rk3 stage " #STAGE));                                                                  \
+    } while (0)
+    OCX_RK(1, C, w.c1, f0);
+    OCX_RK(2, w.c1, w.c2, f1);
+    OCX_RK(3, w.c2, C_out, f2);
+#undef OCX_RK
+    if (fx_i) {
+        k_rk_face_column<<<(unsigned)((col + 255) / 256), 256, 0, s>>>(f0, f1, f2, fx_i, col);
+        OCX_TRY(launch_check("rk3 face column"));
+    }
+    return OCX_OK;
+}
+
+int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const uint8_t* ok, const double* ext,
+                    int nt, int nz, int ny, int nx, unsigned flags, double* fx, double* fy, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(c, "c"));
+    OCX_TRY(need(Qx, "Qx"));
+    OCX_TRY(need(Qy, "Qy"));
+    OCX_TRY(need(ok, "ok"));
+    OCX_TRY(need(fx, "fx"));
+    OCX_TRY(need(fy, "fy"));
+    if (flags & ~31u) return fail(OCX_E_ARG, "flags 0x%x not valid for faces_h", flags);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
+    if (flags & WENO_JS)
+        k_fct_stage<0, false, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, ok, nullptr, fx, fy,
+                                                             nullptr, nullptr, nt, nz, ny, nx, 0.0, flags);
+    else
+        k_fct_stage<0, true, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, ok, nullptr, fx, fy,
+                                                            nullptr, nullptr, nt, nz, ny, nx, 0.0, flags);
+    return launch_check("faces_h");
+}
+
+int ocx_upwind_faces_h_f64(const double* c, const double* Qx, const double* Qy, const double* ext, int nt, int nz,
+                           int ny, int nx, unsigned flags, double* fx, double* fy, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(c, "c"));
+    OCX_TRY(need(Qx, "Qx"));
+    OCX_TRY(need(Qy, "Qy"));
+    OCX_TRY(need(fx, "fx"));
+    OCX_TRY(need(fy, "fy"));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t total = (size_t)nt * nz * ny * (nx + 1) + (size_t)nt * nz * (ny + 1) * nx;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 20);
+    k_upwind_faces<<<blocks, 256, 0, s>>>(c, ext, Qx, Qy, fx, fy, nt, nz, ny, nx, flags);
+    return launch_check("upwind_faces_h");
+}
+
+int ocx_rhs_f64(const double* c, const double* Qx, const double* Qy, const double* Fz, const uint8_t* ok,
+                const double* ext, int nt, int nz, int ny, int nx, unsigned flags, int vert, double* out,
+                void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(c, "c"));
+    OCX_TRY(need(Qx, "Qx"));
+    OCX_TRY(need(Qy, "Qy"));
+    OCX_TRY(need(ok, "ok"));
+    OCX_TRY(need(out, "out"));
+    if (vert) OCX_TRY(need(Fz, "Fz"));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
+    const bool js = flags & WENO_JS;
+    if (js && vert)
+        k_rk_stage<0, false, true, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr,
+                                                                  out, nullptr, -1, nt, nz, ny, nx, 0.0, flags);
+    else if (js)
+        k_rk_stage<0, false, false, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok,
+                                                                   nullptr, out, nullptr, -1, nt, nz, ny, nx, 0.0,
+                                                                   flags);
+    else if (vert)
+        k_rk_stage<0, true, true, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr,
+                                                                 out, nullptr, -1, nt, nz, ny, nx, 0.0, flags);
+    else
+        k_rk_stage<0, true, false, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr,
+                                                                  out, nullptr, -1, nt, nz, ny, nx, 0.0, flags);
+    return launch_check("rhs");
+}
+
+int ocx_fct_coefficients_f64(const double* c0, const double* cL, const double* V1, const double* Ax,
+                             const double* Ay, const uint8_t* ok, double dt, int nt, int nz, int ny, int nx,
+                             double* Cx, double* Cy, void* workspace, size_t workspace_bytes, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(c0, "c0"));
+    OCX_TRY(need(cL, "cL"));
+    OCX_TRY(need(V1, "V1"));
+    OCX_TRY(need(Ax, "Ax"));
+    OCX_TRY(need(Ay, "Ay"));
+    OCX_TRY(need(ok, "ok"));
+    OCX_TRY(need(Cx, "Cx"));
+    OCX_TRY(need(Cy, "Cy"));
+    const size_t S = (size_t)nt * nz * ny * nx;
+    if (!workspace || workspace_bytes < 2 * S * sizeof(double))
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed", 2 * S * sizeof(double));
+    double* r_in = static_cast<double*>(workspace);
+    double* r_out = r_in + S;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
+    k_fct_ratio<TX, TY><<<grid, block, 0, s>>>(c0, cL, V1, Ax, Ay, ok, r_in, r_out, nt, nz, ny, nx, dt);
+    OCX_TRY(launch_check("fct ratios"));
+    const size_t total = (size_t)nt * nz * ny * (nx + 1) + (size_t)nt * nz * (ny + 1) * nx;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 20);
+    k_fct_face_coeff<<<blocks, 256, 0, s>>>(Ax, Ay, r_in, r_out, Cx, Cy, nt, nz, ny, nx);
+    return launch_check("fct face coefficients");
+}
+
+int ocx_periodic_pad(const void* src, void* dst, size_t lead, int sy, int sx, int kind, int pad_y, int pad_x,
+                     int elem, int crop, void* stream) {
+    using namespace ocx;
+    OCX_TRY(need(src, "src"));
+    OCX_TRY(need(dst, "dst"));
+    if (sy < 1 || sx < 1 || pad_x < 0 || pad_y < 0 || kind < 0 || kind > 2 || (elem != 8 && elem != 1))
+        return fail(OCX_E_ARG, "bad pad arguments (sy=%d sx=%d kind=%d pad=%d,%d elem=%d)", sy, sx, kind, pad_y, pad_x,
+                    elem);
+    const int fx = kind == 1, fy = kind == 2;
+    int dy, dx, ny_u = sy - fy, nx_u = sx - fx;
+    if (crop) {
+        dy = sy - 2 * pad_y;
+        dx = sx - 2 * pad_x;
+        if (dy < 1 || dx < 1) return fail(OCX_E_ARG, "crop of %dx%d by %d,%d is empty", sy, sx, pad_y, pad_x);
+    } else {
+        dy = sy + 2 * pad_y;
+        dx = sx + 2 * pad_x;
+        if ((pad_y && ny_u < 1) || (pad_x && nx_u < 1)) return fail(OCX_E_ARG, "nothing to wrap");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t total = lead * (size_t)dy * dx;
+    if (total == 0) return OCX_OK;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 1u << 20);
+    if (elem == 8)
+        k_pad<double><<<blocks, 256, 0, s>>>(static_cast<const double*>(src), static_cast<double*>(dst), lead, sy, sx, dy,
+                                             dx, pad_y, pad_x, ny_u, nx_u, crop);
+    else
+        k_pad<uint8_t><<<blocks, 256, 0, s>>>(static_cast<const uint8_t*>(src), static_cast<uint8_t*>(dst), lead, sy,
+                                              sx, dy, dx, pad_y, pad_x, ny_u, nx_u, crop);
+    return launch_check("periodic pad");
+}
+
+int ocx_profile_enable(int on) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof_on = on != 0;
+    return OCX_OK;
+}
+
+int ocx_profile_collect(int max_kernels, const char** names, double* ms_total, long long* launches) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    std::vector<const char*> keys;
+    std::vector<double> ms;
+    std::vector<long long> cnt;
+    for (auto& r : g_prof) {
+        if (hipEventSynchronize(r.b) != hipSuccess) return fail(OCX_E_HIP, "profile event wait failed");
+        float e = 0.f;
+        if (hipEventElapsedTime(&e, r.a, r.b) != hipSuccess) return fail(OCX_E_HIP, "profile event time failed");
+        size_t m = 0;
+        while (m < keys.size() && std::strcmp(keys[m], r.name) != 0) ++m;
+        if (m == keys.size()) {
+            keys.push_back(r.name);
+            ms.push_back(0.0);
+            cnt.push_back(0);
+        }
+        ms[m] += e;
+        cnt[m] += 1;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof.clear();
+    const int n = (int)keys.size();
+    for (int m = 0; m < n && m < max_kernels; ++m) {
+        if (names) names[m] = keys[m];
+        if (ms_total) ms_total[m] = ms[m];
+        if (launches) launches[m] = cnt[m];
+    }
+    return n;
+}
+
+int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(c, "c"));
+    OCX_TRY(need(V, "V"));
+    OCX_TRY(need(out, "out"));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    k_content<<<nt, 1024, 0, s>>>(c, V, out, (size_t)nz * ny * nx);
+    return launch_check("content");
+}
+
+}  // extern "C"

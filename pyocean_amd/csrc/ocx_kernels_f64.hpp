@@ -1,0 +1,635 @@
+// Plain FP64 tracer-advection kernels for gfx950: SSP-RK3 stages with WENO5 faces, the Zalesak FCT limiter and
+// the h*T flux-divergence update, on the masked C-grid (reference: core/dynamics/sigma_baroclinic.py:265-530,
+// 856-875; core/components/tracers.py:208-375).
+//
+// Layout (C-contiguous, k = 0 at the bed): stacks (nt, n, ny, nx); cells V0, V1, ok (n, ny, nx); faces
+// Qx (n, ny, nx+1), Qy (n, ny+1, nx), Fz (n+1, ny, nx); face stacks fx (nt, n, ny, nx+1), fy (nt, n, ny+1, nx).
+//
+// Every block owns a TX x TY tile of one (tracer, layer) plane.  It stages the input tile with a 3-cell halo in
+// LDS (x-contiguous rows, one 64-lane wavefront per row for TX = 64), reconstructs each face of the tile once into
+// LDS, and differences the faces of its own cells.  A face on a tile border is computed by both blocks that touch
+// it (for the divergence) but written to global memory only by its owner (faces lo..lo+TX-1 of the tile, plus the
+// domain's last face), so read-modify-write of face stacks never races.
+#pragma once
+#include "ocx_weno.hpp"
+
+namespace ocx {
+
+// ------------------------------------------------------------------------------------------------ tile loads
+template <int TX, int TY, int H>
+__device__ __forceinline__ void load_tile(double* sc, uint8_t* so, const double* __restrict__ src,
+                                          const uint8_t* __restrict__ okp, int i0, int j0, int ny, int nx) {
+    constexpr int LX = TX + 2 * H, LY = TY + 2 * H;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    for (int idx = tid; idx < LY * LX; idx += TX * TY) {
+        const int ly = idx / LX, lx = idx - ly * LX;
+        const int j = j0 - H + ly, i = i0 - H + lx;
+        const bool in = j >= 0 && j < ny && i >= 0 && i < nx;
+        sc[idx] = in ? src[(size_t)j * nx + i] : 0.0;
+        if (so) so[idx] = in ? okp[(size_t)j * nx + i] : (uint8_t)0;
+    }
+}
+
+// WENO face fluxes q * c_face of the tile's faces into LDS (sfx: TY rows x TX+1 faces; sfy: TY+1 faces x TX
+// columns), from the input tile sc/so (halo 3).  Rules of _advect_faces_h (sigma_baroclinic.py:265-321):
+// interior faces with q == 0 stay 0; open edges take ext on inflow, the edge cell on outflow; closed edges 0.
+// ext == nullptr: the input itself is the exterior.
+template <bool Z, int TX, int TY>
+__device__ __forceinline__ void weno_tile_faces(double* sfx, double* sfy, const double* sc, const uint8_t* so,
+                                                const double* __restrict__ Qxk, const double* __restrict__ Qyk,
+                                                const double* __restrict__ extp, int i0, int j0, int ny, int nx,
+                                                unsigned flags) {
+    constexpr int H = 3, LX = TX + 2 * H;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    for (int idx = tid; idx < TY * (TX + 1); idx += TX * TY) {
+        const int ly = idx / (TX + 1), lf = idx - ly * (TX + 1);
+        const int j = j0 + ly, f = i0 + lf;
+        double val = 0.0;
+        if (j < ny && f <= nx) {
+            const double q = Qxk[(size_t)j * (nx + 1) + f];
+            const double* row = sc + (ly + H) * LX;
+            const uint8_t* orow = so + (ly + H) * LX;
+            if (f > 0 && f < nx) {
+                if (q != 0.0) val = q * lds_face<Z>(row, orow, 1, lf + H, q);
+            } else if (f == 0) {
+                if (flags & OPEN_W) {
+                    const double cc = row[lf + H];
+                    const double e = extp ? extp[(size_t)j * nx] : cc;
+                    val = q * (q > 0.0 ? e : cc);
+                }
+            } else if (flags & OPEN_E) {
+                const double cc = row[lf + H - 1];
+                const double e = extp ? extp[(size_t)j * nx + nx - 1] : cc;
+                val = q * (q < 0.0 ? e : cc);
+            }
+        }
+        sfx[idx] = val;
+    }
+    for (int idx = tid; idx < (TY + 1) * TX; idx += TX * TY) {
+        const int lf = idx / TX, lx = idx - lf * TX;
+        const int f = j0 + lf, i = i0 + lx;
+        double val = 0.0;
+        if (i < nx && f <= ny) {
+            const double q = Qyk[(size_t)f * nx + i];
+            const double* col = sc + lx + H;
+            const uint8_t* ocol = so + lx + H;
+            if (f > 0 && f < ny) {
+                if (q != 0.0) val = q * lds_face<Z>(col, ocol, LX, lf + H, q);
+            } else if (f == 0) {
+                if (flags & OPEN_S) {
+                    const double cc = col[(lf + H) * LX];
+                    const double e = extp ? extp[i] : cc;
+                    val = q * (q > 0.0 ? e : cc);
+                }
+            } else if (flags & OPEN_N) {
+                const double cc = col[(lf + H - 1) * LX];
+                const double e = extp ? extp[(size_t)(ny - 1) * nx + i] : cc;
+                val = q * (q < 0.0 ? e : cc);
+            }
+        }
+        sfy[idx] = val;
+    }
+}
+
+// Donor-cell face flux F_L = q * c_up of x-face f (row pointers of C and of the exterior values), rules of
+// _fct_fluxes (sigma_baroclinic.py:482-491): interior q >= 0 takes the low side; open edges as the WENO faces;
+// closed edges 0.
+__device__ __forceinline__ double donor_x(const double* __restrict__ crow, const double* __restrict__ erow,
+                                          double q, int f, int nx, unsigned flags) {
+    if (f > 0 && f < nx) return q * (q >= 0 ? crow[f - 1] : crow[f]);
+    if (f == 0) return (flags & OPEN_W) ? q * (q > 0 ? erow[0] : crow[0]) : 0.0;
+    return (flags & OPEN_E) ? q * (q < 0 ? erow[nx - 1] : crow[nx - 1]) : 0.0;
+}
+
+// Same for y-face f of column i (cp, ep: plane pointers).
+__device__ __forceinline__ double donor_y(const double* __restrict__ cp, const double* __restrict__ ep, double q,
+                                          int f, int i, int ny, int nx, unsigned flags) {
+    if (f > 0 && f < ny) return q * (q >= 0 ? cp[(size_t)(f - 1) * nx + i] : cp[(size_t)f * nx + i]);
+    if (f == 0) return (flags & OPEN_S) ? q * (q > 0 ? ep[i] : cp[i]) : 0.0;
+    return (flags & OPEN_N) ? q * (q < 0 ? ep[(size_t)(ny - 1) * nx + i] : cp[(size_t)(ny - 1) * nx + i]) : 0.0;
+}
+
+__device__ __forceinline__ double dmax(double a, double b) { return b > a ? b : a; }
+__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+
+// ------------------------------------------------------------------------------------------------ FCT stages
+// MODE 0: faces only (fx, fy of cin; all owned faces written).
+// MODE 1: FCT stage 1 - F0 = faces(C) written to fx/fy, q1 = q0 - dt div F0, c1 = q1/Va (safe).
+// MODE 2: FCT stage 2 - F1 = faces(c1), fx/fy <- F0 + F1 in place, q2 = 0.75 q0 + 0.25 (q1 - dt div F1),
+//         c2 = q2/Vb (safe).  (_fct_stage, sigma_baroclinic.py:452-468, with Va = V0 + dV, Vb = V0 + dV/2.)
+template <int MODE, bool Z, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_fct_stage(const double* __restrict__ cin, const double* __restrict__ C,
+                                                      const double* __restrict__ ext, const double* __restrict__ V0,
+                                                      const double* __restrict__ V1, const double* __restrict__ Qx,
+                                                      const double* __restrict__ Qy, const uint8_t* __restrict__ ok,
+                                                      const double* __restrict__ qprev, double* __restrict__ fx,
+                                                      double* __restrict__ fy, double* __restrict__ qout,
+                                                      double* __restrict__ cout, int nt, int n, int ny, int nx,
+                                                      double dt, unsigned flags) {
+    constexpr int H = 3, LX = TX + 2 * H, LY = TY + 2 * H;
+    __shared__ double sc[LY * LX];
+    __shared__ uint8_t so[LY * LX];
+    __shared__ double sfx[TY * (TX + 1)];
+    __shared__ double sfy[(TY + 1) * TX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const size_t plane = (size_t)ny * nx;
+    for (int p = blockIdx.z; p < nt * n; p += gridDim.z) {
+        const int k = p % n;
+        const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
+        __syncthreads();
+        load_tile<TX, TY, H>(sc, so, cin + pc, ok + kc, i0, j0, ny, nx);
+        __syncthreads();
+        weno_tile_faces<Z, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
+                                   ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
+        __syncthreads();
+        // owned faces to global memory
+        for (int idx = tid; idx < TY * (TX + 1); idx += TX * TY) {
+            const int ly = idx / (TX + 1), lf = idx - ly * (TX + 1);
+            const int j = j0 + ly, f = i0 + lf;
+            if (j < ny && f <= nx && (lf < TX || f == nx)) {
+                const size_t g = ((size_t)p * ny + j) * (nx + 1) + f;
+                fx[g] = MODE == 2 ? fx[g] + sfx[idx] : sfx[idx];
+            }
+        }
+        for (int idx = tid; idx < (TY + 1) * TX; idx += TX * TY) {
+            const int lf = idx / TX, lx = idx - lf * TX;
+            const int f = j0 + lf, i = i0 + lx;
+            if (i < nx && f <= ny && (lf < TY || f == ny)) {
+                const size_t g = ((size_t)p * (ny + 1) + f) * nx + i;
+                fy[g] = MODE == 2 ? fy[g] + sfy[idx] : sfy[idx];
+            }
+        }
+        if (MODE != 0) {
+            const int j = j0 + ty, i = i0 + tx;
+            if (j < ny && i < nx) {
+                const double d = sfx[ty * (TX + 1) + tx + 1] - sfx[ty * (TX + 1) + tx] + sfy[(ty + 1) * TX + tx] -
+                                 sfy[ty * TX + tx];
+                const size_t x = pc + (size_t)j * nx + i, xv = kc + (size_t)j * nx + i;
+                const double c0 = C[x], v0 = V0[xv], v1 = V1[xv];
+                const double q0 = v0 * c0, dV = v1 - v0;
+                double q, V;
+                if (MODE == 1) {
+                    q = 0.0 * q0 + 1.0 * (q0 - dt * d);
+                    V = v0 + dV;
+                    qout[x] = q;
+                } else {
+                    q = 0.75 * q0 + 0.25 * (qprev[x] - dt * d);
+                    V = v0 + 0.5 * dV;
+                }
+                cout[x] = V > 0 ? q / V : c0;
+            }
+        }
+    }
+}
+
+// FCT stage 3 and the high/low-order split: F2 = faces(c2), F_H = (F0 + F1)/6 + 2 F2/3 (S = F0 + F1 from stage 2),
+// donor-cell F_L from C, A = F_H - F_L written to ax/ay (owned faces), and the low-order solution
+// c_L = (q0 - dt div F_L)/V1 (safe) written to cL (_fct_fluxes :471-511 and _fct_stage :452-468).
+// UNLIM (C = 1, advect_fct_numpy's unlimited branch, tracers.py:286-289): out = (q0 - dt div F_H)/V1 (safe),
+// and the face column fxi of F_H.
+template <bool Z, bool UNLIM, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_fct_high(const double* __restrict__ c2, const double* __restrict__ C,
+                                                     const double* __restrict__ ext, const double* __restrict__ V0,
+                                                     const double* __restrict__ V1, const double* __restrict__ Qx,
+                                                     const double* __restrict__ Qy, const uint8_t* __restrict__ ok,
+                                                     const double* __restrict__ Sx, const double* __restrict__ Sy,
+                                                     double* __restrict__ ax, double* __restrict__ ay,
+                                                     double* __restrict__ cl_out, double* __restrict__ fxi,
+                                                     int flux_face_i, int nt, int n, int ny, int nx, double dt,
+                                                     unsigned flags) {
+    constexpr int H = 3, LX = TX + 2 * H, LY = TY + 2 * H;
+    __shared__ double sc[LY * LX];
+    __shared__ uint8_t so[LY * LX];
+    __shared__ double sfx[TY * (TX + 1)];
+    __shared__ double sfy[(TY + 1) * TX];
+    __shared__ double slx[TY * (TX + 1)];
+    __shared__ double sly[(TY + 1) * TX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const size_t plane = (size_t)ny * nx;
+    for (int p = blockIdx.z; p < nt * n; p += gridDim.z) {
+        const int k = p % n;
+        const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
+        const double* Qxk = Qx + (size_t)k * ny * (nx + 1);
+        const double* Qyk = Qy + (size_t)k * (ny + 1) * nx;
+        const double* cp = C + pc;
+        const double* ep = ext ? ext + pc : cp;          // donor exterior: inflow, else C
+        __syncthreads();
+        load_tile<TX, TY, H>(sc, so, c2 + pc, ok + kc, i0, j0, ny, nx);
+        __syncthreads();
+        weno_tile_faces<Z, TX, TY>(sfx, sfy, sc, so, Qxk, Qyk, ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
+        __syncthreads();
+        for (int idx = tid; idx < TY * (TX + 1); idx += TX * TY) {
+            const int ly = idx / (TX + 1), lf = idx - ly * (TX + 1);
+            const int j = j0 + ly, f = i0 + lf;
+            if (j < ny && f <= nx) {
+                const size_t g = ((size_t)p * ny + j) * (nx + 1) + f;
+                const double fh = Sx[g] / 6.0 + 2.0 * sfx[idx] / 3.0;
+                if (UNLIM) {
+                    sfx[idx] = fh;
+                    if (fxi && f == flux_face_i) fxi[(size_t)p * ny + j] = fh;
+                } else {
+                    const double fl = donor_x(cp + (size_t)j * nx, ep + (size_t)j * nx, Qxk[(size_t)j * (nx + 1) + f],
+                                              f, nx, flags);
+                    slx[idx] = fl;
+                    const double a = fh - fl;
+                    if (lf < TX || f == nx) ax[g] = a;
+                }
+            }
+        }
+        for (int idx = tid; idx < (TY + 1) * TX; idx += TX * TY) {
+            const int lf = idx / TX, lx = idx - lf * TX;
+            const int f = j0 + lf, i = i0 + lx;
+            if (i < nx && f <= ny) {
+                const size_t g = ((size_t)p * (ny + 1) + f) * nx + i;
+                const double fh = Sy[g] / 6.0 + 2.0 * sfy[idx] / 3.0;
+                if (UNLIM) {
+                    sfy[idx] = fh;
+                } else {
+                    const double fl = donor_y(cp, ep, Qyk[(size_t)f * nx + i], f, i, ny, nx, flags);
+                    sly[idx] = fl;
+                    const double a = fh - fl;
+                    if (lf < TY || f == ny) ay[g] = a;
+                }
+            }
+        }
+        __syncthreads();
+        const int j = j0 + ty, i = i0 + tx;
+        if (j < ny && i < nx) {
+            const double* gx = UNLIM ? sfx : slx;
+            const double* gy = UNLIM ? sfy : sly;
+            const double d = gx[ty * (TX + 1) + tx + 1] - gx[ty * (TX + 1) + tx] + gy[(ty + 1) * TX + tx] -
+                             gy[ty * TX + tx];
+            const size_t x = pc + (size_t)j * nx + i, xv = kc + (size_t)j * nx + i;
+            const double c0 = C[x], v0 = V0[xv], v1 = V1[xv];
+            const double q0 = v0 * c0;
+            const double q = UNLIM ? q0 - dt * d : 0.0 * q0 + 1.0 * (q0 - dt * d);
+            cl_out[x] = v1 > 0 ? q / v1 : c0;
+        }
+    }
+}
+
+// Zalesak ratios (eqs. 7-13) of each ok cell: bounds over the cell, i +- 1, j +- 1 and k +- 1 (ok cells only) of
+// C and c_L; P+- from the antidiffusive fluxes on the cell's faces; R = min(1, Q/P) or 0
+// (_fct_coeff_kernel, sigma_baroclinic.py:413-431).  Cells not ok get R+- = 0.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_fct_ratio(const double* __restrict__ C, const double* __restrict__ cL,
+                                                      const double* __restrict__ V1, const double* __restrict__ ax,
+                                                      const double* __restrict__ ay, const uint8_t* __restrict__ ok,
+                                                      double* __restrict__ r_in, double* __restrict__ r_out, int nt,
+                                                      int n, int ny, int nx, double dt) {
+    const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TY + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const size_t plane = (size_t)ny * nx;
+    for (int p = blockIdx.z; p < nt * n; p += gridDim.z) {
+        const int t = p / n, k = p - t * n;
+        const size_t xc = (size_t)j * nx + i, x = (size_t)p * plane + xc;
+        double ri = 0.0, ro = 0.0;
+        if (ok[(size_t)k * plane + xc]) {
+            double cmax = -INFINITY, cmin = INFINITY;
+            const int dk[7] = {0, 0, 0, 0, 0, -1, 1}, dj[7] = {0, 0, 0, -1, 1, 0, 0}, di[7] = {0, -1, 1, 0, 0, 0, 0};
+#pragma unroll
+            for (int m = 0; m < 7; ++m) {
+                const int kk = k + dk[m], jj = j + dj[m], ii = i + di[m];
+                if (kk >= 0 && kk < n && jj >= 0 && jj < ny && ii >= 0 && ii < nx) {
+                    const size_t xn = (size_t)jj * nx + ii;
+                    if (ok[(size_t)kk * plane + xn]) {
+                        const size_t g = ((size_t)t * n + kk) * plane + xn;
+                        const double a = C[g], b = cL[g];
+                        cmax = dmax(dmax(cmax, a), b);
+                        cmin = dmin(dmin(cmin, a), b);
+                    }
+                }
+            }
+            const size_t gx = ((size_t)p * ny + j) * (nx + 1) + i;
+            const size_t gy = ((size_t)p * (ny + 1) + j) * nx + i;
+            const double aw = ax[gx], ae = ax[gx + 1], as = ay[gy], an = ay[gy + nx];
+            const double p_in = dt * (dmax(aw, 0.0) - dmin(ae, 0.0) + dmax(as, 0.0) - dmin(an, 0.0));
+            const double p_out = dt * (dmax(ae, 0.0) - dmin(aw, 0.0) + dmax(an, 0.0) - dmin(as, 0.0));
+            const double v1 = V1[(size_t)k * plane + xc], cl = cL[x];
+            const double q_in = dmax(v1 * (cmax - cl), 0.0);
+            const double q_out = dmax(v1 * (cl - cmin), 0.0);
+            ri = p_in > 0.0 ? dmin(1.0, q_in / p_in) : 0.0;
+            ro = p_out > 0.0 ? dmin(1.0, q_out / p_out) : 0.0;
+        }
+        r_in[x] = ri;
+        r_out[x] = ro;
+    }
+}
+
+// Face coefficients from the ratios of the two adjacent cells (_fct_coeff_kernel :436-447): x-face f (interior)
+// C = min(R_in[f], R_out[f-1]) if A >= 0 else min(R_in[f-1], R_out[f]); edge faces 0.
+__device__ __forceinline__ double coeff(double a, double rin_hi, double rout_lo, double rin_lo, double rout_hi) {
+    return a >= 0.0 ? dmin(rin_hi, rout_lo) : dmin(rin_lo, rout_hi);
+}
+
+// Final update (eq. 14, _fct_final :514-530): G = F_L + C A on the four faces, q = q0 - dt div G, out = q/V1
+// (safe).  Side outputs: the face column fxi (flux_face_i) and the open-edge face fluxes (gw, ge: (nt, n, ny);
+// gs, gn: (nt, n, nx)) for the content budget.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_fct_final(const double* __restrict__ C, const double* __restrict__ ext,
+                                                      const double* __restrict__ V0, const double* __restrict__ V1,
+                                                      const double* __restrict__ Qx, const double* __restrict__ Qy,
+                                                      const double* __restrict__ ax, const double* __restrict__ ay,
+                                                      const double* __restrict__ r_in, const double* __restrict__ r_out,
+                                                      double* __restrict__ out, double* __restrict__ fxi,
+                                                      int flux_face_i, double* __restrict__ gedge, int nt, int n,
+                                                      int ny, int nx, double dt, unsigned flags) {
+    const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TY + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const size_t plane = (size_t)ny * nx;
+    for (int p = blockIdx.z; p < nt * n; p += gridDim.z) {
+        const int k = p % n;
+        const size_t pc = (size_t)p * plane, xc = (size_t)j * nx + i, x = pc + xc;
+        const double* cp = C + pc;
+        const double* ep = ext ? ext + pc : cp;
+        const double* ri = r_in + pc;
+        const double* ro = r_out + pc;
+        const double* Qxr = Qx + ((size_t)k * ny + j) * (nx + 1);
+        const double* Qyk = Qy + (size_t)k * (ny + 1) * nx;
+        const size_t gx = ((size_t)p * ny + j) * (nx + 1) + i;
+        const size_t gy = ((size_t)p * (ny + 1) + j) * nx + i;
+        // west (f = i) and east (f = i + 1) faces
+        const double aw = ax[gx], ae = ax[gx + 1], as = ay[gy], an = ay[gy + nx];
+        const double cw = (i > 0) ? coeff(aw, ri[xc], ro[xc - 1], ri[xc - 1], ro[xc]) : 0.0;
+        const double ce = (i + 1 < nx) ? coeff(ae, ri[xc + 1], ro[xc], ri[xc], ro[xc + 1]) : 0.0;
+        const double cs = (j > 0) ? coeff(as, ri[xc], ro[xc - nx], ri[xc - nx], ro[xc]) : 0.0;
+        const double cn = (j + 1 < ny) ? coeff(an, ri[xc + nx], ro[xc], ri[xc], ro[xc + nx]) : 0.0;
+        const double* crow = cp + (size_t)j * nx;
+        const double* erow = ep + (size_t)j * nx;
+        const double Gw = donor_x(crow, erow, Qxr[i], i, nx, flags) + cw * aw;
+        const double Ge = donor_x(crow, erow, Qxr[i + 1], i + 1, nx, flags) + ce * ae;
+        const double Gs = donor_y(cp, ep, Qyk[(size_t)j * nx + i], j, i, ny, nx, flags) + cs * as;
+        const double Gn = donor_y(cp, ep, Qyk[(size_t)(j + 1) * nx + i], j + 1, i, ny, nx, flags) + cn * an;
+        const double d = Ge - Gw + Gn - Gs;
+        const double c0 = cp[xc];
+        const double q = V0[(size_t)k * plane + xc] * c0 - dt * d;
+        const double v1 = V1[(size_t)k * plane + xc];
+        out[x] = v1 > 0 ? q / v1 : c0;
+        if (fxi) {
+            if (flux_face_i == i) fxi[(size_t)p * ny + j] = Gw;
+            if (flux_face_i == nx && i == nx - 1) fxi[(size_t)p * ny + j] = Ge;
+        }
+        if (gedge) {
+            const size_t nyl = (size_t)nt * n * ny, nxl = (size_t)nt * n * nx;
+            if (i == 0) gedge[(size_t)p * ny + j] = Gw;
+            if (i == nx - 1) gedge[nyl + (size_t)p * ny + j] = Ge;
+            if (j == 0) gedge[2 * nyl + (size_t)p * nx + i] = Gs;
+            if (j == ny - 1) gedge[2 * nyl + nxl + (size_t)p * nx + i] = Gn;
+        }
+    }
+}
+
+// Vertical WENO face flux of interface kf (between layers kf-1 and kf, 1 <= kf <= n-1) of one column; same
+// validity rules along the layer index with each layer's own ok (_advect_div, sigma_baroclinic.py:335-353).
+template <bool Z>
+__device__ __forceinline__ double zface(const double* __restrict__ ccol, const uint8_t* __restrict__ ocol,
+                                        size_t plane, int n, int kf, double q) {
+    const bool pos = q >= 0.0;
+    int id[5];
+    id[0] = pos ? kf - 3 : kf + 2;
+    id[1] = pos ? kf - 2 : kf + 1;
+    id[2] = pos ? kf - 1 : kf;
+    id[3] = pos ? kf : kf - 1;
+    id[4] = pos ? kf + 1 : kf - 2;
+    double v[5];
+    bool b[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const bool in = id[m] >= 0 && id[m] < n;
+        v[m] = in ? ccol[(size_t)id[m] * plane] : 0.0;
+        b[m] = in && ocol[(size_t)id[m] * plane] != 0;
+    }
+    return q * face_value<Z>(v[0], v[1], v[2], v[3], v[4], b[0], b[1], b[2], b[3], b[4]);
+}
+
+// SSP-RK3 stage of TracerAdvection.advect (tracers.py:342-375, _rk_stage sigma_baroclinic.py:856-875) with the
+// fused RHS r = -div(Q c) of _advect_div (x, y and, with VERT, z faces).
+// STAGE 0: r only.  1: q1 = q0 + dt r, c1 = q1/Va.  2: q2 = 0.75 q0 + 0.25 (q1 + dt r), c2 = q2/Vb (q in place).
+// 3: q3 = q0/3 + 2/3 (q2 + dt r), out = q3/V1.  fxs (optional): the x-face column flux_face_i of this stage.
+template <int STAGE, bool Z, bool VERT, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ cin, const double* __restrict__ C,
+                                                     const double* __restrict__ ext, const double* __restrict__ V0,
+                                                     const double* __restrict__ V1, const double* __restrict__ Qx,
+                                                     const double* __restrict__ Qy, const double* __restrict__ Fz,
+                                                     const uint8_t* __restrict__ ok, double* __restrict__ q,
+                                                     double* __restrict__ cout, double* __restrict__ fxs,
+                                                     int flux_face_i, int nt, int n, int ny, int nx, double dt,
+                                                     unsigned flags) {
+    constexpr int H = 3, LX = TX + 2 * H, LY = TY + 2 * H;
+    __shared__ double sc[LY * LX];
+    __shared__ uint8_t so[LY * LX];
+    __shared__ double sfx[TY * (TX + 1)];
+    __shared__ double sfy[(TY + 1) * TX];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const size_t plane = (size_t)ny * nx;
+    for (int p = blockIdx.z; p < nt * n; p += gridDim.z) {
+        const int t = p / n, k = p - t * n;
+        const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
+        __syncthreads();
+        load_tile<TX, TY, H>(sc, so, cin + pc, ok + kc, i0, j0, ny, nx);
+        __syncthreads();
+        weno_tile_faces<Z, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
+                                   ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
+        __syncthreads();
+        const int j = j0 + ty, i = i0 + tx;
+        if (j < ny && i < nx) {
+            const size_t xc = (size_t)j * nx + i, x = pc + xc, xv = kc + xc;
+            double d = sfx[ty * (TX + 1) + tx + 1] - sfx[ty * (TX + 1) + tx] + sfy[(ty + 1) * TX + tx] -
+                       sfy[ty * TX + tx];
+            if (VERT) {
+                const double* ccol = cin + (size_t)t * n * plane + xc;
+                const uint8_t* ocol = ok + xc;
+                double ft = 0.0, fb = 0.0;
+                if (k + 1 < n) {
+                    const double qz = Fz[(size_t)(k + 1) * plane + xc];
+                    if (qz != 0.0) ft = zface<Z>(ccol, ocol, plane, n, k + 1, qz);
+                }
+                if (k > 0) {
+                    const double qz = Fz[(size_t)k * plane + xc];
+                    if (qz != 0.0) fb = zface<Z>(ccol, ocol, plane, n, k, qz);
+                }
+                d = d + ft - fb;
+            }
+            const double r = -d;
+            if (fxs) {
+                if (flux_face_i == i) fxs[(size_t)p * ny + j] = sfx[ty * (TX + 1) + tx];
+                if (flux_face_i == nx && i == nx - 1) fxs[(size_t)p * ny + j] = sfx[ty * (TX + 1) + tx + 1];
+            }
+            if (STAGE == 0) {
+                cout[x] = r;
+            } else {
+                const double c0 = C[x], v0 = V0[xv], v1 = V1[xv];
+                const double q0 = v0 * c0, dV = v1 - v0;
+                double qn, V;
+                if (STAGE == 1) {
+                    qn = q0 + dt * r;
+                    V = v0 + dV;
+                    q[x] = qn;
+                } else if (STAGE == 2) {
+                    qn = 0.75 * q0 + 0.25 * (q[x] + dt * r);
+                    V = v0 + 0.5 * dV;
+                    q[x] = qn;
+                } else {
+                    qn = q0 / 3.0 + 2.0 / 3.0 * (q[x] + dt * r);
+                    V = v1;
+                }
+                cout[x] = V > 0 ? qn / V : c0;
+            }
+        }
+    }
+}
+
+// (F(c0) + F(c1))/6 + 2 F(c2)/3 of the three stages' face columns (tracers.py:375).
+__global__ void k_rk_face_column(const double* __restrict__ f0, const double* __restrict__ f1,
+                                 const double* __restrict__ f2, double* __restrict__ out, size_t count) {
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < count) out[x] = (f0[x] + f1[x]) / 6.0 + 2.0 * f2[x] / 3.0;
+}
+
+// Donor-cell face fluxes of every face (TracerAdvection.upwind_faces_h, tracers.py:178-193).
+__global__ void k_upwind_faces(const double* __restrict__ c, const double* __restrict__ ext,
+                               const double* __restrict__ Qx, const double* __restrict__ Qy, double* __restrict__ fx,
+                               double* __restrict__ fy, int nt, int n, int ny, int nx, unsigned flags) {
+    const size_t nfx = (size_t)nt * n * ny * (nx + 1), nfy = (size_t)nt * n * (ny + 1) * nx;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nfx + nfy; g += (size_t)gridDim.x * blockDim.x) {
+        if (g < nfx) {
+            const int f = (int)(g % (nx + 1));
+            const size_t r = g / (nx + 1);
+            const int j = (int)(r % ny);
+            const size_t p = r / ny;
+            const int k = (int)(p % n);
+            const double* crow = c + (p * ny + j) * nx;
+            const double* erow = (ext ? ext : c) + (p * ny + j) * nx;
+            fx[g] = donor_x(crow, erow, Qx[((size_t)k * ny + j) * (nx + 1) + f], f, nx, flags);
+        } else {
+            const size_t h = g - nfx;
+            const int i = (int)(h % nx);
+            const size_t r = h / nx;
+            const int f = (int)(r % (ny + 1));
+            const size_t p = r / (ny + 1);
+            const int k = (int)(p % n);
+            const double* cp = c + p * ny * nx;
+            const double* ep = (ext ? ext : c) + p * ny * nx;
+            fy[h] = donor_y(cp, ep, Qy[((size_t)k * (ny + 1) + f) * nx + i], f, i, ny, nx, flags);
+        }
+    }
+}
+
+// Face coefficients Cx, Cy from the ratios (for the coefficient entry point; the step fuses them into
+// k_fct_final).
+__global__ void k_fct_face_coeff(const double* __restrict__ ax, const double* __restrict__ ay,
+                                 const double* __restrict__ r_in, const double* __restrict__ r_out,
+                                 double* __restrict__ cx, double* __restrict__ cy, int nt, int n, int ny, int nx) {
+    const size_t nfx = (size_t)nt * n * ny * (nx + 1), nfy = (size_t)nt * n * (ny + 1) * nx;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < nfx + nfy; g += (size_t)gridDim.x * blockDim.x) {
+        if (g < nfx) {
+            const int f = (int)(g % (nx + 1));
+            const size_t r = g / (nx + 1);
+            const size_t row = r * nx;
+            cx[g] = (f > 0 && f < nx) ? coeff(ax[g], r_in[row + f], r_out[row + f - 1], r_in[row + f - 1], r_out[row + f])
+                                      : 0.0;
+        } else {
+            const size_t h = g - nfx;
+            const int i = (int)(h % nx);
+            const size_t r = h / nx;
+            const int f = (int)(r % (ny + 1));
+            const size_t p = r / (ny + 1);
+            const size_t pc = p * ny * nx;
+            cy[h] = (f > 0 && f < ny) ? coeff(ay[h], r_in[pc + (size_t)f * nx + i], r_out[pc + (size_t)(f - 1) * nx + i],
+                                              r_in[pc + (size_t)(f - 1) * nx + i], r_out[pc + (size_t)f * nx + i])
+                                      : 0.0;
+        }
+    }
+}
+
+// Per-tracer sums of the open-edge face fluxes: sum_W G - sum_E G + sum_S G - sum_N G (tracers.py:65-78).  One
+// block per tracer, fixed summation order (deterministic).
+__global__ void k_open_flux(const double* __restrict__ gedge, double* __restrict__ out, int nt, int n, int ny, int nx,
+                            unsigned flags) {
+    __shared__ double red[256];
+    const int t = blockIdx.x;
+    const size_t nyl = (size_t)nt * n * ny, nxl = (size_t)nt * n * nx;
+    const size_t my = (size_t)n * ny, mx = (size_t)n * nx;
+    double s = 0.0;
+    for (size_t e = threadIdx.x; e < my; e += blockDim.x) {
+        const size_t g = (size_t)t * my + e;
+        if (flags & OPEN_W) s += gedge[g];
+        if (flags & OPEN_E) s -= gedge[nyl + g];
+    }
+    for (size_t e = threadIdx.x; e < mx; e += blockDim.x) {
+        const size_t g = (size_t)t * mx + e;
+        if (flags & OPEN_S) s += gedge[2 * nyl + g];
+        if (flags & OPEN_N) s -= gedge[2 * nyl + nxl + g];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[t] = red[0];
+}
+
+// dst[0..count) = value (a kernel, not hipMemsetAsync, so that it may sit in a captured graph).
+__global__ void k_fill(double* __restrict__ dst, double value, size_t count) {
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < count; x += (size_t)gridDim.x * blockDim.x)
+        dst[x] = value;
+}
+
+// Per-tracer content sum_k,j,i V[k,j,i] * c[t,k,j,i] in FP64, one block of 1024 threads per tracer: compensated
+// (Neumaier) partial sums in a fixed order, then a fixed-shape tree, so the result is deterministic and its
+// rounding error stays near one ulp of the sum.
+__device__ __forceinline__ void neumaier_add(double& s, double& comp, double x) {
+    const double t = s + x;
+    comp += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__global__ __launch_bounds__(1024) void k_content(const double* __restrict__ c, const double* __restrict__ V,
+                                                  double* __restrict__ out, size_t cells) {
+    __shared__ double red[1024], rc[1024];
+    const int t = blockIdx.x;
+    double s = 0.0, comp = 0.0;
+    for (size_t x = threadIdx.x; x < cells; x += blockDim.x) neumaier_add(s, comp, V[x] * c[(size_t)t * cells + x]);
+    red[threadIdx.x] = s;
+    rc[threadIdx.x] = comp;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double a = red[threadIdx.x], ca = rc[threadIdx.x] + rc[threadIdx.x + w];
+            neumaier_add(a, ca, red[threadIdx.x + w]);
+            red[threadIdx.x] = a;
+            rc[threadIdx.x] = ca;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[t] = red[0] + rc[0];
+}
+
+// Periodic halo (core/model/periodic.py:71-95): gather of ``lead`` slabs of a (sy, sx) array into (dy, dx) with
+// source index (pos - halo) mod n on each periodic axis (n = unique cells or faces), or the crop (inverse).
+template <typename T>
+__global__ void k_pad(const T* __restrict__ src, T* __restrict__ dst, size_t lead, int sy, int sx, int dy, int dx,
+                      int halo_y, int halo_x, int ny_unique, int nx_unique, int crop) {
+    const size_t total = lead * (size_t)dy * dx;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+        const int ii = (int)(g % dx);
+        const size_t r = g / dx;
+        const int jj = (int)(r % dy);
+        const size_t l = r / dy;
+        int si, sj;
+        if (crop) {
+            si = ii + halo_x;
+            sj = jj + halo_y;
+        } else {
+            si = halo_x ? ((ii - halo_x) % nx_unique + nx_unique) % nx_unique : ii;
+            sj = halo_y ? ((jj - halo_y) % ny_unique + ny_unique) % ny_unique : jj;
+        }
+        dst[g] = src[(l * sy + sj) * (size_t)sx + si];
+    }
+}
+
+}  // namespace ocx

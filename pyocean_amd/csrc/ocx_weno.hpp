@@ -1,0 +1,72 @@
+// Device-side face reconstruction of the tracer advection (gfx950).
+//
+// FP64 arithmetic with the operand order of core/dynamics/sigma_baroclinic.py:204-233 (_wval): every product and
+// sum rounded on its own (the library is compiled with -ffp-contract=off) and IEEE-correct divisions, so a face
+// value equals the CPU reference's bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ocx {
+
+constexpr unsigned OPEN_W = 1u, OPEN_E = 2u, OPEN_S = 4u, OPEN_N = 8u;
+constexpr unsigned WENO_JS = 16u;      // WENO5-JS instead of WENO5-Z
+constexpr unsigned UNLIMITED = 32u;    // FCT coefficients = 1 (the RK3 result)
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// WENO5 value from the upwind-ordered stencil v1..v5 (v3 = upwind cell).  Z: WENO5-Z (Borges et al. 2008,
+// tau5 = |b2 - b0|, eps = 1e-40); otherwise WENO5-JS (Jiang & Shu 1996, eps = 1e-6).  d = (0.1, 0.6, 0.3).
+template <bool Z>
+__device__ __forceinline__ double weno5(double v1, double v2, double v3, double v4, double v5) {
+    const double q0 = (2.0 * v1 - 7.0 * v2 + 11.0 * v3) / 6.0;
+    const double q1 = (-v2 + 5.0 * v3 + 2.0 * v4) / 6.0;
+    const double q2 = (2.0 * v3 + 5.0 * v4 - v5) / 6.0;
+    constexpr double c1312 = 13.0 / 12.0;
+    const double b0 = c1312 * sq(v1 - 2.0 * v2 + v3) + 0.25 * sq(v1 - 4.0 * v2 + 3.0 * v3);
+    const double b1 = c1312 * sq(v2 - 2.0 * v3 + v4) + 0.25 * sq(v2 - v4);
+    const double b2 = c1312 * sq(v3 - 2.0 * v4 + v5) + 0.25 * sq(3.0 * v3 - 4.0 * v4 + v5);
+    double a0, a1, a2;
+    if (Z) {
+        const double tau = fabs(b2 - b0);
+        a0 = 0.1 * (1.0 + tau / (1e-40 + b0));
+        a1 = 0.6 * (1.0 + tau / (1e-40 + b1));
+        a2 = 0.3 * (1.0 + tau / (1e-40 + b2));
+    } else {
+        a0 = 0.1 / sq(1e-6 + b0);
+        a1 = 0.6 / sq(1e-6 + b1);
+        a2 = 0.3 / sq(1e-6 + b2);
+    }
+    return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
+// WENO3 from v2..v4 (candidates (-v2 + 3 v3)/2 and (v3 + v4)/2, d = (1/3, 2/3), eps = 1e-6).
+__device__ __forceinline__ double weno3(double v2, double v3, double v4) {
+    const double c0 = (-v2 + 3.0 * v3) / 2.0, c1 = (v3 + v4) / 2.0;
+    const double e0 = (1.0 / 3.0) / sq(1e-6 + sq(v3 - v2));
+    const double e1 = (2.0 / 3.0) / sq(1e-6 + sq(v4 - v3));
+    return (e0 * c0 + e1 * c1) / (e0 + e1);
+}
+
+// Face value with the validity rule of the reference: WENO5 when all five cells are valid, WENO3 when v2..v4
+// are, else the upwind cell.  Cells outside the domain carry value 0 and validity 0.
+template <bool Z>
+__device__ __forceinline__ double face_value(double v1, double v2, double v3, double v4, double v5,
+                                             bool k1, bool k2, bool k3, bool k4, bool k5) {
+    if (k1 && k2 && k3 && k4 && k5) return weno5<Z>(v1, v2, v3, v4, v5);
+    if (k2 && k3 && k4) return weno3(v2, v3, v4);
+    return v3;
+}
+
+// Face value between cells c-1 and c of a line held in LDS (values s, validity o, element stride st), upwind
+// side from the sign of q (q >= 0: low side).  The line must hold valid indices c-3 .. c+2.
+template <bool Z>
+__device__ __forceinline__ double lds_face(const double* s, const uint8_t* o, int st, int c, double q) {
+    const bool pos = q >= 0.0;
+    const int i1 = pos ? c - 3 : c + 2, i2 = pos ? c - 2 : c + 1, i3 = pos ? c - 1 : c;
+    const int i4 = pos ? c : c - 1, i5 = pos ? c + 1 : c - 2;
+    return face_value<Z>(s[i1 * st], s[i2 * st], s[i3 * st], s[i4 * st], s[i5 * st],
+                         o[i1 * st] != 0, o[i2 * st] != 0, o[i3 * st] != 0, o[i4 * st] != 0, o[i5 * st] != 0);
+}
+
+}  // namespace ocx

@@ -1,0 +1,227 @@
+"""
+Device-resident entry points: torch tensors in HBM in and out, the HIP kernels of ``libocx.so`` underneath.
+
+torch is used only as plumbing: it owns the device memory (its caching allocator) and the current stream.  Every
+function checks shapes on the host and raises ValueError before any launch, so a kernel never sees a shape it does
+not expect.  Tensors must be FP64 (``ok``: uint8 or bool), C-contiguous, on the current GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _native
+from ._native import check, lib
+
+_F64 = torch.float64
+
+
+def require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyocean_amd runs on an AMD GPU (ROCm); no GPU is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _need(t, shape, name, dtype=_F64):
+    if t is None:
+        raise ValueError(f"{name} is None")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name}: must be a contiguous GPU tensor")
+    return t
+
+
+def _ok(ok, shape):
+    if ok.dtype == torch.bool:
+        ok = ok.view(torch.uint8)
+    return _need(ok, shape, "ok", torch.uint8)
+
+
+class Workspace:
+    """A growing device scratch buffer reused across calls (one per stream user)."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, nbytes: int) -> torch.Tensor:
+        if self.buf is None or self.buf.numel() < nbytes:
+            self.buf = None
+            self.buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=require_gpu())
+        return self.buf
+
+
+_WS = Workspace()
+
+
+def flags_of(open_mask: int = 0, js: bool = False, unlimited: bool = False) -> int:
+    return int(open_mask) | (_native.WENO_JS if js else 0) | (_native.UNLIMITED if unlimited else 0)
+
+
+def advect_fct(C, V0, V1, Qx, Qy, ok, dt, inflow=None, flags=0, flux_face_i=None, open_flux=False, out=None,
+               workspace: Workspace | None = None):
+    """
+    One SSP-RK3 + FCT step on device tensors (``ocx_advect_fct_f64``).  Returns ``out`` or (out, fx_i, open_flux)
+    when ``flux_face_i`` / ``open_flux`` are asked for (None where not).
+    """
+    nt, n, ny, nx = C.shape
+    _need(C, (nt, n, ny, nx), "C")
+    for t, s, nm in ((V0, (n, ny, nx), "V0"), (V1, (n, ny, nx), "V1"), (Qx, (n, ny, nx + 1), "Qx"),
+                     (Qy, (n, ny + 1, nx), "Qy")):
+        _need(t, s, nm)
+    ok = _ok(ok, (n, ny, nx))
+    if inflow is not None:
+        _need(inflow, C.shape, "inflow")
+    out = torch.empty_like(C) if out is None else _need(out, C.shape, "out")
+    fxi = torch.empty((nt, n, ny), dtype=_F64, device=C.device) if flux_face_i is not None else None
+    of = torch.empty((nt,), dtype=_F64, device=C.device) if open_flux else None
+    L = lib()
+    nb = L.ocx_fct_workspace_bytes(nt, n, ny, nx)
+    ws = (workspace or _WS).get(nb)
+    rc = L.ocx_advect_fct_f64(_ptr(C), _ptr(V0), _ptr(V1), _ptr(Qx), _ptr(Qy), _ptr(ok), _ptr(inflow), nt, n, ny,
+                              nx, float(dt), int(flags), -1 if flux_face_i is None else int(flux_face_i), _ptr(out),
+                              _ptr(fxi), _ptr(of), _ptr(ws), ws.numel(), _stream())
+    check(rc, "ocx_advect_fct_f64")
+    if flux_face_i is None and not open_flux:
+        return out
+    return out, fxi, of
+
+
+def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, flags=0, vert=None, flux_face_i=None, out=None,
+               workspace: Workspace | None = None):
+    """One SSP-RK3 step (x, y and z sweeps) on device tensors (``ocx_advect_rk3_f64``)."""
+    nt, n, ny, nx = C.shape
+    _need(C, (nt, n, ny, nx), "C")
+    for t, s, nm in ((V0, (n, ny, nx), "V0"), (V1, (n, ny, nx), "V1"), (Qx, (n, ny, nx + 1), "Qx"),
+                     (Qy, (n, ny + 1, nx), "Qy")):
+        _need(t, s, nm)
+    if Fz is not None:
+        _need(Fz, (n + 1, ny, nx), "Fz")
+    if vert is None:
+        vert = Fz is not None and bool(torch.any(Fz != 0.0).item())
+    ok = _ok(ok, (n, ny, nx))
+    if inflow is not None:
+        _need(inflow, C.shape, "inflow")
+    out = torch.empty_like(C) if out is None else _need(out, C.shape, "out")
+    fxi = torch.empty((nt, n, ny), dtype=_F64, device=C.device) if flux_face_i is not None else None
+    L = lib()
+    nb = L.ocx_rk3_workspace_bytes(nt, n, ny, nx)
+    ws = (workspace or _WS).get(nb)
+    rc = L.ocx_advect_rk3_f64(_ptr(C), _ptr(V0), _ptr(V1), _ptr(Qx), _ptr(Qy), _ptr(Fz if vert else None), _ptr(ok),
+                              _ptr(inflow), nt, n, ny, nx, float(dt), int(flags), int(bool(vert)),
+                              -1 if flux_face_i is None else int(flux_face_i), _ptr(out), _ptr(fxi), _ptr(ws),
+                              ws.numel(), _stream())
+    check(rc, "ocx_advect_rk3_f64")
+    return out if flux_face_i is None else (out, fxi)
+
+
+def faces_h(c, Qx, Qy, ok, ext=None, flags=0):
+    """WENO horizontal face fluxes (fx, fy) of the stack c (``ocx_faces_h_f64``)."""
+    nt, n, ny, nx = c.shape
+    _need(c, (nt, n, ny, nx), "c")
+    _need(Qx, (n, ny, nx + 1), "Qx")
+    _need(Qy, (n, ny + 1, nx), "Qy")
+    ok = _ok(ok, (n, ny, nx))
+    if ext is not None:
+        _need(ext, c.shape, "ext")
+    fx = torch.empty((nt, n, ny, nx + 1), dtype=_F64, device=c.device)
+    fy = torch.empty((nt, n, ny + 1, nx), dtype=_F64, device=c.device)
+    check(lib().ocx_faces_h_f64(_ptr(c), _ptr(Qx), _ptr(Qy), _ptr(ok), _ptr(ext), nt, n, ny, nx, int(flags), _ptr(fx),
+                                _ptr(fy), _stream()), "ocx_faces_h_f64")
+    return fx, fy
+
+
+def upwind_faces_h(c, Qx, Qy, ext=None, flags=0):
+    """Donor-cell face fluxes (``ocx_upwind_faces_h_f64``)."""
+    nt, n, ny, nx = c.shape
+    _need(c, (nt, n, ny, nx), "c")
+    _need(Qx, (n, ny, nx + 1), "Qx")
+    _need(Qy, (n, ny + 1, nx), "Qy")
+    if ext is not None:
+        _need(ext, c.shape, "ext")
+    fx = torch.empty((nt, n, ny, nx + 1), dtype=_F64, device=c.device)
+    fy = torch.empty((nt, n, ny + 1, nx), dtype=_F64, device=c.device)
+    check(lib().ocx_upwind_faces_h_f64(_ptr(c), _ptr(Qx), _ptr(Qy), _ptr(ext), nt, n, ny, nx, int(flags), _ptr(fx),
+                                       _ptr(fy), _stream()), "ocx_upwind_faces_h_f64")
+    return fx, fy
+
+
+def rhs(c, Qx, Qy, Fz, ok, ext=None, flags=0, vert=None):
+    """Advection RHS -div(Q c) (``ocx_rhs_f64``)."""
+    nt, n, ny, nx = c.shape
+    _need(c, (nt, n, ny, nx), "c")
+    _need(Qx, (n, ny, nx + 1), "Qx")
+    _need(Qy, (n, ny + 1, nx), "Qy")
+    if Fz is not None:
+        _need(Fz, (n + 1, ny, nx), "Fz")
+    if vert is None:
+        vert = Fz is not None and bool(torch.any(Fz != 0.0).item())
+    ok = _ok(ok, (n, ny, nx))
+    if ext is not None:
+        _need(ext, c.shape, "ext")
+    out = torch.empty_like(c)
+    check(lib().ocx_rhs_f64(_ptr(c), _ptr(Qx), _ptr(Qy), _ptr(Fz if vert else None), _ptr(ok), _ptr(ext), nt, n, ny,
+                            nx, int(flags), int(bool(vert)), _ptr(out), _stream()), "ocx_rhs_f64")
+    return out
+
+
+def fct_coefficients(c0, cL, V1, Ax, Ay, ok, dt):
+    """Zalesak coefficients (Cx, Cy) (``ocx_fct_coefficients_f64``)."""
+    nt, n, ny, nx = c0.shape
+    _need(c0, (nt, n, ny, nx), "c0")
+    _need(cL, (nt, n, ny, nx), "cL")
+    _need(V1, (n, ny, nx), "V1")
+    _need(Ax, (nt, n, ny, nx + 1), "Ax")
+    _need(Ay, (nt, n, ny + 1, nx), "Ay")
+    ok = _ok(ok, (n, ny, nx))
+    Cx = torch.empty_like(Ax)
+    Cy = torch.empty_like(Ay)
+    ws = _WS.get(2 * c0.numel() * 8)
+    check(lib().ocx_fct_coefficients_f64(_ptr(c0), _ptr(cL), _ptr(V1), _ptr(Ax), _ptr(Ay), _ptr(ok), float(dt), nt, n,
+                                         ny, nx, _ptr(Cx), _ptr(Cy), _ptr(ws), ws.numel(), _stream()),
+          "ocx_fct_coefficients_f64")
+    return Cx, Cy
+
+
+def periodic_pad(a, kind: str, halo: int, px: bool, py: bool, crop: bool = False):
+    """Wrapped halo on the periodic axes (``ocx_periodic_pad``); ``kind`` 'c', 'u' or 'v'; ``crop``: inverse."""
+    if not (px or py):
+        return a
+    if not a.is_contiguous():
+        a = a.contiguous()
+    kid = {"c": 0, "u": 1, "v": 2}[kind]
+    sy, sx = a.shape[-2], a.shape[-1]
+    lead = a.numel() // (sy * sx)
+    hy, hx = (halo if py else 0), (halo if px else 0)
+    sign = -1 if crop else 1
+    dy, dx = sy + 2 * hy * sign, sx + 2 * hx * sign
+    if dy < 1 or dx < 1:
+        raise ValueError(f"periodic crop of {sy}x{sx} by {hy},{hx}")
+    out = torch.empty(a.shape[:-2] + (dy, dx), dtype=a.dtype, device=a.device)
+    elem = 8 if a.dtype == _F64 else 1
+    if a.dtype not in (_F64, torch.uint8):
+        raise ValueError(f"periodic pad of dtype {a.dtype}")
+    check(lib().ocx_periodic_pad(_ptr(a), _ptr(out), lead, sy, sx, kid, hy, hx, elem, int(crop), _stream()),
+          "ocx_periodic_pad")
+    return out
+
+
+def content(c, V):
+    """Per-tracer content sum V c (FP64, deterministic) (``ocx_content_f64``)."""
+    nt, n, ny, nx = c.shape
+    _need(c, (nt, n, ny, nx), "c")
+    _need(V, (n, ny, nx), "V")
+    out = torch.empty((nt,), dtype=_F64, device=c.device)
+    check(lib().ocx_content_f64(_ptr(c), _ptr(V), nt, n, ny, nx, _ptr(out), _stream()), "ocx_content_f64")
+    return out

@@ -1,0 +1,206 @@
+"""
+Drop-in ``TracerAdvection`` (role TRACER_ADVECTION) of pyOCEAN's 3-D core, running on the GPU.
+
+Same constructor, attributes, method names, argument meanings, return types and errors as
+``core/components/tracers.py:50-375``: NumPy arrays in (any layout or dtype, coerced to contiguous FP64 like the
+reference), freshly allocated NumPy arrays out, no state kept from one call to the next except
+``last_open_flux``.  Underneath, each call copies its inputs to HBM once, runs the HIP kernels of ``libocx.so``
+and copies the result back; ``pyocean_amd.ops`` has the same operations on device tensors for callers that keep
+their state resident.
+
+Index order (tracer, k, j, i), k = 0 at the bed; faces Qx (N, ny, nx+1), Qy (N, ny+1, nx), Fz (N+1, ny, nx).
+
+Scope: the WENO5 schemes ('weno5' = WENO5-Z, 'weno5js') with SSP-RK3, with or without the FCT limiter, open or
+periodic edges.  The reference's other options (upwind3/upwind1 faces, centred faces, the 'pc' predictor-corrector)
+are outside this library and raise NotImplementedError rather than run anywhere else.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import ops
+from ._native import OPEN_E, OPEN_N, OPEN_S, OPEN_W
+from .contract import Issue, Role
+from .periodic import NONE, TRACER_HALO
+
+_EDGE_BITS = {"west": OPEN_W, "east": OPEN_E, "south": OPEN_S, "north": OPEN_N}
+SCHEMES = ("weno5", "weno5js", "upwind3", "upwind1")
+TIMES = ("ssprk3", "pc")
+
+
+class OpenEdges:
+    """Minimal stand-in for the open-boundary component: only ``open`` (tuple of edge names) is read."""
+
+    def __init__(self, open_=()):
+        self.open = tuple(open_)
+
+
+def _dev(a, dtype=np.float64):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(ops.require_gpu())
+
+
+def _host(t):
+    return t.cpu().numpy()
+
+
+class TracerAdvection:
+    name = "tracer_advection"
+    role = Role.TRACER_ADVECTION
+    reads = ("T", "tr", "Qx", "Qy", "Fz")
+    writes = ("T", "tr")
+    per = NONE                       # periodic axes (pyocean_amd.periodic), set by the model
+
+    def __init__(self, *, scheme: str, time: str, dt: float, ob) -> None:
+        """tracers.py:57-63."""
+        self.tracer_scheme, self.tracer_time, self.dt, self.ob = scheme, time, dt, ob
+        self.record_open_flux = False
+        self.last_open_flux: np.ndarray | None = None
+        self._ws = ops.Workspace()
+
+    # ------------------------------------------------------------------ contract (tracers.py:104-108)
+    def check(self, ctx) -> list[Issue]:
+        return []
+
+    def state_fields(self) -> dict[str, np.ndarray]:
+        return {}
+
+    # ------------------------------------------------------------------ helpers
+    def _flags(self, unlimited=False) -> int:
+        if self.tracer_scheme not in ("weno5", "weno5js"):
+            raise NotImplementedError(f"tracer_scheme {self.tracer_scheme!r}: this GPU library implements the "
+                                      "WENO5 schemes ('weno5', 'weno5js') only")
+        m = 0
+        for e in getattr(self.ob, "open", ()) or ():
+            m |= _EDGE_BITS[e]
+        return ops.flags_of(m, js=self.tracer_scheme == "weno5js", unlimited=unlimited)
+
+    def _pad(self, t, kind):
+        per = self.per
+        return ops.periodic_pad(t, kind, TRACER_HALO, per.x, per.y)
+
+    def _crop(self, t):
+        per = self.per
+        return ops.periodic_pad(t, "c", TRACER_HALO, per.x, per.y, crop=True)
+
+    @staticmethod
+    def _shapes(C, V0, V1, Qx, Qy, Fz=None):
+        nt, n, ny, nx = C.shape
+        for a, s, nm in ((V0, (n, ny, nx), "V0"), (V1, (n, ny, nx), "V1"), (Qx, (n, ny, nx + 1), "Qx"),
+                         (Qy, (n, ny + 1, nx), "Qy")):
+            if np.shape(a) != s:
+                raise ValueError(f"{nm}: shape {np.shape(a)} vs tracer layers {C.shape[1:]} (expected {s})")
+        if Fz is not None and np.shape(Fz) != (n + 1, ny, nx):
+            raise ValueError(f"Fz: shape {np.shape(Fz)}, expected {(n + 1, ny, nx)}")
+
+    # ------------------------------------------------------------------ the step with FCT (tracers.py:208-254)
+    def advect_fct(self, C: np.ndarray, V0: np.ndarray, V1: np.ndarray, Qx: np.ndarray, Qy: np.ndarray,
+                   ok: np.ndarray, unlimited: bool = False, inflow: np.ndarray | None = None,
+                   flux_face_i: int | None = None):
+        """
+        SSP-RK3 WENO5 advection of the stack C (ntr, N, ny, nx) with Zalesak (1979) FCT, on the GPU.
+        ``unlimited``: C = 1 (the RK3 result).  ``flux_face_i``: also return the step's effective flux through
+        u-face column i, F_L + C A (F_H when unlimited), (ntr, N, ny): (C_new, fx_i).
+        """
+        C = np.ascontiguousarray(C, dtype=float)
+        self._shapes(C, V0, V1, Qx, Qy)
+        flags = self._flags(unlimited)
+        okc = np.broadcast_to(ok, C.shape[1:])
+        d = {k: _dev(v) for k, v in (("C", C), ("V0", V0), ("V1", V1), ("Qx", Qx), ("Qy", Qy))}
+        d["ok"] = _dev(okc, np.uint8)
+        d["inflow"] = None if inflow is None else _dev(inflow)
+        fi = flux_face_i
+        if self.per:
+            H = TRACER_HALO
+            d = {k: (None if v is None else self._pad(v, {"Qx": "u", "Qy": "v"}.get(k, "c"))) for k, v in d.items()}
+            fi = None if fi is None else fi + (H if self.per.x else 0)
+        want_open = self.record_open_flux and not unlimited
+        res = ops.advect_fct(d["C"], d["V0"], d["V1"], d["Qx"], d["Qy"], d["ok"], float(self.dt), d["inflow"], flags,
+                             flux_face_i=fi, open_flux=want_open, workspace=self._ws)
+        out, fxi, of = (res, None, None) if not isinstance(res, tuple) else res
+        if self.per:
+            out = self._crop(out)
+            if fxi is not None and self.per.y:
+                fxi = fxi[..., TRACER_HALO:fxi.shape[-1] - TRACER_HALO].contiguous()
+        if want_open:
+            self.last_open_flux = _host(of)
+        if flux_face_i is None:
+            return _host(out)
+        return _host(out), _host(fxi)
+
+    # ------------------------------------------------------------------ the step without limiter (tracers.py:342-375)
+    def advect(self, C: np.ndarray, V0: np.ndarray, V1: np.ndarray, Qx: np.ndarray, Qy: np.ndarray,
+               Fz: np.ndarray, ok: np.ndarray, inflow: np.ndarray | None = None,
+               flux_face_i: int | None = None):
+        """
+        SSP-RK3 flux-form advection of the stack C with frozen transports Qx, Qy and Fz; layer volumes go linearly
+        from V0 to V1.  ``flux_face_i``: also return (F(c0) + F(c1))/6 + 2F(c2)/3 of u-face column i.
+        """
+        if self.tracer_time == "pc":
+            raise NotImplementedError("tracer_time 'pc' (centred predictor) is not part of this GPU library")
+        C = np.ascontiguousarray(C, dtype=float)
+        if np.shape(V0) != C.shape[1:] or np.shape(V1) != C.shape[1:]:
+            raise ValueError(f"_advect: volume shape {np.shape(V0)}/{np.shape(V1)} vs tracer layers {C.shape[1:]}")
+        self._shapes(C, V0, V1, Qx, Qy, Fz)
+        flags = self._flags()
+        vert = bool(np.any(np.asarray(Fz) != 0.0))
+        okc = np.broadcast_to(ok, C.shape[1:])
+        d = {k: _dev(v) for k, v in (("C", C), ("V0", V0), ("V1", V1), ("Qx", Qx), ("Qy", Qy), ("Fz", Fz))}
+        d["ok"] = _dev(okc, np.uint8)
+        d["inflow"] = None if inflow is None else _dev(inflow)
+        fi = flux_face_i
+        if self.per:
+            H = TRACER_HALO
+            d = {k: (None if v is None else self._pad(v, {"Qx": "u", "Qy": "v"}.get(k, "c"))) for k, v in d.items()}
+            fi = None if fi is None else fi + (H if self.per.x else 0)
+        res = ops.advect_rk3(d["C"], d["V0"], d["V1"], d["Qx"], d["Qy"], d["Fz"], d["ok"], float(self.dt),
+                             d["inflow"], flags, vert=vert, flux_face_i=fi, workspace=self._ws)
+        out, fxi = (res, None) if fi is None else res
+        if self.per:
+            out = self._crop(out)
+            if fxi is not None and self.per.y:
+                fxi = fxi[..., TRACER_HALO:fxi.shape[-1] - TRACER_HALO].contiguous()
+        if flux_face_i is None:
+            return _host(out)
+        return _host(out), _host(fxi)
+
+    # ------------------------------------------------------------------ single-stage helpers (tracers.py:153-206)
+    def faces_h(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, ok: np.ndarray, ext: np.ndarray | None = None,
+                inflow: np.ndarray | None = None):
+        """WENO5 horizontal face fluxes (fx, fy) of the stack; ``ext`` the exterior values (default inflow, else c)."""
+        c = np.ascontiguousarray(c, dtype=float)
+        nt, n, ny, nx = c.shape
+        e = (None if inflow is None else inflow) if ext is None else ext
+        fx, fy = ops.faces_h(_dev(c), _dev(Qx), _dev(Qy), _dev(np.broadcast_to(ok, c.shape[1:]), np.uint8),
+                             None if e is None else _dev(e), self._flags())
+        return _host(fx), _host(fy)
+
+    def upwind_faces_h(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, inflow: np.ndarray | None = None):
+        """Donor-cell face fluxes Q c_up (the FCT low-order flux); open edges take the inflow values on inflow."""
+        c = np.ascontiguousarray(c, dtype=float)
+        fx, fy = ops.upwind_faces_h(_dev(c), _dev(Qx), _dev(Qy), None if inflow is None else _dev(inflow),
+                                    self._flags() & 15)
+        return _host(fx), _host(fy)
+
+    def rhs(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, Fz: np.ndarray, ok: np.ndarray,
+            centred: bool = False, inflow: np.ndarray | None = None) -> np.ndarray:
+        """Advection RHS -div(Q c) with x, y and (any Fz != 0) z WENO faces."""
+        if centred:
+            raise NotImplementedError("centred faces ('pc' predictor) are not part of this GPU library")
+        c = np.ascontiguousarray(c, dtype=float)
+        vert = bool(np.any(np.asarray(Fz) != 0.0))
+        r = ops.rhs(_dev(c), _dev(Qx), _dev(Qy), _dev(Fz), _dev(np.broadcast_to(ok, c.shape[1:]), np.uint8),
+                    None if inflow is None else _dev(inflow), self._flags(), vert=vert)
+        return _host(r)
+
+    def flux_x(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, ok: np.ndarray, centred: bool = False,
+               inflow: np.ndarray | None = None) -> np.ndarray:
+        """x-face fluxes Q c_face (ntr, N, ny, nx+1) that ``rhs`` differences (the TEF transect flux)."""
+        if centred:
+            raise NotImplementedError("centred faces ('pc' predictor) are not part of this GPU library")
+        return self.faces_h(c, Qx, Qy, np.broadcast_to(ok, np.shape(c)[1:]), inflow=inflow)[0]
+
+    def content(self, C: np.ndarray, V: np.ndarray) -> np.ndarray:
+        """Per-tracer content sum V c in FP64 (deterministic order)."""
+        C = np.ascontiguousarray(C, dtype=float)
+        return _host(ops.content(_dev(C), _dev(V)))

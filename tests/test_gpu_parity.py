@@ -1,0 +1,243 @@
+"""
+GPU parity: the HIP path through the C ABI against the oracle on the same seeded inputs.  FP64 results are
+expected bit for bit (the kernels keep the reference's operand order, no contraction, IEEE divisions); masks and
+face indexing are exact by construction and checked through those results.  Full-size runs are checked by
+size-independent properties (conservation, no new extrema, determinism).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import numpy_ref as R
+from pyocean_amd import Periodic, ops, synth
+from pyocean_amd.tracers import OpenEdges, TracerAdvection
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+EDGES = ("west", "east", "south", "north")
+
+
+def _say(*a):
+    print("[gpu case]", *a, flush=True)
+
+
+def _adv(open_mask=0, scheme="weno5", dt=30.0, per=None):
+    adv = TracerAdvection(scheme=scheme, time="ssprk3", dt=dt,
+                          ob=OpenEdges([e for b, e in zip((1, 2, 4, 8), EDGES) if open_mask & b]))
+    if per:
+        adv.per = Periodic(per)
+    return adv
+
+
+@pytest.mark.parametrize("open_", [True, False])
+@pytest.mark.parametrize("variant", ["plain", "inflow", "js", "unlimited"])
+def test_fct_step_matches_the_oracle_bit_for_bit(gpu, oracle_c, open_, variant):
+    case = cases.fused_fct_case(open_=open_)
+    _say("fct", open_, variant)
+    inflow = case["C"] * 0.5 + 1.0 if variant == "inflow" else None
+    adv = _adv(case["open_mask"], scheme="weno5js" if variant == "js" else "weno5", dt=case["dt"])
+    got, fxi = adv.advect_fct(case["C"], case["V0"], case["V1"], case["Qx"], case["Qy"], case["ok"],
+                              unlimited=variant == "unlimited", inflow=inflow, flux_face_i=5)
+    ref = oracle_c.advect_fct(case["C"], case["V0"], case["V1"], case["Qx"], case["Qy"], case["ok"], case["dt"],
+                              inflow=inflow, open_mask=case["open_mask"], z_variant=variant != "js",
+                              unlimited=variant == "unlimited", flux_face_i=5)
+    assert np.array_equal(got, ref["out"])
+    assert np.array_equal(fxi, ref["fx_i"])
+    np.testing.assert_allclose(got, ref["out"], rtol=1e-13, atol=0)      # the reference's own criterion
+
+
+def test_fct_open_flux_record(gpu):
+    case = cases.fused_fct_case(open_=True)
+    _say("open flux")
+    adv = _adv(15, dt=case["dt"])
+    adv.record_open_flux = True
+    adv.advect_fct(case["C"], case["V0"], case["V1"], case["Qx"], case["Qy"], case["ok"])
+    ref = R.advect_fct(case["C"], case["V0"], case["V1"], case["Qx"], case["Qy"], case["ok"], case["dt"],
+                       open_mask=15, open_flux=True)
+    np.testing.assert_allclose(adv.last_open_flux, ref["open_flux"], rtol=1e-13, atol=1e-9)
+
+
+@pytest.mark.parametrize("variant", ["plain", "js", "inflow", "flat"])
+def test_rk3_step_with_z_sweep_matches_the_oracle(gpu, oracle_c, variant):
+    case = cases.fused_fct_case(open_=True, seed=3)
+    _say("rk3", variant)
+    nz, ny, nx = case["V0"].shape
+    Fz = np.random.default_rng(2).normal(0.0, 2e3, (nz + 1, ny, nx))
+    Fz[0] = Fz[-1] = 0.0
+    if variant == "flat":
+        Fz[:] = 0.0
+    inflow = case["C"] + 1.0 if variant == "inflow" else None
+    adv = _adv(case["open_mask"], scheme="weno5js" if variant == "js" else "weno5", dt=case["dt"])
+    got, fxi = adv.advect(case["C"], case["V0"], case["V1"], case["Qx"], case["Qy"], Fz, case["ok"], inflow=inflow,
+                          flux_face_i=nx)
+    ref = oracle_c.advect_rk3(case["C"], case["V0"], case["V1"], case["Qx"], case["Qy"], Fz, case["ok"], case["dt"],
+                              inflow=inflow, open_mask=case["open_mask"], z_variant=variant != "js")
+    assert np.array_equal(got, ref)
+    _, fref = R.advect_rk3(case["C"], case["V0"], case["V1"], case["Qx"], case["Qy"], Fz, case["ok"], case["dt"],
+                           inflow=inflow, open_mask=case["open_mask"], z_variant=variant != "js", flux_face_i=nx)
+    assert np.array_equal(fxi, fref)
+
+
+def test_single_stage_helpers_match_the_oracle(gpu, oracle_c):
+    case = cases.fused_rhs_case()
+    _say("helpers")
+    c = case["c"]
+    adv = _adv(case["open_mask"])
+    ext = c + 0.25
+    fx, fy = adv.faces_h(c, case["Qx"], case["Qy"], case["ok"], ext=ext)
+    rx, ry = R.faces_h(c, case["Qx"], case["Qy"], case["ok"], ext, case["open_mask"])
+    assert np.array_equal(fx, rx) and np.array_equal(fy, ry)
+    ux, uy = adv.upwind_faces_h(c, case["Qx"], case["Qy"], inflow=ext)
+    rx, ry = R.upwind_faces_h(c, case["Qx"], case["Qy"], ext, case["open_mask"])
+    assert np.array_equal(ux, rx) and np.array_equal(uy, ry)
+    r = adv.rhs(c, case["Qx"], case["Qy"], case["Fz"], case["ok"])
+    assert np.array_equal(r, R.rhs(c, case["Qx"], case["Qy"], case["Fz"], case["ok"], c, case["open_mask"]))
+    np.testing.assert_array_equal(adv.flux_x(c, case["Qx"], case["Qy"], case["ok"]),
+                                  R.faces_h(c, case["Qx"], case["Qy"], case["ok"], c, case["open_mask"])[0])
+
+
+def test_fct_coefficients_match_the_oracle(gpu):
+    rng = np.random.default_rng(3)
+    _say("coefficients")
+    nt, n, ny, nx = 2, 3, 70, 67
+    c0 = rng.uniform(0, 1, (nt, n, ny, nx))
+    cL = rng.uniform(0, 1, (nt, n, ny, nx))
+    V1 = rng.uniform(0.5, 2, (n, ny, nx)) * 1e6
+    Ax = rng.normal(0, 1e4, (nt, n, ny, nx + 1))
+    Ax[..., [0, -1]] = 0
+    Ay = rng.normal(0, 1e4, (nt, n, ny + 1, nx))
+    Ay[..., [0, -1], :] = 0
+    ok = rng.uniform(0, 1, (n, ny, nx)) > 0.1
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()        # noqa: E731
+    Cx, Cy = ops.fct_coefficients(d(c0), d(cL), d(V1), d(Ax), d(Ay), d(ok.astype(np.uint8)), 30.0)
+    rx, ry = R.fct_coefficients(c0, cL, V1, Ax, Ay, ok, 30.0)
+    assert np.array_equal(Cx.cpu().numpy(), rx) and np.array_equal(Cy.cpu().numpy(), ry)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1, 1), (1, 2, 1, 7), (3, 5, 37, 70), (2, 3, 130, 129), (1, 4, 5, 200)])
+def test_ragged_and_tiny_shapes(gpu, oracle_c, shape):
+    # sizes that are not multiples of the 64 x 4 tile, single rows/columns, a single cell
+    nt, nz, ny, nx = shape
+    _say("shape", shape)
+    rng = np.random.default_rng(sum(shape))
+    V = rng.uniform(0.5, 1.5, (nz, ny, nx)) * 1e6
+    Qx = rng.uniform(-0.3, 0.3, (nz, ny, nx + 1)) * 1e4
+    Qy = rng.uniform(-0.3, 0.3, (nz, ny + 1, nx)) * 1e4
+    V1 = V - 30.0 * (Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:] - Qy[:, :-1])
+    ok = (V1 > 0) & (rng.random((nz, ny, nx)) > 0.1)
+    C = rng.normal(5.0, 1.0, shape)
+    adv = _adv(15)
+    got = adv.advect_fct(C, V, V1, Qx, Qy, ok)
+    assert np.array_equal(got, oracle_c.advect_fct(C, V, V1, Qx, Qy, ok, 30.0, open_mask=15))
+    Fz = rng.normal(0.0, 1e3, (nz + 1, ny, nx))
+    Fz[0] = Fz[-1] = 0.0
+    got = adv.advect(C, V, V1, Qx, Qy, Fz, ok)
+    assert np.array_equal(got, oracle_c.advect_rk3(C, V, V1, Qx, Qy, Fz, ok, 30.0, open_mask=15))
+
+
+@pytest.mark.parametrize("per", [("x",), ("y",), ("x", "y")])
+def test_periodic_pad_crop_matches_the_oracle(gpu, oracle_c, per):
+    _say("periodic", per)
+    px, py = "x" in per, "y" in per
+    rng = np.random.default_rng(5)
+    nt, nz, ny, nx = 2, 3, 20, 24
+    V = rng.uniform(0.8, 1.2, (nz, ny, nx)) * 1e6
+    Qx = rng.uniform(-0.3, 0.3, (nz, ny, nx + 1)) * 1e4
+    Qy = rng.uniform(-0.3, 0.3, (nz, ny + 1, nx)) * 1e4
+    if px:
+        Qx[..., -1] = Qx[..., 0]
+    else:
+        Qx[..., [0, -1]] = 0.0
+    if py:
+        Qy[:, -1] = Qy[:, 0]
+    else:
+        Qy[:, [0, -1]] = 0.0
+    V1 = V - 30.0 * (Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:] - Qy[:, :-1])
+    ok = V1 > 0
+    C = rng.normal(5.0, 1.0, (nt, nz, ny, nx))
+    adv = _adv(0, per=per)
+    got, fxi = adv.advect_fct(C, V, V1, Qx, Qy, ok, flux_face_i=3)
+    H = R.TRACER_HALO
+    P = lambda a, k: R.pad(a, k, H, px, py)                              # noqa: E731
+    ref = oracle_c.advect_fct(P(C, "c"), P(V, "c"), P(V1, "c"), P(Qx, "u"), P(Qy, "v"), P(ok, "c"), 30.0,
+                              flux_face_i=3 + (H if px else 0))
+    assert np.array_equal(got, R.crop(ref["out"], H, px, py))
+    assert np.array_equal(fxi, ref["fx_i"][..., H:-H] if py else ref["fx_i"])
+    Fz = np.zeros((nz + 1, ny, nx))
+    got = adv.advect(C, V, V1, Qx, Qy, Fz, ok)
+    ref = oracle_c.advect_rk3(P(C, "c"), P(V, "c"), P(V1, "c"), P(Qx, "u"), P(Qy, "v"), P(Fz, "c"), P(ok, "c"), 30.0)
+    assert np.array_equal(got, R.crop(ref, H, px, py))
+
+
+def test_known_answer_table_on_the_gpu(gpu):
+    # the reference's 1-D table (tracers.md:101-107) through the GPU's FCT step, periodic in x
+    _say("KAT")
+    nx = 200
+    c0 = cases.kat_initial(nx)
+    adv = _adv(0, dt=0.4, per=("x",))
+    V = np.ones((1, 1, nx))
+    Qx = np.ones((1, 1, nx + 1))
+    Qy = np.zeros((1, 2, nx))
+    ok = np.ones((1, 1, nx), bool)
+    c = c0[None, None, None, :]
+    for _ in range(500):
+        c = adv.advect_fct(c, V, V, Qx, Qy, ok)
+    c = c[0, 0, 0]
+    assert f"{np.abs(c - c0).sum() / np.abs(c0).sum():.3f}" == "0.050"
+    assert f"{c.max():.4f}" == "1.0000" and f"{c.min():.0e}" == "3e-16"
+
+
+def test_solid_body_steps_match_the_oracle(gpu, oracle_c):
+    # the CPU-runnable benchmark case (128 x 128 x 20, two tracers): 5 steps, bit for bit
+    _say("solid body")
+    case = synth.solid_body()
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()        # noqa: E731
+    C, V0, V1, Qx, Qy, ok = (d(case.C), d(case.V0), d(case.V1), d(case.Qx), d(case.Qy),
+                             d(case.ok.astype(np.uint8)))
+    ref = case.C
+    for _ in range(5):
+        C = ops.advect_fct(C, V0, V1, Qx, Qy, ok, case.dt)
+        ref = oracle_c.advect_fct(ref, case.V0, case.V1, case.Qx, case.Qy, case.ok, case.dt)
+    assert np.array_equal(C.cpu().numpy(), ref)
+
+
+def test_regional_case_matches_the_oracle(gpu, oracle_c):
+    # land mask, z* layers with 1 mm sub-bed layers, curvilinear metrics: 2 steps, bit for bit
+    _say("regional 160x144x12")
+    case = synth.regional(nx=160, ny=144, nz=12, nt=3, seed=11)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()        # noqa: E731
+    args = (d(case.V0), d(case.V1), d(case.Qx), d(case.Qy), d(case.ok.astype(np.uint8)))
+    C = d(case.C)
+    ref = case.C
+    for _ in range(2):
+        C = ops.advect_fct(C, *args, case.dt)
+        ref = oracle_c.advect_fct(ref, case.V0, case.V1, case.Qx, case.Qy, case.ok, case.dt)
+    assert np.array_equal(C.cpu().numpy(), ref)
+    s = synth.regional(nx=160, ny=144, nz=12, nt=2, seed=11, sigma=True)
+    out = ops.advect_rk3(d(s.C), d(s.V0), d(s.V1), d(s.Qx), d(s.Qy), d(s.Fz), d(s.ok.astype(np.uint8)), s.dt)
+    assert np.array_equal(out.cpu().numpy(), oracle_c.advect_rk3(s.C, s.V0, s.V1, s.Qx, s.Qy, s.Fz, s.ok, s.dt))
+
+
+def test_full_size_properties(gpu):
+    # the 1024 x 1024 x 50 regional grid (two tracers to keep the check quick): closed edges conserve content to
+    # 1e-13 per step, the limiter makes no new extrema, and two runs are bit-identical
+    _say("full size 1024x1024x50")
+    case = synth.regional(nx=1024, ny=1024, nz=50, nt=2, seed=0)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()        # noqa: E731
+    V0, V1, Qx, Qy = d(case.V0), d(case.V1), d(case.Qx), d(case.Qy)
+    ok = d(case.ok.astype(np.uint8))
+    C0 = d(case.C)
+    C1 = ops.advect_fct(C0, V0, V1, Qx, Qy, ok, case.dt)
+    C1b = ops.advect_fct(C0, V0, V1, Qx, Qy, ok, case.dt)
+    assert torch.equal(C1, C1b)
+    m0 = ops.content(C0, V0).cpu().numpy()
+    m1 = ops.content(C1, V1).cpu().numpy()
+    np.testing.assert_allclose(m1, m0, rtol=1e-13)
+    okb = torch.broadcast_to(ok.bool(), C0.shape)
+    for t in range(C0.shape[0]):
+        lo, hi = C0[t][okb[t]].min().item(), C0[t][okb[t]].max().item()
+        span = hi - lo
+        assert C1[t][okb[t]].min().item() >= lo - 1e-12 * span and C1[t][okb[t]].max().item() <= hi + 1e-12 * span

@@ -1,0 +1,90 @@
+"""Host-side logic of the drop-in component and the workload generators (CPU only, no kernel launches)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from pyocean_amd import Periodic, synth
+from pyocean_amd.contract import Role
+from pyocean_amd.tracers import OpenEdges, TracerAdvection
+
+
+def _adv(**kw):
+    return TracerAdvection(scheme=kw.get("scheme", "weno5"), time=kw.get("time", "ssprk3"), dt=30.0,
+                           ob=OpenEdges(kw.get("open_", ())))
+
+
+def test_component_contract_matches_the_reference():
+    adv = _adv()
+    assert adv.role == Role.TRACER_ADVECTION and adv.name == "tracer_advection"
+    assert adv.reads == ("T", "tr", "Qx", "Qy", "Fz") and adv.writes == ("T", "tr")
+    assert adv.check(None) == [] and adv.state_fields() == {}
+    assert adv.record_open_flux is False and adv.last_open_flux is None
+    assert not adv.per
+
+
+def test_volume_shape_mismatch_raises_value_error_before_the_gpu():
+    adv = _adv()
+    C = np.zeros((2, 3, 4, 5))
+    with pytest.raises(ValueError, match="volume shape"):
+        adv.advect(C, np.zeros((3, 4, 6)), np.zeros((3, 4, 5)), np.zeros((3, 4, 6)), np.zeros((3, 5, 5)),
+                   np.zeros((4, 4, 5)), np.ones((3, 4, 5), bool))
+    with pytest.raises(ValueError, match="Qx"):
+        adv.advect_fct(C, np.zeros((3, 4, 5)), np.zeros((3, 4, 5)), np.zeros((3, 4, 5)), np.zeros((3, 5, 5)),
+                       np.ones((3, 4, 5), bool))
+
+
+@pytest.mark.parametrize("scheme", ["upwind3", "upwind1"])
+def test_schemes_outside_the_library_fail_loudly(scheme):
+    adv = _adv(scheme=scheme)
+    C = np.zeros((1, 2, 3, 4))
+    with pytest.raises(NotImplementedError, match="WENO5"):
+        adv.advect_fct(C, np.ones((2, 3, 4)), np.ones((2, 3, 4)), np.zeros((2, 3, 5)), np.zeros((2, 4, 4)),
+                       np.ones((2, 3, 4), bool))
+
+
+def test_predictor_corrector_fails_loudly():
+    adv = _adv(time="pc")
+    C = np.zeros((1, 2, 3, 4))
+    with pytest.raises(NotImplementedError, match="pc"):
+        adv.advect(C, np.ones((2, 3, 4)), np.ones((2, 3, 4)), np.zeros((2, 3, 5)), np.zeros((2, 4, 4)),
+                   np.zeros((3, 3, 4)), np.ones((2, 3, 4), bool))
+
+
+def test_periodic_axes():
+    assert not Periodic() and Periodic("x").x and not Periodic("x").y
+    assert Periodic(("y", "x")).axes == ("x", "y")
+    with pytest.raises(ValueError, match="unknown periodic axes"):
+        Periodic(("z",))
+
+
+def test_solid_body_case_is_non_divergent_and_closed():
+    c = synth.solid_body(n=64, nz=4)
+    div = c.Qx[..., 1:] - c.Qx[..., :-1] + c.Qy[:, 1:] - c.Qy[:, :-1]
+    assert np.abs(div).max() <= 1e-9 * np.abs(c.Qx).max()
+    assert not c.Qx[..., [0, -1]].any() and not c.Qy[:, [0, -1]].any()
+    np.testing.assert_allclose(c.V1, c.V0, rtol=1e-13)
+    ux = np.abs(c.Qx).max() / (200.0 / 4 * 1000.0)
+    assert abs(ux * c.dt / 1000.0 - 0.4) < 1e-12                        # Courant 0.4
+    assert set(np.unique(c.C[1])) == {0.0, 1.0}
+
+
+def test_regional_case_is_valid_for_the_limiter():
+    c = synth.regional(nx=96, ny=80, nz=10, nt=4, seed=5)
+    assert 0.1 < c.info["land"] < 0.3
+    assert c.info["max_outflow_fraction"] < 1.0                          # limit_outflow's condition
+    assert (c.V1[c.ok] > 0).all() and not c.ok[c.V0 <= 0].any()
+    land = c.V0[0] <= 0
+    assert not c.Qx[:, :, 1:][:, land].any() and not c.Qx[:, :, :-1][:, land].any()
+    assert not c.Qx[..., [0, -1]].any() and not c.Qy[:, [0, -1]].any()
+    s = synth.regional(nx=96, ny=80, nz=10, nt=1, seed=5, sigma=True)
+    div = s.Qx[..., 1:] - s.Qx[..., :-1] + s.Qy[:, 1:] - s.Qy[:, :-1] + s.Fz[1:] - s.Fz[:-1]
+    np.testing.assert_allclose(s.V1, s.V0 - s.dt * div, rtol=0, atol=1e-6 * s.V0.max())
+    assert not s.Fz[0].any() and not s.Fz[-1].any() and np.abs(s.Fz).max() > 0
+
+
+def test_batch_shapes_follow_the_batched_case():
+    shapes = synth.batch_shapes(64)
+    assert len(shapes) == 64
+    for nx, ny, nz in shapes:
+        assert 96 <= nx <= 768 and 96 <= ny <= 768 and nz in (20, 30, 50, 75)

@@ -1,0 +1,75 @@
+"""The C-ABI library loads on the CPU, exports every symbol include/ocx.h declares, and rejects bad arguments on
+the host before any launch (no GPU needed for any of this)."""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from pyocean_amd import _native
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "ocx.h"
+
+
+def _declared():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ocx_\w+)\s*\(", text)))
+
+
+def test_header_and_binding_declare_the_same_symbols():
+    assert _declared() == sorted(_native.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.lib()
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert lib.ocx_version() == 1
+
+
+def test_workspace_sizes_are_host_computations():
+    lib = _native.lib()
+    nt, nz, ny, nx = 10, 50, 1024, 1024
+    b = lib.ocx_fct_workspace_bytes(nt, nz, ny, nx)
+    S = nt * nz * ny * nx
+    assert 7 * S * 8 <= b <= 7.2 * S * 8                    # 3 cell stacks + 2 x 2 face stacks + edge fluxes
+    assert lib.ocx_rk3_workspace_bytes(nt, nz, ny, nx) >= 3 * S * 8
+    assert lib.ocx_fct_workspace_bytes(0, 1, 1, 1) == 0
+
+
+def _fct(lib, **over):
+    a = dict(C=1, V0=1, V1=1, Qx=1, Qy=1, ok=1, inflow=None, nt=1, nz=1, ny=8, nx=8, dt=1.0, flags=0, fi=-1,
+             out=2, fxi=None, of=None, ws=3, wsb=1 << 40, stream=None)
+    a.update(over)
+    v = lambda x: None if x is None else ctypes.c_void_p(x)                 # noqa: E731  (never dereferenced)
+    return lib.ocx_advect_fct_f64(v(a["C"]), v(a["V0"]), v(a["V1"]), v(a["Qx"]), v(a["Qy"]), v(a["ok"]),
+                                  v(a["inflow"]), a["nt"], a["nz"], a["ny"], a["nx"], a["dt"], a["flags"], a["fi"],
+                                  v(a["out"]), v(a["fxi"]), v(a["of"]), v(a["ws"]), a["wsb"], v(a["stream"]))
+
+
+@pytest.mark.parametrize("over,msg", [
+    (dict(nt=0), "must be positive"),
+    (dict(C=None), "C is NULL"),
+    (dict(ok=None), "ok is NULL"),
+    (dict(out=1), "alias"),
+    (dict(flags=1 << 8), "unknown flag"),
+    (dict(fi=9), "flux_face_i"),
+    (dict(fi=3), "fx_i is NULL"),
+    (dict(ws=None), "workspace"),
+    (dict(wsb=16), "workspace"),
+])
+def test_argument_errors_are_reported_before_any_launch(over, msg):
+    lib = _native.lib()
+    rc = _fct(lib, **over)
+    assert rc in (_native.OCX_E_ARG, _native.OCX_E_WORKSPACE)
+    assert msg in lib.ocx_last_error().decode()
+
+
+def test_check_raises_value_error_for_argument_codes():
+    lib = _native.lib()
+    _fct(lib, nt=0)
+    with pytest.raises(ValueError, match="positive"):
+        _native.check(_native.OCX_E_ARG, "ocx_advect_fct_f64")
