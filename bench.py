@@ -95,6 +95,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy the tracer stack once (device to device) before the steps: a kernel of known bytes "
+                         "for scaling the HBM counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU layout (tracer or domain sharding, halo "
@@ -120,6 +123,10 @@ def main() -> int:
     def step(a, b):
         return ops.advect_fct(a, V0, V1, Qx, Qy, ok, case.dt, out=b, workspace=ws)
 
+    if args.copy_calibration:
+        out.copy_(C)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {C.numel() * 8} bytes", file=sys.stderr, flush=True)
     print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step(C, out)

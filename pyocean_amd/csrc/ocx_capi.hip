@@ -11,6 +11,7 @@
 
 #include "../../include/ocx.h"
 #include "ocx_kernels_f64.hpp"
+#include "ocx_march.hpp"
 
 namespace {
 
@@ -77,8 +78,9 @@ struct Sizes {
     }
 };
 
+// FCT workspace: c1 (later c_L), c2 (later R_in), F0 (later A), S (later R_out), open-edge fluxes.
 struct FctWs {
-    double *c1, *q1, *c2, *fx, *fy, *ax, *ay, *gedge;
+    double *c1, *c2, *fx, *fy, *sx, *sy, *gedge;
 };
 
 size_t fct_layout(void* base, int nt, int nz, int ny, int nx, FctWs* w) {
@@ -86,15 +88,27 @@ size_t fct_layout(void* base, int nt, int nz, int ny, int nx, FctWs* w) {
     Carver cv(base);
     FctWs t;
     t.c1 = cv.take<double>(s.S);
-    t.q1 = cv.take<double>(s.S);
     t.c2 = cv.take<double>(s.S);
     t.fx = cv.take<double>(s.SX);
     t.fy = cv.take<double>(s.SY);
-    t.ax = cv.take<double>(s.SX);
-    t.ay = cv.take<double>(s.SY);
+    t.sx = cv.take<double>(s.SX);
+    t.sy = cv.take<double>(s.SY);
     t.gedge = cv.take<double>(2 * s.P * ny + 2 * s.P * nx);
     if (w) *w = t;
     return cv.used;
+}
+
+constexpr int MARCH_WPB = 4;     // waves per block of the marching kernels
+constexpr int MARCH_RCH = 64;    // rows per chunk
+
+template <int STAGE, bool UNLIM>
+void launch_march(const ocx::MarchArgs& a, bool js, hipStream_t s) {
+    const long long waves = (long long)a.nsx * a.ncy * a.nt * a.n;
+    const unsigned blocks = (unsigned)((waves + MARCH_WPB - 1) / MARCH_WPB);
+    if (js)
+        ocx::k_march<STAGE, false, UNLIM, MARCH_WPB><<<blocks, 64 * MARCH_WPB, 0, s>>>(a);
+    else
+        ocx::k_march<STAGE, true, UNLIM, MARCH_WPB><<<blocks, 64 * MARCH_WPB, 0, s>>>(a);
 }
 
 struct Rk3Ws {
@@ -195,39 +209,57 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
     const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
     const bool js = flags & WENO_JS, unlim = flags & UNLIMITED;
     const int fi = fx_i ? flux_face_i : -1;
-    // stage 1: F0, q1, c1
+    if ((long long)((nx + MS - 1) / MS) * ((ny + MARCH_RCH - 1) / MARCH_RCH) * nt * nz / MARCH_WPB >= (1LL << 31))
+        return fail(OCX_E_ARG, "shape (nt=%d, nz=%d, ny=%d, nx=%d) too large for one launch", nt, nz, ny, nx);
+    MarchArgs a{};
+    a.C = C;
+    a.ext = inflow;
+    a.V0 = V0;
+    a.V1 = V1;
+    a.Qx = Qx;
+    a.Qy = Qy;
+    a.ok = ok;
+    a.fx = w.fx;
+    a.fy = w.fy;
+    a.s_x = w.sx;
+    a.s_y = w.sy;
+    a.ax = w.fx;                     // A overwrites F0, dead after stage 2
+    a.ay = w.fy;
+    a.fxi = nullptr;
+    a.flux_face_i = -1;
+    a.nt = nt;
+    a.n = nz;
+    a.ny = ny;
+    a.nx = nx;
+    a.rch = MARCH_RCH;
+    a.nsx = (nx + MS - 1) / MS;
+    a.ncy = (ny + MARCH_RCH - 1) / MARCH_RCH;
+    a.dt = dt;
+    a.flags = flags;
+    // stage 1: F0, c1
     {
-    ProfScope _ps("fct_stage1", s);
-    if (js)
-        k_fct_stage<1, false, TX, TY><<<grid, block, 0, s>>>(C, C, inflow, V0, V1, Qx, Qy, ok, nullptr, w.fx, w.fy,
-                                                             w.q1, w.c1, nt, nz, ny, nx, dt, flags);
-    else
-        k_fct_stage<1, true, TX, TY><<<grid, block, 0, s>>>(C, C, inflow, V0, V1, Qx, Qy, ok, nullptr, w.fx, w.fy,
-                                                            w.q1, w.c1, nt, nz, ny, nx, dt, flags);
+        ProfScope _ps("fct_stage1", s);
+        a.cin = C;
+        a.cout = w.c1;
+        launch_march<1, false>(a, js, s);
     }
     OCX_TRY(launch_check("fct stage 1"));
-    // stage 2: F0 + F1, c2
+    // stage 2: S = F0 + F1, c2
     {
-    ProfScope _ps("fct_stage2", s);
-    if (js)
-        k_fct_stage<2, false, TX, TY><<<grid, block, 0, s>>>(w.c1, C, inflow, V0, V1, Qx, Qy, ok, w.q1, w.fx, w.fy,
-                                                             nullptr, w.c2, nt, nz, ny, nx, dt, flags);
-    else
-        k_fct_stage<2, true, TX, TY><<<grid, block, 0, s>>>(w.c1, C, inflow, V0, V1, Qx, Qy, ok, w.q1, w.fx, w.fy,
-                                                            nullptr, w.c2, nt, nz, ny, nx, dt, flags);
+        ProfScope _ps("fct_stage2", s);
+        a.cin = w.c1;
+        a.cout = w.c2;
+        launch_march<2, false>(a, js, s);
     }
     OCX_TRY(launch_check("fct stage 2"));
     if (unlim) {
         {
-        ProfScope _ps("fct_unlimited", s);
-        if (js)
-            k_fct_high<false, true, TX, TY><<<grid, block, 0, s>>>(w.c2, C, inflow, V0, V1, Qx, Qy, ok, w.fx, w.fy,
-                                                                   nullptr, nullptr, C_out, fx_i, fi, nt, nz, ny, nx,
-                                                                   dt, flags);
-        else
-            k_fct_high<true, true, TX, TY><<<grid, block, 0, s>>>(w.c2, C, inflow, V0, V1, Qx, Qy, ok, w.fx, w.fy,
-                                                                  nullptr, nullptr, C_out, fx_i, fi, nt, nz, ny, nx,
-                                                                  dt, flags);
+            ProfScope _ps("fct_unlimited", s);
+            a.cin = w.c2;
+            a.cout = C_out;
+            a.fxi = fx_i;
+            a.flux_face_i = fi;
+            launch_march<3, true>(a, js, s);
         }
         OCX_TRY(launch_check("fct unlimited"));
         if (open_flux) {
@@ -237,26 +269,23 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
         }
         return OCX_OK;
     }
-    // stage 3: F2, A = F_H - F_L, c_L (into q1)
+    // stage 3: F2, A = F_H - F_L (into the F0 buffers), c_L (into c1)
     {
-    ProfScope _ps("fct_stage3_split", s);
-    if (js)
-        k_fct_high<false, false, TX, TY><<<grid, block, 0, s>>>(w.c2, C, inflow, V0, V1, Qx, Qy, ok, w.fx, w.fy, w.ax,
-                                                                w.ay, w.q1, nullptr, -1, nt, nz, ny, nx, dt, flags);
-    else
-        k_fct_high<true, false, TX, TY><<<grid, block, 0, s>>>(w.c2, C, inflow, V0, V1, Qx, Qy, ok, w.fx, w.fy, w.ax,
-                                                               w.ay, w.q1, nullptr, -1, nt, nz, ny, nx, dt, flags);
+        ProfScope _ps("fct_stage3_split", s);
+        a.cin = w.c2;
+        a.cout = w.c1;
+        launch_march<3, false>(a, js, s);
     }
     OCX_TRY(launch_check("fct high/low split"));
-    // ratios R+- into c1 / c2
+    // ratios R_in (into c2), R_out (into S's x buffer)
     {
         ProfScope _ps("fct_ratio", s);
-        k_fct_ratio<TX, TY><<<grid, block, 0, s>>>(C, w.q1, V1, w.ax, w.ay, ok, w.c1, w.c2, nt, nz, ny, nx, dt);
+        k_fct_ratio<TX, TY><<<grid, block, 0, s>>>(C, w.c1, V1, w.fx, w.fy, ok, w.c2, w.sx, nt, nz, ny, nx, dt);
     }
     OCX_TRY(launch_check("fct ratios"));
     {
         ProfScope _ps("fct_final", s);
-        k_fct_final<TX, TY><<<grid, block, 0, s>>>(C, inflow, V0, V1, Qx, Qy, w.ax, w.ay, w.c1, w.c2, C_out, fx_i, fi,
+        k_fct_final<TX, TY><<<grid, block, 0, s>>>(C, inflow, V0, V1, Qx, Qy, w.fx, w.fy, w.c2, w.sx, C_out, fx_i, fi,
                                                    open_flux ? w.gedge : nullptr, nt, nz, ny, nx, dt, flags);
     }
     OCX_TRY(launch_check("fct final"));

@@ -1,0 +1,294 @@
+// Wave-marching kernels of the FCT step (gfx950).
+//
+// One 64-lane wavefront owns a strip of 63 cells in x of one (tracer, layer) plane and marches down a chunk of
+// rows.  Lane l sits on column i0 + l: it reconstructs the x-face i0 + l (64 faces for the strip's 63 cells, so
+// every lane is busy and no face is computed twice inside a strip) and the y-face north of its cell.  The y stencil
+// (rows j-3 .. j+3 of the lane's column) lives in registers and slides one row per step, so each input row is read
+// from HBM once; the x stencil of the current row goes through a 69-entry LDS row private to the wave (no block
+// barrier).  The east face of a cell comes from lane + 1 (cross-lane shuffle), the south face from the previous
+// step.  Faces are written to global memory only by their owner (x-faces i0 .. i0+62 and the domain's last face;
+// y-faces jb .. je-1 of the chunk and the domain's last face), so a read-modify-write of a face stack never races.
+//
+// Stages (core/components/tracers.py:208-254 with the kernels of core/dynamics/sigma_baroclinic.py:265-530):
+//   1: F0 = faces(C) -> fx/fy;  c1 = (q0 - dt div F0)/Va (safe)
+//   2: F1 = faces(c1);  S = F0 + F1 -> s_x/s_y (F0 stays read-only: a wave reads F0 on faces its neighbours own);
+//      q1 recomputed from F0 (bit-identical to stage 1's), c2 = (0.75 q0 + 0.25 (q1 - dt div F1))/Vb (safe)
+//   3: F2 = faces(c2);  F_H = (F0 + F1)/6 + 2 F2/3;  donor F_L;  A = F_H - F_L -> ax/ay;  c_L = (q0 - dt div F_L)/V1
+//      (safe) -> cout.  UNLIM: out = (q0 - dt div F_H)/V1 (safe) -> cout and the face column of F_H.
+#pragma once
+#include "ocx_weno.hpp"
+
+namespace ocx {
+
+constexpr int MS = 63;     // own cells per strip
+constexpr int MRB = 72;    // LDS row: columns i0-3 .. i0+65 (69 used)
+
+struct MarchArgs {
+    const double* cin;     // the stage's input values (C, c1, c2)
+    const double* C;       // the step's initial values
+    const double* ext;     // inflow values, or nullptr (stage's own values; C for the donor flux)
+    const double* V0;
+    const double* V1;
+    const double* Qx;
+    const double* Qy;
+    const uint8_t* ok;
+    double* fx;            // F0 (stage 1 out, stage 2 in; read-only in stage 2)
+    double* fy;
+    double* s_x;           // S = F0 + F1 (stage 2 out, stage 3 in)
+    double* s_y;
+    double* ax;            // A (stage 3; may be the F0 buffers, dead by then)
+    double* ay;
+    double* cout;          // c1, c2, c_L (or the unlimited result)
+    double* fxi;           // unlimited: face column of F_H
+    int flux_face_i;
+    int nt, n, ny, nx;
+    int rch;               // rows per chunk
+    int nsx, ncy;          // strips in x, chunks in y
+    double dt;
+    unsigned flags;
+};
+
+__device__ __forceinline__ double shfl_down1(double v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, 64); }
+
+// WENO/edge face flux of a face whose upwind-ordered candidates are u[0..5] = cells f-3 .. f+2 along the axis
+// (validity o[0..5]); lo = cell f-1, hi = cell f; rules of _advect_faces_h.
+template <bool Z>
+__device__ __forceinline__ double line_face(double q, int f, int nf, const double* u, const bool* o, bool open_lo,
+                                            bool open_hi, double ext_lo, double ext_hi) {
+    if (f > 0 && f < nf) {
+        if (q == 0.0) return 0.0;
+        const bool pos = q >= 0.0;
+        const double v1 = pos ? u[0] : u[5], v2 = pos ? u[1] : u[4], v3 = pos ? u[2] : u[3];
+        const double v4 = pos ? u[3] : u[2], v5 = pos ? u[4] : u[1];
+        const bool k1 = pos ? o[0] : o[5], k2 = pos ? o[1] : o[4], k3 = pos ? o[2] : o[3];
+        const bool k4 = pos ? o[3] : o[2], k5 = pos ? o[4] : o[1];
+        return q * face_value<Z>(v1, v2, v3, v4, v5, k1, k2, k3, k4, k5);
+    }
+    if (f == 0) return open_lo ? q * (q > 0.0 ? ext_lo : u[3]) : 0.0;
+    return open_hi ? q * (q < 0.0 ? ext_hi : u[2]) : 0.0;
+}
+
+// donor-cell flux of a face between cells lo and hi (interior), or of an edge face (rules of _fct_fluxes)
+__device__ __forceinline__ double donor(double q, int f, int nf, double lo, double hi, bool open_lo, bool open_hi,
+                                        double ext_lo, double ext_hi) {
+    if (f > 0 && f < nf) return q * (q >= 0 ? lo : hi);
+    if (f == 0) return open_lo ? q * (q > 0 ? ext_lo : hi) : 0.0;
+    return open_hi ? q * (q < 0 ? ext_hi : lo) : 0.0;
+}
+
+template <int STAGE, bool Z, bool UNLIM, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
+    __shared__ double rbuf[WPB][MRB];
+    __shared__ uint8_t obuf[WPB][MRB];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long gw = (long long)blockIdx.x * WPB + wv;
+    const long long nwaves = (long long)a.nsx * a.ncy * a.nt * a.n;
+    if (gw >= nwaves) return;                      // whole wave leaves; no block barrier is used below
+    const int sx = (int)(gw % a.nsx);
+    const long long r1 = gw / a.nsx;
+    const int cy = (int)(r1 % a.ncy);
+    const int p = (int)(r1 / a.ncy);
+    const int k = p % a.n;
+    const int ny = a.ny, nx = a.nx;
+    const int i0 = sx * MS, i = i0 + lane;
+    const int jb = cy * a.rch, je = min(jb + a.rch, ny);
+    const size_t plane = (size_t)ny * nx;
+    const double* cin = a.cin + (size_t)p * plane;
+    const double* Cp = a.C + (size_t)p * plane;
+    const double* extp = a.ext ? a.ext + (size_t)p * plane : nullptr;   // stage faces: nullptr = cin
+    const double* ext0 = a.ext ? a.ext + (size_t)p * plane : Cp;        // donor flux: inflow or C
+    const uint8_t* okp = a.ok + (size_t)k * plane;
+    const double* Qxk = a.Qx + (size_t)k * ny * (nx + 1);
+    const double* Qyk = a.Qy + (size_t)k * (ny + 1) * nx;
+    const double* V0k = a.V0 + (size_t)k * plane;
+    const double* V1k = a.V1 + (size_t)k * plane;
+    double* fxp = a.fx + (size_t)p * ny * (nx + 1);
+    double* fyp = a.fy + (size_t)p * (ny + 1) * nx;
+    double* sxp = a.s_x + (size_t)p * ny * (nx + 1);
+    double* syp = a.s_y + (size_t)p * (ny + 1) * nx;
+    const bool ow = a.flags & OPEN_W, oe = a.flags & OPEN_E, os = a.flags & OPEN_S, on = a.flags & OPEN_N;
+    const bool col_in = i < nx;                     // lane's column is a cell
+    const bool own = lane < MS && col_in;           // lane's cell belongs to this strip
+    const bool xface = i <= nx;                     // lane's x-face exists
+    const bool xown = xface && (lane < MS || i == nx);
+    // halo column of lanes 0..4 in the LDS row: i0-3, i0-2, i0-1, i0+64, i0+65
+    const int hcol = lane < 3 ? i0 - 3 + lane : i0 + 61 + lane;
+    const int hidx = lane < 3 ? lane : 64 + lane;
+    const bool hin = lane < 5 && hcol >= 0 && hcol < nx;
+
+    auto ld = [&](int j, double& v, bool& o) {
+        const bool in = col_in && j >= 0 && j < ny;
+        v = in ? cin[(size_t)j * nx + i] : 0.0;
+        o = in && okp[(size_t)j * nx + i] != 0;
+    };
+    auto ldh = [&](int j, double& v, bool& o) {
+        const bool in = hin && j >= 0 && j < ny;
+        v = in ? cin[(size_t)j * nx + hcol] : 0.0;
+        o = in && okp[(size_t)j * nx + hcol] != 0;
+    };
+
+    double w[7];
+    bool kw[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) ld(jb - 3 + m, w[m], kw[m]);
+    double hv;
+    bool hk;
+    ldh(jb, hv, hk);
+
+    // y-face jb (the chunk's first south face)
+    double fys;
+    {
+        const double q = (col_in) ? Qyk[(size_t)jb * nx + i] : 0.0;
+        const double elo = extp ? (col_in ? extp[i] : 0.0) : w[3];
+        const double ehi = extp ? (col_in ? extp[(size_t)(ny - 1) * nx + i] : 0.0) : w[2];
+        fys = col_in ? line_face<Z>(q, jb, ny, w, kw, os, on, elo, ehi) : 0.0;
+    }
+    // stage 2: F0 on the chunk's first south face; stage 3: S there
+    double f0s = 0.0;
+    if (STAGE == 1 && col_in) fyp[(size_t)jb * nx + i] = fys;                // F0 on the owned face jb
+    if (STAGE >= 2 && col_in) f0s = (STAGE == 2 ? fyp : syp)[(size_t)jb * nx + i];
+    if (STAGE == 2 && col_in) syp[(size_t)jb * nx + i] = f0s + fys;         // S = F0 + F1 on an owned face
+    // stage 3 south-face pieces: F_H and F_L of face jb
+    double fhs = 0.0, fls = 0.0;
+    double cC = 0.0;                                                         // C[j, i]
+    if (STAGE == 3) {
+        if (col_in) {
+            fhs = f0s / 6.0 + 2.0 * fys / 3.0;
+            const double q = Qyk[(size_t)jb * nx + i];
+            const double clo = jb > 0 ? Cp[(size_t)(jb - 1) * nx + i] : 0.0;
+            cC = Cp[(size_t)jb * nx + i];
+            fls = donor(q, jb, ny, clo, cC, os, on, ext0[i], ext0[(size_t)(ny - 1) * nx + i]);
+            if (!UNLIM) a.ay[((size_t)p * (ny + 1) + jb) * nx + i] = fhs - fls;
+        }
+    }
+
+    for (int j = jb; j < je; ++j) {
+        // prefetch the next row of the window and the next halo
+        double wn, hn;
+        bool kn, hkn;
+        ld(j + 4, wn, kn);
+        ldh(j + 1, hn, hkn);
+        // x-face i of row j
+        rbuf[wv][lane + 3] = w[3];
+        obuf[wv][lane + 3] = kw[3];
+        if (lane < 5) {
+            rbuf[wv][hidx] = hv;
+            obuf[wv][hidx] = hk;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double fxw = 0.0;
+        if (xface) {
+            double u[6];
+            bool o[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                u[m] = rbuf[wv][lane + m];
+                o[m] = obuf[wv][lane + m] != 0;
+            }
+            const double q = Qxk[(size_t)j * (nx + 1) + i];
+            const double elo = extp ? extp[(size_t)j * nx] : u[3];
+            const double ehi = extp ? extp[(size_t)j * nx + nx - 1] : u[2];
+            fxw = line_face<Z>(q, i, nx, u, o, ow, oe, elo, ehi);
+        }
+        // y-face j+1 (north of row j): rows j-2 .. j+3 = w[1..6]
+        double fyn = 0.0, qyn = 0.0;
+        if (col_in) {
+            qyn = Qyk[(size_t)(j + 1) * nx + i];
+            const double elo = extp ? extp[i] : w[4];
+            const double ehi = extp ? extp[(size_t)(ny - 1) * nx + i] : w[3];
+            fyn = line_face<Z>(qyn, j + 1, ny, w + 1, kw + 1, os, on, elo, ehi);
+        }
+        const double fxe = shfl_down1(fxw);
+        const bool yown = (j + 1 < je) || (j + 1 == ny);
+        const size_t xc = (size_t)j * nx + i;
+
+        if (STAGE == 1) {
+            if (xown) fxp[(size_t)j * (nx + 1) + i] = fxw;
+            if (col_in && yown) fyp[(size_t)(j + 1) * nx + i] = fyn;
+            if (own) {
+                const double d = fxe - fxw + fyn - fys;
+                const double c0 = Cp[xc], v0 = V0k[xc], v1 = V1k[xc];
+                const double q0 = v0 * c0, dV = v1 - v0;
+                const double qn = 0.0 * q0 + 1.0 * (q0 - a.dt * d);
+                const double V = v0 + dV;
+                a.cout[(size_t)p * plane + xc] = V > 0 ? qn / V : c0;
+            }
+        } else if (STAGE == 2) {
+            double f0w = 0.0, f0n = 0.0;
+            if (xface) f0w = fxp[(size_t)j * (nx + 1) + i];
+            if (col_in) f0n = fyp[(size_t)(j + 1) * nx + i];
+            const double f0e = shfl_down1(f0w);
+            if (xown) sxp[(size_t)j * (nx + 1) + i] = f0w + fxw;
+            if (col_in && yown) syp[(size_t)(j + 1) * nx + i] = f0n + fyn;
+            if (own) {
+                const double c0 = Cp[xc], v0 = V0k[xc], v1 = V1k[xc];
+                const double q0 = v0 * c0, dV = v1 - v0;
+                const double d0 = f0e - f0w + f0n - f0s;
+                const double q1 = 0.0 * q0 + 1.0 * (q0 - a.dt * d0);
+                const double d1 = fxe - fxw + fyn - fys;
+                const double qn = 0.75 * q0 + 0.25 * (q1 - a.dt * d1);
+                const double V = v0 + 0.5 * dV;
+                a.cout[(size_t)p * plane + xc] = V > 0 ? qn / V : c0;
+            }
+            f0s = f0n;
+        } else {
+            // F_H on the lane's x-face and on the north face
+            double sw = 0.0, sn = 0.0;
+            if (xface) sw = sxp[(size_t)j * (nx + 1) + i];
+            if (col_in) sn = syp[(size_t)(j + 1) * nx + i];
+            const double fhw = sw / 6.0 + 2.0 * fxw / 3.0;
+            const double fhn = sn / 6.0 + 2.0 * fyn / 3.0;
+            const double fhe = shfl_down1(fhw);
+            // donor fluxes: x-face i between C[j, i-1] and C[j, i]; y-face j+1 between C[j, i] and C[j+1, i]
+            double cCl = shfl_up1(cC);
+            if (lane == 0) cCl = (i0 > 0 && col_in) ? Cp[xc - 1] : 0.0;
+            const double cCn = (col_in && j + 1 < ny) ? Cp[xc + nx] : 0.0;
+            double flw = 0.0, fln = 0.0;
+            if (xface) {
+                const double q = Qxk[(size_t)j * (nx + 1) + i];
+                const double chi = col_in ? cC : 0.0;
+                flw = donor(q, i, nx, cCl, chi, ow, oe, ext0[(size_t)j * nx], ext0[(size_t)j * nx + nx - 1]);
+            }
+            if (col_in) fln = donor(qyn, j + 1, ny, cC, cCn, os, on, ext0[i], ext0[(size_t)(ny - 1) * nx + i]);
+            const double fle = shfl_down1(flw);
+            if (UNLIM) {
+                if (a.fxi && xown && i == a.flux_face_i) a.fxi[(size_t)p * ny + j] = fhw;
+            } else {
+                if (xown) a.ax[((size_t)p * ny + j) * (nx + 1) + i] = fhw - flw;
+                if (col_in && yown) a.ay[((size_t)p * (ny + 1) + j + 1) * nx + i] = fhn - fln;
+            }
+            if (own) {
+                const double c0 = cC, v0 = V0k[xc], v1 = V1k[xc];
+                const double q0 = v0 * c0;
+                double qn;
+                if (UNLIM) {
+                    const double d = fhe - fhw + fhn - fhs;
+                    qn = q0 - a.dt * d;
+                } else {
+                    const double d = fle - flw + fln - fls;
+                    qn = 0.0 * q0 + 1.0 * (q0 - a.dt * d);
+                }
+                a.cout[(size_t)p * plane + xc] = v1 > 0 ? qn / v1 : c0;
+            }
+            fhs = fhn;
+            fls = fln;
+            cC = cCn;
+        }
+        fys = fyn;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            w[m] = w[m + 1];
+            kw[m] = kw[m + 1];
+        }
+        w[6] = wn;
+        kw[6] = kn;
+        hv = hn;
+        hk = hkn;
+    }
+}
+
+}  // namespace ocx
