@@ -37,11 +37,10 @@ def step_bytes(nt: int) -> float:
 # compulsory bytes per tracer-cell of each kernel of the plain FP64 step (every array it touches moved once):
 # (bytes per tracer-cell, shared bytes per cell amortised over the tracers)
 KERNEL_BYTES = {
-    "fct_stage1": (40.0, 33.0),        # C -> c1, q1, F0 (x, y); Qx, Qy, V0, V1, ok
-    "fct_stage2": (64.0, 33.0),        # c1, q1, C, F0 -> c2, F0 + F1
-    "fct_stage3_split": (56.0, 33.0),  # c2, F0 + F1, C -> A (x, y), c_L
-    "fct_ratio": (48.0, 9.0),          # C, c_L, A -> R+, R-; V1, ok
-    "fct_final": (48.0, 32.0),         # R+-, A, C -> out; Qx, Qy, V0, V1
+    "fct_stage1": (32.0, 33.0),        # C -> c1, F0 (x, y); Qx, Qy, V0, V1, ok
+    "fct_stage2": (56.0, 33.0),        # c1, C, F0 -> c2, S = F0 + F1
+    "fct_stage3_split": (56.0, 33.0),  # c2, S, C -> A (x, y), c_L
+    "fct_limit_final": (40.0, 33.0),   # C, c_L, A -> out; Qx, Qy, V0, V1, ok
 }
 
 

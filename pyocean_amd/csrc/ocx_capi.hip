@@ -12,6 +12,7 @@
 #include "../../include/ocx.h"
 #include "ocx_kernels_f64.hpp"
 #include "ocx_march.hpp"
+#include "ocx_limit.hpp"
 
 namespace {
 
@@ -100,6 +101,7 @@ size_t fct_layout(void* base, int nt, int nz, int ny, int nx, FctWs* w) {
 
 constexpr int MARCH_WPB = 4;     // waves per block of the marching kernels
 constexpr int MARCH_RCH = 64;    // rows per chunk
+constexpr int LIMIT_WPB = 4;     // waves per block of the limiter kernel (consecutive layers)
 
 template <int STAGE, bool UNLIM>
 void launch_march(const ocx::MarchArgs& a, bool js, hipStream_t s) {
@@ -277,18 +279,38 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
         launch_march<3, false>(a, js, s);
     }
     OCX_TRY(launch_check("fct high/low split"));
-    // ratios R_in (into c2), R_out (into S's x buffer)
+    // limiter and final update (c_L in c1, A in the F0 buffers)
     {
-        ProfScope _ps("fct_ratio", s);
-        k_fct_ratio<TX, TY><<<grid, block, 0, s>>>(C, w.c1, V1, w.fx, w.fy, ok, w.c2, w.sx, nt, nz, ny, nx, dt);
+        ProfScope _ps("fct_limit_final", s);
+        LimitArgs l{};
+        l.C = C;
+        l.cL = w.c1;
+        l.ext = inflow;
+        l.V0 = V0;
+        l.V1 = V1;
+        l.Qx = Qx;
+        l.Qy = Qy;
+        l.ok = ok;
+        l.ax = w.fx;
+        l.ay = w.fy;
+        l.out = C_out;
+        l.fxi = fx_i;
+        l.flux_face_i = fi;
+        l.gedge = open_flux ? w.gedge : nullptr;
+        l.nt = nt;
+        l.n = nz;
+        l.ny = ny;
+        l.nx = nx;
+        l.rch = MARCH_RCH;
+        l.nsx = (nx + LS - 1) / LS;
+        l.ncy = (ny + MARCH_RCH - 1) / MARCH_RCH;
+        l.dt = dt;
+        l.flags = flags;
+        const long long waves = (long long)l.nsx * l.ncy * nt * nz;
+        const unsigned blocks = (unsigned)((waves + LIMIT_WPB - 1) / LIMIT_WPB);
+        k_fct_limit<LIMIT_WPB><<<blocks, 64 * LIMIT_WPB, 0, s>>>(l);
     }
-    OCX_TRY(launch_check("fct ratios"));
-    {
-        ProfScope _ps("fct_final", s);
-        k_fct_final<TX, TY><<<grid, block, 0, s>>>(C, inflow, V0, V1, Qx, Qy, w.fx, w.fy, w.c2, w.sx, C_out, fx_i, fi,
-                                                   open_flux ? w.gedge : nullptr, nt, nz, ny, nx, dt, flags);
-    }
-    OCX_TRY(launch_check("fct final"));
+    OCX_TRY(launch_check("fct limiter"));
     if (open_flux) {
         k_open_flux<<<nt, 256, 0, s>>>(w.gedge, open_flux, nt, nz, ny, nx, flags);
         OCX_TRY(launch_check("open flux"));

@@ -48,8 +48,17 @@ struct MarchArgs {
     unsigned flags;
 };
 
-__device__ __forceinline__ double shfl_down1(double v) { return __shfl_down(v, 1, 64); }
-__device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, 64); }
+// Neighbour-lane moves by DPP wavefront shifts (a VALU move, no LDS round trip): lane l receives lane l+1
+// (shfl_down1) or lane l-1 (shfl_up1); the lane without a source keeps its own value.  Every lane of the wave must
+// execute them (they sit outside divergent branches).
+__device__ __forceinline__ int dpp_down1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_up1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ double shfl_down1(double v) {
+    return __hiloint2double(dpp_down1(__double2hiint(v)), dpp_down1(__double2loint(v)));
+}
+__device__ __forceinline__ double shfl_up1(double v) {
+    return __hiloint2double(dpp_up1(__double2hiint(v)), dpp_up1(__double2loint(v)));
+}
 
 // WENO/edge face flux of a face whose upwind-ordered candidates are u[0..5] = cells f-3 .. f+2 along the axis
 // (validity o[0..5]); lo = cell f-1, hi = cell f; rules of _advect_faces_h.
@@ -128,6 +137,35 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
         o = in && okp[(size_t)j * nx + hcol] != 0;
     };
 
+    // inputs of one row j that the loop consumes, fetched one row ahead (the marching loop is latency-bound
+    // otherwise: a wave would wait for these loads on every row)
+    struct RowIn {
+        double qx, qy;           // Qx[j, i], Qy[j+1, i]
+        double c, v0, v1;        // C[j, i], V0, V1 (own cell)
+        double fw, fn;           // stage 2: F0, stage 3: S on x-face (j, i) and y-face (j+1, i)
+        double cl;               // stage 3: C[j, i0-1] (lane 0)
+    };
+    auto ld_row = [&](int j) {
+        RowIn r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (j >= je) return r;
+        const size_t xc = (size_t)j * nx + i;
+        if (xface) r.qx = Qxk[(size_t)j * (nx + 1) + i];
+        if (col_in) {
+            r.qy = Qyk[(size_t)(j + 1) * nx + i];
+            r.c = Cp[xc];
+            r.v0 = V0k[xc];
+            r.v1 = V1k[xc];
+        }
+        if (STAGE >= 2) {
+            const double* gx = STAGE == 2 ? fxp : sxp;
+            const double* gy = STAGE == 2 ? fyp : syp;
+            if (xface) r.fw = gx[(size_t)j * (nx + 1) + i];
+            if (col_in) r.fn = gy[(size_t)(j + 1) * nx + i];
+        }
+        if (STAGE == 3 && lane == 0 && i0 > 0 && col_in) r.cl = Cp[xc - 1];
+        return r;
+    };
+
     double w[7];
     bool kw[7];
 #pragma unroll
@@ -135,6 +173,7 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
     double hv;
     bool hk;
     ldh(jb, hv, hk);
+    RowIn cur = ld_row(jb);
 
     // y-face jb (the chunk's first south face)
     double fys;
@@ -151,20 +190,19 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
     if (STAGE == 2 && col_in) syp[(size_t)jb * nx + i] = f0s + fys;         // S = F0 + F1 on an owned face
     // stage 3 south-face pieces: F_H and F_L of face jb
     double fhs = 0.0, fls = 0.0;
-    double cC = 0.0;                                                         // C[j, i]
     if (STAGE == 3) {
         if (col_in) {
             fhs = f0s / 6.0 + 2.0 * fys / 3.0;
             const double q = Qyk[(size_t)jb * nx + i];
             const double clo = jb > 0 ? Cp[(size_t)(jb - 1) * nx + i] : 0.0;
-            cC = Cp[(size_t)jb * nx + i];
-            fls = donor(q, jb, ny, clo, cC, os, on, ext0[i], ext0[(size_t)(ny - 1) * nx + i]);
+            fls = donor(q, jb, ny, clo, cur.c, os, on, ext0[i], ext0[(size_t)(ny - 1) * nx + i]);
             if (!UNLIM) a.ay[((size_t)p * (ny + 1) + jb) * nx + i] = fhs - fls;
         }
     }
 
     for (int j = jb; j < je; ++j) {
-        // prefetch the next row of the window and the next halo
+        // prefetch the next row's inputs, the next row of the window and the next halo
+        const RowIn nxt = ld_row(j + 1);
         double wn, hn;
         bool kn, hkn;
         ld(j + 4, wn, kn);
@@ -188,15 +226,14 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
                 u[m] = rbuf[wv][lane + m];
                 o[m] = obuf[wv][lane + m] != 0;
             }
-            const double q = Qxk[(size_t)j * (nx + 1) + i];
             const double elo = extp ? extp[(size_t)j * nx] : u[3];
             const double ehi = extp ? extp[(size_t)j * nx + nx - 1] : u[2];
-            fxw = line_face<Z>(q, i, nx, u, o, ow, oe, elo, ehi);
+            fxw = line_face<Z>(cur.qx, i, nx, u, o, ow, oe, elo, ehi);
         }
         // y-face j+1 (north of row j): rows j-2 .. j+3 = w[1..6]
-        double fyn = 0.0, qyn = 0.0;
+        double fyn = 0.0;
+        const double qyn = cur.qy;
         if (col_in) {
-            qyn = Qyk[(size_t)(j + 1) * nx + i];
             const double elo = extp ? extp[i] : w[4];
             const double ehi = extp ? extp[(size_t)(ny - 1) * nx + i] : w[3];
             fyn = line_face<Z>(qyn, j + 1, ny, w + 1, kw + 1, os, on, elo, ehi);
@@ -210,21 +247,19 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
             if (col_in && yown) fyp[(size_t)(j + 1) * nx + i] = fyn;
             if (own) {
                 const double d = fxe - fxw + fyn - fys;
-                const double c0 = Cp[xc], v0 = V0k[xc], v1 = V1k[xc];
+                const double c0 = cur.c, v0 = cur.v0, v1 = cur.v1;
                 const double q0 = v0 * c0, dV = v1 - v0;
                 const double qn = 0.0 * q0 + 1.0 * (q0 - a.dt * d);
                 const double V = v0 + dV;
                 a.cout[(size_t)p * plane + xc] = V > 0 ? qn / V : c0;
             }
         } else if (STAGE == 2) {
-            double f0w = 0.0, f0n = 0.0;
-            if (xface) f0w = fxp[(size_t)j * (nx + 1) + i];
-            if (col_in) f0n = fyp[(size_t)(j + 1) * nx + i];
+            const double f0w = cur.fw, f0n = cur.fn;
             const double f0e = shfl_down1(f0w);
             if (xown) sxp[(size_t)j * (nx + 1) + i] = f0w + fxw;
             if (col_in && yown) syp[(size_t)(j + 1) * nx + i] = f0n + fyn;
             if (own) {
-                const double c0 = Cp[xc], v0 = V0k[xc], v1 = V1k[xc];
+                const double c0 = cur.c, v0 = cur.v0, v1 = cur.v1;
                 const double q0 = v0 * c0, dV = v1 - v0;
                 const double d0 = f0e - f0w + f0n - f0s;
                 const double q1 = 0.0 * q0 + 1.0 * (q0 - a.dt * d0);
@@ -236,21 +271,18 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
             f0s = f0n;
         } else {
             // F_H on the lane's x-face and on the north face
-            double sw = 0.0, sn = 0.0;
-            if (xface) sw = sxp[(size_t)j * (nx + 1) + i];
-            if (col_in) sn = syp[(size_t)(j + 1) * nx + i];
-            const double fhw = sw / 6.0 + 2.0 * fxw / 3.0;
-            const double fhn = sn / 6.0 + 2.0 * fyn / 3.0;
+            const double fhw = cur.fw / 6.0 + 2.0 * fxw / 3.0;
+            const double fhn = cur.fn / 6.0 + 2.0 * fyn / 3.0;
             const double fhe = shfl_down1(fhw);
             // donor fluxes: x-face i between C[j, i-1] and C[j, i]; y-face j+1 between C[j, i] and C[j+1, i]
+            const double cC = cur.c;
             double cCl = shfl_up1(cC);
-            if (lane == 0) cCl = (i0 > 0 && col_in) ? Cp[xc - 1] : 0.0;
-            const double cCn = (col_in && j + 1 < ny) ? Cp[xc + nx] : 0.0;
+            if (lane == 0) cCl = cur.cl;
+            const double cCn = (j + 1 < je) ? nxt.c : ((col_in && j + 1 < ny) ? Cp[xc + nx] : 0.0);
             double flw = 0.0, fln = 0.0;
             if (xface) {
-                const double q = Qxk[(size_t)j * (nx + 1) + i];
                 const double chi = col_in ? cC : 0.0;
-                flw = donor(q, i, nx, cCl, chi, ow, oe, ext0[(size_t)j * nx], ext0[(size_t)j * nx + nx - 1]);
+                flw = donor(cur.qx, i, nx, cCl, chi, ow, oe, ext0[(size_t)j * nx], ext0[(size_t)j * nx + nx - 1]);
             }
             if (col_in) fln = donor(qyn, j + 1, ny, cC, cCn, os, on, ext0[i], ext0[(size_t)(ny - 1) * nx + i]);
             const double fle = shfl_down1(flw);
@@ -261,7 +293,7 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
                 if (col_in && yown) a.ay[((size_t)p * (ny + 1) + j + 1) * nx + i] = fhn - fln;
             }
             if (own) {
-                const double c0 = cC, v0 = V0k[xc], v1 = V1k[xc];
+                const double c0 = cC, v0 = cur.v0, v1 = cur.v1;
                 const double q0 = v0 * c0;
                 double qn;
                 if (UNLIM) {
@@ -275,7 +307,6 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
             }
             fhs = fhn;
             fls = fln;
-            cC = cCn;
         }
         fys = fyn;
         __builtin_amdgcn_wave_barrier();
@@ -288,6 +319,7 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
         kw[6] = kn;
         hv = hn;
         hk = hkn;
+        cur = nxt;
     }
 }
 

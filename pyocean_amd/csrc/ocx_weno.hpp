@@ -15,13 +15,43 @@ constexpr unsigned UNLIMITED = 32u;    // FCT coefficients = 1 (the RK3 result)
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
+// Correctly rounded x / b for a constant divisor b with rb = RN(1/b): q = RN(x rb) is within an ulp of x/b, the
+// remainder x - b q is exact under an FMA, and RN(q + (x - b q) rb) = RN(x/b) (Markstein's correction; needs rb
+// within half an ulp of 1/b and no underflow, so tiny |x| takes the IEEE division).
+__device__ __forceinline__ double div_const(double x, double b, double rb) {
+    if (fabs(x) < 0x1p-960 && x != 0.0) return x / b;
+    const double q = x * rb;
+    const double r = __fma_rn(-b, q, x);
+    return __fma_rn(r, rb, q);
+}
+
+// x / y for y > 0 by the Newton-Raphson sequence that the compiler emits for an IEEE FP64 division (reciprocal,
+// two refinements, quotient, one residual correction), without the range scaling and special-value fix-up steps.
+// Where those steps would not change anything (neither operand near the overflow or denormal range) the result is
+// the IEEE quotient bit for bit; callers guarantee that range.
+__device__ __forceinline__ double div_nr(double x, double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double e = __fma_rn(-y, r, 1.0);
+    r = __fma_rn(r, e, r);
+    e = __fma_rn(-y, r, 1.0);
+    r = __fma_rn(r, e, r);
+    const double q = x * r;
+    const double t = __fma_rn(-y, q, x);
+    return __fma_rn(t, r, q);
+}
+
 // WENO5 value from the upwind-ordered stencil v1..v5 (v3 = upwind cell).  Z: WENO5-Z (Borges et al. 2008,
 // tau5 = |b2 - b0|, eps = 1e-40); otherwise WENO5-JS (Jiang & Shu 1996, eps = 1e-6).  d = (0.1, 0.6, 0.3).
-template <bool Z>
+// FAST: the divisions by div_const / div_nr (identical results for stencil values in the guarded range).
+template <bool Z, bool FAST>
 __device__ __forceinline__ double weno5(double v1, double v2, double v3, double v4, double v5) {
-    const double q0 = (2.0 * v1 - 7.0 * v2 + 11.0 * v3) / 6.0;
-    const double q1 = (-v2 + 5.0 * v3 + 2.0 * v4) / 6.0;
-    const double q2 = (2.0 * v3 + 5.0 * v4 - v5) / 6.0;
+    constexpr double r6 = 1.0 / 6.0;
+    const double n0 = 2.0 * v1 - 7.0 * v2 + 11.0 * v3;
+    const double n1 = -v2 + 5.0 * v3 + 2.0 * v4;
+    const double n2 = 2.0 * v3 + 5.0 * v4 - v5;
+    const double q0 = FAST ? div_const(n0, 6.0, r6) : n0 / 6.0;
+    const double q1 = FAST ? div_const(n1, 6.0, r6) : n1 / 6.0;
+    const double q2 = FAST ? div_const(n2, 6.0, r6) : n2 / 6.0;
     constexpr double c1312 = 13.0 / 12.0;
     const double b0 = c1312 * sq(v1 - 2.0 * v2 + v3) + 0.25 * sq(v1 - 4.0 * v2 + 3.0 * v3);
     const double b1 = c1312 * sq(v2 - 2.0 * v3 + v4) + 0.25 * sq(v2 - v4);
@@ -29,32 +59,42 @@ __device__ __forceinline__ double weno5(double v1, double v2, double v3, double 
     double a0, a1, a2;
     if (Z) {
         const double tau = fabs(b2 - b0);
-        a0 = 0.1 * (1.0 + tau / (1e-40 + b0));
-        a1 = 0.6 * (1.0 + tau / (1e-40 + b1));
-        a2 = 0.3 * (1.0 + tau / (1e-40 + b2));
+        a0 = 0.1 * (1.0 + (FAST ? div_nr(tau, 1e-40 + b0) : tau / (1e-40 + b0)));
+        a1 = 0.6 * (1.0 + (FAST ? div_nr(tau, 1e-40 + b1) : tau / (1e-40 + b1)));
+        a2 = 0.3 * (1.0 + (FAST ? div_nr(tau, 1e-40 + b2) : tau / (1e-40 + b2)));
     } else {
-        a0 = 0.1 / sq(1e-6 + b0);
-        a1 = 0.6 / sq(1e-6 + b1);
-        a2 = 0.3 / sq(1e-6 + b2);
+        a0 = FAST ? div_nr(0.1, sq(1e-6 + b0)) : 0.1 / sq(1e-6 + b0);
+        a1 = FAST ? div_nr(0.6, sq(1e-6 + b1)) : 0.6 / sq(1e-6 + b1);
+        a2 = FAST ? div_nr(0.3, sq(1e-6 + b2)) : 0.3 / sq(1e-6 + b2);
     }
-    return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+    const double num = a0 * q0 + a1 * q1 + a2 * q2, den = a0 + a1 + a2;
+    return FAST ? div_nr(num, den) : num / den;
 }
 
 // WENO3 from v2..v4 (candidates (-v2 + 3 v3)/2 and (v3 + v4)/2, d = (1/3, 2/3), eps = 1e-6).
+template <bool FAST>
 __device__ __forceinline__ double weno3(double v2, double v3, double v4) {
     const double c0 = (-v2 + 3.0 * v3) / 2.0, c1 = (v3 + v4) / 2.0;
-    const double e0 = (1.0 / 3.0) / sq(1e-6 + sq(v3 - v2));
-    const double e1 = (2.0 / 3.0) / sq(1e-6 + sq(v4 - v3));
-    return (e0 * c0 + e1 * c1) / (e0 + e1);
+    const double y0 = sq(1e-6 + sq(v3 - v2)), y1 = sq(1e-6 + sq(v4 - v3));
+    const double e0 = FAST ? div_nr(1.0 / 3.0, y0) : (1.0 / 3.0) / y0;
+    const double e1 = FAST ? div_nr(2.0 / 3.0, y1) : (2.0 / 3.0) / y1;
+    const double num = e0 * c0 + e1 * c1, den = e0 + e1;
+    return FAST ? div_nr(num, den) : num / den;
 }
 
 // Face value with the validity rule of the reference: WENO5 when all five cells are valid, WENO3 when v2..v4
-// are, else the upwind cell.  Cells outside the domain carry value 0 and validity 0.
+// are, else the upwind cell.  Cells outside the domain carry value 0 and validity 0.  The fast divisions are used
+// when the stencil's largest magnitude m lies in [2^-400, 2^100] (or m = 0): then no operand of a division is near
+// the overflow or denormal range (the smoothness indicators stay below 2^210, the weight sums above 2^-430, a
+// nonzero numerator above 2^-520), which is where the plain sequence and the IEEE division can differ.
 template <bool Z>
 __device__ __forceinline__ double face_value(double v1, double v2, double v3, double v4, double v5,
                                              bool k1, bool k2, bool k3, bool k4, bool k5) {
-    if (k1 && k2 && k3 && k4 && k5) return weno5<Z>(v1, v2, v3, v4, v5);
-    if (k2 && k3 && k4) return weno3(v2, v3, v4);
+    const double m = fmax(fmax(fmax(fabs(v1), fabs(v2)), fmax(fabs(v3), fabs(v4))), fabs(v5));
+    const bool fast = m <= 0x1p100 && (m >= 0x1p-400 || m == 0.0);
+    if (k1 && k2 && k3 && k4 && k5)
+        return fast ? weno5<Z, true>(v1, v2, v3, v4, v5) : weno5<Z, false>(v1, v2, v3, v4, v5);
+    if (k2 && k3 && k4) return fast ? weno3<true>(v2, v3, v4) : weno3<false>(v2, v3, v4);
     return v3;
 }
 
