@@ -41,8 +41,10 @@ extern "C" {
 const char* ocx_last_error(void);
 int ocx_version(void);
 
-/* Workspace (device bytes) of ocx_advect_fct_f64 / ocx_advect_rk3_f64 for one call of that shape. */
+/* Workspace (device bytes) of ocx_advect_fct_f64 / ocx_advect_fct_split / ocx_advect_rk3_f64 for one call of that
+ * shape. */
 size_t ocx_fct_workspace_bytes(int nt, int nz, int ny, int nx);
+size_t ocx_fct_split_workspace_bytes(int nt, int nz, int ny, int nx);
 size_t ocx_rk3_workspace_bytes(int nt, int nz, int ny, int nx);
 
 /* One SSP-RK3 + Zalesak FCT step of the stack C.
@@ -56,6 +58,17 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
                        const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx, double dt,
                        unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux,
                        void* workspace, size_t workspace_bytes, void* stream);
+
+/* The same step in split precision (no counterpart in the reference; the precision form of the same algorithm):
+ * the WENO reconstruction (smoothness indicators, weights, face values and fluxes) in FP32, the stage values c1, c2
+ * and the face fluxes F0, F0 + F1, A stored in FP32; the h*T accumulation, the low-order solution, the Zalesak
+ * ratios, the limited update and content sums in FP64.  Conservation and the FCT bounds hold as in FP64; values
+ * differ from the FP64 step by the FP32 reconstruction error.  Same arguments as ocx_advect_fct_f64 except that
+ * OCX_UNLIMITED is rejected. */
+int ocx_advect_fct_split(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                         const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx, double dt,
+                         unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux,
+                         void* workspace, size_t workspace_bytes, void* stream);
 
 /* One SSP-RK3 step without limiter, x, y and (if vert) z sweeps.  Replaces TracerAdvection.advect
  * (core/components/tracers.py:342-375) with _advect_div and _rk_stage (sigma_baroclinic.py:324-364, 856-875).

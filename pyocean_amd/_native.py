@@ -9,7 +9,9 @@ from __future__ import annotations
 import ctypes
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libocx.so"
+import os
+
+LIB_PATH = Path(os.environ.get("OCX_LIB", Path(__file__).resolve().parent / "lib" / "libocx.so"))
 
 OPEN_W, OPEN_E, OPEN_S, OPEN_N = 1, 2, 4, 8
 WENO_JS = 16
@@ -24,8 +26,10 @@ SIGNATURES = {
     "ocx_last_error": (ctypes.c_char_p, []),
     "ocx_version": (I, []),
     "ocx_fct_workspace_bytes": (Z, [I, I, I, I]),
+    "ocx_fct_split_workspace_bytes": (Z, [I, I, I, I]),
     "ocx_rk3_workspace_bytes": (Z, [I, I, I, I]),
     "ocx_advect_fct_f64": (I, [P, P, P, P, P, P, P, I, I, I, I, D, U, I, P, P, P, P, Z, P]),
+    "ocx_advect_fct_split": (I, [P, P, P, P, P, P, P, I, I, I, I, D, U, I, P, P, P, P, Z, P]),
     "ocx_advect_rk3_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, U, I, I, P, P, P, Z, P]),
     "ocx_faces_h_f64": (I, [P, P, P, P, P, I, I, I, I, U, P, P, P]),
     "ocx_upwind_faces_h_f64": (I, [P, P, P, P, I, I, I, I, U, P, P, P]),

@@ -79,38 +79,49 @@ struct Sizes {
     }
 };
 
-// FCT workspace: c1 (later c_L), c2 (later R_in), F0 (later A), S (later R_out), open-edge fluxes.
+// FCT workspace.  FP64: c1 (later c_L), c2, F0 (later A), S, open-edge fluxes.  Split: c1, c2 and the face
+// stacks in FP32, c_L in its own FP64 stack.
 struct FctWs {
-    double *c1, *c2, *fx, *fy, *sx, *sy, *gedge;
+    void *c1, *c2, *fx, *fy, *sx, *sy;
+    double *cl, *gedge;
 };
 
+template <class PR>
 size_t fct_layout(void* base, int nt, int nz, int ny, int nx, FctWs* w) {
+    using ST = typename PR::st;
+    using FL = typename PR::fl;
     Sizes s(nt, nz, ny, nx);
     Carver cv(base);
     FctWs t;
-    t.c1 = cv.take<double>(s.S);
-    t.c2 = cv.take<double>(s.S);
-    t.fx = cv.take<double>(s.SX);
-    t.fy = cv.take<double>(s.SY);
-    t.sx = cv.take<double>(s.SX);
-    t.sy = cv.take<double>(s.SY);
+    t.c1 = cv.take<ST>(s.S);
+    t.c2 = cv.take<ST>(s.S);
+    t.fx = cv.take<FL>(s.SX);
+    t.fy = cv.take<FL>(s.SY);
+    t.sx = cv.take<FL>(s.SX);
+    t.sy = cv.take<FL>(s.SY);
+    t.cl = PR::split ? cv.take<double>(s.S) : static_cast<double*>(t.c1);
     t.gedge = cv.take<double>(2 * s.P * ny + 2 * s.P * nx);
     if (w) *w = t;
     return cv.used;
 }
 
-constexpr int MARCH_WPB = 4;     // waves per block of the marching kernels
 constexpr int MARCH_RCH = 64;    // rows per chunk
-constexpr int LIMIT_WPB = 4;     // waves per block of the limiter kernel (consecutive layers)
 
-template <int STAGE, bool UNLIM>
+// waves per workgroup = tracers per group: the tracers split into groups of at most MAXW, as even as possible
+int tracer_group(int nt) {
+    const int groups = (nt + ocx::MAXW - 1) / ocx::MAXW;
+    return (nt + groups - 1) / groups;
+}
+
+template <int STAGE, bool UNLIM, class PR>
 void launch_march(const ocx::MarchArgs& a, bool js, hipStream_t s) {
-    const long long waves = (long long)a.nsx * a.ncy * a.nt * a.n;
-    const unsigned blocks = (unsigned)((waves + MARCH_WPB - 1) / MARCH_WPB);
+    const int wpb = tracer_group(a.nt);
+    const long long ntb = (a.nt + wpb - 1) / wpb;
+    const unsigned blocks = (unsigned)(ntb * a.nsx * a.ncy * a.n);
     if (js)
-        ocx::k_march<STAGE, false, UNLIM, MARCH_WPB><<<blocks, 64 * MARCH_WPB, 0, s>>>(a);
+        ocx::k_march<STAGE, false, UNLIM, PR><<<blocks, 64 * wpb, 0, s>>>(a);
     else
-        ocx::k_march<STAGE, true, UNLIM, MARCH_WPB><<<blocks, 64 * MARCH_WPB, 0, s>>>(a);
+        ocx::k_march<STAGE, true, UNLIM, PR><<<blocks, 64 * wpb, 0, s>>>(a);
 }
 
 struct Rk3Ws {
@@ -176,7 +187,12 @@ int ocx_version(void) { return 1; }
 
 size_t ocx_fct_workspace_bytes(int nt, int nz, int ny, int nx) {
     if (check_shape(nt, nz, ny, nx) != OCX_OK) return 0;
-    return fct_layout(nullptr, nt, nz, ny, nx, nullptr);
+    return fct_layout<ocx::PrecF64>(nullptr, nt, nz, ny, nx, nullptr);
+}
+
+size_t ocx_fct_split_workspace_bytes(int nt, int nz, int ny, int nx) {
+    if (check_shape(nt, nz, ny, nx) != OCX_OK) return 0;
+    return fct_layout<ocx::PrecSplit>(nullptr, nt, nz, ny, nx, nullptr);
 }
 
 size_t ocx_rk3_workspace_bytes(int nt, int nz, int ny, int nx) {
@@ -184,10 +200,15 @@ size_t ocx_rk3_workspace_bytes(int nt, int nz, int ny, int nx) {
     return rk3_layout(nullptr, nt, nz, ny, nx, nullptr);
 }
 
-int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
-                       const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx, double dt,
-                       unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux,
-                       void* workspace, size_t workspace_bytes, void* stream) {
+}  // extern "C"
+
+namespace {
+
+template <class PR>
+int advect_fct_impl(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                    const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx, double dt,
+                    unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux, void* workspace,
+                    size_t workspace_bytes, void* stream) {
     using namespace ocx;
     OCX_TRY(check_shape(nt, nz, ny, nx));
     OCX_TRY(need(C, "C"));
@@ -202,16 +223,17 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
     if (flux_face_i != -1 && (flux_face_i < 0 || flux_face_i > nx))
         return fail(OCX_E_ARG, "flux_face_i %d outside 0..%d", flux_face_i, nx);
     if (flux_face_i >= 0 && !fx_i) return fail(OCX_E_ARG, "fx_i is NULL with flux_face_i = %d", flux_face_i);
-    const size_t wsb = fct_layout(nullptr, nt, nz, ny, nx, nullptr);
+    if (PR::split && (flags & UNLIMITED))
+        return fail(OCX_E_ARG, "the unlimited step has no split-precision form");
+    const size_t wsb = fct_layout<PR>(nullptr, nt, nz, ny, nx, nullptr);
     if (!workspace || workspace_bytes < wsb)
         return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed, %zu given", wsb, workspace ? workspace_bytes : 0);
     FctWs w;
-    fct_layout(workspace, nt, nz, ny, nx, &w);
+    fct_layout<PR>(workspace, nt, nz, ny, nx, &w);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
     const bool js = flags & WENO_JS, unlim = flags & UNLIMITED;
     const int fi = fx_i ? flux_face_i : -1;
-    if ((long long)((nx + MS - 1) / MS) * ((ny + MARCH_RCH - 1) / MARCH_RCH) * nt * nz / MARCH_WPB >= (1LL << 31))
+    if ((long long)((nx + LS - 1) / LS) * ((ny + MARCH_RCH - 1) / MARCH_RCH) * nt * nz >= (1LL << 31))
         return fail(OCX_E_ARG, "shape (nt=%d, nz=%d, ny=%d, nx=%d) too large for one launch", nt, nz, ny, nx);
     MarchArgs a{};
     a.C = C;
@@ -243,7 +265,7 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
         ProfScope _ps("fct_stage1", s);
         a.cin = C;
         a.cout = w.c1;
-        launch_march<1, false>(a, js, s);
+        launch_march<1, false, PR>(a, js, s);
     }
     OCX_TRY(launch_check("fct stage 1"));
     // stage 2: S = F0 + F1, c2
@@ -251,7 +273,7 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
         ProfScope _ps("fct_stage2", s);
         a.cin = w.c1;
         a.cout = w.c2;
-        launch_march<2, false>(a, js, s);
+        launch_march<2, false, PR>(a, js, s);
     }
     OCX_TRY(launch_check("fct stage 2"));
     if (unlim) {
@@ -261,7 +283,7 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
             a.cout = C_out;
             a.fxi = fx_i;
             a.flux_face_i = fi;
-            launch_march<3, true>(a, js, s);
+            launch_march<3, true, PR>(a, js, s);
         }
         OCX_TRY(launch_check("fct unlimited"));
         if (open_flux) {
@@ -275,8 +297,8 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
     {
         ProfScope _ps("fct_stage3_split", s);
         a.cin = w.c2;
-        a.cout = w.c1;
-        launch_march<3, false>(a, js, s);
+        a.cout = w.cl;
+        launch_march<3, false, PR>(a, js, s);
     }
     OCX_TRY(launch_check("fct high/low split"));
     // limiter and final update (c_L in c1, A in the F0 buffers)
@@ -284,7 +306,7 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
         ProfScope _ps("fct_limit_final", s);
         LimitArgs l{};
         l.C = C;
-        l.cL = w.c1;
+        l.cL = w.cl;
         l.ext = inflow;
         l.V0 = V0;
         l.V1 = V1;
@@ -306,9 +328,10 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
         l.ncy = (ny + MARCH_RCH - 1) / MARCH_RCH;
         l.dt = dt;
         l.flags = flags;
-        const long long waves = (long long)l.nsx * l.ncy * nt * nz;
-        const unsigned blocks = (unsigned)((waves + LIMIT_WPB - 1) / LIMIT_WPB);
-        k_fct_limit<LIMIT_WPB><<<blocks, 64 * LIMIT_WPB, 0, s>>>(l);
+        const int wpb = tracer_group(nt);
+        const long long ntb = (nt + wpb - 1) / wpb;
+        const unsigned blocks = (unsigned)(ntb * l.nsx * l.ncy * nz);
+        k_fct_limit<PR><<<blocks, 64 * wpb, 0, s>>>(l);
     }
     OCX_TRY(launch_check("fct limiter"));
     if (open_flux) {
@@ -316,6 +339,26 @@ int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, cons
         OCX_TRY(launch_check("open flux"));
     }
     return OCX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocx_advect_fct_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                       const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx, double dt,
+                       unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+    return advect_fct_impl<ocx::PrecF64>(C, V0, V1, Qx, Qy, ok, inflow, nt, nz, ny, nx, dt, flags, flux_face_i,
+                                         C_out, fx_i, open_flux, workspace, workspace_bytes, stream);
+}
+
+int ocx_advect_fct_split(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                         const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx, double dt,
+                         unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux,
+                         void* workspace, size_t workspace_bytes, void* stream) {
+    return advect_fct_impl<ocx::PrecSplit>(C, V0, V1, Qx, Qy, ok, inflow, nt, nz, ny, nx, dt, flags, flux_face_i,
+                                           C_out, fx_i, open_flux, workspace, workspace_bytes, stream);
 }
 
 int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,

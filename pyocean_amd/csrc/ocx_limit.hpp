@@ -7,7 +7,7 @@
 // core/dynamics/sigma_baroclinic.py:407-431) and finishes row j: face coefficients from the ratios of the two
 // adjacent cells (:436-447), G = F_L + C A, q = q0 - dt div G, out = q/V1 (safe) (_fct_final :514-530).  R+- of
 // the west/east neighbours come from the adjacent lanes, those of rows j-1, j+1 from a register ring, so R+- never
-// goes through HBM.  Waves are numbered plane-fastest: the waves of layers k-1, k, k+1 of one strip run side by
+// goes through HBM.  The waves of a workgroup are the tracers of one layer; the workgroups of layers k-1, k, k+1
 // side in one workgroup, so the k +- 1 reads of C and c_L hit the caches of that compute unit.
 #pragma once
 #include "ocx_march.hpp"
@@ -25,8 +25,8 @@ struct LimitArgs {
     const double* Qx;
     const double* Qy;
     const uint8_t* ok;
-    const double* ax;      // antidiffusive fluxes A
-    const double* ay;
+    const void* ax;        // antidiffusive fluxes A (the precision's flux type)
+    const void* ay;
     double* out;
     double* fxi;           // optional face column flux_face_i of G
     int flux_face_i;
@@ -55,18 +55,25 @@ struct RowR {
     double aw, an;           // A on the lane's west face and on its north face
 };
 
-template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_fct_limit(LimitArgs a) {
+// Work decomposition: a workgroup holds the waves of up to MAXW tracers of one (layer, strip, chunk) (shared
+// per-layer fields from the CU's caches); logical blocks run layer-fastest, so the workgroups of layers k-1, k, k+1
+// of a strip run side by side on one XCD and the k +- 1 reads of C and c_L hit its L2.
+template <class PR>
+__global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
+    using FL = typename PR::fl;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long long gw = (long long)blockIdx.x * WPB + wv;
-    const long long P = (long long)a.nt * a.n;
-    const long long nwaves = P * a.nsx * a.ncy;
-    if (gw >= nwaves) return;                      // whole wave leaves; no block barrier is used
-    const int p = (int)(gw % P);
-    const long long r1 = gw / P;
-    const int sx = (int)(r1 % a.nsx);
-    const int cy = (int)(r1 / a.nsx);
-    const int k = p % a.n;
+    const int wpb = blockDim.x >> 6;
+    const int ntb = (a.nt + wpb - 1) / wpb;
+    long long L = xcd_logical(blockIdx.x, gridDim.x);
+    const int tg = (int)(L % ntb);
+    L /= ntb;
+    const int k = (int)(L % a.n);
+    L /= a.n;
+    const int sx = (int)(L % a.nsx);
+    const int cy = (int)(L / a.nsx);
+    const int t = tg * wpb + wv;
+    if (t >= a.nt || cy >= a.ncy) return;         // whole wave leaves; no block barrier is used
+    const int p = t * a.n + k;
     const int ny = a.ny, nx = a.nx, n = a.n;
     const int i0 = sx * LS, i = i0 - 2 + lane;
     const int jb = cy * a.rch, je = min(jb + a.rch, ny);
@@ -82,8 +89,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fct_limit(LimitArgs a) {
     const double* V1k = a.V1 + (size_t)k * plane;
     const double* Qxk = a.Qx + (size_t)k * ny * (nx + 1);
     const double* Qyk = a.Qy + (size_t)k * (ny + 1) * nx;
-    const double* axp = a.ax + (size_t)p * ny * (nx + 1);
-    const double* ayp = a.ay + (size_t)p * (ny + 1) * nx;
+    const FL* axp = static_cast<const FL*>(a.ax) + (size_t)p * ny * (nx + 1);
+    const FL* ayp = static_cast<const FL*>(a.ay) + (size_t)p * (ny + 1) * nx;
     const bool has_dn = k > 0, has_up = k + 1 < n;
     const double dt = a.dt;
     const unsigned flags = a.flags;
@@ -101,10 +108,10 @@ __global__ __launch_bounds__(64 * WPB) void k_fct_limit(LimitArgs a) {
         const bool rin_dom = r >= 0 && r < ny;
         if (!rin_dom) return q;
         const size_t x = (size_t)r * nx + i;
-        if (i >= 0 && i <= nx) q.aw = axp[(size_t)r * (nx + 1) + i];
+        if (i >= 0 && i <= nx) q.aw = (double)axp[(size_t)r * (nx + 1) + i];
         if (col_in && rlane) {
-            q.an = ayp[x + nx];
-            q.as = ayp[x];
+            q.an = (double)ayp[x + nx];
+            q.as = (double)ayp[x];
             q.v1 = V1k[x];
             if (has_dn) {
                 q.od = okk[x - plane] != 0;

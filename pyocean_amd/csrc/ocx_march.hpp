@@ -16,6 +16,8 @@
 //   3: F2 = faces(c2);  F_H = (F0 + F1)/6 + 2 F2/3;  donor F_L;  A = F_H - F_L -> ax/ay;  c_L = (q0 - dt div F_L)/V1
 //      (safe) -> cout.  UNLIM: out = (q0 - dt div F_H)/V1 (safe) -> cout and the face column of F_H.
 #pragma once
+#include <type_traits>
+
 #include "ocx_weno.hpp"
 
 namespace ocx {
@@ -24,7 +26,7 @@ constexpr int MS = 63;     // own cells per strip
 constexpr int MRB = 72;    // LDS row: columns i0-3 .. i0+65 (69 used)
 
 struct MarchArgs {
-    const double* cin;     // the stage's input values (C, c1, c2)
+    const void* cin;       // the stage's input values: C (double), or c1, c2 (the precision's stage type)
     const double* C;       // the step's initial values
     const double* ext;     // inflow values, or nullptr (stage's own values; C for the donor flux)
     const double* V0;
@@ -32,13 +34,13 @@ struct MarchArgs {
     const double* Qx;
     const double* Qy;
     const uint8_t* ok;
-    double* fx;            // F0 (stage 1 out, stage 2 in; read-only in stage 2)
-    double* fy;
-    double* s_x;           // S = F0 + F1 (stage 2 out, stage 3 in)
-    double* s_y;
-    double* ax;            // A (stage 3; may be the F0 buffers, dead by then)
-    double* ay;
-    double* cout;          // c1, c2, c_L (or the unlimited result)
+    void* fx;              // F0 (stage 1 out, stage 2 in; read-only in stage 2)      [flux type]
+    void* fy;
+    void* s_x;             // S = F0 + F1 (stage 2 out, stage 3 in)                     [flux type]
+    void* s_y;
+    void* ax;              // A (stage 3; may be the F0 buffers, dead by then)         [flux type]
+    void* ay;
+    void* cout;            // c1, c2 [stage type]; c_L or the unlimited result [double]
     double* fxi;           // unlimited: face column of F_H
     int flux_face_i;
     int nt, n, ny, nx;
@@ -60,22 +62,89 @@ __device__ __forceinline__ double shfl_up1(double v) {
     return __hiloint2double(dpp_up1(__double2hiint(v)), dpp_up1(__double2loint(v)));
 }
 
-// WENO/edge face flux of a face whose upwind-ordered candidates are u[0..5] = cells f-3 .. f+2 along the axis
-// (validity o[0..5]); lo = cell f-1, hi = cell f; rules of _advect_faces_h.
+// Precision of a step.  F64: everything in FP64, bit-identical to the reference.  Split: stage values c1, c2 and
+// the face fluxes F0, F0 + F1, A stored in FP32 and the WENO reconstruction (smoothness indicators, weights, face
+// values, fluxes) computed in FP32; the h*T accumulation q = a q0 + b (q' - dt div F), c_L, the FCT ratios, the
+// limited update and the content sums stay FP64.
+struct PrecF64 {
+    using st = double;     // stored stage values
+    using fl = double;     // stored face fluxes
+    using w = double;      // reconstruction arithmetic
+    static constexpr bool split = false;
+};
+struct PrecSplit {
+    using st = float;
+    using fl = float;
+    using w = float;
+    static constexpr bool split = true;
+};
+
+__device__ __forceinline__ float sqf(float x) { return x * x; }
+
+// FP32 WENO5 value (same formulas; hardware reciprocals, contraction allowed).  WENO-Z's eps = 1e-40 is below the
+// FP32 normal range, so eps = 1e-30 (1 + m^2) with m the stencil's largest magnitude: it only keeps a vanishing
+// indicator from dividing by zero, and tau/(eps + beta) stays below 4e30 whatever the data.
 template <bool Z>
-__device__ __forceinline__ double line_face(double q, int f, int nf, const double* u, const bool* o, bool open_lo,
-                                            bool open_hi, double ext_lo, double ext_hi) {
+__device__ __forceinline__ float weno5f(float v1, float v2, float v3, float v4, float v5) {
+#pragma clang fp contract(fast)
+    const float r6 = 1.0f / 6.0f;
+    const float q0 = (2.0f * v1 - 7.0f * v2 + 11.0f * v3) * r6;
+    const float q1 = (-v2 + 5.0f * v3 + 2.0f * v4) * r6;
+    const float q2 = (2.0f * v3 + 5.0f * v4 - v5) * r6;
+    const float c1312 = 13.0f / 12.0f;
+    const float b0 = c1312 * sqf(v1 - 2.0f * v2 + v3) + 0.25f * sqf(v1 - 4.0f * v2 + 3.0f * v3);
+    const float b1 = c1312 * sqf(v2 - 2.0f * v3 + v4) + 0.25f * sqf(v2 - v4);
+    const float b2 = c1312 * sqf(v3 - 2.0f * v4 + v5) + 0.25f * sqf(3.0f * v3 - 4.0f * v4 + v5);
+    float a0, a1, a2;
+    if (Z) {
+        const float m = fmaxf(fmaxf(fmaxf(fabsf(v1), fabsf(v2)), fmaxf(fabsf(v3), fabsf(v4))), fabsf(v5));
+        const float eps = 1e-30f * (1.0f + m * m);
+        const float tau = fabsf(b2 - b0);
+        a0 = 0.1f + 0.1f * tau * __builtin_amdgcn_rcpf(eps + b0);
+        a1 = 0.6f + 0.6f * tau * __builtin_amdgcn_rcpf(eps + b1);
+        a2 = 0.3f + 0.3f * tau * __builtin_amdgcn_rcpf(eps + b2);
+    } else {
+        a0 = 0.1f * __builtin_amdgcn_rcpf(sqf(1e-6f + b0));
+        a1 = 0.6f * __builtin_amdgcn_rcpf(sqf(1e-6f + b1));
+        a2 = 0.3f * __builtin_amdgcn_rcpf(sqf(1e-6f + b2));
+    }
+    return (a0 * q0 + a1 * q1 + a2 * q2) * __builtin_amdgcn_rcpf(a0 + a1 + a2);
+}
+
+__device__ __forceinline__ float weno3f(float v2, float v3, float v4) {
+#pragma clang fp contract(fast)
+    const float c0 = (-v2 + 3.0f * v3) * 0.5f, c1 = (v3 + v4) * 0.5f;
+    const float e0 = (1.0f / 3.0f) * __builtin_amdgcn_rcpf(sqf(1e-6f + sqf(v3 - v2)));
+    const float e1 = (2.0f / 3.0f) * __builtin_amdgcn_rcpf(sqf(1e-6f + sqf(v4 - v3)));
+    return (e0 * c0 + e1 * c1) * __builtin_amdgcn_rcpf(e0 + e1);
+}
+
+template <bool Z>
+__device__ __forceinline__ float face_value(float v1, float v2, float v3, float v4, float v5, bool k1, bool k2,
+                                            bool k3, bool k4, bool k5) {
+    if (k1 && k2 && k3 && k4 && k5) return weno5f<Z>(v1, v2, v3, v4, v5);
+    if (k2 && k3 && k4) return weno3f(v2, v3, v4);
+    return v3;
+}
+
+// WENO/edge face flux of a face whose upwind-ordered candidates are u[0..5] = cells f-3 .. f+2 along the axis
+// (validity o[0..5]); lo = cell f-1, hi = cell f; rules of _advect_faces_h.  W: the reconstruction type (the flux
+// q * value is formed in it too).
+template <bool Z, class W>
+__device__ __forceinline__ W line_face(double qd, int f, int nf, const W* u, const bool* o, bool open_lo,
+                                       bool open_hi, W ext_lo, W ext_hi) {
+    const W q = (W)qd;
     if (f > 0 && f < nf) {
-        if (q == 0.0) return 0.0;
-        const bool pos = q >= 0.0;
-        const double v1 = pos ? u[0] : u[5], v2 = pos ? u[1] : u[4], v3 = pos ? u[2] : u[3];
-        const double v4 = pos ? u[3] : u[2], v5 = pos ? u[4] : u[1];
+        if (qd == 0.0) return (W)0;
+        const bool pos = qd >= 0.0;
+        const W v1 = pos ? u[0] : u[5], v2 = pos ? u[1] : u[4], v3 = pos ? u[2] : u[3];
+        const W v4 = pos ? u[3] : u[2], v5 = pos ? u[4] : u[1];
         const bool k1 = pos ? o[0] : o[5], k2 = pos ? o[1] : o[4], k3 = pos ? o[2] : o[3];
         const bool k4 = pos ? o[3] : o[2], k5 = pos ? o[4] : o[1];
         return q * face_value<Z>(v1, v2, v3, v4, v5, k1, k2, k3, k4, k5);
     }
-    if (f == 0) return open_lo ? q * (q > 0.0 ? ext_lo : u[3]) : 0.0;
-    return open_hi ? q * (q < 0.0 ? ext_hi : u[2]) : 0.0;
+    if (f == 0) return open_lo ? q * (qd > 0.0 ? ext_lo : u[3]) : (W)0;
+    return open_hi ? q * (qd < 0.0 ? ext_hi : u[2]) : (W)0;
 }
 
 // donor-cell flux of a face between cells lo and hi (interior), or of an edge face (rules of _fct_fluxes)
@@ -86,24 +155,40 @@ __device__ __forceinline__ double donor(double q, int f, int nf, double lo, doub
     return open_hi ? q * (q < 0 ? ext_hi : lo) : 0.0;
 }
 
-template <int STAGE, bool Z, bool UNLIM, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
-    __shared__ double rbuf[WPB][MRB];
-    __shared__ uint8_t obuf[WPB][MRB];
+// Work decomposition: a workgroup holds the waves of up to MAXW tracers of one (layer, strip, chunk), so the
+// per-layer fields (Qx, Qy, V0, V1, ok) they all read come from the compute unit's caches after the first wave;
+// logical blocks run strip-fastest, each XCD through a contiguous range.
+#ifndef OCX_MAXW
+#define OCX_MAXW 5
+#endif
+constexpr int MAXW = OCX_MAXW;
+
+template <int STAGE, bool Z, bool UNLIM, class PR>
+__global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
+    using W = typename PR::w;
+    using ST = typename PR::st;
+    using FL = typename PR::fl;
+    using IN = typename std::conditional<STAGE == 1, double, ST>::type;   // type of the stage's input values
+    __shared__ W rbuf[MAXW][MRB];
+    __shared__ uint8_t obuf[MAXW][MRB];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long long gw = (long long)blockIdx.x * WPB + wv;
-    const long long nwaves = (long long)a.nsx * a.ncy * a.nt * a.n;
-    if (gw >= nwaves) return;                      // whole wave leaves; no block barrier is used below
-    const int sx = (int)(gw % a.nsx);
-    const long long r1 = gw / a.nsx;
-    const int cy = (int)(r1 % a.ncy);
-    const int p = (int)(r1 / a.ncy);
-    const int k = p % a.n;
+    const int wpb = blockDim.x >> 6;
+    const int ntb = (a.nt + wpb - 1) / wpb;
+    long long L = xcd_logical(blockIdx.x, gridDim.x);
+    const int tg = (int)(L % ntb);
+    L /= ntb;
+    const int sx = (int)(L % a.nsx);
+    L /= a.nsx;
+    const int cy = (int)(L % a.ncy);
+    const int k = (int)(L / a.ncy);
+    const int t = tg * wpb + wv;
+    if (t >= a.nt || k >= a.n) return;            // whole wave leaves; no block barrier is used below
+    const int p = t * a.n + k;
     const int ny = a.ny, nx = a.nx;
     const int i0 = sx * MS, i = i0 + lane;
     const int jb = cy * a.rch, je = min(jb + a.rch, ny);
     const size_t plane = (size_t)ny * nx;
-    const double* cin = a.cin + (size_t)p * plane;
+    const IN* cin = static_cast<const IN*>(a.cin) + (size_t)p * plane;
     const double* Cp = a.C + (size_t)p * plane;
     const double* extp = a.ext ? a.ext + (size_t)p * plane : nullptr;   // stage faces: nullptr = cin
     const double* ext0 = a.ext ? a.ext + (size_t)p * plane : Cp;        // donor flux: inflow or C
@@ -112,10 +197,12 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
     const double* Qyk = a.Qy + (size_t)k * (ny + 1) * nx;
     const double* V0k = a.V0 + (size_t)k * plane;
     const double* V1k = a.V1 + (size_t)k * plane;
-    double* fxp = a.fx + (size_t)p * ny * (nx + 1);
-    double* fyp = a.fy + (size_t)p * (ny + 1) * nx;
-    double* sxp = a.s_x + (size_t)p * ny * (nx + 1);
-    double* syp = a.s_y + (size_t)p * (ny + 1) * nx;
+    FL* fxp = static_cast<FL*>(a.fx) + (size_t)p * ny * (nx + 1);
+    FL* fyp = static_cast<FL*>(a.fy) + (size_t)p * (ny + 1) * nx;
+    FL* sxp = static_cast<FL*>(a.s_x) + (size_t)p * ny * (nx + 1);
+    FL* syp = static_cast<FL*>(a.s_y) + (size_t)p * (ny + 1) * nx;
+    FL* axp = static_cast<FL*>(a.ax) + (size_t)p * ny * (nx + 1);
+    FL* ayp = static_cast<FL*>(a.ay) + (size_t)p * (ny + 1) * nx;
     const bool ow = a.flags & OPEN_W, oe = a.flags & OPEN_E, os = a.flags & OPEN_S, on = a.flags & OPEN_N;
     const bool col_in = i < nx;                     // lane's column is a cell
     const bool own = lane < MS && col_in;           // lane's cell belongs to this strip
@@ -126,16 +213,18 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
     const int hidx = lane < 3 ? lane : 64 + lane;
     const bool hin = lane < 5 && hcol >= 0 && hcol < nx;
 
-    auto ld = [&](int j, double& v, bool& o) {
+    auto ld = [&](int j, W& v, bool& o) {
         const bool in = col_in && j >= 0 && j < ny;
-        v = in ? cin[(size_t)j * nx + i] : 0.0;
+        v = in ? (W)cin[(size_t)j * nx + i] : (W)0;
         o = in && okp[(size_t)j * nx + i] != 0;
     };
-    auto ldh = [&](int j, double& v, bool& o) {
+    auto ldh = [&](int j, W& v, bool& o) {
         const bool in = hin && j >= 0 && j < ny;
-        v = in ? cin[(size_t)j * nx + hcol] : 0.0;
+        v = in ? (W)cin[(size_t)j * nx + hcol] : (W)0;
         o = in && okp[(size_t)j * nx + hcol] != 0;
     };
+    // exterior value of an edge face: the inflow, or (nullptr) the stage's own edge cell
+    auto ex = [&](size_t x, W own_val) -> W { return extp ? (W)extp[x] : own_val; };
 
     // inputs of one row j that the loop consumes, fetched one row ahead (the marching loop is latency-bound
     // otherwise: a wave would wait for these loads on every row)
@@ -157,20 +246,20 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
             r.v1 = V1k[xc];
         }
         if (STAGE >= 2) {
-            const double* gx = STAGE == 2 ? fxp : sxp;
-            const double* gy = STAGE == 2 ? fyp : syp;
-            if (xface) r.fw = gx[(size_t)j * (nx + 1) + i];
-            if (col_in) r.fn = gy[(size_t)(j + 1) * nx + i];
+            const FL* gx = STAGE == 2 ? fxp : sxp;
+            const FL* gy = STAGE == 2 ? fyp : syp;
+            if (xface) r.fw = (double)gx[(size_t)j * (nx + 1) + i];
+            if (col_in) r.fn = (double)gy[(size_t)(j + 1) * nx + i];
         }
         if (STAGE == 3 && lane == 0 && i0 > 0 && col_in) r.cl = Cp[xc - 1];
         return r;
     };
 
-    double w[7];
+    W w[7];
     bool kw[7];
 #pragma unroll
     for (int m = 0; m < 7; ++m) ld(jb - 3 + m, w[m], kw[m]);
-    double hv;
+    W hv;
     bool hk;
     ldh(jb, hv, hk);
     RowIn cur = ld_row(jb);
@@ -179,31 +268,32 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
     double fys;
     {
         const double q = (col_in) ? Qyk[(size_t)jb * nx + i] : 0.0;
-        const double elo = extp ? (col_in ? extp[i] : 0.0) : w[3];
-        const double ehi = extp ? (col_in ? extp[(size_t)(ny - 1) * nx + i] : 0.0) : w[2];
-        fys = col_in ? line_face<Z>(q, jb, ny, w, kw, os, on, elo, ehi) : 0.0;
+        const W elo = col_in ? ex(i, w[3]) : (W)0;
+        const W ehi = col_in ? ex((size_t)(ny - 1) * nx + i, w[2]) : (W)0;
+        fys = col_in ? (double)line_face<Z>(q, jb, ny, w, kw, os, on, elo, ehi) : 0.0;
     }
-    // stage 2: F0 on the chunk's first south face; stage 3: S there
+    // stage 1: F0 on the owned face jb; stage 2: F0 there and S = F0 + F1; stage 3: S there
     double f0s = 0.0;
-    if (STAGE == 1 && col_in) fyp[(size_t)jb * nx + i] = fys;                // F0 on the owned face jb
-    if (STAGE >= 2 && col_in) f0s = (STAGE == 2 ? fyp : syp)[(size_t)jb * nx + i];
-    if (STAGE == 2 && col_in) syp[(size_t)jb * nx + i] = f0s + fys;         // S = F0 + F1 on an owned face
+    if (STAGE == 1 && col_in) fyp[(size_t)jb * nx + i] = (FL)fys;
+    if (STAGE >= 2 && col_in) f0s = (double)(STAGE == 2 ? fyp : syp)[(size_t)jb * nx + i];
+    if (STAGE == 2 && col_in) syp[(size_t)jb * nx + i] = (FL)(f0s + fys);
     // stage 3 south-face pieces: F_H and F_L of face jb
     double fhs = 0.0, fls = 0.0;
     if (STAGE == 3) {
         if (col_in) {
-            fhs = f0s / 6.0 + 2.0 * fys / 3.0;
+            fhs = PR::split ? (double)((float)f0s * (1.0f / 6.0f) + 2.0f * (float)fys * (1.0f / 3.0f))
+                            : f0s / 6.0 + 2.0 * fys / 3.0;
             const double q = Qyk[(size_t)jb * nx + i];
             const double clo = jb > 0 ? Cp[(size_t)(jb - 1) * nx + i] : 0.0;
             fls = donor(q, jb, ny, clo, cur.c, os, on, ext0[i], ext0[(size_t)(ny - 1) * nx + i]);
-            if (!UNLIM) a.ay[((size_t)p * (ny + 1) + jb) * nx + i] = fhs - fls;
+            if (!UNLIM) ayp[(size_t)jb * nx + i] = (FL)(fhs - fls);
         }
     }
 
     for (int j = jb; j < je; ++j) {
         // prefetch the next row's inputs, the next row of the window and the next halo
         const RowIn nxt = ld_row(j + 1);
-        double wn, hn;
+        W wn, hn;
         bool kn, hkn;
         ld(j + 4, wn, kn);
         ldh(j + 1, hn, hkn);
@@ -219,45 +309,41 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double fxw = 0.0;
         if (xface) {
-            double u[6];
+            W u[6];
             bool o[6];
 #pragma unroll
             for (int m = 0; m < 6; ++m) {
                 u[m] = rbuf[wv][lane + m];
                 o[m] = obuf[wv][lane + m] != 0;
             }
-            const double elo = extp ? extp[(size_t)j * nx] : u[3];
-            const double ehi = extp ? extp[(size_t)j * nx + nx - 1] : u[2];
-            fxw = line_face<Z>(cur.qx, i, nx, u, o, ow, oe, elo, ehi);
+            fxw = (double)line_face<Z>(cur.qx, i, nx, u, o, ow, oe, ex((size_t)j * nx, u[3]),
+                                       ex((size_t)j * nx + nx - 1, u[2]));
         }
         // y-face j+1 (north of row j): rows j-2 .. j+3 = w[1..6]
         double fyn = 0.0;
         const double qyn = cur.qy;
-        if (col_in) {
-            const double elo = extp ? extp[i] : w[4];
-            const double ehi = extp ? extp[(size_t)(ny - 1) * nx + i] : w[3];
-            fyn = line_face<Z>(qyn, j + 1, ny, w + 1, kw + 1, os, on, elo, ehi);
-        }
+        if (col_in) fyn = (double)line_face<Z>(qyn, j + 1, ny, w + 1, kw + 1, os, on, ex(i, w[4]),
+                                               ex((size_t)(ny - 1) * nx + i, w[3]));
         const double fxe = shfl_down1(fxw);
         const bool yown = (j + 1 < je) || (j + 1 == ny);
         const size_t xc = (size_t)j * nx + i;
 
         if (STAGE == 1) {
-            if (xown) fxp[(size_t)j * (nx + 1) + i] = fxw;
-            if (col_in && yown) fyp[(size_t)(j + 1) * nx + i] = fyn;
+            if (xown) fxp[(size_t)j * (nx + 1) + i] = (FL)fxw;
+            if (col_in && yown) fyp[(size_t)(j + 1) * nx + i] = (FL)fyn;
             if (own) {
                 const double d = fxe - fxw + fyn - fys;
                 const double c0 = cur.c, v0 = cur.v0, v1 = cur.v1;
                 const double q0 = v0 * c0, dV = v1 - v0;
                 const double qn = 0.0 * q0 + 1.0 * (q0 - a.dt * d);
                 const double V = v0 + dV;
-                a.cout[(size_t)p * plane + xc] = V > 0 ? qn / V : c0;
+                static_cast<ST*>(a.cout)[(size_t)p * plane + xc] = (ST)(V > 0 ? qn / V : c0);
             }
         } else if (STAGE == 2) {
             const double f0w = cur.fw, f0n = cur.fn;
             const double f0e = shfl_down1(f0w);
-            if (xown) sxp[(size_t)j * (nx + 1) + i] = f0w + fxw;
-            if (col_in && yown) syp[(size_t)(j + 1) * nx + i] = f0n + fyn;
+            if (xown) sxp[(size_t)j * (nx + 1) + i] = (FL)(f0w + fxw);
+            if (col_in && yown) syp[(size_t)(j + 1) * nx + i] = (FL)(f0n + fyn);
             if (own) {
                 const double c0 = cur.c, v0 = cur.v0, v1 = cur.v1;
                 const double q0 = v0 * c0, dV = v1 - v0;
@@ -266,13 +352,19 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
                 const double d1 = fxe - fxw + fyn - fys;
                 const double qn = 0.75 * q0 + 0.25 * (q1 - a.dt * d1);
                 const double V = v0 + 0.5 * dV;
-                a.cout[(size_t)p * plane + xc] = V > 0 ? qn / V : c0;
+                static_cast<ST*>(a.cout)[(size_t)p * plane + xc] = (ST)(V > 0 ? qn / V : c0);
             }
             f0s = f0n;
         } else {
             // F_H on the lane's x-face and on the north face
-            const double fhw = cur.fw / 6.0 + 2.0 * fxw / 3.0;
-            const double fhn = cur.fn / 6.0 + 2.0 * fyn / 3.0;
+            double fhw, fhn;
+            if (PR::split) {
+                fhw = (double)((float)cur.fw * (1.0f / 6.0f) + 2.0f * (float)fxw * (1.0f / 3.0f));
+                fhn = (double)((float)cur.fn * (1.0f / 6.0f) + 2.0f * (float)fyn * (1.0f / 3.0f));
+            } else {
+                fhw = cur.fw / 6.0 + 2.0 * fxw / 3.0;
+                fhn = cur.fn / 6.0 + 2.0 * fyn / 3.0;
+            }
             const double fhe = shfl_down1(fhw);
             // donor fluxes: x-face i between C[j, i-1] and C[j, i]; y-face j+1 between C[j, i] and C[j+1, i]
             const double cC = cur.c;
@@ -289,8 +381,8 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
             if (UNLIM) {
                 if (a.fxi && xown && i == a.flux_face_i) a.fxi[(size_t)p * ny + j] = fhw;
             } else {
-                if (xown) a.ax[((size_t)p * ny + j) * (nx + 1) + i] = fhw - flw;
-                if (col_in && yown) a.ay[((size_t)p * (ny + 1) + j + 1) * nx + i] = fhn - fln;
+                if (xown) axp[(size_t)j * (nx + 1) + i] = (FL)(fhw - flw);
+                if (col_in && yown) ayp[(size_t)(j + 1) * nx + i] = (FL)(fhn - fln);
             }
             if (own) {
                 const double c0 = cC, v0 = cur.v0, v1 = cur.v1;
@@ -303,7 +395,7 @@ __global__ __launch_bounds__(64 * WPB) void k_march(MarchArgs a) {
                     const double d = fle - flw + fln - fls;
                     qn = 0.0 * q0 + 1.0 * (q0 - a.dt * d);
                 }
-                a.cout[(size_t)p * plane + xc] = v1 > 0 ? qn / v1 : c0;
+                static_cast<double*>(a.cout)[(size_t)p * plane + xc] = v1 > 0 ? qn / v1 : c0;
             }
             fhs = fhn;
             fls = fln;

@@ -15,6 +15,14 @@ constexpr unsigned UNLIMITED = 32u;    // FCT coefficients = 1 (the RK3 result)
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): physical block b runs on XCD b % 8.
+// This maps b to a logical index so that every XCD works through one contiguous range of logical blocks; blocks
+// that are neighbours in logical order then share an L2 (a bijection for any block count).
+__device__ __forceinline__ long long xcd_logical(long long b, long long nb) {
+    const long long q = nb / 8, r = nb % 8, x = b % 8, s = b / 8;
+    return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
+
 // Correctly rounded x / b for a constant divisor b with rb = RN(1/b): q = RN(x rb) is within an ulp of x/b, the
 // remainder x - b q is exact under an FMA, and RN(q + (x - b q) rb) = RN(x/b) (Markstein's correction; needs rb
 // within half an ulp of 1/b and no underflow, so tiny |x| takes the IEEE division).

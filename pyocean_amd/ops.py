@@ -69,12 +69,18 @@ def flags_of(open_mask: int = 0, js: bool = False, unlimited: bool = False) -> i
     return int(open_mask) | (_native.WENO_JS if js else 0) | (_native.UNLIMITED if unlimited else 0)
 
 
+PRECISIONS = ("f64", "split")
+
+
 def advect_fct(C, V0, V1, Qx, Qy, ok, dt, inflow=None, flags=0, flux_face_i=None, open_flux=False, out=None,
-               workspace: Workspace | None = None):
+               workspace: Workspace | None = None, precision: str = "f64"):
     """
-    One SSP-RK3 + FCT step on device tensors (``ocx_advect_fct_f64``).  Returns ``out`` or (out, fx_i, open_flux)
-    when ``flux_face_i`` / ``open_flux`` are asked for (None where not).
+    One SSP-RK3 + FCT step on device tensors: ``precision`` "f64" (``ocx_advect_fct_f64``, bit-identical to the
+    reference) or "split" (``ocx_advect_fct_split``: FP32 reconstruction, FP64 accumulation and limiter).  Returns
+    ``out`` or (out, fx_i, open_flux) when ``flux_face_i`` / ``open_flux`` are asked for (None where not).
     """
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision {precision!r}: use one of {PRECISIONS}")
     nt, n, ny, nx = C.shape
     _need(C, (nt, n, ny, nx), "C")
     for t, s, nm in ((V0, (n, ny, nx), "V0"), (V1, (n, ny, nx), "V1"), (Qx, (n, ny, nx + 1), "Qx"),
@@ -87,12 +93,14 @@ def advect_fct(C, V0, V1, Qx, Qy, ok, dt, inflow=None, flags=0, flux_face_i=None
     fxi = torch.empty((nt, n, ny), dtype=_F64, device=C.device) if flux_face_i is not None else None
     of = torch.empty((nt,), dtype=_F64, device=C.device) if open_flux else None
     L = lib()
-    nb = L.ocx_fct_workspace_bytes(nt, n, ny, nx)
+    split = precision == "split"
+    nb = (L.ocx_fct_split_workspace_bytes if split else L.ocx_fct_workspace_bytes)(nt, n, ny, nx)
     ws = (workspace or _WS).get(nb)
-    rc = L.ocx_advect_fct_f64(_ptr(C), _ptr(V0), _ptr(V1), _ptr(Qx), _ptr(Qy), _ptr(ok), _ptr(inflow), nt, n, ny,
-                              nx, float(dt), int(flags), -1 if flux_face_i is None else int(flux_face_i), _ptr(out),
-                              _ptr(fxi), _ptr(of), _ptr(ws), ws.numel(), _stream())
-    check(rc, "ocx_advect_fct_f64")
+    fn = L.ocx_advect_fct_split if split else L.ocx_advect_fct_f64
+    rc = fn(_ptr(C), _ptr(V0), _ptr(V1), _ptr(Qx), _ptr(Qy), _ptr(ok), _ptr(inflow), nt, n, ny, nx, float(dt),
+            int(flags), -1 if flux_face_i is None else int(flux_face_i), _ptr(out), _ptr(fxi), _ptr(of), _ptr(ws),
+            ws.numel(), _stream())
+    check(rc, "ocx_advect_fct_split" if split else "ocx_advect_fct_f64")
     if flux_face_i is None and not open_flux:
         return out
     return out, fxi, of

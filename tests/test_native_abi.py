@@ -35,7 +35,9 @@ def test_workspace_sizes_are_host_computations():
     nt, nz, ny, nx = 10, 50, 1024, 1024
     b = lib.ocx_fct_workspace_bytes(nt, nz, ny, nx)
     S = nt * nz * ny * nx
-    assert 7 * S * 8 <= b <= 7.2 * S * 8                    # 3 cell stacks + 2 x 2 face stacks + edge fluxes
+    assert 6 * S * 8 <= b <= 6.1 * S * 8                    # 2 cell stacks + 2 x 2 face stacks + edge fluxes
+    bs = lib.ocx_fct_split_workspace_bytes(nt, nz, ny, nx)
+    assert 6 * S * 4 + S * 8 <= bs <= 6.1 * S * 4 + S * 8  # FP32 stages and fluxes, FP64 c_L
     assert lib.ocx_rk3_workspace_bytes(nt, nz, ny, nx) >= 3 * S * 8
     assert lib.ocx_fct_workspace_bytes(0, 1, 1, 1) == 0
 
