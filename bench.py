@@ -28,19 +28,28 @@ HBM_PEAK = 8.0e12                 # MI355X HBM3E, bytes/s
 METRIC = "tracer-cell updates/s (1024²×50, 10 tracers); % HBM roofline; p50/p99"
 
 
-def step_bytes(nt: int) -> float:
+def step_bytes(nt: int, precision: str = "f64") -> float:
     """Algorithmic HBM bytes per tracer-cell update of one FCT step (three dependent RK3 sweeps, FCT fused into the
-    third): tracer streams 48 B read + 24 B written, shared Qx, Qy, V0, V1 (32 B) and ok (1 B) read once per sweep."""
-    return 72.0 + 99.0 / nt
+    third): tracer streams 48 B read + 24 B written, shared Qx, Qy, V0, V1 (32 B) and ok (1 B) read once per sweep;
+    with c1 and c2 stored in FP32, 52 B instead of 72."""
+    return (52.0 if precision == "split" else 72.0) + 99.0 / nt
 
 
-# compulsory bytes per tracer-cell of each kernel of the plain FP64 step (every array it touches moved once):
+# compulsory bytes per tracer-cell of each kernel of the step (every array it touches moved once):
 # (bytes per tracer-cell, shared bytes per cell amortised over the tracers)
 KERNEL_BYTES = {
-    "fct_stage1": (32.0, 33.0),        # C -> c1, F0 (x, y); Qx, Qy, V0, V1, ok
-    "fct_stage2": (56.0, 33.0),        # c1, C, F0 -> c2, S = F0 + F1
-    "fct_stage3_split": (56.0, 33.0),  # c2, S, C -> A (x, y), c_L
-    "fct_limit_final": (40.0, 33.0),   # C, c_L, A -> out; Qx, Qy, V0, V1, ok
+    "f64": {
+        "fct_stage1": (32.0, 33.0),        # C -> c1, F0 (x, y); Qx, Qy, V0, V1, ok
+        "fct_stage2": (56.0, 33.0),        # c1, C, F0 -> c2, S = F0 + F1
+        "fct_stage3_split": (56.0, 33.0),  # c2, S, C -> A (x, y), c_L
+        "fct_limit_final": (40.0, 33.0),   # C, c_L, A -> out; Qx, Qy, V0, V1, ok
+    },
+    "split": {                             # c1, c2, F0, S, A in FP32; C, c_L, out in FP64
+        "fct_stage1": (20.0, 33.0),
+        "fct_stage2": (32.0, 33.0),
+        "fct_stage3_split": (36.0, 33.0),
+        "fct_limit_final": (32.0, 33.0),
+    },
 }
 
 
@@ -91,6 +100,11 @@ def main() -> int:
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--nz", type=int, default=50)
     ap.add_argument("--nt", type=int, default=10)
+    ap.add_argument("--precision", choices=("f64", "split"), default="f64",
+                    help="f64: the plain FP64 drop-in (bit-identical to the reference); split: FP32 reconstruction "
+                         "with FP64 accumulation and limiter")
+    ap.add_argument("--rch", type=int, default=0, help="rows per chunk of the marching kernels (0: default)")
+    ap.add_argument("--wpb", type=int, default=0, help="tracers (wavefronts) per workgroup (0: default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
@@ -107,6 +121,9 @@ def main() -> int:
 
     from pyocean_amd import _native, ops, synth
 
+    if args.rch or args.wpb:
+        from pyocean_amd import batch as _batch
+        _batch.set_tuning(args.rch, args.wpb)
     print(f"[bench] generating regional {args.nx}x{args.ny}x{args.nz} nt={args.nt}", file=sys.stderr, flush=True)
     case = synth.regional(nx=args.nx, ny=args.ny, nz=args.nz, nt=args.nt, seed=0)
     dev = torch.device("cuda", 0)
@@ -120,7 +137,7 @@ def main() -> int:
     updates = nt * nz * ny * nx
 
     def step(a, b):
-        return ops.advect_fct(a, V0, V1, Qx, Qy, ok, case.dt, out=b, workspace=ws)
+        return ops.advect_fct(a, V0, V1, Qx, Qy, ok, case.dt, out=b, workspace=ws, precision=args.precision)
 
     if args.copy_calibration:
         out.copy_(C)
@@ -154,12 +171,12 @@ def main() -> int:
     value = updates * args.steps / wall
     dom = max(prof, key=lambda k: prof[k][0])
     dom_ms = prof[dom][0] / prof[dom][1]
-    per_tc, shared = KERNEL_BYTES.get(dom, (float("nan"), 0.0))
+    per_tc, shared = KERNEL_BYTES[args.precision].get(dom, (float("nan"), 0.0))
     alg = updates * (per_tc + shared / nt)
     achieved = alg / (dom_ms * 1e-3)
-    workload = f"regional{args.nx}x{args.ny}x{args.nz}nt{args.nt}"
+    workload = f"regional{args.nx}x{args.ny}x{args.nz}nt{args.nt}" + ("" if args.precision == "f64" else "split")
     traffic = read_traffic(dom, workload)
-    B = step_bytes(nt)
+    B = step_bytes(nt, args.precision)
     res = {
         "metric": METRIC,
         "value": value,
@@ -173,11 +190,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "f64" else "f32/f64",
         "data": "synthetic (seeded: curvilinear metrics, land mask, z* layers, k^-3 stream function, 10 tracers)",
-        "config": {"workload": f"regional {nx}x{ny}x{nz}, {nt} tracers, plain FP64 WENO5-Z + Zalesak FCT + SSP-RK3",
+        "config": {"workload": f"regional {nx}x{ny}x{nz}, {nt} tracers, "
+                               + ("plain FP64" if args.precision == "f64" else "FP32/FP64-split")
+                               + " WENO5-Z + Zalesak FCT + SSP-RK3",
                    "nx": nx, "ny": ny, "nz": nz, "nt": nt, "dt": case.dt, "land_fraction": case.info["land"],
-                   "wet_cells": case.info["wet_cells"], "form": "plain_f64"},
+                   "wet_cells": case.info["wet_cells"],
+                   "form": "plain_f64" if args.precision == "f64" else "split_f32_f64"},
         "wet_updates_per_s": value * case.info["wet_cells"] / (nz * ny * nx),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,

@@ -109,6 +109,47 @@ int ocx_periodic_pad(const void* src, void* dst, size_t lead, int sy, int sx, in
 /* Per-tracer content sum_k,j,i V c (FP64, deterministic order): out (nt). */
 int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* stream);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * Batched form (no counterpart in the reference, which advances one domain per call): many independent domains
+ * (ensemble members or regional domains of uneven nt, nz, ny, nx, each FP64 or split precision) advanced by one
+ * FCT step with one persistent launch per stage (four launches in all), optionally replayed as a captured hipGraph.
+ * Each domain's result is the one ocx_advect_fct_f64 / ocx_advect_fct_split gives alone (same kernels, same
+ * per-domain arithmetic).  The state ping-pongs: parity 0 reads C and writes C_out, parity 1 the reverse.
+ * No inflow-less restriction; no fx_i / open_flux side outputs. */
+typedef struct ocx_domain {
+    double* C;                    /* (nt, nz, ny, nx), state buffer A */
+    double* C_out;                /* (nt, nz, ny, nx), state buffer B */
+    const double* V0;
+    const double* V1;
+    const double* Qx;
+    const double* Qy;
+    const double* inflow;         /* NULL: no exterior values */
+    const uint8_t* ok;
+    int nt, nz, ny, nx;
+    double dt;
+    unsigned flags;               /* OCX_OPEN_*, OCX_WENO_JS */
+    int precision;                /* 0: FP64 (bit-identical to the reference), 1: split */
+    void* workspace;              /* ocx_fct_workspace_bytes / ocx_fct_split_workspace_bytes of the domain */
+    size_t workspace_bytes;
+} ocx_domain;
+
+typedef struct ocx_batch ocx_batch;
+
+/* rows_per_chunk: rows one wavefront marches (0: 64); waves_per_group: wavefronts per workgroup, the tracers of one
+ * layer (0: 2, at most 4); groups_per_cu: resident workgroups per compute unit of the persistent grid (0: 4).  Copies
+ * the domain table to the device (synchronous). */
+int ocx_batch_create(const ocx_domain* domains, int ndomains, int rows_per_chunk, int waves_per_group,
+                     int groups_per_cu, ocx_batch** out);
+int ocx_batch_step(ocx_batch* batch, int parity, void* stream);
+/* Captures the step of this parity into a hipGraph on its first call (stream must not be the NULL stream), then
+ * launches the graph. */
+int ocx_batch_graph_step(ocx_batch* batch, int parity, void* stream);
+int ocx_batch_destroy(ocx_batch* batch);
+
+/* Launch shape of the single-domain entry points (for the autotuner; no counterpart in the reference):
+ * rows_per_chunk and waves_per_group as for ocx_batch_create; 0 restores the default.  Process-wide. */
+int ocx_set_tuning(int rows_per_chunk, int waves_per_group);
+
 /* Per-kernel timing (no counterpart in the reference; for benchmarks): while enabled, every kernel launch of this
  * library is bracketed by a hipEvent pair on its stream.  ocx_profile_collect waits for the recorded events,
  * returns the number of distinct kernels and fills up to max_kernels entries (static name strings, total

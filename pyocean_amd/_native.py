@@ -37,9 +37,21 @@ SIGNATURES = {
     "ocx_fct_coefficients_f64": (I, [P, P, P, P, P, P, D, I, I, I, I, P, P, P, Z, P]),
     "ocx_periodic_pad": (I, [P, P, Z, I, I, I, I, I, I, I, P]),
     "ocx_content_f64": (I, [P, P, I, I, I, I, P, P]),
+    "ocx_batch_create": (I, [P, I, I, I, I, P]),
+    "ocx_batch_step": (I, [P, I, P]),
+    "ocx_batch_graph_step": (I, [P, I, P]),
+    "ocx_batch_destroy": (I, [P]),
+    "ocx_set_tuning": (I, [I, I]),
     "ocx_profile_enable": (I, [I]),
     "ocx_profile_collect": (I, [I, P, P, P]),
 }
+
+
+class OcxDomain(ctypes.Structure):
+    """ocx_domain of include/ocx.h (one member of a batch)."""
+    _fields_ = [("C", P), ("C_out", P), ("V0", P), ("V1", P), ("Qx", P), ("Qy", P), ("inflow", P), ("ok", P),
+                ("nt", I), ("nz", I), ("ny", I), ("nx", I), ("dt", D), ("flags", U), ("precision", I),
+                ("workspace", P), ("workspace_bytes", Z)]
 
 
 class OcxError(RuntimeError):

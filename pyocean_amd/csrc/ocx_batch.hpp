@@ -1,0 +1,96 @@
+// Batched FCT step: many independent domains (ensemble members or regional domains of uneven nx x ny x nz, FP64
+// or split precision) advanced by one persistent launch per stage.
+//
+// Every domain contributes the logical blocks its own launch would have (march_blocks / limit_blocks); the batch
+// concatenates them in domain order (prefix sums) and a fixed grid of workgroups walks them: workgroup b takes the
+// contiguous item range [b * per, (b + 1) * per) in XCD-contiguous order, so consecutive items of a domain (the
+// strips of a chunk, the layers k-1, k, k+1 of the limiter) run on one compute unit one after the other.  Each item
+// runs exactly the per-domain code (march_block / limit_block) with the domain's own sizes and precision, so a
+// domain's result does not depend on what else is in the batch.
+#pragma once
+#include "ocx_limit.hpp"
+
+namespace ocx {
+
+struct BatchDomain {
+    MarchArgs m;           // shared fields of the three stages (cin/cout/fxi set per stage)
+    LimitArgs l;           // limiter (out set per parity)
+    const double* C;       // the two state buffers: step parity 0 reads C and writes C_out, parity 1 the reverse
+    double* C_out;
+    void* c1;              // stage buffers from the domain's workspace
+    void* c2;
+    double* cl;
+    int split;             // 1: split precision
+    int js;                // 1: WENO5-JS
+};
+
+__device__ __forceinline__ int find_domain(const long long* prefix, int ndom, long long item) {
+    int lo = 0, hi = ndom;                     // prefix[0] = 0 <= item < prefix[ndom]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= item) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __restrict__ dom,
+                                                           const long long* __restrict__ prefix, int ndom,
+                                                           long long total, long long per, int parity) {
+    __shared__ double rbuf[MAXW][MRB];
+    __shared__ uint8_t obuf[MAXW][MRB];
+    const int wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    const long long b0 = xcd_logical(blockIdx.x, gridDim.x) * per;
+    const long long b1 = b0 + per < total ? b0 + per : total;
+    for (long long it = b0; it < b1; ++it) {
+        const int d = find_domain(prefix, ndom, it);
+        const BatchDomain& D = dom[d];
+        MarchArgs a = D.m;
+        const double* Cin = parity ? D.C_out : D.C;
+        a.C = Cin;
+        if (STAGE == 1) {
+            a.cin = Cin;
+            a.cout = D.c1;
+        } else if (STAGE == 2) {
+            a.cin = D.c1;
+            a.cout = D.c2;
+        } else {
+            a.cin = D.c2;
+            a.cout = D.cl;
+        }
+        const long long L = it - prefix[d];
+        if (D.split) {
+            if (D.js)
+                march_block<STAGE, false, false, PrecSplit>(a, L, wv, wpb, reinterpret_cast<float*>(rbuf[wv]), obuf[wv]);
+            else
+                march_block<STAGE, true, false, PrecSplit>(a, L, wv, wpb, reinterpret_cast<float*>(rbuf[wv]), obuf[wv]);
+        } else {
+            if (D.js)
+                march_block<STAGE, false, false, PrecF64>(a, L, wv, wpb, rbuf[wv], obuf[wv]);
+            else
+                march_block<STAGE, true, false, PrecF64>(a, L, wv, wpb, rbuf[wv], obuf[wv]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * MAXW) void k_limit_batch(const BatchDomain* __restrict__ dom,
+                                                           const long long* __restrict__ prefix, int ndom,
+                                                           long long total, long long per, int parity) {
+    const int wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    const long long b0 = xcd_logical(blockIdx.x, gridDim.x) * per;
+    const long long b1 = b0 + per < total ? b0 + per : total;
+    for (long long it = b0; it < b1; ++it) {
+        const int d = find_domain(prefix, ndom, it);
+        const BatchDomain& D = dom[d];
+        LimitArgs l = D.l;
+        l.C = parity ? D.C_out : D.C;
+        l.out = parity ? const_cast<double*>(D.C) : D.C_out;
+        const long long L = it - prefix[d];
+        if (D.split)
+            limit_block<PrecSplit>(l, L, wv, wpb);
+        else
+            limit_block<PrecF64>(l, L, wv, wpb);
+    }
+}
+
+}  // namespace ocx

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -13,6 +14,7 @@
 #include "ocx_kernels_f64.hpp"
 #include "ocx_march.hpp"
 #include "ocx_limit.hpp"
+#include "ocx_batch.hpp"
 
 namespace {
 
@@ -105,11 +107,23 @@ size_t fct_layout(void* base, int nt, int nz, int ny, int nx, FctWs* w) {
     return cv.used;
 }
 
-constexpr int MARCH_RCH = 64;    // rows per chunk
+// launch shape of the single-domain calls (ocx_set_tuning): rows per chunk, waves per workgroup (0: default)
+std::atomic<int> g_rch{0}, g_wpb{0};
+constexpr int DEFAULT_RCH = 64;
+constexpr int DEFAULT_WPB = 2;    // two tracers of a layer per workgroup (measured best at 1024 x 1024 x 50 x 10)
 
-// waves per workgroup = tracers per group: the tracers split into groups of at most MAXW, as even as possible
-int tracer_group(int nt) {
-    const int groups = (nt + ocx::MAXW - 1) / ocx::MAXW;
+int rows_per_chunk() {
+    const int r = g_rch.load();
+    return r > 0 ? r : DEFAULT_RCH;
+}
+
+// waves per workgroup = tracers per group: the tracers split into groups of at most `cap` (<= MAXW), as even as
+// possible
+int tracer_group(int nt, int cap = 0) {
+    if (cap <= 0) cap = g_wpb.load();
+    if (cap <= 0) cap = DEFAULT_WPB;
+    if (cap > ocx::MAXW) cap = ocx::MAXW;
+    const int groups = (nt + cap - 1) / cap;
     return (nt + groups - 1) / groups;
 }
 
@@ -162,6 +176,11 @@ struct ProfScope {
     ProfRec rec{};
     ProfScope(const char* name, hipStream_t st) : s(st), on(g_prof_on) {
         if (!on) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (st && hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            on = false;                // no event nodes inside a captured graph
+            return;
+        }
         rec.name = name;
         if (hipEventCreate(&rec.a) != hipSuccess || hipEventCreate(&rec.b) != hipSuccess) {
             on = false;
@@ -233,7 +252,8 @@ int advect_fct_impl(const double* C, const double* V0, const double* V1, const d
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool js = flags & WENO_JS, unlim = flags & UNLIMITED;
     const int fi = fx_i ? flux_face_i : -1;
-    if ((long long)((nx + LS - 1) / LS) * ((ny + MARCH_RCH - 1) / MARCH_RCH) * nt * nz >= (1LL << 31))
+    const int rch = rows_per_chunk();
+    if ((long long)((nx + LS - 1) / LS) * ((ny + rch - 1) / rch) * nt * nz >= (1LL << 31))
         return fail(OCX_E_ARG, "shape (nt=%d, nz=%d, ny=%d, nx=%d) too large for one launch", nt, nz, ny, nx);
     MarchArgs a{};
     a.C = C;
@@ -255,9 +275,9 @@ int advect_fct_impl(const double* C, const double* V0, const double* V1, const d
     a.n = nz;
     a.ny = ny;
     a.nx = nx;
-    a.rch = MARCH_RCH;
+    a.rch = rch;
     a.nsx = (nx + MS - 1) / MS;
-    a.ncy = (ny + MARCH_RCH - 1) / MARCH_RCH;
+    a.ncy = (ny + rch - 1) / rch;
     a.dt = dt;
     a.flags = flags;
     // stage 1: F0, c1
@@ -323,9 +343,9 @@ int advect_fct_impl(const double* C, const double* V0, const double* V1, const d
         l.n = nz;
         l.ny = ny;
         l.nx = nx;
-        l.rch = MARCH_RCH;
+        l.rch = rch;
         l.nsx = (nx + LS - 1) / LS;
-        l.ncy = (ny + MARCH_RCH - 1) / MARCH_RCH;
+        l.ncy = (ny + rch - 1) / rch;
         l.dt = dt;
         l.flags = flags;
         const int wpb = tracer_group(nt);
@@ -585,6 +605,221 @@ int ocx_profile_collect(int max_kernels, const char** names, double* ms_total, l
         if (launches) launches[m] = cnt[m];
     }
     return n;
+}
+
+}  // extern "C"
+
+struct ocx_batch {
+    int ndom = 0, wpb = 0, rch = 0, gpc = 0, ncu = 0;
+    ocx::BatchDomain* d_dom = nullptr;
+    long long* d_pm = nullptr;
+    long long* d_pl = nullptr;
+    long long tm = 0, tl = 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+void batch_free(ocx_batch* b) {
+    if (!b) return;
+    for (int q = 0; q < 2; ++q) {
+        if (b->exec[q]) (void)hipGraphExecDestroy(b->exec[q]);
+        if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
+    }
+    if (b->d_dom) (void)hipFree(b->d_dom);
+    if (b->d_pm) (void)hipFree(b->d_pm);
+    if (b->d_pl) (void)hipFree(b->d_pl);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocx_set_tuning(int rows_per_chunk, int waves_per_group) {
+    if (rows_per_chunk < 0 || rows_per_chunk > (1 << 20) || waves_per_group < 0 || waves_per_group > ocx::MAXW)
+        return fail(OCX_E_ARG, "tuning (rows_per_chunk=%d, waves_per_group=%d) out of range (waves <= %d)",
+                    rows_per_chunk, waves_per_group, ocx::MAXW);
+    g_rch.store(rows_per_chunk);
+    g_wpb.store(waves_per_group);
+    return OCX_OK;
+}
+
+int ocx_batch_create(const ocx_domain* domains, int ndomains, int rows_per_chunk, int waves_per_group,
+                     int groups_per_cu, ocx_batch** out) {
+    using namespace ocx;
+    OCX_TRY(need(out, "out"));
+    *out = nullptr;
+    if (ndomains < 1 || !domains) return fail(OCX_E_ARG, "no domains");
+    if (rows_per_chunk < 0 || waves_per_group < 0 || waves_per_group > MAXW || groups_per_cu < 0)
+        return fail(OCX_E_ARG, "bad tuning (rows_per_chunk=%d, waves_per_group=%d, groups_per_cu=%d)",
+                    rows_per_chunk, waves_per_group, groups_per_cu);
+    const int rch = rows_per_chunk > 0 ? rows_per_chunk : DEFAULT_RCH;
+    int maxnt = 1;
+    for (int d = 0; d < ndomains; ++d) maxnt = std::max(maxnt, domains[d].nt);
+    const int wpb = tracer_group(maxnt, waves_per_group);
+    std::vector<BatchDomain> h(ndomains);
+    std::vector<long long> pm(ndomains + 1, 0), pl(ndomains + 1, 0);
+    for (int d = 0; d < ndomains; ++d) {
+        const ocx_domain& D = domains[d];
+        OCX_TRY(check_shape(D.nt, D.nz, D.ny, D.nx));
+        if (!D.C || !D.C_out || !D.V0 || !D.V1 || !D.Qx || !D.Qy || !D.ok || !D.workspace)
+            return fail(OCX_E_ARG, "domain %d: a required pointer is NULL", d);
+        if (D.C == D.C_out) return fail(OCX_E_ARG, "domain %d: C and C_out alias", d);
+        if (D.flags & ~31u) return fail(OCX_E_ARG, "domain %d: flags 0x%x not valid in a batch", d, D.flags);
+        if (D.precision != 0 && D.precision != 1) return fail(OCX_E_ARG, "domain %d: precision %d", d, D.precision);
+        const bool split = D.precision == 1;
+        const size_t need_b = split ? fct_layout<PrecSplit>(nullptr, D.nt, D.nz, D.ny, D.nx, nullptr)
+                                    : fct_layout<PrecF64>(nullptr, D.nt, D.nz, D.ny, D.nx, nullptr);
+        if (D.workspace_bytes < need_b)
+            return fail(OCX_E_WORKSPACE, "domain %d: workspace of %zu bytes needed, %zu given", d, need_b,
+                        D.workspace_bytes);
+        FctWs w;
+        if (split) fct_layout<PrecSplit>(D.workspace, D.nt, D.nz, D.ny, D.nx, &w);
+        else fct_layout<PrecF64>(D.workspace, D.nt, D.nz, D.ny, D.nx, &w);
+        BatchDomain& B = h[d];
+        B = BatchDomain{};
+        MarchArgs& a = B.m;
+        a.ext = D.inflow;
+        a.V0 = D.V0;
+        a.V1 = D.V1;
+        a.Qx = D.Qx;
+        a.Qy = D.Qy;
+        a.ok = D.ok;
+        a.fx = w.fx;
+        a.fy = w.fy;
+        a.s_x = w.sx;
+        a.s_y = w.sy;
+        a.ax = w.fx;
+        a.ay = w.fy;
+        a.fxi = nullptr;
+        a.flux_face_i = -1;
+        a.nt = D.nt;
+        a.n = D.nz;
+        a.ny = D.ny;
+        a.nx = D.nx;
+        a.rch = rch;
+        a.nsx = (D.nx + MS - 1) / MS;
+        a.ncy = (D.ny + rch - 1) / rch;
+        a.dt = D.dt;
+        a.flags = D.flags;
+        LimitArgs& l = B.l;
+        l.cL = w.cl;
+        l.ext = D.inflow;
+        l.V0 = D.V0;
+        l.V1 = D.V1;
+        l.Qx = D.Qx;
+        l.Qy = D.Qy;
+        l.ok = D.ok;
+        l.ax = w.fx;
+        l.ay = w.fy;
+        l.fxi = nullptr;
+        l.flux_face_i = -1;
+        l.gedge = nullptr;
+        l.nt = D.nt;
+        l.n = D.nz;
+        l.ny = D.ny;
+        l.nx = D.nx;
+        l.rch = rch;
+        l.nsx = (D.nx + LS - 1) / LS;
+        l.ncy = (D.ny + rch - 1) / rch;
+        l.dt = D.dt;
+        l.flags = D.flags;
+        B.C = D.C;
+        B.C_out = D.C_out;
+        B.c1 = w.c1;
+        B.c2 = w.c2;
+        B.cl = w.cl;
+        B.split = split;
+        B.js = (D.flags & WENO_JS) != 0;
+        pm[d + 1] = pm[d] + march_blocks(a, wpb);
+        pl[d + 1] = pl[d] + limit_blocks(l, wpb);
+    }
+    ocx_batch* b = new ocx_batch();
+    b->ndom = ndomains;
+    b->wpb = wpb;
+    b->rch = rch;
+    b->gpc = groups_per_cu > 0 ? groups_per_cu : 4;
+    b->tm = pm[ndomains];
+    b->tl = pl[ndomains];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || b->ncu < 1) {
+        batch_free(b);
+        return fail(OCX_E_HIP, "cannot query the compute-unit count");
+    }
+    if (hipMalloc(&b->d_dom, sizeof(BatchDomain) * ndomains) != hipSuccess ||
+        hipMalloc(&b->d_pm, sizeof(long long) * (ndomains + 1)) != hipSuccess ||
+        hipMalloc(&b->d_pl, sizeof(long long) * (ndomains + 1)) != hipSuccess) {
+        batch_free(b);
+        return fail(OCX_E_HIP, "batch table allocation failed");
+    }
+    if (hipMemcpy(b->d_dom, h.data(), sizeof(BatchDomain) * ndomains, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_pm, pm.data(), sizeof(long long) * (ndomains + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_pl, pl.data(), sizeof(long long) * (ndomains + 1), hipMemcpyHostToDevice) != hipSuccess) {
+        batch_free(b);
+        return fail(OCX_E_HIP, "batch table copy failed");
+    }
+    *out = b;
+    return OCX_OK;
+}
+
+int ocx_batch_step(ocx_batch* b, int parity, void* stream) {
+    using namespace ocx;
+    OCX_TRY(need(b, "batch"));
+    if (parity != 0 && parity != 1) return fail(OCX_E_ARG, "parity %d", parity);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long slots = (long long)b->ncu * b->gpc;
+    const long long perm = (b->tm + slots - 1) / slots, perl = (b->tl + slots - 1) / slots;
+    const unsigned gm = (unsigned)((b->tm + perm - 1) / perm), gl = (unsigned)((b->tl + perl - 1) / perl);
+    const unsigned threads = 64u * b->wpb;
+    {
+        ProfScope _ps("batch_stage1", s);
+        k_march_batch<1><<<gm, threads, 0, s>>>(b->d_dom, b->d_pm, b->ndom, b->tm, perm, parity);
+    }
+    OCX_TRY(launch_check("batch stage 1"));
+    {
+        ProfScope _ps("batch_stage2", s);
+        k_march_batch<2><<<gm, threads, 0, s>>>(b->d_dom, b->d_pm, b->ndom, b->tm, perm, parity);
+    }
+    OCX_TRY(launch_check("batch stage 2"));
+    {
+        ProfScope _ps("batch_stage3_split", s);
+        k_march_batch<3><<<gm, threads, 0, s>>>(b->d_dom, b->d_pm, b->ndom, b->tm, perm, parity);
+    }
+    OCX_TRY(launch_check("batch stage 3"));
+    {
+        ProfScope _ps("batch_limit_final", s);
+        k_limit_batch<<<gl, threads, 0, s>>>(b->d_dom, b->d_pl, b->ndom, b->tl, perl, parity);
+    }
+    return launch_check("batch limiter");
+}
+
+int ocx_batch_graph_step(ocx_batch* b, int parity, void* stream) {
+    OCX_TRY(need(b, "batch"));
+    OCX_TRY(need(stream, "stream (graph capture needs a created stream)"));
+    if (parity != 0 && parity != 1) return fail(OCX_E_ARG, "parity %d", parity);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!b->exec[parity]) {
+        OCX_TRY(hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture"));
+        const int rc = ocx_batch_step(b, parity, stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != OCX_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        OCX_TRY(hip_check(e, "end capture"));
+        b->graph[parity] = g;
+        OCX_TRY(hip_check(hipGraphInstantiate(&b->exec[parity], g, nullptr, nullptr, 0), "graph instantiate"));
+    }
+    return hip_check(hipGraphLaunch(b->exec[parity], s), "graph launch");
+}
+
+int ocx_batch_destroy(ocx_batch* b) {
+    batch_free(b);
+    return OCX_OK;
 }
 
 int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* stream) {

@@ -47,7 +47,7 @@ struct RatioIn {
 
 // loads of one row j that the final update needs (fetched one row ahead)
 struct FinalIn {
-    double qx, qys, qyn, v0;
+    double qx, qys, qyn, v0, v1;
 };
 
 struct RowR {
@@ -58,13 +58,17 @@ struct RowR {
 // Work decomposition: a workgroup holds the waves of up to MAXW tracers of one (layer, strip, chunk) (shared
 // per-layer fields from the CU's caches); logical blocks run layer-fastest, so the workgroups of layers k-1, k, k+1
 // of a strip run side by side on one XCD and the k +- 1 reads of C and c_L hit its L2.
+// Logical blocks of one limiter launch over a domain: tracer groups x layers x strips x chunks.
+__host__ __device__ inline long long limit_blocks(const LimitArgs& a, int wpb) {
+    return (long long)((a.nt + wpb - 1) / wpb) * a.n * a.nsx * a.ncy;
+}
+
+// One logical block L of the limiter (wave wv of wpb).
 template <class PR>
-__global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
+__device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int wv, int wpb) {
     using FL = typename PR::fl;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int wpb = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
     const int ntb = (a.nt + wpb - 1) / wpb;
-    long long L = xcd_logical(blockIdx.x, gridDim.x);
     const int tg = (int)(L % ntb);
     L /= ntb;
     const int k = (int)(L % a.n);
@@ -95,47 +99,78 @@ __global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
     const double dt = a.dt;
     const unsigned flags = a.flags;
 
-    // C, c_L and validity (in the domain and ok) of the lane's column in row r
-    auto ldc = [&](int r, double& c, double& l, bool& o) {
+    // Loads are issued unconditionally from addresses that are always inside the arrays and masked one loop step
+    // later, when the row is used, so a wave never waits on a load right after issuing it.
+    struct RawC {
+        double c, l;
+        uint8_t o;
+        bool in;
+    };
+    auto ldc = [&](int r) {
         const bool in = col_in && r >= 0 && r < ny;
-        const size_t x = (size_t)r * nx + i;
-        c = in ? Cp[x] : 0.0;
-        l = in ? cLp[x] : 0.0;
-        o = in && okk[x] != 0;
+        const size_t x = in ? (size_t)r * nx + i : 0;
+        return RawC{Cp[x], cLp[x], okk[x], in};
+    };
+    auto usec = [&](const RawC& q, double& c, double& l, bool& o) {
+        c = q.in ? q.c : 0.0;
+        l = q.in ? q.l : 0.0;
+        o = q.in && q.o != 0;
+    };
+    struct RawR {
+        RatioIn q;
+        uint8_t od, ou;
+        bool row, aw_in, cell;
     };
     auto ld_ratio = [&](int r) {
-        RatioIn q{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false, false};
-        const bool rin_dom = r >= 0 && r < ny;
-        if (!rin_dom) return q;
-        const size_t x = (size_t)r * nx + i;
-        if (i >= 0 && i <= nx) q.aw = (double)axp[(size_t)r * (nx + 1) + i];
-        if (col_in && rlane) {
-            q.an = (double)ayp[x + nx];
-            q.as = (double)ayp[x];
-            q.v1 = V1k[x];
-            if (has_dn) {
-                q.od = okk[x - plane] != 0;
-                q.cd = Cp[x - plane];
-                q.ld = cLp[x - plane];
-            }
-            if (has_up) {
-                q.ou = okk[x + plane] != 0;
-                q.cu = Cp[x + plane];
-                q.lu = cLp[x + plane];
-            }
-        }
+        RawR w;
+        w.row = r >= 0 && r < ny;
+        w.aw_in = w.row && i >= 0 && i <= nx;
+        w.cell = w.row && col_in && rlane;
+        const size_t x = w.cell ? (size_t)r * nx + i : 0;
+        w.q.aw = (double)axp[w.aw_in ? (size_t)r * (nx + 1) + i : 0];
+        w.q.an = (double)ayp[w.cell ? x + nx : 0];
+        w.q.as = (double)ayp[x];
+        w.q.v1 = V1k[x];
+        const size_t xd = (w.cell && has_dn) ? x - plane : 0, xu = (w.cell && has_up) ? x + plane : 0;
+        w.od = okk[xd];
+        w.q.cd = Cp[xd];
+        w.q.ld = cLp[xd];
+        w.ou = okk[xu];
+        w.q.cu = Cp[xu];
+        w.q.lu = cLp[xu];
+        return w;
+    };
+    auto use_ratio = [&](const RawR& w) {
+        RatioIn q = w.q;
+        if (!w.aw_in) q.aw = 0.0;
+        if (!w.cell) q.an = q.as = q.v1 = 0.0;
+        q.od = w.cell && has_dn && w.od != 0;
+        q.ou = w.cell && has_up && w.ou != 0;
+        if (!q.od) q.cd = q.ld = 0.0;
+        if (!q.ou) q.cu = q.lu = 0.0;
         return q;
     };
+    struct RawF {
+        FinalIn f;
+        bool xf, ow_;
+    };
     auto ld_final = [&](int j) {
-        FinalIn f{0.0, 0.0, 0.0, 0.0};
-        if (j >= je) return f;
-        const size_t x = (size_t)j * nx + i;
-        if (i >= 0 && i <= nx) f.qx = Qxk[(size_t)j * (nx + 1) + i];
-        if (own) {
-            f.qys = Qyk[x];
-            f.qyn = Qyk[x + nx];
-            f.v0 = V0k[x];
-        }
+        RawF w;
+        const bool row = j < je;
+        w.xf = row && i >= 0 && i <= nx;
+        w.ow_ = row && own;
+        const size_t x = w.ow_ ? (size_t)j * nx + i : 0;
+        w.f.qx = Qxk[w.xf ? (size_t)j * (nx + 1) + i : 0];
+        w.f.qys = Qyk[x];
+        w.f.qyn = Qyk[w.ow_ ? x + nx : 0];
+        w.f.v0 = V0k[x];
+        w.f.v1 = V1k[x];
+        return w;
+    };
+    auto use_final = [&](const RawF& w) {
+        FinalIn f = w.f;
+        if (!w.xf) f.qx = 0.0;
+        if (!w.ow_) f.qys = f.qyn = f.v0 = f.v1 = 0.0;
         return f;
     };
     // R+- of row r from the lane's column values of rows r-1, r, r+1 and the row's prefetched loads
@@ -174,23 +209,21 @@ __global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
     double cr[5], lr[5];
     bool orr[5];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) ldc(jb - 1 + m, cr[m], lr[m], orr[m]);
+    for (int m = 0; m < 5; ++m) usec(ldc(jb - 1 + m), cr[m], lr[m], orr[m]);
     double cm2, lm2;
     bool om2;
-    ldc(jb - 2, cm2, lm2, om2);
-    RowR Rm = ratios(jb - 1, ld_ratio(jb - 1), cm2, lm2, om2, cr[0], lr[0], orr[0], cr[1], lr[1], orr[1]);
-    RowR Rc = ratios(jb, ld_ratio(jb), cr[0], lr[0], orr[0], cr[1], lr[1], orr[1], cr[2], lr[2], orr[2]);
-    RatioIn qn1 = ld_ratio(jb + 1);                 // inputs of R(j+1)
-    FinalIn fc = ld_final(jb);                      // inputs of row j's update
+    usec(ldc(jb - 2), cm2, lm2, om2);
+    RowR Rm = ratios(jb - 1, use_ratio(ld_ratio(jb - 1)), cm2, lm2, om2, cr[0], lr[0], orr[0], cr[1], lr[1], orr[1]);
+    RowR Rc = ratios(jb, use_ratio(ld_ratio(jb)), cr[0], lr[0], orr[0], cr[1], lr[1], orr[1], cr[2], lr[2], orr[2]);
+    RatioIn qn1 = use_ratio(ld_ratio(jb + 1));      // inputs of R(j+1)
+    FinalIn fc = use_final(ld_final(jb));           // inputs of row j's update
     const bool ow = flags & OPEN_W, oe = flags & OPEN_E, osb = flags & OPEN_S, onb = flags & OPEN_N;
 
     for (int j = jb; j < je; ++j) {
         // one row ahead: the ring row j+4, the ratio inputs of row j+2, the update inputs of row j+1
-        double c4, l4;
-        bool o4;
-        ldc(j + 4, c4, l4, o4);
-        const RatioIn qn2 = ld_ratio(j + 2);
-        const FinalIn fn = ld_final(j + 1);
+        const RawC c4 = ldc(j + 4);
+        const RawR qn2 = ld_ratio(j + 2);
+        const RawF fn = ld_final(j + 1);
         // ratios of row j+1 (rows j .. j+2 = cr[1..3])
         const RowR Rn = ratios(j + 1, qn1, cr[1], lr[1], orr[1], cr[2], lr[2], orr[2], cr[3], lr[3], orr[3]);
         // finish row j
@@ -208,15 +241,15 @@ __global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
             const double cfs = j > 0 ? coeff(as_, Rc.rin, Rm.rout, Rm.rin, Rc.rout) : 0.0;
             const double cfn = j + 1 < ny ? coeff(an, Rn.rin, Rc.rout, Rc.rin, Rn.rout) : 0.0;
             const double* erow = ext0 + (size_t)j * nx;
-            const double Gw = donor(fc.qx, i, nx, cw, c0, ow, oe, erow[0], erow[nx - 1]) + cfw * aw;
-            const double Ge = donor(qxe, i + 1, nx, c0, ce, ow, oe, erow[0], erow[nx - 1]) + cfe * ae;
-            const double Gs = donor(fc.qys, j, ny, cr[0], c0, osb, onb, ext0[i], ext0[(size_t)(ny - 1) * nx + i]) +
-                              cfs * as_;
-            const double Gn = donor(fc.qyn, j + 1, ny, c0, cr[2], osb, onb, ext0[i],
-                                    ext0[(size_t)(ny - 1) * nx + i]) + cfn * an;
+            const double* ecol_s = ext0 + i;
+            const double* ecol_n = ext0 + (size_t)(ny - 1) * nx + i;
+            const double Gw = donor(fc.qx, i, nx, cw, c0, ow, oe, erow, erow + nx - 1) + cfw * aw;
+            const double Ge = donor(qxe, i + 1, nx, c0, ce, ow, oe, erow, erow + nx - 1) + cfe * ae;
+            const double Gs = donor(fc.qys, j, ny, cr[0], c0, osb, onb, ecol_s, ecol_n) + cfs * as_;
+            const double Gn = donor(fc.qyn, j + 1, ny, c0, cr[2], osb, onb, ecol_s, ecol_n) + cfn * an;
             const double d = Ge - Gw + Gn - Gs;
             const double q = fc.v0 * c0 - dt * d;
-            const double v1 = V1k[xc];
+            const double v1 = fc.v1;
             a.out[(size_t)p * plane + xc] = v1 > 0 ? q / v1 : c0;
             if (a.fxi) {
                 if (a.flux_face_i == i) a.fxi[(size_t)p * ny + j] = Gw;
@@ -233,18 +266,21 @@ __global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
         // advance
         Rm = Rc;
         Rc = Rn;
-        qn1 = qn2;
-        fc = fn;
+        qn1 = use_ratio(qn2);
+        fc = use_final(fn);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             cr[m] = cr[m + 1];
             lr[m] = lr[m + 1];
             orr[m] = orr[m + 1];
         }
-        cr[4] = c4;
-        lr[4] = l4;
-        orr[4] = o4;
+        usec(c4, cr[4], lr[4], orr[4]);
     }
+}
+
+template <class PR>
+__global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
+    limit_block<PR>(a, xcd_logical(blockIdx.x, gridDim.x), threadIdx.x >> 6, blockDim.x >> 6);
 }
 
 }  // namespace ocx

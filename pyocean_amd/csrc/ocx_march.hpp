@@ -147,34 +147,38 @@ __device__ __forceinline__ W line_face(double qd, int f, int nf, const W* u, con
     return open_hi ? q * (qd < 0.0 ? ext_hi : u[2]) : (W)0;
 }
 
-// donor-cell flux of a face between cells lo and hi (interior), or of an edge face (rules of _fct_fluxes)
+// donor-cell flux of a face between cells lo and hi (interior), or of an edge face (rules of _fct_fluxes); the
+// exterior values are read (ext_lo[0] / ext_hi[0]) only on an open edge face
 __device__ __forceinline__ double donor(double q, int f, int nf, double lo, double hi, bool open_lo, bool open_hi,
-                                        double ext_lo, double ext_hi) {
+                                        const double* ext_lo, const double* ext_hi) {
     if (f > 0 && f < nf) return q * (q >= 0 ? lo : hi);
-    if (f == 0) return open_lo ? q * (q > 0 ? ext_lo : hi) : 0.0;
-    return open_hi ? q * (q < 0 ? ext_hi : lo) : 0.0;
+    if (f == 0) return open_lo ? q * (q > 0 ? ext_lo[0] : hi) : 0.0;
+    return open_hi ? q * (q < 0 ? ext_hi[0] : lo) : 0.0;
 }
 
 // Work decomposition: a workgroup holds the waves of up to MAXW tracers of one (layer, strip, chunk), so the
 // per-layer fields (Qx, Qy, V0, V1, ok) they all read come from the compute unit's caches after the first wave;
 // logical blocks run strip-fastest, each XCD through a contiguous range.
 #ifndef OCX_MAXW
-#define OCX_MAXW 5
+#define OCX_MAXW 4
 #endif
 constexpr int MAXW = OCX_MAXW;
 
+// Logical blocks of one marching launch over a domain: tracer groups x strips x chunks x layers.
+__host__ __device__ inline long long march_blocks(const MarchArgs& a, int wpb) {
+    return (long long)((a.nt + wpb - 1) / wpb) * a.nsx * a.ncy * a.n;
+}
+
+// One logical block L of a marching stage (wave wv of wpb).  rbuf/obuf: this wave's LDS row.
 template <int STAGE, bool Z, bool UNLIM, class PR>
-__global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
+__device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int wv, int wpb, typename PR::w* rbw,
+                                            uint8_t* obw) {
     using W = typename PR::w;
     using ST = typename PR::st;
     using FL = typename PR::fl;
     using IN = typename std::conditional<STAGE == 1, double, ST>::type;   // type of the stage's input values
-    __shared__ W rbuf[MAXW][MRB];
-    __shared__ uint8_t obuf[MAXW][MRB];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int wpb = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
     const int ntb = (a.nt + wpb - 1) / wpb;
-    long long L = xcd_logical(blockIdx.x, gridDim.x);
     const int tg = (int)(L % ntb);
     L /= ntb;
     const int sx = (int)(L % a.nsx);
@@ -213,56 +217,84 @@ __global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
     const int hidx = lane < 3 ? lane : 64 + lane;
     const bool hin = lane < 5 && hcol >= 0 && hcol < nx;
 
-    auto ld = [&](int j, W& v, bool& o) {
-        const bool in = col_in && j >= 0 && j < ny;
-        v = in ? (W)cin[(size_t)j * nx + i] : (W)0;
-        o = in && okp[(size_t)j * nx + i] != 0;
+    // Loads are split into an issue half (unconditional, from an address that is always inside the array) and a
+    // use half (mask and convert); the use half of a prefetched row runs one loop step later, so the wave never
+    // waits on a load right after issuing it.
+    struct Raw {
+        IN v;
+        uint8_t o;
+        bool in;
     };
-    auto ldh = [&](int j, W& v, bool& o) {
+    auto ld = [&](int j) {
+        const bool in = col_in && j >= 0 && j < ny;
+        const size_t x = in ? (size_t)j * nx + i : 0;
+        return Raw{cin[x], okp[x], in};
+    };
+    auto ldh = [&](int j) {
         const bool in = hin && j >= 0 && j < ny;
-        v = in ? (W)cin[(size_t)j * nx + hcol] : (W)0;
-        o = in && okp[(size_t)j * nx + hcol] != 0;
+        const size_t x = in ? (size_t)j * nx + hcol : 0;
+        return Raw{cin[x], okp[x], in};
+    };
+    auto use = [&](const Raw& r, W& v, bool& o) {
+        v = r.in ? (W)r.v : (W)0;
+        o = r.in && r.o != 0;
     };
     // exterior value of an edge face: the inflow, or (nullptr) the stage's own edge cell
     auto ex = [&](size_t x, W own_val) -> W { return extp ? (W)extp[x] : own_val; };
 
-    // inputs of one row j that the loop consumes, fetched one row ahead (the marching loop is latency-bound
-    // otherwise: a wave would wait for these loads on every row)
+    // inputs of one row j that the loop consumes, fetched one row ahead (raw: masked when used)
     struct RowIn {
         double qx, qy;           // Qx[j, i], Qy[j+1, i]
         double c, v0, v1;        // C[j, i], V0, V1 (own cell)
-        double fw, fn;           // stage 2: F0, stage 3: S on x-face (j, i) and y-face (j+1, i)
+        FL fw, fn;               // stage 2: F0, stage 3: S on x-face (j, i) and y-face (j+1, i)
         double cl;               // stage 3: C[j, i0-1] (lane 0)
+        bool xf, ci;             // the x-face exists, the cell exists (and j is a row of the chunk)
     };
     auto ld_row = [&](int j) {
-        RowIn r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (j >= je) return r;
-        const size_t xc = (size_t)j * nx + i;
-        if (xface) r.qx = Qxk[(size_t)j * (nx + 1) + i];
-        if (col_in) {
-            r.qy = Qyk[(size_t)(j + 1) * nx + i];
-            r.c = Cp[xc];
-            r.v0 = V0k[xc];
-            r.v1 = V1k[xc];
-        }
+        RowIn r;
+        const bool row = j < je;
+        r.xf = row && xface;
+        r.ci = row && col_in;
+        const size_t xc = r.ci ? (size_t)j * nx + i : 0;
+        const size_t xf = r.xf ? (size_t)j * (nx + 1) + i : 0;
+        const size_t yf = r.ci ? (size_t)(j + 1) * nx + i : 0;
+        r.qx = Qxk[xf];
+        r.qy = Qyk[yf];
+        r.c = Cp[xc];
+        r.v0 = V0k[xc];
+        r.v1 = V1k[xc];
         if (STAGE >= 2) {
             const FL* gx = STAGE == 2 ? fxp : sxp;
             const FL* gy = STAGE == 2 ? fyp : syp;
-            if (xface) r.fw = (double)gx[(size_t)j * (nx + 1) + i];
-            if (col_in) r.fn = (double)gy[(size_t)(j + 1) * nx + i];
+            r.fw = gx[xf];
+            r.fn = gy[yf];
+        } else {
+            r.fw = r.fn = (FL)0;
         }
-        if (STAGE == 3 && lane == 0 && i0 > 0 && col_in) r.cl = Cp[xc - 1];
+        r.cl = Cp[(STAGE == 3 && lane == 0 && i0 > 0 && r.ci) ? xc - 1 : 0];
+        return r;
+    };
+    auto mask_row = [&](RowIn r) {
+        if (!(STAGE == 3 && lane == 0 && i0 > 0 && r.ci)) r.cl = 0.0;
+        if (!r.xf) {
+            r.qx = 0.0;
+            r.fw = (FL)0;
+        }
+        if (!r.ci) {
+            r.qy = r.c = r.v0 = r.v1 = 0.0;
+            r.fn = (FL)0;
+        }
         return r;
     };
 
     W w[7];
     bool kw[7];
 #pragma unroll
-    for (int m = 0; m < 7; ++m) ld(jb - 3 + m, w[m], kw[m]);
+    for (int m = 0; m < 7; ++m) use(ld(jb - 3 + m), w[m], kw[m]);
     W hv;
     bool hk;
-    ldh(jb, hv, hk);
-    RowIn cur = ld_row(jb);
+    use(ldh(jb), hv, hk);
+    RowIn cur = mask_row(ld_row(jb));
 
     // y-face jb (the chunk's first south face)
     double fys;
@@ -285,24 +317,21 @@ __global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
                             : f0s / 6.0 + 2.0 * fys / 3.0;
             const double q = Qyk[(size_t)jb * nx + i];
             const double clo = jb > 0 ? Cp[(size_t)(jb - 1) * nx + i] : 0.0;
-            fls = donor(q, jb, ny, clo, cur.c, os, on, ext0[i], ext0[(size_t)(ny - 1) * nx + i]);
+            fls = donor(q, jb, ny, clo, cur.c, os, on, ext0 + i, ext0 + (size_t)(ny - 1) * nx + i);
             if (!UNLIM) ayp[(size_t)jb * nx + i] = (FL)(fhs - fls);
         }
     }
 
     for (int j = jb; j < je; ++j) {
         // prefetch the next row's inputs, the next row of the window and the next halo
-        const RowIn nxt = ld_row(j + 1);
-        W wn, hn;
-        bool kn, hkn;
-        ld(j + 4, wn, kn);
-        ldh(j + 1, hn, hkn);
+        const RowIn nxt_raw = ld_row(j + 1);
+        const Raw wn_raw = ld(j + 4), hn_raw = ldh(j + 1);
         // x-face i of row j
-        rbuf[wv][lane + 3] = w[3];
-        obuf[wv][lane + 3] = kw[3];
+        rbw[lane + 3] = w[3];
+        obw[lane + 3] = kw[3];
         if (lane < 5) {
-            rbuf[wv][hidx] = hv;
-            obuf[wv][hidx] = hk;
+            rbw[hidx] = hv;
+            obw[hidx] = hk;
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -313,8 +342,8 @@ __global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
             bool o[6];
 #pragma unroll
             for (int m = 0; m < 6; ++m) {
-                u[m] = rbuf[wv][lane + m];
-                o[m] = obuf[wv][lane + m] != 0;
+                u[m] = rbw[lane + m];
+                o[m] = obw[lane + m] != 0;
             }
             fxw = (double)line_face<Z>(cur.qx, i, nx, u, o, ow, oe, ex((size_t)j * nx, u[3]),
                                        ex((size_t)j * nx + nx - 1, u[2]));
@@ -340,7 +369,7 @@ __global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
                 static_cast<ST*>(a.cout)[(size_t)p * plane + xc] = (ST)(V > 0 ? qn / V : c0);
             }
         } else if (STAGE == 2) {
-            const double f0w = cur.fw, f0n = cur.fn;
+            const double f0w = (double)cur.fw, f0n = (double)cur.fn;
             const double f0e = shfl_down1(f0w);
             if (xown) sxp[(size_t)j * (nx + 1) + i] = (FL)(f0w + fxw);
             if (col_in && yown) syp[(size_t)(j + 1) * nx + i] = (FL)(f0n + fyn);
@@ -362,21 +391,22 @@ __global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
                 fhw = (double)((float)cur.fw * (1.0f / 6.0f) + 2.0f * (float)fxw * (1.0f / 3.0f));
                 fhn = (double)((float)cur.fn * (1.0f / 6.0f) + 2.0f * (float)fyn * (1.0f / 3.0f));
             } else {
-                fhw = cur.fw / 6.0 + 2.0 * fxw / 3.0;
-                fhn = cur.fn / 6.0 + 2.0 * fyn / 3.0;
+                fhw = (double)cur.fw / 6.0 + 2.0 * fxw / 3.0;
+                fhn = (double)cur.fn / 6.0 + 2.0 * fyn / 3.0;
             }
             const double fhe = shfl_down1(fhw);
             // donor fluxes: x-face i between C[j, i-1] and C[j, i]; y-face j+1 between C[j, i] and C[j+1, i]
             const double cC = cur.c;
             double cCl = shfl_up1(cC);
             if (lane == 0) cCl = cur.cl;
-            const double cCn = (j + 1 < je) ? nxt.c : ((col_in && j + 1 < ny) ? Cp[xc + nx] : 0.0);
+            const double cCn = (j + 1 < je) ? (nxt_raw.ci ? nxt_raw.c : 0.0)
+                                             : ((col_in && j + 1 < ny) ? Cp[xc + nx] : 0.0);
             double flw = 0.0, fln = 0.0;
             if (xface) {
                 const double chi = col_in ? cC : 0.0;
-                flw = donor(cur.qx, i, nx, cCl, chi, ow, oe, ext0[(size_t)j * nx], ext0[(size_t)j * nx + nx - 1]);
+                flw = donor(cur.qx, i, nx, cCl, chi, ow, oe, ext0 + (size_t)j * nx, ext0 + (size_t)j * nx + nx - 1);
             }
-            if (col_in) fln = donor(qyn, j + 1, ny, cC, cCn, os, on, ext0[i], ext0[(size_t)(ny - 1) * nx + i]);
+            if (col_in) fln = donor(qyn, j + 1, ny, cC, cCn, os, on, ext0 + i, ext0 + (size_t)(ny - 1) * nx + i);
             const double fle = shfl_down1(flw);
             if (UNLIM) {
                 if (a.fxi && xown && i == a.flux_face_i) a.fxi[(size_t)p * ny + j] = fhw;
@@ -407,12 +437,18 @@ __global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
             w[m] = w[m + 1];
             kw[m] = kw[m + 1];
         }
-        w[6] = wn;
-        kw[6] = kn;
-        hv = hn;
-        hk = hkn;
-        cur = nxt;
+        use(wn_raw, w[6], kw[6]);
+        use(hn_raw, hv, hk);
+        cur = mask_row(nxt_raw);
     }
+}
+
+template <int STAGE, bool Z, bool UNLIM, class PR>
+__global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
+    __shared__ typename PR::w rbuf[MAXW][MRB];
+    __shared__ uint8_t obuf[MAXW][MRB];
+    const int wv = threadIdx.x >> 6;
+    march_block<STAGE, Z, UNLIM, PR>(a, xcd_logical(blockIdx.x, gridDim.x), wv, blockDim.x >> 6, rbuf[wv], obuf[wv]);
 }
 
 }  // namespace ocx
