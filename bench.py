@@ -59,16 +59,18 @@ def read_traffic(kernel: str, workload: str):
     path = ROOT / "profiles" / f"traffic_{workload}.txt"
     if not path.exists():
         return None
-    scale, fetch, write = 1.0, None, None
+    sf, sw, fetch, write = 1.0, 1.0, None, None
     for line in path.read_text().splitlines():
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == "scale":
-            scale = float(parts[1])
+        if len(parts) >= 2 and parts[0] == "scale_fetch":
+            sf = float(parts[1])
+        if len(parts) >= 2 and parts[0] == "scale_write":
+            sw = float(parts[1])
         if len(parts) >= 3 and parts[0] == kernel:
             fetch, write = float(parts[1]), float(parts[2])
     if fetch is None:
         return None
-    return (fetch + write) * 1024.0 * scale
+    return (fetch * sf + write * sw) * 1024.0
 
 
 def cpu_baseline(case, nt_sample: int, reps: int):
