@@ -93,6 +93,131 @@ def cpu_baseline(case, nt_sample: int, reps: int):
                       f"OpenMP over (tracer, layer) planes"}
 
 
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_batch(args) -> int:
+    """64 regional domains (nx, ny log-uniform in [96, 768], nz in {20, 30, 50, 75}, ten tracers, seeds 100 + d),
+    FP64 and split members, advanced by the batched step as a captured hipGraph."""
+    import torch
+
+    from pyocean_amd import _native, autotune, batch, ops, serve, synth
+
+    shapes = synth.batch_shapes(args.domains, seed=7)
+    _log(f"generating {len(shapes)} domains")
+    grids = serve.make_grids(shapes, nt=args.nt, seed=100, workers=args.workers, log=_log)
+    prec = [("f64" if d % 2 == 0 else "split") if args.mix == "mixed" else args.mix for d in range(len(grids))]
+    doms = [batch.Domain(C=g.C0, V0=g.V0, V1=g.V1, Qx=g.Qx, Qy=g.Qy, ok=g.ok, dt=g.dt, precision=p)
+            for g, p in zip(grids, prec)]
+    updates = sum(d.cells for d in doms)
+    shape = autotune.Shape(args.rch, args.wpb, 0)
+    tuned = None
+    if args.autotune:
+        _log("autotuning the launch shape")
+        shape, table, _ = autotune.tune_batch(doms, reps=1, graph=not args.eager, use_cache=False, log=_log)
+        tuned = {"best": shape.tile, "groups_per_cu": shape.groups_per_cu,
+                 "table_ms": {f"{s.tile}/g{s.groups_per_cu}": round(ms, 3) for s, ms in table}}
+    b = batch.Batch(doms, rows_per_chunk=shape.rows_per_chunk, waves_per_group=shape.waves_per_group,
+                    groups_per_cu=shape.groups_per_cu)
+    stream = torch.cuda.Stream()                      # graph capture needs a created stream
+    torch.cuda.set_stream(stream)
+    graph = not args.eager
+    _log(f"{updates / 1e9:.2f} G tracer-cells per step; warmup {args.warmup}")
+    for _ in range(max(args.warmup, 1)):
+        b.step(graph=graph, stream=stream)
+    # per-kernel durations from two eager steps
+    torch.cuda.synchronize()
+    _native.profile_enable(True)
+    _native.profile_collect()
+    for _ in range(2):
+        b.step(stream=stream)
+    torch.cuda.synchronize()
+    _native.profile_enable(False)
+    prof = _native.profile_collect()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        b.step(graph=graph, stream=stream)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = np.array([a.elapsed_time(c) for a, c in ev])
+    value = updates * args.steps / wall
+    names = {"batch_stage1": "fct_stage1", "batch_stage2": "fct_stage2", "batch_stage3_split": "fct_stage3_split",
+             "batch_limit_final": "fct_limit_final"}
+    kernels = {k: v[0] / v[1] for k, v in prof.items()}
+    dom = max(kernels, key=kernels.get)
+
+    def kbytes(kname):
+        tot = 0.0
+        for d in doms:
+            per_tc, shared = KERNEL_BYTES[d.precision][names.get(kname, "fct_stage1")]
+            tot += d.cells * (per_tc + shared / d.C.shape[0])
+        return tot
+
+    alg = kbytes(dom)
+    achieved = alg / (kernels[dom] * 1e-3)
+    B = sum(d.cells * step_bytes(d.C.shape[0], d.precision) for d in doms) / updates
+    b.close()
+    res = {
+        "metric": "tracer-cell updates/s (batched: 64 regional domains of uneven size, one persistent launch)",
+        "value": value, "unit": "tracer-cell updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "p50_ms": float(np.percentile(step_ms, 50)),
+        "p99_ms": float(np.percentile(step_ms, 99)), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": {"mixed": "f64 + f32/f64", "split": "f32/f64", "f64": "f64"}[args.mix],
+        "data": "synthetic (seeded regional domains, seeds 100 + d)",
+        "config": {"workload": f"{len(doms)} regional domains, nx, ny log-uniform in [96, 768], nz in "
+                               f"{{20, 30, 50, 75}}, {args.nt} tracers, members {args.mix}, "
+                               + ("eager launches" if args.eager else "hipGraph step"),
+                   "tracer_cells": updates, "rows_per_chunk": shape.rows_per_chunk,
+                   "waves_per_group": shape.waves_per_group, "groups_per_cu": shape.groups_per_cu},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None, "alg_bytes_per_launch": alg,
+                     "avg_launch_ms": kernels[dom]},
+        "step_roofline": {"bytes_per_update": B, "achieved": value * B / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "frac": value * B / HBM_PEAK},
+        "kernels_ms": kernels,
+        "autotune": tuned,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_serve(args) -> int:
+    """The serving stream: Poisson arrivals at ``--rate`` jobs/s, grids as in the batch form plus a 1536 x 1536 x 50
+    domain drawn with probability 0.02, steps log-uniform in [10, 500], split precision, HBM budget 200 GB."""
+    from pyocean_amd import serve, synth
+
+    shapes = synth.batch_shapes(args.domains, seed=7) + [(1536, 1536, 50)]
+    _log(f"generating {len(shapes)} grids")
+    grids = serve.make_grids(shapes, nt=args.nt, seed=100, workers=args.workers, log=_log)
+    jobs = serve.poisson_jobs(args.jobs, args.rate, len(grids), steps_lo=10, steps_hi=500, big_grid=len(grids) - 1,
+                              big_prob=0.02, seed=7)
+    srv = serve.Server(grids, hbm_budget=args.budget_gb * 1e9, max_active=64, precision="split",
+                       rows_per_chunk=args.rch, waves_per_group=args.wpb)
+    _log(f"serving {len(jobs)} jobs at {args.rate} jobs/s")
+    st = srv.run(jobs, log=_log)
+    res = {
+        "metric": "serving: tracer-cell updates/s and per-job latency p50/p99 (Poisson arrivals, continuous batching)",
+        "value": st["updates_per_s"], "unit": "tracer-cell updates/s", "n_gpus": 1, "steps": st["batched_steps"],
+        "warmup": 0, "ms_per_step": st["wall_s"] / max(st["batched_steps"], 1) * 1e3,
+        "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
+        "data": "synthetic (seeded regional domains, seeded Poisson stream)",
+        "config": {"workload": f"{len(jobs)} jobs, Poisson {args.rate}/s, {len(grids) - 1} regional grids + "
+                               "1536x1536x50 (p = 0.02), steps log-uniform in [10, 500], split precision, "
+                               f"HBM budget {args.budget_gb:.0f} GB, at most 64 jobs in a batch"},
+        "serve": st,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,18 +235,35 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--form", choices=("step", "batch", "serve"), default="step",
+                    help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "
+                         "persistent launch; serve: the Poisson job stream through the continuous-batching loop")
+    ap.add_argument("--mix", choices=("mixed", "split", "f64"), default="mixed",
+                    help="batch form: member precisions (mixed: even members FP64, odd split)")
+    ap.add_argument("--domains", type=int, default=64, help="batch/serve forms: number of regional domains")
+    ap.add_argument("--autotune", action="store_true", help="batch form: sweep the launch shapes first")
+    ap.add_argument("--eager", action="store_true", help="batch form: eager launches instead of the captured graph")
+    ap.add_argument("--jobs", type=int, default=500, help="serve form: number of jobs")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve form: mean arrivals per second")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve form: HBM budget")
+    ap.add_argument("--workers", type=int, default=16, help="batch/serve forms: host processes generating grids")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy the tracer stack once (device to device) before the steps: a kernel of known bytes "
                          "for scaling the HBM counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU layout (tracer or domain sharding, halo "
-              "exchange) is described in DESIGN.md but not launched", file=sys.stderr)
+        print("bench.py: one GPU only; sharding the tracers, the domains or the (y, x) plane over several GPUs is "
+              "not launched here, so --gpus must be 1", file=sys.stderr)
         return 2
 
     import torch
 
     from pyocean_amd import _native, ops, synth
+
+    if args.form == "batch":
+        return bench_batch(args)
+    if args.form == "serve":
+        return bench_serve(args)
 
     if args.rch or args.wpb:
         from pyocean_amd import batch as _batch

@@ -30,6 +30,8 @@ class Domain:
     flags: int = 0
     precision: str = "f64"
     inflow: torch.Tensor | None = None
+    C_alt: torch.Tensor | None = None        # the second state buffer (allocated by the batch when None)
+    workspace: torch.Tensor | None = None    # uint8 device workspace (allocated by the batch when None)
 
     @property
     def cells(self) -> int:
@@ -69,8 +71,11 @@ class Batch:
                 ops._need(dom.inflow, dom.C.shape, "inflow")
             if dom.precision not in ops.PRECISIONS:
                 raise ValueError(f"domain {d}: precision {dom.precision!r}")
-            other = torch.empty_like(dom.C)
-            ws = torch.empty(max(dom.workspace_bytes(), 256), dtype=torch.uint8, device=dom.C.device)
+            other = dom.C_alt if dom.C_alt is not None else torch.empty_like(dom.C)
+            ops._need(other, dom.C.shape, "C_alt")
+            ws = dom.workspace
+            if ws is None or ws.numel() < dom.workspace_bytes():
+                ws = torch.empty(max(dom.workspace_bytes(), 256), dtype=torch.uint8, device=dom.C.device)
             self._bufs.append((other, ws, ok))
             p = lambda t: None if t is None else t.data_ptr()                # noqa: E731
             descs[d] = OcxDomain(p(dom.C), p(other), p(dom.V0), p(dom.V1), p(dom.Qx), p(dom.Qy), p(dom.inflow), p(ok),
@@ -86,6 +91,14 @@ class Batch:
     def state(self, d: int) -> torch.Tensor:
         """The current state of domain d."""
         return self.domains[d].C if self.parity == 0 else self._bufs[d][0]
+
+    def spare(self, d: int) -> torch.Tensor:
+        """The other state buffer of domain d (the next step's output)."""
+        return self._bufs[d][0] if self.parity == 0 else self.domains[d].C
+
+    def workspace(self, d: int) -> torch.Tensor:
+        """The workspace of domain d."""
+        return self._bufs[d][1]
 
     def step(self, graph: bool = False, stream: torch.cuda.Stream | None = None) -> None:
         """One FCT step of every domain (eagerly, or as the captured graph of this parity)."""

@@ -88,3 +88,19 @@ def test_batch_shapes_follow_the_batched_case():
     assert len(shapes) == 64
     for nx, ny, nz in shapes:
         assert 96 <= nx <= 768 and 96 <= ny <= 768 and nz in (20, 30, 50, 75)
+
+
+def test_poisson_job_stream_follows_the_serving_case():
+    from pyocean_amd import serve
+    jobs = serve.poisson_jobs(4000, rate=20.0, n_grids=65, big_grid=64, big_prob=0.02, seed=7)
+    t = np.array([j.arrival for j in jobs])
+    steps = np.array([j.steps for j in jobs])
+    grids = np.array([j.grid for j in jobs])
+    assert np.all(np.diff(t) >= 0) and abs(len(jobs) / t[-1] - 20.0) < 1.0          # mean rate
+    gaps = np.diff(t)
+    assert abs(gaps.std() / gaps.mean() - 1.0) < 0.1                                 # exponential gaps: CV = 1
+    assert steps.min() >= 10 and steps.max() <= 500
+    assert abs(np.median(np.log(steps)) - 0.5 * (np.log(10) + np.log(500))) < 0.15   # log-uniform
+    assert 0.01 < np.mean(grids == 64) < 0.03 and set(grids[grids != 64]) == set(range(64))
+    again = serve.poisson_jobs(4000, rate=20.0, n_grids=65, big_grid=64, big_prob=0.02, seed=7)
+    assert [(j.arrival, j.steps, j.grid) for j in again] == [(j.arrival, j.steps, j.grid) for j in jobs]
