@@ -218,6 +218,90 @@ def bench_serve(args) -> int:
     return 0
 
 
+def bench_precision(args) -> int:
+    """The split form against the plain FP64 form (bit-identical to the oracle) on the same inputs, stepped side by
+    side: per-tracer max and RMS error relative to the tracer's initial range, content drift and global bounds."""
+    import torch
+
+    from pyocean_amd import ops, synth
+
+    marks = sorted(int(v) for v in args.checkpoints.split(","))
+    _log(f"generating regional {args.nx}x{args.ny}x{args.nz} nt={args.nt}")
+    case = synth.regional(nx=args.nx, ny=args.ny, nz=args.nz, nt=args.nt, seed=0)
+    dev = torch.device("cuda", 0)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)     # noqa: E731
+    V0, V1, Qx, Qy, ok = d(case.V0), d(case.V1), d(case.Qx), d(case.Qy), d(case.ok.astype(np.uint8))
+    wet = ok.bool()
+    C0 = d(case.C)
+    ref, ref_o = C0.clone(), torch.empty_like(C0)
+    spl, spl_o = C0.clone(), torch.empty_like(C0)
+    ws_r, ws_s = ops.Workspace(), ops.Workspace()
+    nt = C0.shape[0]
+    w = wet.unsqueeze(0).expand_as(C0)
+    lo = torch.where(w, C0, torch.full_like(C0, np.inf)).amin(dim=(1, 2, 3))
+    hi = torch.where(w, C0, torch.full_like(C0, -np.inf)).amax(dim=(1, 2, 3))
+    span = (hi - lo).cpu().numpy()
+    m0 = ops.content(C0, V0).cpu().numpy()
+    out = {}
+    k = 0
+    t_ref = t_spl = 0.0
+    for mark in marks:
+        while k < mark:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            c = torch.cuda.Event(enable_timing=True)
+            a.record()
+            ops.advect_fct(ref, V0, V1, Qx, Qy, ok, case.dt, out=ref_o, workspace=ws_r, precision="f64")
+            b.record()
+            ops.advect_fct(spl, V0, V1, Qx, Qy, ok, case.dt, out=spl_o, workspace=ws_s, precision="split")
+            c.record()
+            ref, ref_o = ref_o, ref
+            spl, spl_o = spl_o, spl
+            k += 1
+            if k % 100 == 0:
+                torch.cuda.synchronize()
+                t_ref += a.elapsed_time(b)
+                t_spl += b.elapsed_time(c)
+                _log(f"step {k}")
+        torch.cuda.synchronize()
+        diff = torch.where(w, (spl - ref).abs(), torch.zeros_like(ref))
+        nwet = w.sum(dim=(1, 2, 3)).double()
+        emax = diff.amax(dim=(1, 2, 3)).cpu().numpy()
+        erms = torch.sqrt((diff * diff).sum(dim=(1, 2, 3)) / nwet).cpu().numpy()
+        # every step maps (V0, c^n) to (V1, c^{n+1}) with the same volumes, so the conserved quantity of the last
+        # step is sum V0 c^{n-1} -> sum V1 c^n (spl_o still holds c^{n-1})
+        ms = ops.content(spl, V1).cpu().numpy()
+        ms_prev = ops.content(spl_o, V0).cpu().numpy()
+        mr = ops.content(ref, V1).cpu().numpy()
+        smin = torch.where(w, spl, torch.full_like(spl, np.inf)).amin(dim=(1, 2, 3)).cpu().numpy()
+        smax = torch.where(w, spl, torch.full_like(spl, -np.inf)).amax(dim=(1, 2, 3)).cpu().numpy()
+        lo_n, hi_n = lo.cpu().numpy(), hi.cpu().numpy()
+        out[str(mark)] = {
+            "max_rel_err": (emax / span).tolist(),
+            "rms_rel_err": (erms / span).tolist(),
+            "split_last_step_content_rel": (np.abs(ms - ms_prev) / np.abs(ms_prev)).tolist(),
+            "split_vs_f64_content_rel": (np.abs(ms - mr) / np.abs(mr)).tolist(),
+            "split_below_initial_min": (np.maximum(lo_n - smin, 0) / span).tolist(),
+            "split_above_initial_max": (np.maximum(smax - hi_n, 0) / span).tolist(),
+        }
+        _log(f"after {mark} steps: max rel err {np.max(emax / span):.3e}, rms {np.max(erms / span):.3e}")
+    res = {
+        "metric": "split vs FP64: per-tracer max and RMS error relative to the tracer's initial range",
+        "value": float(max(max(v["max_rel_err"]) for v in out.values())), "unit": "relative error",
+        "n_gpus": 1, "steps": marks[-1], "warmup": 0, "higher_is_better": False,
+        "dtype": "f32/f64 against f64",
+        "data": "synthetic (seeded regional case)",
+        "config": {"workload": f"regional {args.nx}x{args.ny}x{args.nz}, {nt} tracers [T, S, 8 passive], closed "
+                               "edges, the same (V0, V1, Qx, Qy) every step",
+                   "tracers": ["T", "S"] + [f"tr{m}" for m in range(nt - 2)],
+                   "initial_span": span.tolist(), "first_step_content": m0.tolist()},
+        "checkpoints": out,
+        "ms_per_step_sampled": {"f64": t_ref / max(k // 100, 1), "split": t_spl / max(k // 100, 1)},
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,9 +319,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--form", choices=("step", "batch", "serve"), default="step",
+    ap.add_argument("--form", choices=("step", "batch", "serve", "precision"), default="step",
                     help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "
-                         "persistent launch; serve: the Poisson job stream through the continuous-batching loop")
+                         "persistent launch; serve: the Poisson job stream through the continuous-batching loop; "
+                         "precision: split against FP64 after --checkpoints steps")
+    ap.add_argument("--checkpoints", default="200,2000", help="precision form: step counts to compare at")
     ap.add_argument("--mix", choices=("mixed", "split", "f64"), default="mixed",
                     help="batch form: member precisions (mixed: even members FP64, odd split)")
     ap.add_argument("--domains", type=int, default=64, help="batch/serve forms: number of regional domains")
@@ -264,6 +350,8 @@ def main() -> int:
         return bench_batch(args)
     if args.form == "serve":
         return bench_serve(args)
+    if args.form == "precision":
+        return bench_precision(args)
 
     if args.rch or args.wpb:
         from pyocean_amd import batch as _batch

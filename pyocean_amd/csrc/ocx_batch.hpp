@@ -33,7 +33,9 @@ __device__ __forceinline__ int find_domain(const long long* prefix, int ndom, lo
     return lo;
 }
 
-template <int STAGE>
+// PM: which member precisions the batch holds (1: FP64 only, 2: split only, 3: both); a batch of one precision
+// compiles only that path, so its kernels need fewer registers and keep more waves resident
+template <int STAGE, int PM>
 __global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __restrict__ dom,
                                                            const long long* __restrict__ prefix, int ndom,
                                                            long long total, long long per, int parity) {
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __
             a.cout = D.cl;
         }
         const long long L = it - prefix[d];
-        if (D.split) {
+        if (PM == 2 || (PM == 3 && D.split)) {
             if (D.js)
                 march_block<STAGE, false, false, PrecSplit>(a, L, wv, wpb, reinterpret_cast<float*>(rbuf[wv]), obuf[wv]);
             else
@@ -73,6 +75,7 @@ __global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __
     }
 }
 
+template <int PM>
 __global__ __launch_bounds__(64 * MAXW) void k_limit_batch(const BatchDomain* __restrict__ dom,
                                                            const long long* __restrict__ prefix, int ndom,
                                                            long long total, long long per, int parity) {
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(64 * MAXW) void k_limit_batch(const BatchDomain* __
         l.C = parity ? D.C_out : D.C;
         l.out = parity ? const_cast<double*>(D.C) : D.C_out;
         const long long L = it - prefix[d];
-        if (D.split)
+        if (PM == 2 || (PM == 3 && D.split))
             limit_block<PrecSplit>(l, L, wv, wpb);
         else
             limit_block<PrecF64>(l, L, wv, wpb);

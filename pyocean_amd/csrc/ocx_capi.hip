@@ -610,7 +610,9 @@ int ocx_profile_collect(int max_kernels, const char** names, double* ms_total, l
 }  // extern "C"
 
 struct ocx_batch {
-    int ndom = 0, wpb = 0, rch = 0, gpc = 0, ncu = 0;
+    int ndom = 0, wpb = 0, rch = 0, ncu = 0;
+    int pm = 0;            // member precisions: 1 FP64 only, 2 split only, 3 both
+    int gpc[4] = {};       // resident workgroups per compute unit of stage 1, 2, 3 and the limiter
     ocx::BatchDomain* d_dom = nullptr;
     long long* d_pm = nullptr;
     long long* d_pl = nullptr;
@@ -620,6 +622,20 @@ struct ocx_batch {
 };
 
 namespace {
+
+template <int PM>
+void batch_kernels_pm(const void** k) {
+    k[0] = reinterpret_cast<const void*>(&ocx::k_march_batch<1, PM>);
+    k[1] = reinterpret_cast<const void*>(&ocx::k_march_batch<2, PM>);
+    k[2] = reinterpret_cast<const void*>(&ocx::k_march_batch<3, PM>);
+    k[3] = reinterpret_cast<const void*>(&ocx::k_limit_batch<PM>);
+}
+
+void batch_kernels(int pm, const void** k) {
+    if (pm == 1) batch_kernels_pm<1>(k);
+    else if (pm == 2) batch_kernels_pm<2>(k);
+    else batch_kernels_pm<3>(k);
+}
 
 void batch_free(ocx_batch* b) {
     if (!b) return;
@@ -740,14 +756,26 @@ int ocx_batch_create(const ocx_domain* domains, int ndomains, int rows_per_chunk
     b->ndom = ndomains;
     b->wpb = wpb;
     b->rch = rch;
-    b->gpc = groups_per_cu > 0 ? groups_per_cu : 4;
     b->tm = pm[ndomains];
     b->tl = pl[ndomains];
+    for (int d = 0; d < ndomains; ++d) b->pm |= domains[d].precision == 1 ? 2 : 1;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || b->ncu < 1) {
         batch_free(b);
         return fail(OCX_E_HIP, "cannot query the compute-unit count");
+    }
+    // the persistent grid: groups_per_cu workgroups per compute unit, or (0) as many as the kernel's register and
+    // LDS use let stay resident
+    const void* kern[4];
+    batch_kernels(b->pm, kern);
+    for (int q = 0; q < 4; ++q) {
+        int n = groups_per_cu;
+        if (n <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern[q], 64 * wpb, 0) != hipSuccess) {
+            batch_free(b);
+            return fail(OCX_E_HIP, "occupancy query failed");
+        }
+        b->gpc[q] = n > 0 ? n : 1;
     }
     if (hipMalloc(&b->d_dom, sizeof(BatchDomain) * ndomains) != hipSuccess ||
         hipMalloc(&b->d_pm, sizeof(long long) * (ndomains + 1)) != hipSuccess ||
@@ -770,30 +798,23 @@ int ocx_batch_step(ocx_batch* b, int parity, void* stream) {
     OCX_TRY(need(b, "batch"));
     if (parity != 0 && parity != 1) return fail(OCX_E_ARG, "parity %d", parity);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const long long slots = (long long)b->ncu * b->gpc;
-    const long long perm = (b->tm + slots - 1) / slots, perl = (b->tl + slots - 1) / slots;
-    const unsigned gm = (unsigned)((b->tm + perm - 1) / perm), gl = (unsigned)((b->tl + perl - 1) / perl);
-    const unsigned threads = 64u * b->wpb;
-    {
-        ProfScope _ps("batch_stage1", s);
-        k_march_batch<1><<<gm, threads, 0, s>>>(b->d_dom, b->d_pm, b->ndom, b->tm, perm, parity);
+    static const char* const names[4] = {"batch_stage1", "batch_stage2", "batch_stage3_split", "batch_limit_final"};
+    const void* kern[4];
+    batch_kernels(b->pm, kern);
+    const dim3 block(64u * b->wpb);
+    for (int q = 0; q < 4; ++q) {
+        // workgroup g walks the contiguous item range [g * per, (g + 1) * per)
+        const long long total = q < 3 ? b->tm : b->tl;
+        const long long slots = (long long)b->ncu * b->gpc[q];
+        long long per = (total + slots - 1) / slots;
+        const dim3 grid((unsigned)((total + per - 1) / per));
+        const long long* prefix = q < 3 ? b->d_pm : b->d_pl;
+        void* args[] = {&b->d_dom, const_cast<long long**>(&prefix), &b->ndom, const_cast<long long*>(&total), &per,
+                        &parity};
+        ProfScope _ps(names[q], s);
+        OCX_TRY(hip_check(hipLaunchKernel(kern[q], grid, block, args, 0, s), names[q]));
     }
-    OCX_TRY(launch_check("batch stage 1"));
-    {
-        ProfScope _ps("batch_stage2", s);
-        k_march_batch<2><<<gm, threads, 0, s>>>(b->d_dom, b->d_pm, b->ndom, b->tm, perm, parity);
-    }
-    OCX_TRY(launch_check("batch stage 2"));
-    {
-        ProfScope _ps("batch_stage3_split", s);
-        k_march_batch<3><<<gm, threads, 0, s>>>(b->d_dom, b->d_pm, b->ndom, b->tm, perm, parity);
-    }
-    OCX_TRY(launch_check("batch stage 3"));
-    {
-        ProfScope _ps("batch_limit_final", s);
-        k_limit_batch<<<gl, threads, 0, s>>>(b->d_dom, b->d_pl, b->ndom, b->tl, perl, parity);
-    }
-    return launch_check("batch limiter");
+    return launch_check("batch step");
 }
 
 int ocx_batch_graph_step(ocx_batch* b, int parity, void* stream) {
