@@ -41,7 +41,7 @@ __global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __
                                                            long long total, long long per, int parity) {
     __shared__ double rbuf[MAXW][MRB];
     __shared__ uint8_t obuf[MAXW][MRB];
-    const int wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    const int wv = wave_id(), wpb = blockDim.x >> 6;
     const long long b0 = xcd_logical(blockIdx.x, gridDim.x) * per;
     const long long b1 = b0 + per < total ? b0 + per : total;
     for (long long it = b0; it < b1; ++it) {
@@ -79,7 +79,7 @@ template <int PM>
 __global__ __launch_bounds__(64 * MAXW) void k_limit_batch(const BatchDomain* __restrict__ dom,
                                                            const long long* __restrict__ prefix, int ndom,
                                                            long long total, long long per, int parity) {
-    const int wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    const int wv = wave_id(), wpb = blockDim.x >> 6;
     const long long b0 = xcd_logical(blockIdx.x, gridDim.x) * per;
     const long long b1 = b0 + per < total ? b0 + per : total;
     for (long long it = b0; it < b1; ++it) {

@@ -2,13 +2,19 @@
 //
 // One wavefront owns a strip of 60 cells of one (tracer, layer) plane and marches down a chunk of rows.  Lane l sits
 // on column i0 - 2 + l: lanes 2..61 are the strip's cells, lanes 1 and 62 the halo columns whose ratios R+- the
-// edge cells need, lanes 0 and 63 the columns those ratios take their bounds from.  At row j the wave computes R+- of row j+1 (bounds over the cell, i +- 1, j +- 1, k +- 1 of C
-// and c_L, ok cells only; P+- from the antidiffusive fluxes on the cell's faces; _fct_coeff_kernel,
-// core/dynamics/sigma_baroclinic.py:407-431) and finishes row j: face coefficients from the ratios of the two
-// adjacent cells (:436-447), G = F_L + C A, q = q0 - dt div G, out = q/V1 (safe) (_fct_final :514-530).  R+- of
-// the west/east neighbours come from the adjacent lanes, those of rows j-1, j+1 from a register ring, so R+- never
-// goes through HBM.  The waves of a workgroup are the tracers of one layer; the workgroups of layers k-1, k, k+1
-// side in one workgroup, so the k +- 1 reads of C and c_L hit the caches of that compute unit.
+// edge cells need, lanes 0 and 63 the columns those ratios take their bounds from.  At row j the wave computes R+-
+// of row j+1 (_fct_coeff_kernel, core/dynamics/sigma_baroclinic.py:407-431: bounds over the cell, i +- 1, j +- 1 and
+// k +- 1 of C and c_L, ok cells only; P+- from the antidiffusive fluxes on the cell's faces) and finishes row j
+// (_fct_final :514-530): face coefficients from the ratios of the two adjacent cells (:436-447), G = F_L + C A,
+// q = q0 - dt div G, out = q/V1 (safe).
+//
+// Each cell's bound contribution (max and min of C and c_L, or -inf/+inf when the cell is not valid) is formed once
+// when its row enters the register ring and reaches the neighbouring lanes by DPP shifts, so the 7-point bounds are
+// twelve max/min instructions.  R+- of the west/east neighbours come from the adjacent lanes and those of rows
+// j-1, j+1 from registers, so R+- never goes through HBM.  Each face flux G is formed once: the east face of a cell
+// is the west face of the next lane, the south face of row j is the north face of row j-1 of the same lane.
+// Loads address a wave-uniform plane base with 32-bit lane offsets (rows and columns outside the plane clamped to an
+// in-bounds element whose value the validity masks discard).
 #pragma once
 #include "ocx_march.hpp"
 
@@ -37,30 +43,29 @@ struct LimitArgs {
     unsigned flags;
 };
 
-// loads of one row r that the ratios of that row need (fetched one row ahead)
-struct RatioIn {
-    double aw, an, as;       // A on the lane's west, north and south faces
-    double cd, ld, cu, lu;   // C and c_L in the layers below and above
-    double v1;
-    bool od, ou;             // validity below / above
-};
-
-// loads of one row j that the final update needs (fetched one row ahead)
-struct FinalIn {
-    double qx, qys, qyn, v0, v1;
-};
-
-struct RowR {
-    double rin, rout;        // ratios of the lane's cell in this row
-    double aw, an;           // A on the lane's west face and on its north face
-};
-
 // Work decomposition: a workgroup holds the waves of up to MAXW tracers of one (layer, strip, chunk) (shared
 // per-layer fields from the CU's caches); logical blocks run layer-fastest, so the workgroups of layers k-1, k, k+1
 // of a strip run side by side on one XCD and the k +- 1 reads of C and c_L hit its L2.
 // Logical blocks of one limiter launch over a domain: tracer groups x layers x strips x chunks.
 __host__ __device__ inline long long limit_blocks(const LimitArgs& a, int wpb) {
     return (long long)((a.nt + wpb - 1) / wpb) * a.n * a.nsx * a.ncy;
+}
+
+// A load at a byte offset from a wave-uniform base: the address is the scalar base plus a 32-bit lane offset, so no
+// 64-bit address arithmetic runs on the vector ALUs.  Every offset passed here lies inside the array (rows and
+// columns outside the domain are clamped to row 0 / column 0 and their values are never used).
+template <class T>
+__device__ __forceinline__ T ldo(const T* base, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
+// Upper and lower bound contributions of one cell to the 7-point bounds of its neighbours: max and min of C and
+// c_L where the cell is valid, -inf / +inf (no contribution) elsewhere.
+struct Bnd {
+    double hi, lo;
+};
+__device__ __forceinline__ Bnd bnd(bool v, double c, double l) {
+    return v ? Bnd{fmax(c, l), fmin(c, l)} : Bnd{-INFINITY, INFINITY};
 }
 
 // One logical block L of the limiter (wave wv of wpb).
@@ -83,174 +88,177 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
     const int jb = cy * a.rch, je = min(jb + a.rch, ny);
     const size_t plane = (size_t)ny * nx;
     const bool col_in = i >= 0 && i < nx;
+    const bool xcol = i >= 0 && i <= nx;
     const bool own = lane >= 2 && lane < 2 + LS && col_in;
-    const bool rlane = lane >= 1 && lane <= 62;    // lanes whose ratios are used
+    const double dt = a.dt;
+    const unsigned flags = a.flags;
+    const bool ow = flags & OPEN_W, oe = flags & OPEN_E, osb = flags & OPEN_S, onb = flags & OPEN_N;
+    const bool has_dn = k > 0, has_up = k + 1 < n;
+    // wave-uniform plane bases
     const double* Cp = a.C + (size_t)p * plane;
     const double* cLp = a.cL + (size_t)p * plane;
-    const double* ext0 = a.ext ? a.ext + (size_t)p * plane : Cp;
+    const double* Cd = has_dn ? Cp - plane : Cp;
+    const double* Ld = has_dn ? cLp - plane : cLp;
+    const double* Cu = has_up ? Cp + plane : Cp;
+    const double* Lu = has_up ? cLp + plane : cLp;
     const uint8_t* okk = a.ok + (size_t)k * plane;
+    const uint8_t* okd = has_dn ? okk - plane : okk;
+    const uint8_t* oku = has_up ? okk + plane : okk;
     const double* V0k = a.V0 + (size_t)k * plane;
     const double* V1k = a.V1 + (size_t)k * plane;
     const double* Qxk = a.Qx + (size_t)k * ny * (nx + 1);
     const double* Qyk = a.Qy + (size_t)k * (ny + 1) * nx;
     const FL* axp = static_cast<const FL*>(a.ax) + (size_t)p * ny * (nx + 1);
     const FL* ayp = static_cast<const FL*>(a.ay) + (size_t)p * (ny + 1) * nx;
-    const bool has_dn = k > 0, has_up = k + 1 < n;
-    const double dt = a.dt;
-    const unsigned flags = a.flags;
+    const double* ext0 = a.ext ? a.ext + (size_t)p * plane : Cp;
+    double* outp = a.out + (size_t)p * plane;
+    // lane byte offsets inside a row (column 0 for lanes outside the row)
+    const uint32_t o8 = col_in ? (uint32_t)i * 8u : 0u, o1 = col_in ? (uint32_t)i : 0u;
+    const uint32_t ox8 = xcol ? (uint32_t)i * 8u : 0u;
+    const uint32_t of = col_in ? (uint32_t)i * (uint32_t)sizeof(FL) : 0u;
+    const uint32_t oxf = xcol ? (uint32_t)i * (uint32_t)sizeof(FL) : 0u;
+    const uint32_t rc8 = (uint32_t)nx * 8u, rx8 = (uint32_t)(nx + 1) * 8u;
+    const uint32_t rcf = (uint32_t)nx * (uint32_t)sizeof(FL), rxf = (uint32_t)(nx + 1) * (uint32_t)sizeof(FL);
+    auto rowc = [&](int r) -> uint32_t { return (r >= 0 && r < ny) ? (uint32_t)r : 0u; };     // cell / x-face rows
+    auto rowy = [&](int r) -> uint32_t { return (r >= 0 && r <= ny) ? (uint32_t)r : 0u; };    // y-face rows
 
-    // Loads are issued unconditionally from addresses that are always inside the arrays and masked one loop step
-    // later, when the row is used, so a wave never waits on a load right after issuing it.
-    struct RawC {
+    // Loads are issued one loop step before their use, so a wave never waits on a load right after issuing it.
+    struct RingRaw {
         double c, l;
         uint8_t o;
-        bool in;
     };
-    auto ldc = [&](int r) {
-        const bool in = col_in && r >= 0 && r < ny;
-        const size_t x = in ? (size_t)r * nx + i : 0;
-        return RawC{Cp[x], cLp[x], okk[x], in};
+    auto ld_ring = [&](int r) {
+        const uint32_t rr = rowc(r);
+        return RingRaw{ldo(Cp, o8 + rr * rc8), ldo(cLp, o8 + rr * rc8), ldo(okk, o1 + rr * (uint32_t)nx)};
     };
-    auto usec = [&](const RawC& q, double& c, double& l, bool& o) {
-        c = q.in ? q.c : 0.0;
-        l = q.in ? q.l : 0.0;
-        o = q.in && q.o != 0;
-    };
-    struct RawR {
-        RatioIn q;
+    auto row_in = [&](int r) { return r >= 0 && r < ny; };
+    // ratio inputs of row r: A on the west face (r, i) and north face (r+1, i), V1, and the layers below / above
+    struct RatRaw {
+        FL aw, an;
+        double v1, cd, ld, cu, lu;
         uint8_t od, ou;
-        bool row, aw_in, cell;
     };
-    auto ld_ratio = [&](int r) {
-        RawR w;
-        w.row = r >= 0 && r < ny;
-        w.aw_in = w.row && i >= 0 && i <= nx;
-        w.cell = w.row && col_in && rlane;
-        const size_t x = w.cell ? (size_t)r * nx + i : 0;
-        w.q.aw = (double)axp[w.aw_in ? (size_t)r * (nx + 1) + i : 0];
-        w.q.an = (double)ayp[w.cell ? x + nx : 0];
-        w.q.as = (double)ayp[x];
-        w.q.v1 = V1k[x];
-        const size_t xd = (w.cell && has_dn) ? x - plane : 0, xu = (w.cell && has_up) ? x + plane : 0;
-        w.od = okk[xd];
-        w.q.cd = Cp[xd];
-        w.q.ld = cLp[xd];
-        w.ou = okk[xu];
-        w.q.cu = Cp[xu];
-        w.q.lu = cLp[xu];
-        return w;
+    auto ld_rat = [&](int r) {
+        const uint32_t rr = rowc(r), ry = rowy(r + 1);
+        const uint32_t xc = o8 + rr * rc8, xb = o1 + rr * (uint32_t)nx;
+        return RatRaw{ldo(axp, oxf + rr * rxf), ldo(ayp, of + ry * rcf), ldo(V1k, xc), ldo(Cd, xc), ldo(Ld, xc),
+                      ldo(Cu, xc), ldo(Lu, xc), ldo(okd, xb), ldo(oku, xb)};
     };
-    auto use_ratio = [&](const RawR& w) {
-        RatioIn q = w.q;
-        if (!w.aw_in) q.aw = 0.0;
-        if (!w.cell) q.an = q.as = q.v1 = 0.0;
-        q.od = w.cell && has_dn && w.od != 0;
-        q.ou = w.cell && has_up && w.ou != 0;
-        if (!q.od) q.cd = q.ld = 0.0;
-        if (!q.ou) q.cu = q.lu = 0.0;
-        return q;
+    struct Rat {
+        double aw, an, v1;
+        Bnd d, u;
     };
-    struct RawF {
-        FinalIn f;
-        bool xf, ow_;
+    auto use_rat = [&](const RatRaw& w, int r) {
+        const bool in = col_in && row_in(r);
+        return Rat{(double)w.aw, (double)w.an, w.v1, bnd(in && has_dn && w.od != 0, w.cd, w.ld),
+                   bnd(in && has_up && w.ou != 0, w.cu, w.lu)};
     };
-    auto ld_final = [&](int j) {
-        RawF w;
-        const bool row = j < je;
-        w.xf = row && i >= 0 && i <= nx;
-        w.ow_ = row && own;
-        const size_t x = w.ow_ ? (size_t)j * nx + i : 0;
-        w.f.qx = Qxk[w.xf ? (size_t)j * (nx + 1) + i : 0];
-        w.f.qys = Qyk[x];
-        w.f.qyn = Qyk[w.ow_ ? x + nx : 0];
-        w.f.v0 = V0k[x];
-        w.f.v1 = V1k[x];
-        return w;
+    // update inputs of row j: V0, Qx on the west face, Qy on the north face
+    struct FinRaw {
+        double v0, qx, qyn;
     };
-    auto use_final = [&](const RawF& w) {
-        FinalIn f = w.f;
-        if (!w.xf) f.qx = 0.0;
-        if (!w.ow_) f.qys = f.qyn = f.v0 = f.v1 = 0.0;
-        return f;
+    auto ld_fin = [&](int j) {
+        const uint32_t rr = rowc(j);
+        return FinRaw{ldo(V0k, o8 + rr * rc8), ldo(Qxk, ox8 + rr * rx8), ldo(Qyk, o8 + rowy(j + 1) * rc8)};
     };
-    // R+- of row r from the lane's column values of rows r-1, r, r+1 and the row's prefetched loads
-    auto ratios = [&](int r, const RatioIn& q, double cs, double ls, bool os, double cc, double lc, bool oc,
-                      double cn, double ln, bool on) -> RowR {
-        RowR res{0.0, 0.0, q.aw, q.an};
-        const double cw = shfl_up1(cc), lw = shfl_up1(lc);
-        const double ce = shfl_down1(cc), le = shfl_down1(lc);
-        const int owi = dpp_up1((int)oc), oei = dpp_down1((int)oc);
+
+    // R+- of row r (_fct_coeff_kernel, core/dynamics/sigma_baroclinic.py:407-431): bounds over the cell, i +- 1
+    // (neighbour lanes), r +- 1 (bS, bN) and k +- 1 (q.d, q.u), valid cells only; as = A on the south face
+    struct RR {
+        double rin, rout;
+    };
+    auto ratios = [&](const Bnd& b, const Bnd& bS, const Bnd& bN, const Rat& q, double as, double lc, bool oc) -> RR {
+        const double hw = shfl_up1(b.hi), he = shfl_down1(b.hi);
+        const double lw = shfl_up1(b.lo), le = shfl_down1(b.lo);
         const double ae = shfl_down1(q.aw);
-        if (!(rlane && oc && r >= 0 && r < ny)) return res;      // oc implies the cell is in the domain
-        double cmax = -INFINITY, cmin = INFINITY;
-        auto take = [&](bool v, double c0, double l0) {
-            if (v) {
-                cmax = dmax(dmax(cmax, c0), l0);
-                cmin = dmin(dmin(cmin, c0), l0);
-            }
-        };
-        take(true, cc, lc);
-        take(owi != 0, cw, lw);
-        take(oei != 0, ce, le);
-        take(os, cs, ls);
-        take(on, cn, ln);
-        take(q.od, q.cd, q.ld);
-        take(q.ou, q.cu, q.lu);
-        const double p_in = dt * (dmax(q.aw, 0.0) - dmin(ae, 0.0) + dmax(q.as, 0.0) - dmin(q.an, 0.0));
-        const double p_out = dt * (dmax(ae, 0.0) - dmin(q.aw, 0.0) + dmax(q.an, 0.0) - dmin(q.as, 0.0));
+        const double cmax = fmax(fmax(fmax(b.hi, hw), fmax(he, bS.hi)), fmax(fmax(bN.hi, q.d.hi), q.u.hi));
+        const double cmin = fmin(fmin(fmin(b.lo, lw), fmin(le, bS.lo)), fmin(fmin(bN.lo, q.d.lo), q.u.lo));
+        const double p_in = dt * (dmax(q.aw, 0.0) - dmin(ae, 0.0) + dmax(as, 0.0) - dmin(q.an, 0.0));
+        const double p_out = dt * (dmax(ae, 0.0) - dmin(q.aw, 0.0) + dmax(q.an, 0.0) - dmin(as, 0.0));
         const double q_in = dmax(q.v1 * (cmax - lc), 0.0);
         const double q_out = dmax(q.v1 * (lc - cmin), 0.0);
-        res.rin = p_in > 0.0 ? dmin(1.0, q_in / p_in) : 0.0;
-        res.rout = p_out > 0.0 ? dmin(1.0, q_out / p_out) : 0.0;
+        RR res;
+        res.rin = (oc && p_in > 0.0) ? dmin(1.0, q_in / p_in) : 0.0;
+        res.rout = (oc && p_out > 0.0) ? dmin(1.0, q_out / p_out) : 0.0;
         return res;
     };
 
-    // ring of the lane's column: C, c_L, validity of rows j-1 .. j+3 (row j+3 is one row ahead of its use)
-    double cr[5], lr[5];
-    bool orr[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) usec(ldc(jb - 1 + m), cr[m], lr[m], orr[m]);
-    double cm2, lm2;
-    bool om2;
-    usec(ldc(jb - 2), cm2, lm2, om2);
-    RowR Rm = ratios(jb - 1, use_ratio(ld_ratio(jb - 1)), cm2, lm2, om2, cr[0], lr[0], orr[0], cr[1], lr[1], orr[1]);
-    RowR Rc = ratios(jb, use_ratio(ld_ratio(jb)), cr[0], lr[0], orr[0], cr[1], lr[1], orr[1], cr[2], lr[2], orr[2]);
-    RatioIn qn1 = use_ratio(ld_ratio(jb + 1));      // inputs of R(j+1)
-    FinalIn fc = use_final(ld_final(jb));           // inputs of row j's update
-    const bool ow = flags & OPEN_W, oe = flags & OPEN_E, osb = flags & OPEN_S, onb = flags & OPEN_N;
+    // ring of the lane's column, rows j .. j+2: C, c_L, validity and bound contributions
+    double cr[3], lr[3];
+    bool orr[3];
+    Bnd br[3];
+    auto ring_in = [&](const RingRaw& w, int r, int m) {
+        cr[m] = w.c;
+        lr[m] = w.l;
+        orr[m] = col_in && row_in(r) && w.o != 0;
+        br[m] = bnd(orr[m], w.c, w.l);
+    };
+    // rows jb-2 and jb-1 for R(jb-1), R(jb)
+    const RingRaw rm2 = ld_ring(jb - 2), rm1 = ld_ring(jb - 1), r0 = ld_ring(jb), r1 = ld_ring(jb + 1);
+    const Rat qm1 = use_rat(ld_rat(jb - 1), jb - 1);
+    const Rat q0 = use_rat(ld_rat(jb), jb);
+    Rat q1 = use_rat(ld_rat(jb + 1), jb + 1);
+    const double as_m1 = (double)ldo(ayp, of + rowy(jb - 1) * rcf);     // A on the south face of row jb-1
+    ring_in(rm2, jb - 2, 0);
+    ring_in(rm1, jb - 1, 1);
+    ring_in(r0, jb, 2);
+    const RR Rm = ratios(br[1], br[0], br[2], qm1, as_m1, lr[1], orr[1]);
+    const double cm1 = cr[1];                                           // C(jb-1)
+    ring_in(r1, jb + 1, 0);                                             // rows jb .. jb+1 in slots 2, 0
+    RR Rc = ratios(br[2], br[1], br[0], q0, qm1.an, lr[2], orr[2]);
+    // reorder the ring to rows jb, jb+1 in slots 0, 1 and load row jb+2 into slot 2
+    {
+        const double c0 = cr[2], l0 = lr[2];
+        const bool o0 = orr[2];
+        const Bnd b0 = br[2];
+        cr[1] = cr[0];
+        lr[1] = lr[0];
+        orr[1] = orr[0];
+        br[1] = br[0];
+        cr[0] = c0;
+        lr[0] = l0;
+        orr[0] = o0;
+        br[0] = b0;
+        ring_in(ld_ring(jb + 2), jb + 2, 2);
+    }
+    FinRaw fc = ld_fin(jb);
+    double aw0 = q0.aw, an0 = q0.an, v1_0 = q0.v1;                     // row jb: A west / north, V1
+    // G on the south face of row jb (_fct_final :514-530 with the face coefficient of :436-447)
+    double Gs = 0.0;
+    if (own) {
+        const double* ecol_s = ext0 + i;
+        const double* ecol_n = ext0 + (size_t)(ny - 1) * nx + i;
+        const double qys = ldo(Qyk, o8 + rowy(jb) * rc8);
+        const double cfs = jb > 0 ? coeff(qm1.an, Rc.rin, Rm.rout, Rm.rin, Rc.rout) : 0.0;
+        Gs = donor(qys, jb, ny, cm1, cr[0], osb, onb, ecol_s, ecol_n) + cfs * qm1.an;
+    }
 
     for (int j = jb; j < je; ++j) {
-        // one row ahead: the ring row j+4, the ratio inputs of row j+2, the update inputs of row j+1
-        const RawC c4 = ldc(j + 4);
-        const RawR qn2 = ld_ratio(j + 2);
-        const RawF fn = ld_final(j + 1);
-        // ratios of row j+1 (rows j .. j+2 = cr[1..3])
-        const RowR Rn = ratios(j + 1, qn1, cr[1], lr[1], orr[1], cr[2], lr[2], orr[2], cr[3], lr[3], orr[3]);
+        // one row ahead: ring row j+3, ratio inputs of row j+2, update inputs of row j+1
+        const RingRaw rn = ld_ring(j + 3);
+        const RatRaw qn = ld_rat(j + 2);
+        const FinRaw fn = ld_fin(j + 1);
+        // ratios of row j+1 (ring rows j .. j+2)
+        const RR Rn = ratios(br[1], br[0], br[2], q1, an0, lr[1], orr[1]);
         // finish row j
         const double rin_w = shfl_up1(Rc.rin), rout_w = shfl_up1(Rc.rout);
-        const double rin_e = shfl_down1(Rc.rin), rout_e = shfl_down1(Rc.rout);
-        const double ae = shfl_down1(Rc.aw);
-        const double cw = shfl_up1(cr[1]), ce = shfl_down1(cr[1]);
-        const double qxe = shfl_down1(fc.qx);
+        const double c0 = cr[0];
+        const double cw = shfl_up1(c0);
+        // G on the lane's west face (lanes whose column is 0 .. nx): donor flux + coefficient x A
+        const double* erow = ext0 + (size_t)j * nx;
+        const double cfw = (i > 0 && i < nx) ? coeff(aw0, Rc.rin, rout_w, rin_w, Rc.rout) : 0.0;
+        const double Gw = donor(fc.qx, i, nx, cw, c0, ow, oe, erow, erow + nx - 1) + cfw * aw0;
+        const double Ge = shfl_down1(Gw);
+        double Gn = 0.0;
         if (own) {
-            const size_t xc = (size_t)j * nx + i;
-            const double c0 = cr[1];
-            const double aw = Rc.aw, as_ = Rm.an, an = Rc.an;
-            const double cfw = i > 0 ? coeff(aw, Rc.rin, rout_w, rin_w, Rc.rout) : 0.0;
-            const double cfe = i + 1 < nx ? coeff(ae, rin_e, Rc.rout, Rc.rin, rout_e) : 0.0;
-            const double cfs = j > 0 ? coeff(as_, Rc.rin, Rm.rout, Rm.rin, Rc.rout) : 0.0;
-            const double cfn = j + 1 < ny ? coeff(an, Rn.rin, Rc.rout, Rc.rin, Rn.rout) : 0.0;
-            const double* erow = ext0 + (size_t)j * nx;
             const double* ecol_s = ext0 + i;
             const double* ecol_n = ext0 + (size_t)(ny - 1) * nx + i;
-            const double Gw = donor(fc.qx, i, nx, cw, c0, ow, oe, erow, erow + nx - 1) + cfw * aw;
-            const double Ge = donor(qxe, i + 1, nx, c0, ce, ow, oe, erow, erow + nx - 1) + cfe * ae;
-            const double Gs = donor(fc.qys, j, ny, cr[0], c0, osb, onb, ecol_s, ecol_n) + cfs * as_;
-            const double Gn = donor(fc.qyn, j + 1, ny, c0, cr[2], osb, onb, ecol_s, ecol_n) + cfn * an;
+            const double cfn = j + 1 < ny ? coeff(an0, Rn.rin, Rc.rout, Rc.rin, Rn.rout) : 0.0;
+            Gn = donor(fc.qyn, j + 1, ny, c0, cr[1], osb, onb, ecol_s, ecol_n) + cfn * an0;
             const double d = Ge - Gw + Gn - Gs;
             const double q = fc.v0 * c0 - dt * d;
-            const double v1 = fc.v1;
-            a.out[(size_t)p * plane + xc] = v1 > 0 ? q / v1 : c0;
+            const double v1 = v1_0;
+            outp[(size_t)j * nx + i] = v1 > 0 ? q / v1 : c0;
             if (a.fxi) {
                 if (a.flux_face_i == i) a.fxi[(size_t)p * ny + j] = Gw;
                 if (a.flux_face_i == nx && i == nx - 1) a.fxi[(size_t)p * ny + j] = Ge;
@@ -264,23 +272,28 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
             }
         }
         // advance
-        Rm = Rc;
+        Gs = Gn;
         Rc = Rn;
-        qn1 = use_ratio(qn2);
-        fc = use_final(fn);
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            cr[m] = cr[m + 1];
-            lr[m] = lr[m + 1];
-            orr[m] = orr[m + 1];
-        }
-        usec(c4, cr[4], lr[4], orr[4]);
+        aw0 = q1.aw;
+        an0 = q1.an;
+        v1_0 = q1.v1;
+        q1 = use_rat(qn, j + 2);
+        fc = fn;
+        cr[0] = cr[1];
+        lr[0] = lr[1];
+        orr[0] = orr[1];
+        br[0] = br[1];
+        cr[1] = cr[2];
+        lr[1] = lr[2];
+        orr[1] = orr[2];
+        br[1] = br[2];
+        ring_in(rn, j + 3, 2);
     }
 }
 
 template <class PR>
 __global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
-    limit_block<PR>(a, xcd_logical(blockIdx.x, gridDim.x), threadIdx.x >> 6, blockDim.x >> 6);
+    limit_block<PR>(a, xcd_logical(blockIdx.x, gridDim.x), wave_id(), blockDim.x >> 6);
 }
 
 }  // namespace ocx

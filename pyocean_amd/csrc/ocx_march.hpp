@@ -447,7 +447,7 @@ template <int STAGE, bool Z, bool UNLIM, class PR>
 __global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
     __shared__ typename PR::w rbuf[MAXW][MRB];
     __shared__ uint8_t obuf[MAXW][MRB];
-    const int wv = threadIdx.x >> 6;
+    const int wv = wave_id();
     march_block<STAGE, Z, UNLIM, PR>(a, xcd_logical(blockIdx.x, gridDim.x), wv, blockDim.x >> 6, rbuf[wv], obuf[wv]);
 }
 

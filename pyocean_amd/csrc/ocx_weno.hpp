@@ -23,6 +23,10 @@ __device__ __forceinline__ long long xcd_logical(long long b, long long nb) {
     return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
 }
 
+// Index of the calling wavefront in its workgroup, as a wave-uniform (scalar) value: everything derived from it
+// (tracer, plane base pointers) stays in scalar registers instead of being recomputed per lane.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // Correctly rounded x / b for a constant divisor b with rb = RN(1/b): q = RN(x rb) is within an ulp of x/b, the
 // remainder x - b q is exact under an FMA, and RN(q + (x - b q) rb) = RN(x/b) (Markstein's correction; needs rb
 // within half an ulp of 1/b and no underflow, so tiny |x| takes the IEEE division).
