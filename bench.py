@@ -104,7 +104,7 @@ def bench_batch(args) -> int:
 
     from pyocean_amd import _native, autotune, batch, ops, serve, synth
 
-    shapes = synth.batch_shapes(args.domains, seed=7)
+    shapes = synth.batch_shapes(args.domains, seed=7) if not args.same_shape else [(args.nx, args.ny, args.nz)] * args.domains
     _log(f"generating {len(shapes)} domains")
     grids = serve.make_grids(shapes, nt=args.nt, seed=100, workers=args.workers, log=_log)
     prec = [("f64" if d % 2 == 0 else "split") if args.mix == "mixed" else args.mix for d in range(len(grids))]
@@ -235,7 +235,11 @@ def bench_precision(args) -> int:
     C0 = d(case.C)
     ref, ref_o = C0.clone(), torch.empty_like(C0)
     spl, spl_o = C0.clone(), torch.empty_like(C0)
-    ws_r, ws_s = ops.Workspace(), ops.Workspace()
+    # control: the FP64 form from the initial state rounded to FP32 (a perturbation of the size of one FP32
+    # rounding, about what one split step introduces), to separate the problem's own sensitivity to small
+    # perturbations from the split form's arithmetic
+    ctl, ctl_o = C0.float().double(), torch.empty_like(C0)
+    ws_r, ws_s, ws_c = ops.Workspace(), ops.Workspace(), ops.Workspace()
     nt = C0.shape[0]
     w = wet.unsqueeze(0).expand_as(C0)
     lo = torch.where(w, C0, torch.full_like(C0, np.inf)).amin(dim=(1, 2, 3))
@@ -255,8 +259,10 @@ def bench_precision(args) -> int:
             b.record()
             ops.advect_fct(spl, V0, V1, Qx, Qy, ok, case.dt, out=spl_o, workspace=ws_s, precision="split")
             c.record()
+            ops.advect_fct(ctl, V0, V1, Qx, Qy, ok, case.dt, out=ctl_o, workspace=ws_c, precision="f64")
             ref, ref_o = ref_o, ref
             spl, spl_o = spl_o, spl
+            ctl, ctl_o = ctl_o, ctl
             k += 1
             if k % 100 == 0:
                 torch.cuda.synchronize()
@@ -268,6 +274,18 @@ def bench_precision(args) -> int:
         nwet = w.sum(dim=(1, 2, 3)).double()
         emax = diff.amax(dim=(1, 2, 3)).cpu().numpy()
         erms = torch.sqrt((diff * diff).sum(dim=(1, 2, 3)) / nwet).cpu().numpy()
+        # where the largest split error sits: layer, row, column and that cell's volume against the median
+        tw = int(np.argmax(emax / span))
+        flat = int(torch.argmax(diff[tw]).item())
+        kk, jj, ii = np.unravel_index(flat, diff.shape[1:])
+        v1med = float(V1[wet].median().item())
+        where = {"tracer": tw, "k": int(kk), "j": int(jj), "i": int(ii),
+                 "V1_over_median": float(V1[kk, jj, ii].item()) / v1med,
+                 "V0_over_median": float(V0[kk, jj, ii].item()) / v1med}
+        dctl = torch.where(w, (ctl - ref).abs(), torch.zeros_like(ref))
+        cmax_ = dctl.amax(dim=(1, 2, 3)).cpu().numpy()
+        crms = torch.sqrt((dctl * dctl).sum(dim=(1, 2, 3)) / nwet).cpu().numpy()
+        del dctl
         # every step maps (V0, c^n) to (V1, c^{n+1}) with the same volumes, so the conserved quantity of the last
         # step is sum V0 c^{n-1} -> sum V1 c^n (spl_o still holds c^{n-1})
         ms = ops.content(spl, V1).cpu().numpy()
@@ -279,12 +297,17 @@ def bench_precision(args) -> int:
         out[str(mark)] = {
             "max_rel_err": (emax / span).tolist(),
             "rms_rel_err": (erms / span).tolist(),
+            "largest_split_error_at": where,
+            "control_f64_from_f32_rounded_state_max_rel": (cmax_ / span).tolist(),
+            "control_f64_from_f32_rounded_state_rms_rel": (crms / span).tolist(),
             "split_last_step_content_rel": (np.abs(ms - ms_prev) / np.abs(ms_prev)).tolist(),
             "split_vs_f64_content_rel": (np.abs(ms - mr) / np.abs(mr)).tolist(),
             "split_below_initial_min": (np.maximum(lo_n - smin, 0) / span).tolist(),
             "split_above_initial_max": (np.maximum(smax - hi_n, 0) / span).tolist(),
         }
-        _log(f"after {mark} steps: max rel err {np.max(emax / span):.3e}, rms {np.max(erms / span):.3e}")
+        del diff
+        _log(f"after {mark} steps: max rel err {np.max(emax / span):.3e}, rms {np.max(erms / span):.3e}; control "
+             f"max {np.max(cmax_ / span):.3e}, rms {np.max(crms / span):.3e}")
     res = {
         "metric": "split vs FP64: per-tracer max and RMS error relative to the tracer's initial range",
         "value": float(max(max(v["max_rel_err"]) for v in out.values())), "unit": "relative error",
@@ -323,11 +346,12 @@ def main() -> int:
                     help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "
                          "persistent launch; serve: the Poisson job stream through the continuous-batching loop; "
                          "precision: split against FP64 after --checkpoints steps")
-    ap.add_argument("--checkpoints", default="200,2000", help="precision form: step counts to compare at")
+    ap.add_argument("--checkpoints", default="1,10,200,2000", help="precision form: step counts to compare at")
     ap.add_argument("--mix", choices=("mixed", "split", "f64"), default="mixed",
                     help="batch form: member precisions (mixed: even members FP64, odd split)")
     ap.add_argument("--domains", type=int, default=64, help="batch/serve forms: number of regional domains")
     ap.add_argument("--autotune", action="store_true", help="batch form: sweep the launch shapes first")
+    ap.add_argument("--same-shape", action="store_true", help="batch form: every domain --nx x --ny x --nz")
     ap.add_argument("--eager", action="store_true", help="batch form: eager launches instead of the captured graph")
     ap.add_argument("--jobs", type=int, default=500, help="serve form: number of jobs")
     ap.add_argument("--rate", type=float, default=20.0, help="serve form: mean arrivals per second")

@@ -2,11 +2,12 @@
 // or split precision) advanced by one persistent launch per stage.
 //
 // Every domain contributes the logical blocks its own launch would have (march_blocks / limit_blocks); the batch
-// concatenates them in domain order (prefix sums) and a fixed grid of workgroups walks them: workgroup b takes the
-// contiguous item range [b * per, (b + 1) * per) in XCD-contiguous order, so consecutive items of a domain (the
-// strips of a chunk, the layers k-1, k, k+1 of the limiter) run on one compute unit one after the other.  Each item
-// runs exactly the per-domain code (march_block / limit_block) with the domain's own sizes and precision, so a
-// domain's result does not depend on what else is in the batch.
+// concatenates them in domain order (prefix sums) and a fixed grid of G workgroups, as many as stay resident, walks
+// them grid-stride: workgroup g takes items g, g + G, g + 2G, ... with g numbered XCD-contiguously, so the items in
+// flight at any moment are neighbours in item order (the tracer groups and layers of one strip of one domain) and
+// share the per-layer fields and the limiter's k +- 1 planes through the XCD's L2, as a one-domain launch does.
+// Each item runs exactly the per-domain code (march_block / limit_block) with the domain's own sizes and
+// precision, so a domain's result does not depend on what else is in the batch.
 #pragma once
 #include "ocx_limit.hpp"
 
@@ -38,13 +39,11 @@ __device__ __forceinline__ int find_domain(const long long* prefix, int ndom, lo
 template <int STAGE, int PM>
 __global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __restrict__ dom,
                                                            const long long* __restrict__ prefix, int ndom,
-                                                           long long total, long long per, int parity) {
+                                                           long long total, int parity) {
     __shared__ double rbuf[MAXW][MRB];
     __shared__ uint8_t obuf[MAXW][MRB];
     const int wv = wave_id(), wpb = blockDim.x >> 6;
-    const long long b0 = xcd_logical(blockIdx.x, gridDim.x) * per;
-    const long long b1 = b0 + per < total ? b0 + per : total;
-    for (long long it = b0; it < b1; ++it) {
+    for (long long it = xcd_logical(blockIdx.x, gridDim.x); it < total; it += gridDim.x) {
         const int d = find_domain(prefix, ndom, it);
         const BatchDomain& D = dom[d];
         MarchArgs a = D.m;
@@ -78,11 +77,9 @@ __global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __
 template <int PM>
 __global__ __launch_bounds__(64 * MAXW) void k_limit_batch(const BatchDomain* __restrict__ dom,
                                                            const long long* __restrict__ prefix, int ndom,
-                                                           long long total, long long per, int parity) {
+                                                           long long total, int parity) {
     const int wv = wave_id(), wpb = blockDim.x >> 6;
-    const long long b0 = xcd_logical(blockIdx.x, gridDim.x) * per;
-    const long long b1 = b0 + per < total ? b0 + per : total;
-    for (long long it = b0; it < b1; ++it) {
+    for (long long it = xcd_logical(blockIdx.x, gridDim.x); it < total; it += gridDim.x) {
         const int d = find_domain(prefix, ndom, it);
         const BatchDomain& D = dom[d];
         LimitArgs l = D.l;

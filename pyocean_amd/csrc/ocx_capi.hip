@@ -803,13 +803,12 @@ int ocx_batch_step(ocx_batch* b, int parity, void* stream) {
     batch_kernels(b->pm, kern);
     const dim3 block(64u * b->wpb);
     for (int q = 0; q < 4; ++q) {
-        // workgroup g walks the contiguous item range [g * per, (g + 1) * per)
+        // a resident grid walking the items grid-stride
         const long long total = q < 3 ? b->tm : b->tl;
         const long long slots = (long long)b->ncu * b->gpc[q];
-        long long per = (total + slots - 1) / slots;
-        const dim3 grid((unsigned)((total + per - 1) / per));
+        const dim3 grid((unsigned)(total < slots ? total : slots));
         const long long* prefix = q < 3 ? b->d_pm : b->d_pl;
-        void* args[] = {&b->d_dom, const_cast<long long**>(&prefix), &b->ndom, const_cast<long long*>(&total), &per,
+        void* args[] = {&b->d_dom, const_cast<long long**>(&prefix), &b->ndom, const_cast<long long*>(&total),
                         &parity};
         ProfScope _ps(names[q], s);
         OCX_TRY(hip_check(hipLaunchKernel(kern[q], grid, block, args, 0, s), names[q]));
