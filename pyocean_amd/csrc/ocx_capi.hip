@@ -616,6 +616,7 @@ struct ocx_batch {
     ocx::BatchDomain* d_dom = nullptr;
     long long* d_pm = nullptr;
     long long* d_pl = nullptr;
+    unsigned long long* d_ctr = nullptr;   // work-queue counters: NQ per kernel of the step
     long long tm = 0, tl = 0;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -646,6 +647,7 @@ void batch_free(ocx_batch* b) {
     if (b->d_dom) (void)hipFree(b->d_dom);
     if (b->d_pm) (void)hipFree(b->d_pm);
     if (b->d_pl) (void)hipFree(b->d_pl);
+    if (b->d_ctr) (void)hipFree(b->d_ctr);
     delete b;
 }
 
@@ -779,7 +781,8 @@ int ocx_batch_create(const ocx_domain* domains, int ndomains, int rows_per_chunk
     }
     if (hipMalloc(&b->d_dom, sizeof(BatchDomain) * ndomains) != hipSuccess ||
         hipMalloc(&b->d_pm, sizeof(long long) * (ndomains + 1)) != hipSuccess ||
-        hipMalloc(&b->d_pl, sizeof(long long) * (ndomains + 1)) != hipSuccess) {
+        hipMalloc(&b->d_pl, sizeof(long long) * (ndomains + 1)) != hipSuccess ||
+        hipMalloc(&b->d_ctr, sizeof(unsigned long long) * 4 * NQ) != hipSuccess) {
         batch_free(b);
         return fail(OCX_E_HIP, "batch table allocation failed");
     }
@@ -802,14 +805,18 @@ int ocx_batch_step(ocx_batch* b, int parity, void* stream) {
     const void* kern[4];
     batch_kernels(b->pm, kern);
     const dim3 block(64u * b->wpb);
+    // the work-queue counters start at zero (a kernel, not a memset, so that the step can be graph-captured)
+    k_zero_u64<<<1, 64, 0, s>>>(b->d_ctr, 4 * NQ);
+    OCX_TRY(launch_check("batch queue reset"));
     for (int q = 0; q < 4; ++q) {
-        // a resident grid walking the items grid-stride
+        // a resident grid taking items from the work queues
         const long long total = q < 3 ? b->tm : b->tl;
         const long long slots = (long long)b->ncu * b->gpc[q];
         const dim3 grid((unsigned)(total < slots ? total : slots));
         const long long* prefix = q < 3 ? b->d_pm : b->d_pl;
+        unsigned long long* ctr = b->d_ctr + (size_t)q * NQ;
         void* args[] = {&b->d_dom, const_cast<long long**>(&prefix), &b->ndom, const_cast<long long*>(&total),
-                        &parity};
+                        &parity, &ctr};
         ProfScope _ps(names[q], s);
         OCX_TRY(hip_check(hipLaunchKernel(kern[q], grid, block, args, 0, s), names[q]));
     }
