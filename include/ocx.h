@@ -106,8 +106,12 @@ int ocx_fct_coefficients_f64(const double* c0, const double* cL, const double* V
 int ocx_periodic_pad(const void* src, void* dst, size_t lead, int sy, int sx, int kind, int pad_y, int pad_x,
                      int elem, int crop, void* stream);
 
-/* Per-tracer content sum_k,j,i V c (FP64, deterministic order): out (nt). */
-int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* stream);
+/* Per-tracer content sum_k,j,i V c (FP64, compensated, in a fixed order that does not depend on the launch):
+ * out (nt).  The global content sums of the reference's conservation checks (tracers.py:359-375 callers,
+ * test_sigma_ale.py:1342-1361).  workspace: ocx_content_workspace_bytes(nt) bytes of device memory. */
+size_t ocx_content_workspace_bytes(int nt);
+int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* workspace,
+                    size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Batched form (no counterpart in the reference, which advances one domain per call): many independent domains

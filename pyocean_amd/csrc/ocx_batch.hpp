@@ -132,12 +132,15 @@ __global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __
     });
 }
 
+// blockDim.x = 64 * wpb * LG; dynamic LDS limit_lds_bytes(wpb)
 template <int PM>
-__global__ __launch_bounds__(64 * MAXW) void k_limit_batch(const BatchDomain* __restrict__ dom,
-                                                           const long long* __restrict__ prefix, int ndom,
-                                                           long long total, int parity,
-                                                           unsigned long long* __restrict__ ctr) {
-    const int wv = wave_id(), wpb = blockDim.x >> 6;
+__global__ __launch_bounds__(64 * MAXW * LG) void k_limit_batch(const BatchDomain* __restrict__ dom,
+                                                                const long long* __restrict__ prefix, int ndom,
+                                                                long long total, int parity,
+                                                                unsigned long long* __restrict__ ctr) {
+    extern __shared__ double limit_lds[];
+    Bnd* lds = reinterpret_cast<Bnd*>(limit_lds);
+    const int wv = wave_id(), wpb = (blockDim.x >> 6) / LG;
     drain(ctr, total, [&](long long it) {
         const int d = find_domain(prefix, ndom, it);
         const BatchDomain& D = dom[d];
@@ -146,9 +149,9 @@ __global__ __launch_bounds__(64 * MAXW) void k_limit_batch(const BatchDomain* __
         l.out = parity ? const_cast<double*>(D.C) : D.C_out;
         const long long L = it - prefix[d];
         if (PM == 2 || (PM == 3 && D.split))
-            limit_block<PrecSplit>(l, L, wv, wpb);
+            limit_block<PrecSplit>(l, L, wv, wpb, lds);
         else
-            limit_block<PrecF64>(l, L, wv, wpb);
+            limit_block<PrecF64>(l, L, wv, wpb, lds);
     });
 }
 

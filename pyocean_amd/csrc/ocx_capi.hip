@@ -48,6 +48,9 @@ int check_shape(int nt, int nz, int ny, int nx) {
         return fail(OCX_E_ARG, "shape (nt=%d, nz=%d, ny=%d, nx=%d) must be positive", nt, nz, ny, nx);
     if ((long long)nt * nz > (1LL << 30) || ny > (1 << 24) || nx > (1 << 24))
         return fail(OCX_E_ARG, "shape (nt=%d, nz=%d, ny=%d, nx=%d) too large", nt, nz, ny, nx);
+    // the kernels address one (tracer, layer) plane with 32-bit byte offsets
+    if ((long long)(ny + 1) * (nx + 1) * 8 >= (1LL << 31))
+        return fail(OCX_E_ARG, "plane ny=%d x nx=%d too large (at most 2^28 cells per layer)", ny, nx);
     return OCX_OK;
 }
 
@@ -349,9 +352,8 @@ int advect_fct_impl(const double* C, const double* V0, const double* V1, const d
         l.dt = dt;
         l.flags = flags;
         const int wpb = tracer_group(nt);
-        const long long ntb = (nt + wpb - 1) / wpb;
-        const unsigned blocks = (unsigned)(ntb * l.nsx * l.ncy * nz);
-        k_fct_limit<PR><<<blocks, 64 * wpb, 0, s>>>(l);
+        const unsigned blocks = (unsigned)limit_blocks(l, wpb);
+        k_fct_limit<PR><<<blocks, 64 * wpb * LG, limit_lds_bytes(wpb), s>>>(l);
     }
     OCX_TRY(launch_check("fct limiter"));
     if (open_flux) {
@@ -773,7 +775,9 @@ int ocx_batch_create(const ocx_domain* domains, int ndomains, int rows_per_chunk
     batch_kernels(b->pm, kern);
     for (int q = 0; q < 4; ++q) {
         int n = groups_per_cu;
-        if (n <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern[q], 64 * wpb, 0) != hipSuccess) {
+        const int threads = q < 3 ? 64 * wpb : 64 * wpb * LG;
+        const size_t lds = q < 3 ? 0 : limit_lds_bytes(wpb);
+        if (n <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern[q], threads, lds) != hipSuccess) {
             batch_free(b);
             return fail(OCX_E_HIP, "occupancy query failed");
         }
@@ -818,7 +822,9 @@ int ocx_batch_step(ocx_batch* b, int parity, void* stream) {
         void* args[] = {&b->d_dom, const_cast<long long**>(&prefix), &b->ndom, const_cast<long long*>(&total),
                         &parity, &ctr};
         ProfScope _ps(names[q], s);
-        OCX_TRY(hip_check(hipLaunchKernel(kern[q], grid, block, args, 0, s), names[q]));
+        const dim3 blk = q < 3 ? block : dim3(64u * b->wpb * LG);
+        const size_t lds = q < 3 ? 0 : limit_lds_bytes(b->wpb);
+        OCX_TRY(hip_check(hipLaunchKernel(kern[q], grid, blk, args, lds, s), names[q]));
     }
     return launch_check("batch step");
 }
@@ -849,14 +855,26 @@ int ocx_batch_destroy(ocx_batch* b) {
     return OCX_OK;
 }
 
-int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* stream) {
+size_t ocx_content_workspace_bytes(int nt) {
+    return nt > 0 ? (size_t)nt * ocx::CONTENT_PARTS * 2 * sizeof(double) : 0;
+}
+
+int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* workspace,
+                    size_t workspace_bytes, void* stream) {
     using namespace ocx;
     OCX_TRY(check_shape(nt, nz, ny, nx));
     OCX_TRY(need(c, "c"));
     OCX_TRY(need(V, "V"));
     OCX_TRY(need(out, "out"));
+    const size_t wsb = ocx_content_workspace_bytes(nt);
+    if (!workspace || workspace_bytes < wsb)
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed, %zu given", wsb, workspace ? workspace_bytes : 0);
+    if (nt > 65535) return fail(OCX_E_ARG, "nt=%d: at most 65535 tracers per content call", nt);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    k_content<<<nt, 1024, 0, s>>>(c, V, out, (size_t)nz * ny * nx);
+    double* part = static_cast<double*>(workspace);
+    k_content_part<<<dim3(CONTENT_PARTS, nt), 256, 0, s>>>(c, V, part, (size_t)nz * ny * nx);
+    OCX_TRY(launch_check("content partials"));
+    k_content_fold<<<nt, CONTENT_PARTS, 0, s>>>(part, out);
     return launch_check("content");
 }
 

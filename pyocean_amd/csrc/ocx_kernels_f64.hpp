@@ -588,12 +588,12 @@ __device__ __forceinline__ void neumaier_add(double& s, double& comp, double x) 
     s = t;
 }
 
-__global__ __launch_bounds__(1024) void k_content(const double* __restrict__ c, const double* __restrict__ V,
-                                                  double* __restrict__ out, size_t cells) {
-    __shared__ double red[1024], rc[1024];
-    const int t = blockIdx.x;
-    double s = 0.0, comp = 0.0;
-    for (size_t x = threadIdx.x; x < cells; x += blockDim.x) neumaier_add(s, comp, V[x] * c[(size_t)t * cells + x]);
+// Content sums in two passes with a fixed partition, so the result does not depend on the launch: pass 1 gives
+// each of CONTENT_PARTS blocks per tracer one contiguous slice of the cells (compensated sum over its threads'
+// strided elements, then a fixed tree over the block); pass 2 folds a tracer's CONTENT_PARTS partials the same way.
+constexpr int CONTENT_PARTS = 256;
+
+__device__ __forceinline__ void block_fold(double& s, double& comp, double* red, double* rc) {
     red[threadIdx.x] = s;
     rc[threadIdx.x] = comp;
     __syncthreads();
@@ -606,7 +606,36 @@ __global__ __launch_bounds__(1024) void k_content(const double* __restrict__ c, 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[t] = red[0] + rc[0];
+    s = red[0];
+    comp = rc[0];
+}
+
+// grid (CONTENT_PARTS, nt), 256 threads: part[t][b] = (sum, compensation) of slice b of tracer t
+__global__ __launch_bounds__(256) void k_content_part(const double* __restrict__ c, const double* __restrict__ V,
+                                                      double* __restrict__ part, size_t cells) {
+    __shared__ double red[256], rc[256];
+    const int t = blockIdx.y, b = blockIdx.x;
+    const size_t per = (cells + CONTENT_PARTS - 1) / CONTENT_PARTS;
+    const size_t x0 = (size_t)b * per, x1 = x0 + per < cells ? x0 + per : cells;
+    const double* ct = c + (size_t)t * cells;
+    double s = 0.0, comp = 0.0;
+    for (size_t x = x0 + threadIdx.x; x < x1; x += blockDim.x) neumaier_add(s, comp, V[x] * ct[x]);
+    block_fold(s, comp, red, rc);
+    if (threadIdx.x == 0) {
+        part[((size_t)t * CONTENT_PARTS + b) * 2] = s;
+        part[((size_t)t * CONTENT_PARTS + b) * 2 + 1] = comp;
+    }
+}
+
+// grid nt, CONTENT_PARTS threads: out[t] = the folded partials of tracer t
+__global__ __launch_bounds__(CONTENT_PARTS) void k_content_fold(const double* __restrict__ part,
+                                                                double* __restrict__ out) {
+    __shared__ double red[CONTENT_PARTS], rc[CONTENT_PARTS];
+    const int t = blockIdx.x;
+    double s = part[((size_t)t * CONTENT_PARTS + threadIdx.x) * 2];
+    double comp = part[((size_t)t * CONTENT_PARTS + threadIdx.x) * 2 + 1];
+    block_fold(s, comp, red, rc);
+    if (threadIdx.x == 0) out[t] = s + comp;
 }
 
 // Periodic halo (core/model/periodic.py:71-95): gather of ``lead`` slabs of a (sy, sx) array into (dy, dx) with

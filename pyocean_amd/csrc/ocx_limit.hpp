@@ -43,21 +43,20 @@ struct LimitArgs {
     unsigned flags;
 };
 
-// Work decomposition: a workgroup holds the waves of up to MAXW tracers of one (layer, strip, chunk) (shared
-// per-layer fields from the CU's caches); logical blocks run layer-fastest, so the workgroups of layers k-1, k, k+1
-// of a strip run side by side on one XCD and the k +- 1 reads of C and c_L hit its L2.
-// Logical blocks of one limiter launch over a domain: tracer groups x layers x strips x chunks.
+// Work decomposition: a workgroup holds LG consecutive layers x (up to MAXW) tracers of one (strip, chunk); its
+// waves march the rows in step (one workgroup barrier per row).  Each wave publishes the bound contributions of its
+// newest row in LDS, so a wave reads those of layers k +- 1 from the waves above and below it and only the group's
+// bottom and top layers read their outer neighbour layer from memory; the waves of one layer (its tracers) read the
+// shared per-layer fields together, through the compute unit's cache.
+constexpr int LG = 4;      // layers per limiter workgroup
+
+// Logical blocks of one limiter launch over a domain: tracer groups x layer groups x strips x chunks.
 __host__ __device__ inline long long limit_blocks(const LimitArgs& a, int wpb) {
-    return (long long)((a.nt + wpb - 1) / wpb) * a.n * a.nsx * a.ncy;
+    return (long long)((a.nt + wpb - 1) / wpb) * ((a.n + LG - 1) / LG) * a.nsx * a.ncy;
 }
 
-// A load at a byte offset from a wave-uniform base: the address is the scalar base plus a 32-bit lane offset, so no
-// 64-bit address arithmetic runs on the vector ALUs.  Every offset passed here lies inside the array (rows and
-// columns outside the domain are clamped to row 0 / column 0 and their values are never used).
-template <class T>
-__device__ __forceinline__ T ldo(const T* base, uint32_t off) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
-}
+// LDS of one limiter workgroup: per wave, the bound contributions of three rows (a ring) for its 64 lanes.
+__host__ __device__ constexpr size_t limit_lds_bytes(int wpb) { return (size_t)wpb * LG * 3 * 64 * 2 * sizeof(double); }
 
 // Upper and lower bound contributions of one cell to the 7-point bounds of its neighbours: max and min of C and
 // c_L where the cell is valid, -inf / +inf (no contribution) elsewhere.
@@ -68,20 +67,26 @@ __device__ __forceinline__ Bnd bnd(bool v, double c, double l) {
     return v ? Bnd{fmax(c, l), fmin(c, l)} : Bnd{-INFINITY, INFINITY};
 }
 
-// One logical block L of the limiter (wave wv of wpb).
+// One logical block L of the limiter: wave wv of the workgroup's wpb x LG waves (tracer wv % wpb of the group,
+// layer wv / wpb of the layer group).  Every wave runs the whole row loop, also a wave whose tracer or layer lies
+// past the stack (it computes on clamped in-bounds data and stores nothing), so every wave meets every barrier.
+// lds: the workgroup's limit_lds_bytes(wpb) bytes of LDS.
 template <class PR>
-__device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int wv, int wpb) {
+__device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int wv, int wpb, Bnd* lds) {
     using FL = typename PR::fl;
     const int lane = threadIdx.x & 63;
     const int ntb = (a.nt + wpb - 1) / wpb;
+    const int nkg = (a.n + LG - 1) / LG;
     const int tg = (int)(L % ntb);
     L /= ntb;
-    const int k = (int)(L % a.n);
-    L /= a.n;
+    const int kg = (int)(L % nkg);
+    L /= nkg;
     const int sx = (int)(L % a.nsx);
     const int cy = (int)(L / a.nsx);
-    const int t = tg * wpb + wv;
-    if (t >= a.nt || cy >= a.ncy) return;         // whole wave leaves; no block barrier is used
+    const int tw = wv % wpb, kw = wv / wpb;
+    const int t_raw = tg * wpb + tw, k_raw = kg * LG + kw;
+    const bool active = t_raw < a.nt && k_raw < a.n;
+    const int t = min(t_raw, a.nt - 1), k = min(k_raw, a.n - 1);    // in-bounds planes for an idle wave
     const int p = t * a.n + k;
     const int ny = a.ny, nx = a.nx, n = a.n;
     const int i0 = sx * LS, i = i0 - 2 + lane;
@@ -89,11 +94,18 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
     const size_t plane = (size_t)ny * nx;
     const bool col_in = i >= 0 && i < nx;
     const bool xcol = i >= 0 && i <= nx;
-    const bool own = lane >= 2 && lane < 2 + LS && col_in;
+    const bool own = active && lane >= 2 && lane < 2 + LS && col_in;
     const double dt = a.dt;
     const unsigned flags = a.flags;
     const bool ow = flags & OPEN_W, oe = flags & OPEN_E, osb = flags & OPEN_S, onb = flags & OPEN_N;
     const bool has_dn = k > 0, has_up = k + 1 < n;
+    // layers k -+ 1 from the neighbouring waves of the workgroup, or (group edge) from memory
+    const bool dn_lds = kw > 0, up_lds = kw + 1 < LG && k_raw + 1 < n;
+    const bool dn_mem = has_dn && !dn_lds, up_mem = has_up && !up_lds;
+    Bnd* my = lds + (size_t)wv * 3 * 64;
+    const Bnd* below = lds + (size_t)(wv - wpb) * 3 * 64;           // used only when dn_lds
+    const Bnd* above = lds + (size_t)(wv + wpb) * 3 * 64;           // used only when up_lds
+    auto slot = [](int r) { return ((r % 3) + 3) % 3; };
     // wave-uniform plane bases
     const double* Cp = a.C + (size_t)p * plane;
     const double* cLp = a.cL + (size_t)p * plane;
@@ -121,6 +133,7 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
     const uint32_t rcf = (uint32_t)nx * (uint32_t)sizeof(FL), rxf = (uint32_t)(nx + 1) * (uint32_t)sizeof(FL);
     auto rowc = [&](int r) -> uint32_t { return (r >= 0 && r < ny) ? (uint32_t)r : 0u; };     // cell / x-face rows
     auto rowy = [&](int r) -> uint32_t { return (r >= 0 && r <= ny) ? (uint32_t)r : 0u; };    // y-face rows
+    auto row_in = [&](int r) { return r >= 0 && r < ny; };
 
     // Loads are issued one loop step before their use, so a wave never waits on a load right after issuing it.
     struct RingRaw {
@@ -131,27 +144,43 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
         const uint32_t rr = rowc(r);
         return RingRaw{ldo(Cp, o8 + rr * rc8), ldo(cLp, o8 + rr * rc8), ldo(okk, o1 + rr * (uint32_t)nx)};
     };
-    auto row_in = [&](int r) { return r >= 0 && r < ny; };
-    // ratio inputs of row r: A on the west face (r, i) and north face (r+1, i), V1, and the layers below / above
+    // ratio inputs of row r: A on the west face (r, i) and north face (r+1, i), V1, and (group edge) the layer
+    // below / above from memory
     struct RatRaw {
         FL aw, an;
-        double v1, cd, ld, cu, lu;
-        uint8_t od, ou;
+        double v1, cv, lv;
+        uint8_t ov;
     };
     auto ld_rat = [&](int r) {
         const uint32_t rr = rowc(r), ry = rowy(r + 1);
         const uint32_t xc = o8 + rr * rc8, xb = o1 + rr * (uint32_t)nx;
-        return RatRaw{ldo(axp, oxf + rr * rxf), ldo(ayp, of + ry * rcf), ldo(V1k, xc), ldo(Cd, xc), ldo(Ld, xc),
-                      ldo(Cu, xc), ldo(Lu, xc), ldo(okd, xb), ldo(oku, xb)};
+        RatRaw w{ldo(axp, oxf + rr * rxf), ldo(ayp, of + ry * rcf), ldo(V1k, xc), 0.0, 0.0, 0};
+        if (dn_mem) {
+            w.cv = ldo(Cd, xc);
+            w.lv = ldo(Ld, xc);
+            w.ov = ldo(okd, xb);
+        } else if (up_mem) {
+            w.cv = ldo(Cu, xc);
+            w.lv = ldo(Lu, xc);
+            w.ov = ldo(oku, xb);
+        }
+        return w;
     };
+    // a group-edge wave with both outer neighbours (LG = 1 would need it) cannot occur: LG > 1
+    static_assert(LG > 1, "a layer group has a bottom and a top wave");
     struct Rat {
         double aw, an, v1;
-        Bnd d, u;
+        Bnd mem;                 // the outer neighbour layer's contribution (group-edge waves)
     };
     auto use_rat = [&](const RatRaw& w, int r) {
         const bool in = col_in && row_in(r);
-        return Rat{(double)w.aw, (double)w.an, w.v1, bnd(in && has_dn && w.od != 0, w.cd, w.ld),
-                   bnd(in && has_up && w.ou != 0, w.cu, w.lu)};
+        return Rat{(double)w.aw, (double)w.an, w.v1, bnd(in && (dn_mem || up_mem) && w.ov != 0, w.cv, w.lv)};
+    };
+    // the layers below and above at row r: from LDS (published two or more barriers ago) or from memory
+    auto vert = [&](int r, const Rat& q, Bnd& d, Bnd& u) {
+        const int sl = slot(r);
+        d = dn_lds ? below[sl * 64 + lane] : (dn_mem ? q.mem : Bnd{-INFINITY, INFINITY});
+        u = up_lds ? above[sl * 64 + lane] : (up_mem ? q.mem : Bnd{-INFINITY, INFINITY});
     };
     // update inputs of row j: V0, Qx on the west face, Qy on the north face
     struct FinRaw {
@@ -163,16 +192,17 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
     };
 
     // R+- of row r (_fct_coeff_kernel, core/dynamics/sigma_baroclinic.py:407-431): bounds over the cell, i +- 1
-    // (neighbour lanes), r +- 1 (bS, bN) and k +- 1 (q.d, q.u), valid cells only; as = A on the south face
+    // (neighbour lanes), r +- 1 (bS, bN) and k +- 1 (d, u), valid cells only; as = A on the south face
     struct RR {
         double rin, rout;
     };
-    auto ratios = [&](const Bnd& b, const Bnd& bS, const Bnd& bN, const Rat& q, double as, double lc, bool oc) -> RR {
+    auto ratios = [&](const Bnd& b, const Bnd& bS, const Bnd& bN, const Bnd& d, const Bnd& u, const Rat& q,
+                      double as, double lc, bool oc) -> RR {
         const double hw = shfl_up1(b.hi), he = shfl_down1(b.hi);
         const double lw = shfl_up1(b.lo), le = shfl_down1(b.lo);
         const double ae = shfl_down1(q.aw);
-        const double cmax = fmax(fmax(fmax(b.hi, hw), fmax(he, bS.hi)), fmax(fmax(bN.hi, q.d.hi), q.u.hi));
-        const double cmin = fmin(fmin(fmin(b.lo, lw), fmin(le, bS.lo)), fmin(fmin(bN.lo, q.d.lo), q.u.lo));
+        const double cmax = fmax(fmax(fmax(b.hi, hw), fmax(he, bS.hi)), fmax(fmax(bN.hi, d.hi), u.hi));
+        const double cmin = fmin(fmin(fmin(b.lo, lw), fmin(le, bS.lo)), fmin(fmin(bN.lo, d.lo), u.lo));
         const double p_in = dt * (dmax(q.aw, 0.0) - dmin(ae, 0.0) + dmax(as, 0.0) - dmin(q.an, 0.0));
         const double p_out = dt * (dmax(ae, 0.0) - dmin(q.aw, 0.0) + dmax(q.an, 0.0) - dmin(as, 0.0));
         const double q_in = dmax(q.v1 * (cmax - lc), 0.0);
@@ -193,7 +223,8 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
         orr[m] = col_in && row_in(r) && w.o != 0;
         br[m] = bnd(orr[m], w.c, w.l);
     };
-    // rows jb-2 and jb-1 for R(jb-1), R(jb)
+    auto publish = [&](int r, const Bnd& b) { my[slot(r) * 64 + lane] = b; };
+    // rows jb-2 .. jb+1; rows jb-1, jb, jb+1 published for the layers above and below
     const RingRaw rm2 = ld_ring(jb - 2), rm1 = ld_ring(jb - 1), r0 = ld_ring(jb), r1 = ld_ring(jb + 1);
     const Rat qm1 = use_rat(ld_rat(jb - 1), jb - 1);
     const Rat q0 = use_rat(ld_rat(jb), jb);
@@ -202,10 +233,18 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
     ring_in(rm2, jb - 2, 0);
     ring_in(rm1, jb - 1, 1);
     ring_in(r0, jb, 2);
-    const RR Rm = ratios(br[1], br[0], br[2], qm1, as_m1, lr[1], orr[1]);
+    publish(jb - 1, br[1]);
+    publish(jb, br[2]);
+    const Bnd b_p1 = bnd(col_in && row_in(jb + 1) && r1.o != 0, r1.c, r1.l);
+    publish(jb + 1, b_p1);
+    __syncthreads();
+    Bnd vd, vu;
+    vert(jb - 1, qm1, vd, vu);
+    const RR Rm = ratios(br[1], br[0], br[2], vd, vu, qm1, as_m1, lr[1], orr[1]);
     const double cm1 = cr[1];                                           // C(jb-1)
     ring_in(r1, jb + 1, 0);                                             // rows jb .. jb+1 in slots 2, 0
-    RR Rc = ratios(br[2], br[1], br[0], q0, qm1.an, lr[2], orr[2]);
+    vert(jb, q0, vd, vu);
+    RR Rc = ratios(br[2], br[1], br[0], vd, vu, q0, qm1.an, lr[2], orr[2]);
     // reorder the ring to rows jb, jb+1 in slots 0, 1 and load row jb+2 into slot 2
     {
         const double c0 = cr[2], l0 = lr[2];
@@ -221,6 +260,9 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
         br[0] = b0;
         ring_in(ld_ring(jb + 2), jb + 2, 2);
     }
+    __syncthreads();                                                    // row jb-1's slot is read by everyone
+    publish(jb + 2, br[2]);
+    __syncthreads();
     FinRaw fc = ld_fin(jb);
     double aw0 = q0.aw, an0 = q0.an, v1_0 = q0.v1;                     // row jb: A west / north, V1
     // G on the south face of row jb (_fct_final :514-530 with the face coefficient of :436-447)
@@ -238,8 +280,9 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
         const RingRaw rn = ld_ring(j + 3);
         const RatRaw qn = ld_rat(j + 2);
         const FinRaw fn = ld_fin(j + 1);
-        // ratios of row j+1 (ring rows j .. j+2)
-        const RR Rn = ratios(br[1], br[0], br[2], q1, an0, lr[1], orr[1]);
+        // ratios of row j+1 (ring rows j .. j+2; layers k -+ 1 of row j+1)
+        vert(j + 1, q1, vd, vu);
+        const RR Rn = ratios(br[1], br[0], br[2], vd, vu, q1, an0, lr[1], orr[1]);
         // finish row j
         const double rin_w = shfl_up1(Rc.rin), rout_w = shfl_up1(Rc.rout);
         const double c0 = cr[0];
@@ -258,7 +301,7 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
             const double d = Ge - Gw + Gn - Gs;
             const double q = fc.v0 * c0 - dt * d;
             const double v1 = v1_0;
-            outp[(size_t)j * nx + i] = v1 > 0 ? q / v1 : c0;
+            sto(outp, o8 + (uint32_t)j * rc8, v1 > 0 ? q / v1 : c0);
             if (a.fxi) {
                 if (a.flux_face_i == i) a.fxi[(size_t)p * ny + j] = Gw;
                 if (a.flux_face_i == nx && i == nx - 1) a.fxi[(size_t)p * ny + j] = Ge;
@@ -288,12 +331,17 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
         orr[1] = orr[2];
         br[1] = br[2];
         ring_in(rn, j + 3, 2);
+        publish(j + 3, br[2]);                 // row j's slot: every wave read it in the previous step
+        __syncthreads();
     }
 }
 
+// blockDim.x = 64 * wpb * LG; dynamic LDS limit_lds_bytes(wpb)
 template <class PR>
-__global__ __launch_bounds__(64 * MAXW) void k_fct_limit(LimitArgs a) {
-    limit_block<PR>(a, xcd_logical(blockIdx.x, gridDim.x), wave_id(), blockDim.x >> 6);
+__global__ __launch_bounds__(64 * MAXW * LG) void k_fct_limit(LimitArgs a) {
+    extern __shared__ double limit_lds[];
+    limit_block<PR>(a, xcd_logical(blockIdx.x, gridDim.x), wave_id(), (blockDim.x >> 6) / LG,
+                    reinterpret_cast<Bnd*>(limit_lds));
 }
 
 }  // namespace ocx

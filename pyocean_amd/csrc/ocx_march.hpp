@@ -62,6 +62,33 @@ __device__ __forceinline__ double shfl_up1(double v) {
     return __hiloint2double(dpp_up1(__double2hiint(v)), dpp_up1(__double2loint(v)));
 }
 
+// The stage value c = q / V (V > 0) as the precision stores it: FP64 keeps the IEEE quotient; the split form, which
+// stores c in FP32, multiplies by the reciprocal refined once by Newton-Raphson (relative error near 2^-50 before
+// the rounding to FP32, so the stored value is the FP32 rounding of q / V up to one ulp in rare ties).
+template <class PR>
+__device__ __forceinline__ typename PR::st stage_value(double q, double V, double c0) {
+    if (!(V > 0)) return (typename PR::st)c0;
+    if constexpr (PR::split) {
+        double r = __builtin_amdgcn_rcp(V);
+        r = __fma_rn(r, __fma_rn(-V, r, 1.0), r);
+        return (float)(q * r);
+    } else {
+        return q / V;
+    }
+}
+
+// A load or store at a byte offset from a wave-uniform base: the address is the scalar base plus a 32-bit lane
+// offset, so no 64-bit address arithmetic runs on the vector ALUs.  Every offset passed here lies inside the array
+// (rows and columns outside the plane are clamped to row 0 / column 0 and their values are never used).
+template <class T>
+__device__ __forceinline__ T ldo(const T* base, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void sto(T* base, uint32_t off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
 // Precision of a step.  F64: everything in FP64, bit-identical to the reference.  Split: stage values c1, c2 and
 // the face fluxes F0, F0 + F1, A stored in FP32 and the WENO reconstruction (smoothness indicators, weights, face
 // values, fluxes) computed in FP32; the h*T accumulation q = a q0 + b (q' - dt div F), c_L, the FCT ratios, the
@@ -207,6 +234,7 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
     FL* syp = static_cast<FL*>(a.s_y) + (size_t)p * (ny + 1) * nx;
     FL* axp = static_cast<FL*>(a.ax) + (size_t)p * ny * (nx + 1);
     FL* ayp = static_cast<FL*>(a.ay) + (size_t)p * (ny + 1) * nx;
+    ST* coutp = static_cast<ST*>(a.cout) + (size_t)p * plane;     // stages 1 and 2
     const bool ow = a.flags & OPEN_W, oe = a.flags & OPEN_E, os = a.flags & OPEN_S, on = a.flags & OPEN_N;
     const bool col_in = i < nx;                     // lane's column is a cell
     const bool own = lane < MS && col_in;           // lane's cell belongs to this strip
@@ -216,8 +244,18 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
     const int hcol = lane < 3 ? i0 - 3 + lane : i0 + 61 + lane;
     const int hidx = lane < 3 ? lane : 64 + lane;
     const bool hin = lane < 5 && hcol >= 0 && hcol < nx;
+    // byte offsets of the lane's column inside a row (column 0 for lanes outside it); rows are clamped on the scalar
+    // unit, so each load costs one 32-bit add on the vector ALUs
+    const uint32_t ic = col_in ? (uint32_t)i : 0u, ix = xface ? (uint32_t)i : 0u, ih = hin ? (uint32_t)hcol : 0u;
+    constexpr uint32_t sIN = sizeof(IN), sFL = sizeof(FL);
+    const uint32_t nxu = (uint32_t)nx, nx1 = (uint32_t)nx + 1u;
+    const uint32_t c8 = ic * 8u, cin_b = ic * sIN, x8 = ix * 8u, xfl = ix * sFL, cfl = ic * sFL, h_in = ih * sIN;
+    // lane 0 of a stage-3 wave also reads C west of the strip (the donor flux of the strip's first x-face)
+    const uint32_t cw8 = (STAGE == 3 && lane == 0 && i0 > 0) ? (uint32_t)(i0 - 1) * 8u : c8;
+    auto rowc = [&](int r) -> uint32_t { return (r >= 0 && r < ny) ? (uint32_t)r : 0u; };    // cell / x-face rows
+    auto rowy = [&](int r) -> uint32_t { return (r >= 0 && r <= ny) ? (uint32_t)r : 0u; };   // y-face rows
 
-    // Loads are split into an issue half (unconditional, from an address that is always inside the array) and a
+    // Loads are split into an issue half (unconditional, from an offset that is always inside the array) and a
     // use half (mask and convert); the use half of a prefetched row runs one loop step later, so the wave never
     // waits on a load right after issuing it.
     struct Raw {
@@ -227,13 +265,13 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
     };
     auto ld = [&](int j) {
         const bool in = col_in && j >= 0 && j < ny;
-        const size_t x = in ? (size_t)j * nx + i : 0;
-        return Raw{cin[x], okp[x], in};
+        const uint32_t r = rowc(j);
+        return Raw{ldo(cin, cin_b + r * nxu * sIN), ldo(okp, ic + r * nxu), in};
     };
     auto ldh = [&](int j) {
         const bool in = hin && j >= 0 && j < ny;
-        const size_t x = in ? (size_t)j * nx + hcol : 0;
-        return Raw{cin[x], okp[x], in};
+        const uint32_t r = rowc(j);
+        return Raw{ldo(cin, h_in + r * nxu * sIN), ldo(okp, ih + r * nxu), in};
     };
     auto use = [&](const Raw& r, W& v, bool& o) {
         v = r.in ? (W)r.v : (W)0;
@@ -255,23 +293,22 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
         const bool row = j < je;
         r.xf = row && xface;
         r.ci = row && col_in;
-        const size_t xc = r.ci ? (size_t)j * nx + i : 0;
-        const size_t xf = r.xf ? (size_t)j * (nx + 1) + i : 0;
-        const size_t yf = r.ci ? (size_t)(j + 1) * nx + i : 0;
-        r.qx = Qxk[xf];
-        r.qy = Qyk[yf];
-        r.c = Cp[xc];
-        r.v0 = V0k[xc];
-        r.v1 = V1k[xc];
+        const uint32_t rc = rowc(j), ry = rowy(j + 1);
+        const uint32_t oc = c8 + rc * nxu * 8u;
+        r.qx = ldo(Qxk, x8 + rc * nx1 * 8u);
+        r.qy = ldo(Qyk, c8 + ry * nxu * 8u);
+        r.c = ldo(Cp, oc);
+        r.v0 = ldo(V0k, oc);
+        r.v1 = ldo(V1k, oc);
         if (STAGE >= 2) {
             const FL* gx = STAGE == 2 ? fxp : sxp;
             const FL* gy = STAGE == 2 ? fyp : syp;
-            r.fw = gx[xf];
-            r.fn = gy[yf];
+            r.fw = ldo(gx, xfl + rc * nx1 * sFL);
+            r.fn = ldo(gy, cfl + ry * nxu * sFL);
         } else {
             r.fw = r.fn = (FL)0;
         }
-        r.cl = Cp[(STAGE == 3 && lane == 0 && i0 > 0 && r.ci) ? xc - 1 : 0];
+        r.cl = STAGE == 3 ? ldo(Cp, cw8 + rc * nxu * 8u) : 0.0;
         return r;
     };
     auto mask_row = [&](RowIn r) {
@@ -314,7 +351,7 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
     if (STAGE == 3) {
         if (col_in) {
             fhs = PR::split ? (double)((float)f0s * (1.0f / 6.0f) + 2.0f * (float)fys * (1.0f / 3.0f))
-                            : f0s / 6.0 + 2.0 * fys / 3.0;
+                            : div_const(f0s, 6.0, 1.0 / 6.0) + div_const(2.0 * fys, 3.0, 1.0 / 3.0);
             const double q = Qyk[(size_t)jb * nx + i];
             const double clo = jb > 0 ? Cp[(size_t)(jb - 1) * nx + i] : 0.0;
             fls = donor(q, jb, ny, clo, cur.c, os, on, ext0 + i, ext0 + (size_t)(ny - 1) * nx + i);
@@ -355,24 +392,23 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
                                                ex((size_t)(ny - 1) * nx + i, w[3]));
         const double fxe = shfl_down1(fxw);
         const bool yown = (j + 1 < je) || (j + 1 == ny);
-        const size_t xc = (size_t)j * nx + i;
 
         if (STAGE == 1) {
-            if (xown) fxp[(size_t)j * (nx + 1) + i] = (FL)fxw;
-            if (col_in && yown) fyp[(size_t)(j + 1) * nx + i] = (FL)fyn;
+            if (xown) sto(fxp, xfl + (uint32_t)j * nx1 * sFL, (FL)fxw);
+            if (col_in && yown) sto(fyp, cfl + (uint32_t)(j + 1) * nxu * sFL, (FL)fyn);
             if (own) {
                 const double d = fxe - fxw + fyn - fys;
                 const double c0 = cur.c, v0 = cur.v0, v1 = cur.v1;
                 const double q0 = v0 * c0, dV = v1 - v0;
                 const double qn = 0.0 * q0 + 1.0 * (q0 - a.dt * d);
                 const double V = v0 + dV;
-                static_cast<ST*>(a.cout)[(size_t)p * plane + xc] = (ST)(V > 0 ? qn / V : c0);
+                sto(coutp, ic * (uint32_t)sizeof(ST) + (uint32_t)j * nxu * (uint32_t)sizeof(ST), stage_value<PR>(qn, V, c0));
             }
         } else if (STAGE == 2) {
             const double f0w = (double)cur.fw, f0n = (double)cur.fn;
             const double f0e = shfl_down1(f0w);
-            if (xown) sxp[(size_t)j * (nx + 1) + i] = (FL)(f0w + fxw);
-            if (col_in && yown) syp[(size_t)(j + 1) * nx + i] = (FL)(f0n + fyn);
+            if (xown) sto(sxp, xfl + (uint32_t)j * nx1 * sFL, (FL)(f0w + fxw));
+            if (col_in && yown) sto(syp, cfl + (uint32_t)(j + 1) * nxu * sFL, (FL)(f0n + fyn));
             if (own) {
                 const double c0 = cur.c, v0 = cur.v0, v1 = cur.v1;
                 const double q0 = v0 * c0, dV = v1 - v0;
@@ -381,7 +417,7 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
                 const double d1 = fxe - fxw + fyn - fys;
                 const double qn = 0.75 * q0 + 0.25 * (q1 - a.dt * d1);
                 const double V = v0 + 0.5 * dV;
-                static_cast<ST*>(a.cout)[(size_t)p * plane + xc] = (ST)(V > 0 ? qn / V : c0);
+                sto(coutp, ic * (uint32_t)sizeof(ST) + (uint32_t)j * nxu * (uint32_t)sizeof(ST), stage_value<PR>(qn, V, c0));
             }
             f0s = f0n;
         } else {
@@ -391,8 +427,9 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
                 fhw = (double)((float)cur.fw * (1.0f / 6.0f) + 2.0f * (float)fxw * (1.0f / 3.0f));
                 fhn = (double)((float)cur.fn * (1.0f / 6.0f) + 2.0f * (float)fyn * (1.0f / 3.0f));
             } else {
-                fhw = (double)cur.fw / 6.0 + 2.0 * fxw / 3.0;
-                fhn = (double)cur.fn / 6.0 + 2.0 * fyn / 3.0;
+                // (F0 + F1)/6 + 2 F2/3 with correctly rounded divisions by the constants (div_const)
+                fhw = div_const((double)cur.fw, 6.0, 1.0 / 6.0) + div_const(2.0 * fxw, 3.0, 1.0 / 3.0);
+                fhn = div_const((double)cur.fn, 6.0, 1.0 / 6.0) + div_const(2.0 * fyn, 3.0, 1.0 / 3.0);
             }
             const double fhe = shfl_down1(fhw);
             // donor fluxes: x-face i between C[j, i-1] and C[j, i]; y-face j+1 between C[j, i] and C[j+1, i]
@@ -400,7 +437,7 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
             double cCl = shfl_up1(cC);
             if (lane == 0) cCl = cur.cl;
             const double cCn = (j + 1 < je) ? (nxt_raw.ci ? nxt_raw.c : 0.0)
-                                             : ((col_in && j + 1 < ny) ? Cp[xc + nx] : 0.0);
+                                             : ((col_in && j + 1 < ny) ? ldo(Cp, c8 + rowc(j + 1) * nxu * 8u) : 0.0);
             double flw = 0.0, fln = 0.0;
             if (xface) {
                 const double chi = col_in ? cC : 0.0;
@@ -411,8 +448,8 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
             if (UNLIM) {
                 if (a.fxi && xown && i == a.flux_face_i) a.fxi[(size_t)p * ny + j] = fhw;
             } else {
-                if (xown) axp[(size_t)j * (nx + 1) + i] = (FL)(fhw - flw);
-                if (col_in && yown) ayp[(size_t)(j + 1) * nx + i] = (FL)(fhn - fln);
+                if (xown) sto(axp, xfl + (uint32_t)j * nx1 * sFL, (FL)(fhw - flw));
+                if (col_in && yown) sto(ayp, cfl + (uint32_t)(j + 1) * nxu * sFL, (FL)(fhn - fln));
             }
             if (own) {
                 const double c0 = cC, v0 = cur.v0, v1 = cur.v1;
@@ -425,7 +462,7 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
                     const double d = fle - flw + fln - fls;
                     qn = 0.0 * q0 + 1.0 * (q0 - a.dt * d);
                 }
-                static_cast<double*>(a.cout)[(size_t)p * plane + xc] = v1 > 0 ? qn / v1 : c0;
+                sto(static_cast<double*>(a.cout) + (size_t)p * plane, c8 + (uint32_t)j * nxu * 8u, v1 > 0 ? qn / v1 : c0);
             }
             fhs = fhn;
             fls = fln;

@@ -63,6 +63,7 @@ class Workspace:
 
 
 _WS = Workspace()
+_WS_CONTENT = Workspace()
 
 
 def flags_of(open_mask: int = 0, js: bool = False, unlimited: bool = False) -> int:
@@ -231,5 +232,8 @@ def content(c, V):
     _need(c, (nt, n, ny, nx), "c")
     _need(V, (n, ny, nx), "V")
     out = torch.empty((nt,), dtype=_F64, device=c.device)
-    check(lib().ocx_content_f64(_ptr(c), _ptr(V), nt, n, ny, nx, _ptr(out), _stream()), "ocx_content_f64")
+    L = lib()
+    ws = _WS_CONTENT.get(int(L.ocx_content_workspace_bytes(nt)))
+    check(L.ocx_content_f64(_ptr(c), _ptr(V), nt, n, ny, nx, _ptr(out), _ptr(ws), ws.numel(), _stream()),
+          "ocx_content_f64")
     return out
