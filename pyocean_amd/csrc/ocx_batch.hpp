@@ -134,7 +134,7 @@ __global__ __launch_bounds__(64 * MAXW) void k_march_batch(const BatchDomain* __
 
 // blockDim.x = 64 * wpb * LG; dynamic LDS limit_lds_bytes(wpb)
 template <int PM>
-__global__ __launch_bounds__(64 * MAXW * LG) void k_limit_batch(const BatchDomain* __restrict__ dom,
+__global__ __launch_bounds__(64 * LMAXW * LG) void k_limit_batch(const BatchDomain* __restrict__ dom,
                                                                 const long long* __restrict__ prefix, int ndom,
                                                                 long long total, int parity,
                                                                 unsigned long long* __restrict__ ctr) {

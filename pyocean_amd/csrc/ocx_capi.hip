@@ -130,6 +130,13 @@ int tracer_group(int nt, int cap = 0) {
     return (nt + groups - 1) / groups;
 }
 
+// tracers per limiter workgroup: the same grouping, capped at LMAXW (its workgroups also span LG layers)
+int limit_group(int nt, int cap = 0) {
+    if (cap <= 0) cap = g_wpb.load();
+    if (cap <= 0) cap = DEFAULT_WPB;
+    return tracer_group(nt, cap < ocx::LMAXW ? cap : ocx::LMAXW);
+}
+
 template <int STAGE, bool UNLIM, class PR>
 void launch_march(const ocx::MarchArgs& a, bool js, hipStream_t s) {
     const int wpb = tracer_group(a.nt);
@@ -351,7 +358,7 @@ int advect_fct_impl(const double* C, const double* V0, const double* V1, const d
         l.ncy = (ny + rch - 1) / rch;
         l.dt = dt;
         l.flags = flags;
-        const int wpb = tracer_group(nt);
+        const int wpb = limit_group(nt);
         const unsigned blocks = (unsigned)limit_blocks(l, wpb);
         k_fct_limit<PR><<<blocks, 64 * wpb * LG, limit_lds_bytes(wpb), s>>>(l);
     }
@@ -612,7 +619,7 @@ int ocx_profile_collect(int max_kernels, const char** names, double* ms_total, l
 }  // extern "C"
 
 struct ocx_batch {
-    int ndom = 0, wpb = 0, rch = 0, ncu = 0;
+    int ndom = 0, wpb = 0, wpl = 0, rch = 0, ncu = 0;   // wpl: tracers per limiter workgroup
     int pm = 0;            // member precisions: 1 FP64 only, 2 split only, 3 both
     int gpc[4] = {};       // resident workgroups per compute unit of stage 1, 2, 3 and the limiter
     ocx::BatchDomain* d_dom = nullptr;
@@ -678,7 +685,7 @@ int ocx_batch_create(const ocx_domain* domains, int ndomains, int rows_per_chunk
     const int rch = rows_per_chunk > 0 ? rows_per_chunk : DEFAULT_RCH;
     int maxnt = 1;
     for (int d = 0; d < ndomains; ++d) maxnt = std::max(maxnt, domains[d].nt);
-    const int wpb = tracer_group(maxnt, waves_per_group);
+    const int wpb = tracer_group(maxnt, waves_per_group), wpl = limit_group(maxnt, waves_per_group);
     std::vector<BatchDomain> h(ndomains);
     std::vector<long long> pm(ndomains + 1, 0), pl(ndomains + 1, 0);
     for (int d = 0; d < ndomains; ++d) {
@@ -754,11 +761,12 @@ int ocx_batch_create(const ocx_domain* domains, int ndomains, int rows_per_chunk
         B.split = split;
         B.js = (D.flags & WENO_JS) != 0;
         pm[d + 1] = pm[d] + march_blocks(a, wpb);
-        pl[d + 1] = pl[d] + limit_blocks(l, wpb);
+        pl[d + 1] = pl[d] + limit_blocks(l, wpl);
     }
     ocx_batch* b = new ocx_batch();
     b->ndom = ndomains;
     b->wpb = wpb;
+    b->wpl = wpl;
     b->rch = rch;
     b->tm = pm[ndomains];
     b->tl = pl[ndomains];
@@ -775,8 +783,8 @@ int ocx_batch_create(const ocx_domain* domains, int ndomains, int rows_per_chunk
     batch_kernels(b->pm, kern);
     for (int q = 0; q < 4; ++q) {
         int n = groups_per_cu;
-        const int threads = q < 3 ? 64 * wpb : 64 * wpb * LG;
-        const size_t lds = q < 3 ? 0 : limit_lds_bytes(wpb);
+        const int threads = q < 3 ? 64 * wpb : 64 * wpl * LG;
+        const size_t lds = q < 3 ? 0 : limit_lds_bytes(wpl);
         if (n <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern[q], threads, lds) != hipSuccess) {
             batch_free(b);
             return fail(OCX_E_HIP, "occupancy query failed");
@@ -822,8 +830,8 @@ int ocx_batch_step(ocx_batch* b, int parity, void* stream) {
         void* args[] = {&b->d_dom, const_cast<long long**>(&prefix), &b->ndom, const_cast<long long*>(&total),
                         &parity, &ctr};
         ProfScope _ps(names[q], s);
-        const dim3 blk = q < 3 ? block : dim3(64u * b->wpb * LG);
-        const size_t lds = q < 3 ? 0 : limit_lds_bytes(b->wpb);
+        const dim3 blk = q < 3 ? block : dim3(64u * b->wpl * LG);
+        const size_t lds = q < 3 ? 0 : limit_lds_bytes(b->wpl);
         OCX_TRY(hip_check(hipLaunchKernel(kern[q], grid, blk, args, lds, s), names[q]));
     }
     return launch_check("batch step");

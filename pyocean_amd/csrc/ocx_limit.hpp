@@ -49,6 +49,7 @@ struct LimitArgs {
 // bottom and top layers read their outer neighbour layer from memory; the waves of one layer (its tracers) read the
 // shared per-layer fields together, through the compute unit's cache.
 constexpr int LG = 4;      // layers per limiter workgroup
+constexpr int LMAXW = 4;   // tracers per limiter workgroup at most (LMAXW x LG waves <= 1024 threads)
 
 // Logical blocks of one limiter launch over a domain: tracer groups x layer groups x strips x chunks.
 __host__ __device__ inline long long limit_blocks(const LimitArgs& a, int wpb) {
@@ -338,7 +339,7 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
 
 // blockDim.x = 64 * wpb * LG; dynamic LDS limit_lds_bytes(wpb)
 template <class PR>
-__global__ __launch_bounds__(64 * MAXW * LG) void k_fct_limit(LimitArgs a) {
+__global__ __launch_bounds__(64 * LMAXW * LG) void k_fct_limit(LimitArgs a) {
     extern __shared__ double limit_lds[];
     limit_block<PR>(a, xcd_logical(blockIdx.x, gridDim.x), wave_id(), (blockDim.x >> 6) / LG,
                     reinterpret_cast<Bnd*>(limit_lds));

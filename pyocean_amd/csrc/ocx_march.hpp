@@ -108,49 +108,63 @@ struct PrecSplit {
 
 __device__ __forceinline__ float sqf(float x) { return x * x; }
 
-// FP32 WENO5 value (same formulas; hardware reciprocals, contraction allowed).  WENO-Z's eps = 1e-40 is below the
-// FP32 normal range, so eps = 1e-30 (1 + m^2) with m the stencil's largest magnitude: it only keeps a vanishing
-// indicator from dividing by zero, and tau/(eps + beta) stays below 4e30 whatever the data.
+// FP32 WENO5 value from the upwind-ordered stencil, written on the first differences e1..e4 of v1..v5 (the same
+// scheme up to rounding, with less cancellation and fewer operations):
+//   candidates q_k = v3 + p_k / 6, p0 = 5 e2 - 2 e1, p1 = e2 + 2 e3, p2 = 4 e3 - e4;
+//   4 beta_k = (13/3) s_k^2 + r_k^2 with second differences s_k and r0 = s0 + 2 e2, r1 = e2 + e3, r2 = s2 - 2 e3;
+//   value = v3 + (sum alpha_k p_k) / (6 sum alpha_k).
+// The weights depend on beta only through ratios, so 4 beta, 4 tau and 4 eps (Z) or (4e-6 + 4 beta)^2 (JS) give the
+// reference's weights.  WENO-Z's eps = 1e-40 is below the FP32 normal range, so eps = 1e-30 (1 + m^2) with m the
+// stencil's largest magnitude: it only keeps a vanishing indicator from dividing by zero, and tau/(eps + beta) stays
+// below about 1e31 whatever the data.  Hardware reciprocals, contraction allowed.
+struct Diff5 {
+    float e1, e2, e3, e4;
+};
+__device__ __forceinline__ Diff5 diff5(float v1, float v2, float v3, float v4, float v5) {
+    return Diff5{v2 - v1, v3 - v2, v4 - v3, v5 - v4};
+}
+
 template <bool Z>
-__device__ __forceinline__ float weno5f(float v1, float v2, float v3, float v4, float v5) {
+__device__ __forceinline__ float weno5f(float v1, float v2, float v3, float v4, float v5, const Diff5& d) {
 #pragma clang fp contract(fast)
-    const float r6 = 1.0f / 6.0f;
-    const float q0 = (2.0f * v1 - 7.0f * v2 + 11.0f * v3) * r6;
-    const float q1 = (-v2 + 5.0f * v3 + 2.0f * v4) * r6;
-    const float q2 = (2.0f * v3 + 5.0f * v4 - v5) * r6;
-    const float c1312 = 13.0f / 12.0f;
-    const float b0 = c1312 * sqf(v1 - 2.0f * v2 + v3) + 0.25f * sqf(v1 - 4.0f * v2 + 3.0f * v3);
-    const float b1 = c1312 * sqf(v2 - 2.0f * v3 + v4) + 0.25f * sqf(v2 - v4);
-    const float b2 = c1312 * sqf(v3 - 2.0f * v4 + v5) + 0.25f * sqf(3.0f * v3 - 4.0f * v4 + v5);
+    const float s0 = d.e2 - d.e1, s1 = d.e3 - d.e2, s2 = d.e4 - d.e3;
+    const float r0 = s0 + 2.0f * d.e2, r1 = d.e2 + d.e3, r2 = s2 - 2.0f * d.e3;
+    constexpr float c133 = 13.0f / 3.0f;
+    const float b0 = c133 * s0 * s0 + r0 * r0;
+    const float b1 = c133 * s1 * s1 + r1 * r1;
+    const float b2 = c133 * s2 * s2 + r2 * r2;
+    const float p0 = 5.0f * d.e2 - 2.0f * d.e1, p1 = d.e2 + 2.0f * d.e3, p2 = 4.0f * d.e3 - d.e4;
     float a0, a1, a2;
     if (Z) {
         const float m = fmaxf(fmaxf(fmaxf(fabsf(v1), fabsf(v2)), fmaxf(fabsf(v3), fabsf(v4))), fabsf(v5));
-        const float eps = 1e-30f * (1.0f + m * m);
+        const float eps = 4e-30f * (1.0f + m * m);
         const float tau = fabsf(b2 - b0);
         a0 = 0.1f + 0.1f * tau * __builtin_amdgcn_rcpf(eps + b0);
         a1 = 0.6f + 0.6f * tau * __builtin_amdgcn_rcpf(eps + b1);
         a2 = 0.3f + 0.3f * tau * __builtin_amdgcn_rcpf(eps + b2);
     } else {
-        a0 = 0.1f * __builtin_amdgcn_rcpf(sqf(1e-6f + b0));
-        a1 = 0.6f * __builtin_amdgcn_rcpf(sqf(1e-6f + b1));
-        a2 = 0.3f * __builtin_amdgcn_rcpf(sqf(1e-6f + b2));
+        a0 = 0.1f * __builtin_amdgcn_rcpf(sqf(4e-6f + b0));
+        a1 = 0.6f * __builtin_amdgcn_rcpf(sqf(4e-6f + b1));
+        a2 = 0.3f * __builtin_amdgcn_rcpf(sqf(4e-6f + b2));
     }
-    return (a0 * q0 + a1 * q1 + a2 * q2) * __builtin_amdgcn_rcpf(a0 + a1 + a2);
+    return v3 + (a0 * p0 + a1 * p1 + a2 * p2) * (__builtin_amdgcn_rcpf(a0 + a1 + a2) * (1.0f / 6.0f));
 }
 
-__device__ __forceinline__ float weno3f(float v2, float v3, float v4) {
+// WENO3 from v2..v4: candidates (-v2 + 3 v3)/2 = v3 + e2/2 and (v3 + v4)/2 = v3 + e3/2, d = (1/3, 2/3),
+// eps = 1e-6; value = v3 + (w0 e2 + w1 e3) / 2.
+__device__ __forceinline__ float weno3f(float v3, const Diff5& d) {
 #pragma clang fp contract(fast)
-    const float c0 = (-v2 + 3.0f * v3) * 0.5f, c1 = (v3 + v4) * 0.5f;
-    const float e0 = (1.0f / 3.0f) * __builtin_amdgcn_rcpf(sqf(1e-6f + sqf(v3 - v2)));
-    const float e1 = (2.0f / 3.0f) * __builtin_amdgcn_rcpf(sqf(1e-6f + sqf(v4 - v3)));
-    return (e0 * c0 + e1 * c1) * __builtin_amdgcn_rcpf(e0 + e1);
+    const float g0 = (1.0f / 3.0f) * __builtin_amdgcn_rcpf(sqf(1e-6f + d.e2 * d.e2));
+    const float g1 = (2.0f / 3.0f) * __builtin_amdgcn_rcpf(sqf(1e-6f + d.e3 * d.e3));
+    return v3 + (g0 * d.e2 + g1 * d.e3) * (0.5f * __builtin_amdgcn_rcpf(g0 + g1));
 }
 
 template <bool Z>
 __device__ __forceinline__ float face_value(float v1, float v2, float v3, float v4, float v5, bool k1, bool k2,
                                             bool k3, bool k4, bool k5) {
-    if (k1 && k2 && k3 && k4 && k5) return weno5f<Z>(v1, v2, v3, v4, v5);
-    if (k2 && k3 && k4) return weno3f(v2, v3, v4);
+    const Diff5 d = diff5(v1, v2, v3, v4, v5);
+    if (k1 && k2 && k3 && k4 && k5) return weno5f<Z>(v1, v2, v3, v4, v5, d);
+    if (k2 && k3 && k4) return weno3f(v3, d);
     return v3;
 }
 

@@ -75,3 +75,20 @@ def test_check_raises_value_error_for_argument_codes():
     _fct(lib, nt=0)
     with pytest.raises(ValueError, match="positive"):
         _native.check(_native.OCX_E_ARG, "ocx_advect_fct_f64")
+
+
+def test_content_needs_its_workspace():
+    lib = _native.lib()
+    assert lib.ocx_content_workspace_bytes(10) == 10 * 256 * 16
+    assert lib.ocx_content_workspace_bytes(0) == 0
+    v = ctypes.c_void_p
+    rc = lib.ocx_content_f64(v(1), v(2), 10, 4, 8, 8, v(3), v(4), 16, None)
+    assert rc == _native.OCX_E_WORKSPACE and "workspace" in lib.ocx_last_error().decode()
+    rc = lib.ocx_content_f64(v(1), v(2), 10, 4, 8, 8, None, v(4), 1 << 20, None)
+    assert rc == _native.OCX_E_ARG and "out is NULL" in lib.ocx_last_error().decode()
+
+
+def test_planes_beyond_32_bit_offsets_are_rejected():
+    lib = _native.lib()
+    rc = _fct(lib, ny=20000, nx=20000)
+    assert rc == _native.OCX_E_ARG and "too large" in lib.ocx_last_error().decode()
