@@ -465,11 +465,9 @@ int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const u
     hipStream_t s = static_cast<hipStream_t>(stream);
     const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
     if (flags & WENO_JS)
-        k_fct_stage<0, false, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, ok, nullptr, fx, fy,
-                                                             nullptr, nullptr, nt, nz, ny, nx, 0.0, flags);
+        k_faces<false, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags);
     else
-        k_fct_stage<0, true, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, ok, nullptr, fx, fy,
-                                                            nullptr, nullptr, nt, nz, ny, nx, 0.0, flags);
+        k_faces<true, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags);
     return launch_check("faces_h");
 }
 

@@ -1,6 +1,7 @@
-// Plain FP64 tracer-advection kernels for gfx950: SSP-RK3 stages with WENO5 faces, the Zalesak FCT limiter and
-// the h*T flux-divergence update, on the masked C-grid (reference: core/dynamics/sigma_baroclinic.py:265-530,
-// 856-875; core/components/tracers.py:208-375).
+// Plain FP64 tile kernels for gfx950: the SSP-RK3 step of TracerAdvection.advect with x, y and z WENO5 sweeps, and
+// the standalone helpers (WENO faces, upwind faces, Zalesak coefficients, open-edge fluxes, periodic halo, content
+// sums), on the masked C-grid (reference: core/dynamics/sigma_baroclinic.py:265-449, 856-875;
+// core/components/tracers.py:65-193, 342-375).  The FCT step itself runs in the wave-marching kernels.
 //
 // Layout (C-contiguous, k = 0 at the bed): stacks (nt, n, ny, nx); cells V0, V1, ok (n, ny, nx); faces
 // Qx (n, ny, nx+1), Qy (n, ny+1, nx), Fz (n+1, ny, nx); face stacks fx (nt, n, ny, nx+1), fy (nt, n, ny+1, nx).
@@ -112,20 +113,15 @@ __device__ __forceinline__ double donor_y(const double* __restrict__ cp, const d
 __device__ __forceinline__ double dmax(double a, double b) { return b > a ? b : a; }
 __device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
 
-// ------------------------------------------------------------------------------------------------ FCT stages
-// MODE 0: faces only (fx, fy of cin; all owned faces written).
-// MODE 1: FCT stage 1 - F0 = faces(C) written to fx/fy, q1 = q0 - dt div F0, c1 = q1/Va (safe).
-// MODE 2: FCT stage 2 - F1 = faces(c1), fx/fy <- F0 + F1 in place, q2 = 0.75 q0 + 0.25 (q1 - dt div F1),
-//         c2 = q2/Vb (safe).  (_fct_stage, sigma_baroclinic.py:452-468, with Va = V0 + dV, Vb = V0 + dV/2.)
-template <int MODE, bool Z, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_fct_stage(const double* __restrict__ cin, const double* __restrict__ C,
-                                                      const double* __restrict__ ext, const double* __restrict__ V0,
-                                                      const double* __restrict__ V1, const double* __restrict__ Qx,
-                                                      const double* __restrict__ Qy, const uint8_t* __restrict__ ok,
-                                                      const double* __restrict__ qprev, double* __restrict__ fx,
-                                                      double* __restrict__ fy, double* __restrict__ qout,
-                                                      double* __restrict__ cout, int nt, int n, int ny, int nx,
-                                                      double dt, unsigned flags) {
+// ------------------------------------------------------------------------------------------------ faces
+// WENO face fluxes fx, fy of cin on every owned face (the faces_h helper; _advect_faces_h,
+// sigma_baroclinic.py:265-321).
+template <bool Z, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_faces(const double* __restrict__ cin, const double* __restrict__ ext,
+                                                  const double* __restrict__ Qx, const double* __restrict__ Qy,
+                                                  const uint8_t* __restrict__ ok, double* __restrict__ fx,
+                                                  double* __restrict__ fy, int nt, int n, int ny, int nx,
+                                                  unsigned flags) {
     constexpr int H = 3, LX = TX + 2 * H, LY = TY + 2 * H;
     __shared__ double sc[LY * LX];
     __shared__ uint8_t so[LY * LX];
@@ -143,129 +139,15 @@ __global__ __launch_bounds__(TX* TY) void k_fct_stage(const double* __restrict__
         weno_tile_faces<Z, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
                                    ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
         __syncthreads();
-        // owned faces to global memory
         for (int idx = tid; idx < TY * (TX + 1); idx += TX * TY) {
             const int ly = idx / (TX + 1), lf = idx - ly * (TX + 1);
             const int j = j0 + ly, f = i0 + lf;
-            if (j < ny && f <= nx && (lf < TX || f == nx)) {
-                const size_t g = ((size_t)p * ny + j) * (nx + 1) + f;
-                fx[g] = MODE == 2 ? fx[g] + sfx[idx] : sfx[idx];
-            }
+            if (j < ny && f <= nx && (lf < TX || f == nx)) fx[((size_t)p * ny + j) * (nx + 1) + f] = sfx[idx];
         }
         for (int idx = tid; idx < (TY + 1) * TX; idx += TX * TY) {
             const int lf = idx / TX, lx = idx - lf * TX;
             const int f = j0 + lf, i = i0 + lx;
-            if (i < nx && f <= ny && (lf < TY || f == ny)) {
-                const size_t g = ((size_t)p * (ny + 1) + f) * nx + i;
-                fy[g] = MODE == 2 ? fy[g] + sfy[idx] : sfy[idx];
-            }
-        }
-        if (MODE != 0) {
-            const int j = j0 + ty, i = i0 + tx;
-            if (j < ny && i < nx) {
-                const double d = sfx[ty * (TX + 1) + tx + 1] - sfx[ty * (TX + 1) + tx] + sfy[(ty + 1) * TX + tx] -
-                                 sfy[ty * TX + tx];
-                const size_t x = pc + (size_t)j * nx + i, xv = kc + (size_t)j * nx + i;
-                const double c0 = C[x], v0 = V0[xv], v1 = V1[xv];
-                const double q0 = v0 * c0, dV = v1 - v0;
-                double q, V;
-                if (MODE == 1) {
-                    q = 0.0 * q0 + 1.0 * (q0 - dt * d);
-                    V = v0 + dV;
-                    qout[x] = q;
-                } else {
-                    q = 0.75 * q0 + 0.25 * (qprev[x] - dt * d);
-                    V = v0 + 0.5 * dV;
-                }
-                cout[x] = V > 0 ? q / V : c0;
-            }
-        }
-    }
-}
-
-// FCT stage 3 and the high/low-order split: F2 = faces(c2), F_H = (F0 + F1)/6 + 2 F2/3 (S = F0 + F1 from stage 2),
-// donor-cell F_L from C, A = F_H - F_L written to ax/ay (owned faces), and the low-order solution
-// c_L = (q0 - dt div F_L)/V1 (safe) written to cL (_fct_fluxes :471-511 and _fct_stage :452-468).
-// UNLIM (C = 1, advect_fct_numpy's unlimited branch, tracers.py:286-289): out = (q0 - dt div F_H)/V1 (safe),
-// and the face column fxi of F_H.
-template <bool Z, bool UNLIM, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_fct_high(const double* __restrict__ c2, const double* __restrict__ C,
-                                                     const double* __restrict__ ext, const double* __restrict__ V0,
-                                                     const double* __restrict__ V1, const double* __restrict__ Qx,
-                                                     const double* __restrict__ Qy, const uint8_t* __restrict__ ok,
-                                                     const double* __restrict__ Sx, const double* __restrict__ Sy,
-                                                     double* __restrict__ ax, double* __restrict__ ay,
-                                                     double* __restrict__ cl_out, double* __restrict__ fxi,
-                                                     int flux_face_i, int nt, int n, int ny, int nx, double dt,
-                                                     unsigned flags) {
-    constexpr int H = 3, LX = TX + 2 * H, LY = TY + 2 * H;
-    __shared__ double sc[LY * LX];
-    __shared__ uint8_t so[LY * LX];
-    __shared__ double sfx[TY * (TX + 1)];
-    __shared__ double sfy[(TY + 1) * TX];
-    __shared__ double slx[TY * (TX + 1)];
-    __shared__ double sly[(TY + 1) * TX];
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
-    const size_t plane = (size_t)ny * nx;
-    for (int p = blockIdx.z; p < nt * n; p += gridDim.z) {
-        const int k = p % n;
-        const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
-        const double* Qxk = Qx + (size_t)k * ny * (nx + 1);
-        const double* Qyk = Qy + (size_t)k * (ny + 1) * nx;
-        const double* cp = C + pc;
-        const double* ep = ext ? ext + pc : cp;          // donor exterior: inflow, else C
-        __syncthreads();
-        load_tile<TX, TY, H>(sc, so, c2 + pc, ok + kc, i0, j0, ny, nx);
-        __syncthreads();
-        weno_tile_faces<Z, TX, TY>(sfx, sfy, sc, so, Qxk, Qyk, ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
-        __syncthreads();
-        for (int idx = tid; idx < TY * (TX + 1); idx += TX * TY) {
-            const int ly = idx / (TX + 1), lf = idx - ly * (TX + 1);
-            const int j = j0 + ly, f = i0 + lf;
-            if (j < ny && f <= nx) {
-                const size_t g = ((size_t)p * ny + j) * (nx + 1) + f;
-                const double fh = Sx[g] / 6.0 + 2.0 * sfx[idx] / 3.0;
-                if (UNLIM) {
-                    sfx[idx] = fh;
-                    if (fxi && f == flux_face_i) fxi[(size_t)p * ny + j] = fh;
-                } else {
-                    const double fl = donor_x(cp + (size_t)j * nx, ep + (size_t)j * nx, Qxk[(size_t)j * (nx + 1) + f],
-                                              f, nx, flags);
-                    slx[idx] = fl;
-                    const double a = fh - fl;
-                    if (lf < TX || f == nx) ax[g] = a;
-                }
-            }
-        }
-        for (int idx = tid; idx < (TY + 1) * TX; idx += TX * TY) {
-            const int lf = idx / TX, lx = idx - lf * TX;
-            const int f = j0 + lf, i = i0 + lx;
-            if (i < nx && f <= ny) {
-                const size_t g = ((size_t)p * (ny + 1) + f) * nx + i;
-                const double fh = Sy[g] / 6.0 + 2.0 * sfy[idx] / 3.0;
-                if (UNLIM) {
-                    sfy[idx] = fh;
-                } else {
-                    const double fl = donor_y(cp, ep, Qyk[(size_t)f * nx + i], f, i, ny, nx, flags);
-                    sly[idx] = fl;
-                    const double a = fh - fl;
-                    if (lf < TY || f == ny) ay[g] = a;
-                }
-            }
-        }
-        __syncthreads();
-        const int j = j0 + ty, i = i0 + tx;
-        if (j < ny && i < nx) {
-            const double* gx = UNLIM ? sfx : slx;
-            const double* gy = UNLIM ? sfy : sly;
-            const double d = gx[ty * (TX + 1) + tx + 1] - gx[ty * (TX + 1) + tx] + gy[(ty + 1) * TX + tx] -
-                             gy[ty * TX + tx];
-            const size_t x = pc + (size_t)j * nx + i, xv = kc + (size_t)j * nx + i;
-            const double c0 = C[x], v0 = V0[xv], v1 = V1[xv];
-            const double q0 = v0 * c0;
-            const double q = UNLIM ? q0 - dt * d : 0.0 * q0 + 1.0 * (q0 - dt * d);
-            cl_out[x] = v1 > 0 ? q / v1 : c0;
+            if (i < nx && f <= ny && (lf < TY || f == ny)) fy[((size_t)p * (ny + 1) + f) * nx + i] = sfy[idx];
         }
     }
 }
@@ -322,63 +204,6 @@ __global__ __launch_bounds__(TX* TY) void k_fct_ratio(const double* __restrict__
 // C = min(R_in[f], R_out[f-1]) if A >= 0 else min(R_in[f-1], R_out[f]); edge faces 0.
 __device__ __forceinline__ double coeff(double a, double rin_hi, double rout_lo, double rin_lo, double rout_hi) {
     return a >= 0.0 ? dmin(rin_hi, rout_lo) : dmin(rin_lo, rout_hi);
-}
-
-// Final update (eq. 14, _fct_final :514-530): G = F_L + C A on the four faces, q = q0 - dt div G, out = q/V1
-// (safe).  Side outputs: the face column fxi (flux_face_i) and the open-edge face fluxes (gw, ge: (nt, n, ny);
-// gs, gn: (nt, n, nx)) for the content budget.
-template <int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_fct_final(const double* __restrict__ C, const double* __restrict__ ext,
-                                                      const double* __restrict__ V0, const double* __restrict__ V1,
-                                                      const double* __restrict__ Qx, const double* __restrict__ Qy,
-                                                      const double* __restrict__ ax, const double* __restrict__ ay,
-                                                      const double* __restrict__ r_in, const double* __restrict__ r_out,
-                                                      double* __restrict__ out, double* __restrict__ fxi,
-                                                      int flux_face_i, double* __restrict__ gedge, int nt, int n,
-                                                      int ny, int nx, double dt, unsigned flags) {
-    const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TY + threadIdx.y;
-    if (i >= nx || j >= ny) return;
-    const size_t plane = (size_t)ny * nx;
-    for (int p = blockIdx.z; p < nt * n; p += gridDim.z) {
-        const int k = p % n;
-        const size_t pc = (size_t)p * plane, xc = (size_t)j * nx + i, x = pc + xc;
-        const double* cp = C + pc;
-        const double* ep = ext ? ext + pc : cp;
-        const double* ri = r_in + pc;
-        const double* ro = r_out + pc;
-        const double* Qxr = Qx + ((size_t)k * ny + j) * (nx + 1);
-        const double* Qyk = Qy + (size_t)k * (ny + 1) * nx;
-        const size_t gx = ((size_t)p * ny + j) * (nx + 1) + i;
-        const size_t gy = ((size_t)p * (ny + 1) + j) * nx + i;
-        // west (f = i) and east (f = i + 1) faces
-        const double aw = ax[gx], ae = ax[gx + 1], as = ay[gy], an = ay[gy + nx];
-        const double cw = (i > 0) ? coeff(aw, ri[xc], ro[xc - 1], ri[xc - 1], ro[xc]) : 0.0;
-        const double ce = (i + 1 < nx) ? coeff(ae, ri[xc + 1], ro[xc], ri[xc], ro[xc + 1]) : 0.0;
-        const double cs = (j > 0) ? coeff(as, ri[xc], ro[xc - nx], ri[xc - nx], ro[xc]) : 0.0;
-        const double cn = (j + 1 < ny) ? coeff(an, ri[xc + nx], ro[xc], ri[xc], ro[xc + nx]) : 0.0;
-        const double* crow = cp + (size_t)j * nx;
-        const double* erow = ep + (size_t)j * nx;
-        const double Gw = donor_x(crow, erow, Qxr[i], i, nx, flags) + cw * aw;
-        const double Ge = donor_x(crow, erow, Qxr[i + 1], i + 1, nx, flags) + ce * ae;
-        const double Gs = donor_y(cp, ep, Qyk[(size_t)j * nx + i], j, i, ny, nx, flags) + cs * as;
-        const double Gn = donor_y(cp, ep, Qyk[(size_t)(j + 1) * nx + i], j + 1, i, ny, nx, flags) + cn * an;
-        const double d = Ge - Gw + Gn - Gs;
-        const double c0 = cp[xc];
-        const double q = V0[(size_t)k * plane + xc] * c0 - dt * d;
-        const double v1 = V1[(size_t)k * plane + xc];
-        out[x] = v1 > 0 ? q / v1 : c0;
-        if (fxi) {
-            if (flux_face_i == i) fxi[(size_t)p * ny + j] = Gw;
-            if (flux_face_i == nx && i == nx - 1) fxi[(size_t)p * ny + j] = Ge;
-        }
-        if (gedge) {
-            const size_t nyl = (size_t)nt * n * ny, nxl = (size_t)nt * n * nx;
-            if (i == 0) gedge[(size_t)p * ny + j] = Gw;
-            if (i == nx - 1) gedge[nyl + (size_t)p * ny + j] = Ge;
-            if (j == 0) gedge[2 * nyl + (size_t)p * nx + i] = Gs;
-            if (j == ny - 1) gedge[2 * nyl + nxl + (size_t)p * nx + i] = Gn;
-        }
-    }
 }
 
 // Vertical WENO face flux of interface kf (between layers kf-1 and kf, 1 <= kf <= n-1) of one column; same
