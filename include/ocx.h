@@ -106,6 +106,12 @@ int ocx_fct_coefficients_f64(const double* c0, const double* cL, const double* V
 int ocx_periodic_pad(const void* src, void* dst, size_t lead, int sy, int sx, int kind, int pad_y, int pad_x,
                      int elem, int crop, void* stream);
 
+/* Numerical mixing of the advection, out = V1 (<c^2> - c^2) / dt per cell (accumulate != 0: out += it), with c and
+ * c^2 (nz, ny, nx) two members of the advected stack (T and T^2).  Replaces mix_num (core/model/driver.py:905) and
+ * VarianceBudget.advective (core/components/variance_budget.py:60-62), same operation order. */
+int ocx_mix_num_f64(const double* c, const double* c2, const double* V1, int nz, int ny, int nx, double dt,
+                    int accumulate, double* out, void* stream);
+
 /* Per-tracer content sum_k,j,i V c (FP64, compensated, in a fixed order that does not depend on the launch):
  * out (nt).  The global content sums of the reference's conservation checks (tracers.py:359-375 callers,
  * test_sigma_ale.py:1342-1361).  workspace: ocx_content_workspace_bytes(nt) bytes of device memory. */

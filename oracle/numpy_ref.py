@@ -350,3 +350,13 @@ def crop(a, halo, px, py):
     if py:
         a = a[..., halo:a.shape[-2] - halo, :]
     return np.ascontiguousarray(a)
+
+
+# ------------------------------------------------------------------------------------------------ diagnostics
+def numerical_mixing(c, c2, V1, dt):
+    """
+    Numerical mixing of the advection of c, V1 (<c^2> - c^2) / dt per cell, with <c^2> the advected c^2
+    (Burchard & Rennau 2008, eq. 24): ``VarianceBudget.advective`` (core/components/variance_budget.py:60-62) and
+    ``mix_num`` (core/model/driver.py:905), same operation order.
+    """
+    return V1 * (c2 - c ** 2) / dt

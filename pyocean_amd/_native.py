@@ -36,6 +36,7 @@ SIGNATURES = {
     "ocx_rhs_f64": (I, [P, P, P, P, P, P, I, I, I, I, U, I, P, P]),
     "ocx_fct_coefficients_f64": (I, [P, P, P, P, P, P, D, I, I, I, I, P, P, P, Z, P]),
     "ocx_periodic_pad": (I, [P, P, Z, I, I, I, I, I, I, I, P]),
+    "ocx_mix_num_f64": (I, [P, P, P, I, I, I, D, I, P, P]),
     "ocx_content_workspace_bytes": (Z, [I]),
     "ocx_content_f64": (I, [P, P, I, I, I, I, P, P, Z, P]),
     "ocx_batch_create": (I, [P, I, I, I, I, P]),

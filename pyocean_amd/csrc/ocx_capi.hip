@@ -861,6 +861,21 @@ int ocx_batch_destroy(ocx_batch* b) {
     return OCX_OK;
 }
 
+int ocx_mix_num_f64(const double* c, const double* c2, const double* V1, int nz, int ny, int nx, double dt,
+                    int accumulate, double* out, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(1, nz, ny, nx));
+    OCX_TRY(need(c, "c"));
+    OCX_TRY(need(c2, "c2"));
+    OCX_TRY(need(V1, "V1"));
+    OCX_TRY(need(out, "out"));
+    if (!(dt > 0)) return fail(OCX_E_ARG, "dt = %g must be positive", dt);
+    const size_t cells = (size_t)nz * ny * nx;
+    const unsigned blocks = (unsigned)std::min<size_t>((cells + 255) / 256, 65536);
+    k_mix_num<<<blocks, 256, 0, static_cast<hipStream_t>(stream)>>>(c, c2, V1, dt, accumulate != 0, out, cells);
+    return launch_check("mix_num");
+}
+
 size_t ocx_content_workspace_bytes(int nt) {
     return nt > 0 ? (size_t)nt * ocx::CONTENT_PARTS * 2 * sizeof(double) : 0;
 }

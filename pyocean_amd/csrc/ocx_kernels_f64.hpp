@@ -404,9 +404,21 @@ __global__ void k_fill(double* __restrict__ dst, double value, size_t count) {
         dst[x] = value;
 }
 
-// Per-tracer content sum_k,j,i V[k,j,i] * c[t,k,j,i] in FP64, one block of 1024 threads per tracer: compensated
-// (Neumaier) partial sums in a fixed order, then a fixed-shape tree, so the result is deterministic and its
-// rounding error stays near one ulp of the sum.
+// Numerical mixing of the advection of c (Burchard & Rennau 2008, eq. 24), with c^2 advected in the same stack:
+// m = V1 (<c^2> - c^2) / dt per cell, in the reference's operation order (VarianceBudget.advective,
+// core/components/variance_budget.py:60-62; mix_num, core/model/driver.py:905); accumulate adds it to out.
+__global__ void k_mix_num(const double* __restrict__ c, const double* __restrict__ c2, const double* __restrict__ V1,
+                          double dt, int accumulate, double* __restrict__ out, size_t cells) {
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < cells; x += (size_t)gridDim.x * blockDim.x) {
+        const double cx = c[x];
+        const double m = V1[x] * (c2[x] - cx * cx) / dt;
+        out[x] = accumulate ? out[x] + m : m;
+    }
+}
+
+// Per-tracer content sum_k,j,i V[k,j,i] * c[t,k,j,i] in FP64: compensated (Neumaier) partial sums in a fixed
+// order, then fixed-shape trees, so the result is deterministic and its rounding error stays near one ulp of the
+// sum.
 __device__ __forceinline__ void neumaier_add(double& s, double& comp, double x) {
     const double t = s + x;
     comp += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;

@@ -226,6 +226,24 @@ def periodic_pad(a, kind: str, halo: int, px: bool, py: bool, crop: bool = False
     return out
 
 
+def numerical_mixing(c, c2, V1, dt, out=None, accumulate=False):
+    """Numerical mixing of the advection V1 (<c^2> - c^2) / dt per cell (``ocx_mix_num_f64``): c, c2 (nz, ny, nx)
+    the advected c and c^2 (members of one stack); ``accumulate`` adds to ``out`` (the variance budget's +=).
+    core/model/driver.py:905 (mix_num), core/components/variance_budget.py:60-62."""
+    n, ny, nx = c.shape
+    _need(c, (n, ny, nx), "c")
+    _need(c2, (n, ny, nx), "c2")
+    _need(V1, (n, ny, nx), "V1")
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs out")
+        out = torch.empty_like(c)
+    _need(out, (n, ny, nx), "out")
+    check(lib().ocx_mix_num_f64(_ptr(c), _ptr(c2), _ptr(V1), n, ny, nx, float(dt), int(bool(accumulate)), _ptr(out),
+                                _stream()), "ocx_mix_num_f64")
+    return out
+
+
 def content(c, V):
     """Per-tracer content sum V c (FP64, deterministic) (``ocx_content_f64``)."""
     nt, n, ny, nx = c.shape

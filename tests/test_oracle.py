@@ -442,3 +442,24 @@ def test_rhs_restatements_agree():
     cref.lib().oc_advect_div(P(c), P(case["Qx"]), P(case["Qy"]), P(case["Fz"]), P(okb), P(c), nt, n, ny, nx,
                              case["open_mask"], 1, 1, P(out))
     np.testing.assert_array_equal(out, ref)
+
+
+# ------------------------------------------------------------------------------------------------ numerical mixing
+def test_numerical_mixing_closes_the_variance_budget_of_the_advection(oracle_c):
+    """Advection conserves the content of the c^2 member, so sum V1 c_adv^2 - sum V0 c0^2 = -dt sum mix: the
+    mixing is exactly the variance the step destroyed (variance_budget.py:36-41), and positive for the FCT scheme."""
+    k = cases.fused_fct_case(open_=False, thin=False)
+    c = k["C"][0]
+    out = oracle_c.advect_fct(k["C"], k["V0"], k["V1"], k["Qx"], k["Qy"], k["ok"], k["dt"])
+    mix = R.numerical_mixing(out[0], out[1], k["V1"], k["dt"])
+    lhs = (k["V1"] * out[0] ** 2).sum() - (k["V0"] * c * c).sum()
+    assert abs(lhs + k["dt"] * mix.sum()) <= 1e-12 * (k["V0"] * c * c).sum()
+    assert mix.sum() > 0
+
+
+def test_numerical_mixing_vanishes_for_a_uniform_tracer(oracle_c):
+    k = cases.fused_fct_case(open_=False, thin=False)
+    C = np.stack([np.full_like(k["C"][0], 2.5), np.full_like(k["C"][0], 6.25)])
+    out = oracle_c.advect_fct(C, k["V0"], k["V1"], k["Qx"], k["Qy"], k["ok"], k["dt"])
+    mix = R.numerical_mixing(out[0], out[1], k["V1"], k["dt"])
+    assert np.abs(mix).max() <= 1e-12 * (k["V1"].max() * 6.25 / k["dt"])
