@@ -325,6 +325,64 @@ def bench_precision(args) -> int:
     return 0
 
 
+def bench_diffusion(args) -> int:
+    """Tracer diffusion of the whole stack on the regional grid: the along-layer Laplacian (K_h = 100 m^2/s, the
+    anomaly field T - 10) then the implicit vertical solve (K_v = 1e-4 m^2/s, dz = max(h, 1e-6)), per step."""
+    import torch
+
+    from pyocean_amd import _native, ops, synth
+
+    _log(f"generating regional {args.nx}x{args.ny}x{args.nz} nt={args.nt}")
+    case = synth.regional(nx=args.nx, ny=args.ny, nz=args.nz, nt=args.nt, seed=0)
+    nt, nz, ny, nx = case.C.shape
+    act = case.ok.any(axis=0)
+    h = np.where(act[None], case.V1 / case.V1.max() * 40.0, 0.0)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()       # noqa: E731
+    cx, cy = d(np.ones((ny, nx + 1))), d(np.ones((ny + 1, nx)))
+    okx, oky = d(np.ones((ny, nx + 1), np.uint8)), d(np.ones((ny + 1, nx), np.uint8))
+    C, Tw, V, hd, actd = d(case.C), d(case.C - 10.0), d(case.V1), d(h), d(act.astype(np.uint8))
+    dz = d(np.maximum(h, 1e-6).reshape(nz, -1))
+    nu = d(np.full((nz + 1, ny * nx), 1e-4))
+    out = torch.empty_like(C)
+    ws = ops.Workspace()
+    M = ny * nx
+
+    def step():
+        ops.diffuse_h(C, Tw, hd, actd, cx, cy, okx, oky, V, 100.0, case.dt, out=out)
+        ops.vertical_implicit(out.view(nt, nz, M), dz, nu, None, case.dt, out=out.view(nt, nz, M), workspace=ws)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    _native.profile_enable(True)
+    _native.profile_collect()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _native.profile_enable(False)
+    prof = _native.profile_collect()
+    updates = nt * nz * ny * nx
+    value = updates * args.steps / wall
+    # compulsory bytes per tracer-cell: horizontal T, Tw read and out written (24 B) + h, V (16 B) / nt; vertical
+    # u read and out written twice (the back substitution revisits it: 32 B) + dz, nu, cp (40 B) / nt
+    b_h, b_v = 24.0 + 16.0 / nt, 32.0 + 40.0 / nt
+    res = {
+        "metric": "tracer-cell diffusion updates/s (along-layer Laplacian + implicit vertical, whole stack)",
+        "value": value, "unit": "tracer-cell updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded regional grid)",
+        "config": {"workload": f"regional {nx}x{ny}x{nz}, {nt} tracers, K_h 100 m2/s, K_v 1e-4 m2/s"},
+        "step_roofline": {"bytes_per_update": b_h + b_v, "achieved": value * (b_h + b_v) / 1e9,
+                          "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": value * (b_h + b_v) / HBM_PEAK},
+        "kernels_ms": {k: v[0] / v[1] for k, v in prof.items()},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -342,10 +400,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--form", choices=("step", "batch", "serve", "precision"), default="step",
+    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion"), default="step",
                     help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "
                          "persistent launch; serve: the Poisson job stream through the continuous-batching loop; "
-                         "precision: split against FP64 after --checkpoints steps")
+                         "precision: split against FP64 after --checkpoints steps; diffusion: the along-layer and "
+                         "vertical tracer diffusion of the stack")
     ap.add_argument("--checkpoints", default="1,10,200,2000", help="precision form: step counts to compare at")
     ap.add_argument("--mix", choices=("mixed", "split", "f64"), default="mixed",
                     help="batch form: member precisions (mixed: even members FP64, odd split)")
@@ -376,6 +435,8 @@ def main() -> int:
         return bench_serve(args)
     if args.form == "precision":
         return bench_precision(args)
+    if args.form == "diffusion":
+        return bench_diffusion(args)
 
     if args.rch or args.wpb:
         from pyocean_amd import batch as _batch

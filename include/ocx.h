@@ -106,6 +106,26 @@ int ocx_fct_coefficients_f64(const double* c0, const double* cL, const double* V
 int ocx_periodic_pad(const void* src, void* dst, size_t lead, int sy, int sx, int kind, int pad_y, int pad_x,
                      int elem, int crop, void* stream);
 
+/* Explicit Laplacian diffusion along the layers of an ALE step (_diffuse_ale_kernel,
+ * core/dynamics/sigma_baroclinic.py:533-561, as TracerDiffusion.diffuse_h calls it, core/components/tracers.py:420-446),
+ * same per-element arithmetic, for a stack of nt tracers: T, Tw (nt, nz, ny, nx) the field and the field diffused
+ * (the anomaly or T itself); h (nz, ny, nx) layer thickness already zeroed on inactive cells; act (ny, nx);
+ * cx = L_u/dx_u (ny, nx+1), cy = L_v/dy_v (ny+1, nx); okx (ny, nx+1), oky (ny+1, nx) face masks (bytes);
+ * V (nz, ny, nx) volumes.  out (nt, nz, ny, nx) must not alias T or Tw. */
+int ocx_diffuse_h_f64(const double* T, const double* Tw, const double* h, const uint8_t* act, const double* cx,
+                      const double* cy, const uint8_t* okx, const uint8_t* oky, const double* V, int nt, int nz, int ny,
+                      int nx, double kh, double dt, double* out, void* stream);
+
+/* Implicit vertical diffusion of nt stacked fields (vertical_implicit / _thomas_columns,
+ * core/dynamics/sigma_wd.py:57-99), same arithmetic: u, out (nt, n, columns) (out may be u); dz (n, columns);
+ * nu (n+1, columns) interface diffusivities; r_bed NULL, (columns) on layer 0 (r_layers = 1) or (n, columns)
+ * (r_layers = n); tau_top NULL or (columns) surface flux into the top layer.  workspace:
+ * ocx_vertical_implicit_workspace_bytes(n, columns) bytes (the forward-sweep coefficients, shared by the fields). */
+size_t ocx_vertical_implicit_workspace_bytes(int n, long long columns);
+int ocx_vertical_implicit_f64(const double* u, const double* dz, const double* nu, const double* r_bed, int r_layers,
+                              const double* tau_top, int nt, int n, long long columns, double dt, double* out,
+                              void* workspace, size_t workspace_bytes, void* stream);
+
 /* Numerical mixing of the advection, out = V1 (<c^2> - c^2) / dt per cell (accumulate != 0: out += it), with c and
  * c^2 (nz, ny, nx) two members of the advected stack (T and T^2).  Replaces mix_num (core/model/driver.py:905) and
  * VarianceBudget.advective (core/components/variance_budget.py:60-62), same operation order. */

@@ -360,3 +360,70 @@ def numerical_mixing(c, c2, V1, dt):
     ``mix_num`` (core/model/driver.py:905), same operation order.
     """
     return V1 * (c2 - c ** 2) / dt
+
+
+# ------------------------------------------------------------------------------------------------ tracer diffusion
+def diffuse_h_ale(T, Tw, h, act, cx, cy, okx, oky, V, kh, dt):
+    """
+    Explicit Laplacian diffusion along the layers, ALE form: the ALE branch of ``TracerDiffusion.diffuse_h_numpy``
+    (core/components/tracers.py:450-481, with ``face_thickness``, core/vertical/ale.py:66-79), whose per-element
+    arithmetic ``_diffuse_ale_kernel`` (core/dynamics/sigma_baroclinic.py:533-561) repeats.  T, Tw (..., n, ny, nx);
+    h (n, ny, nx) zeroed on inactive cells; cx = L_u/dx_u (ny, nx+1), cy = L_v/dy_v (ny+1, nx); okx, oky face masks.
+    """
+    n, ny, nx = h.shape
+    hu = np.zeros((n, ny, nx + 1))
+    hv = np.zeros((n, ny + 1, nx))
+    hu[..., 1:-1] = np.where((act[:, :-1] & act[:, 1:])[None], np.minimum(h[..., :-1], h[..., 1:]), 0.0)
+    hv[:, 1:-1, :] = np.where((act[:-1, :] & act[1:, :])[None], np.minimum(h[:, :-1, :], h[:, 1:, :]), 0.0)
+    lead = T.shape[:-3]
+    Fx = np.zeros(lead + (n, ny, nx + 1))
+    Fy = np.zeros(lead + (n, ny + 1, nx))
+    Fx[..., 1:-1] = -kh * cx[None, :, 1:-1] * hu[..., 1:-1] * (Tw[..., 1:] - Tw[..., :-1])
+    Fy[..., 1:-1, :] = -kh * cy[None, 1:-1, :] * hv[:, 1:-1, :] * (Tw[..., 1:, :] - Tw[..., :-1, :])
+    Fx = np.where(okx[None], Fx, 0.0)
+    Fy = np.where(oky[None], Fy, 0.0)
+    div = Fx[..., 1:] - Fx[..., :-1] + Fy[..., 1:, :] - Fy[..., :-1, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(V > 0, T - dt * div / np.where(V > 0, V, 1.0), T)
+
+
+def vertical_implicit(u, dz, nu, r_bed, dt, tau_top=None):
+    """
+    Backward-Euler vertical diffusion per column, Thomas algorithm: ``_vertical_implicit_numpy`` and its compiled
+    twin ``_thomas_columns`` (core/dynamics/sigma_wd.py:57-88, 112-155; same arithmetic), with the compiled entry's
+    zero surface flux when ``tau_top`` is None (``vertical_implicit`` :91-99).  u, dz (n, M); nu (n+1, M);
+    r_bed (M,) on layer 0 or (n, M).  A leading stack axis of u is solved field by field.
+    """
+    u = np.asarray(u, float)
+    if u.ndim == 3:
+        return np.stack([vertical_implicit(x, dz, nu, r_bed, dt, tau_top) for x in u])
+    n, M = u.shape
+    r = np.asarray(r_bed, float)
+    rates = np.ascontiguousarray(np.broadcast_to(r, (n, M))) if r.ndim == 2 else np.zeros((n, M))
+    if r.ndim != 2:
+        rates[0] = np.broadcast_to(r, (M,))
+    h = 0.5 * (dz[1:] + dz[:-1])
+    c = dt * nu[1:-1] / h
+    lower = np.zeros_like(u)
+    upper = np.zeros_like(u)
+    lower[1:] = -c
+    upper[:-1] = -c
+    diag = dz.copy()
+    diag[1:] += c
+    diag[:-1] += c
+    diag += dt * rates
+    rhs = dz * u
+    rhs[-1] = rhs[-1] + dt * (np.zeros(M) if tau_top is None else tau_top)
+    cp = np.zeros_like(u)
+    dp = np.zeros_like(u)
+    cp[0] = upper[0] / diag[0]
+    dp[0] = rhs[0] / diag[0]
+    for k in range(1, n):
+        m = diag[k] - lower[k] * cp[k - 1]
+        cp[k] = upper[k] / m
+        dp[k] = (rhs[k] - lower[k] * dp[k - 1]) / m
+    out = np.empty_like(u)
+    out[-1] = dp[-1]
+    for k in range(n - 2, -1, -1):
+        out[k] = dp[k] - cp[k] * out[k + 1]
+    return out

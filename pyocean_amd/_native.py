@@ -21,7 +21,7 @@ SPLIT = 64
 OCX_E_ARG, OCX_E_HIP, OCX_E_WORKSPACE = -1, -2, -3
 
 # (name, restype, argtypes) of every exported symbol; the CPU tests check that each one is exported
-P, I, U, D, Z = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_double, ctypes.c_size_t
+P, I, U, D, Z, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_double, ctypes.c_size_t, ctypes.c_longlong
 SIGNATURES = {
     "ocx_last_error": (ctypes.c_char_p, []),
     "ocx_version": (I, []),
@@ -37,6 +37,9 @@ SIGNATURES = {
     "ocx_fct_coefficients_f64": (I, [P, P, P, P, P, P, D, I, I, I, I, P, P, P, Z, P]),
     "ocx_periodic_pad": (I, [P, P, Z, I, I, I, I, I, I, I, P]),
     "ocx_mix_num_f64": (I, [P, P, P, I, I, I, D, I, P, P]),
+    "ocx_diffuse_h_f64": (I, [P, P, P, P, P, P, P, P, P, I, I, I, I, D, D, P, P]),
+    "ocx_vertical_implicit_workspace_bytes": (Z, [I, L]),
+    "ocx_vertical_implicit_f64": (I, [P, P, P, P, I, P, I, I, L, D, P, P, Z, P]),
     "ocx_content_workspace_bytes": (Z, [I]),
     "ocx_content_f64": (I, [P, P, I, I, I, I, P, P, Z, P]),
     "ocx_batch_create": (I, [P, I, I, I, I, P]),

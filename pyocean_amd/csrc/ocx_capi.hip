@@ -15,6 +15,7 @@
 #include "ocx_march.hpp"
 #include "ocx_limit.hpp"
 #include "ocx_batch.hpp"
+#include "ocx_diffuse.hpp"
 
 namespace {
 
@@ -859,6 +860,56 @@ int ocx_batch_graph_step(ocx_batch* b, int parity, void* stream) {
 int ocx_batch_destroy(ocx_batch* b) {
     batch_free(b);
     return OCX_OK;
+}
+
+int ocx_diffuse_h_f64(const double* T, const double* Tw, const double* h, const uint8_t* act, const double* cx,
+                      const double* cy, const uint8_t* okx, const uint8_t* oky, const double* V, int nt, int nz, int ny,
+                      int nx, double kh, double dt, double* out, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    const void* need_p[] = {T, Tw, h, act, cx, cy, okx, oky, V, out};
+    const char* need_n[] = {"T", "Tw", "h", "act", "cx", "cy", "okx", "oky", "V", "out"};
+    for (int q = 0; q < 10; ++q) OCX_TRY(need(need_p[q], need_n[q]));
+    if (out == T || out == Tw) return fail(OCX_E_ARG, "out must not alias T or Tw (neighbours are read)");
+    if (nz > 65535) return fail(OCX_E_ARG, "nz = %d layers: at most 65535", nz);
+    const dim3 grid((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz), block(64, 4);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ProfScope _ps("diffuse_h", s);
+    k_diffuse_h<<<grid, block, 0, s>>>(T, Tw, h, act, cx, cy, okx, oky, V, nt, nz, ny, nx, kh, dt, out);
+    return launch_check("diffuse_h");
+}
+
+size_t ocx_vertical_implicit_workspace_bytes(int n, long long columns) {
+    return n > 0 && columns > 0 ? (size_t)n * (size_t)columns * sizeof(double) : 0;
+}
+
+int ocx_vertical_implicit_f64(const double* u, const double* dz, const double* nu, const double* r_bed, int r_layers,
+                              const double* tau_top, int nt, int n, long long columns, double dt, double* out,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+    using namespace ocx;
+    if (nt < 1 || n < 1 || columns < 1) return fail(OCX_E_ARG, "shape (nt=%d, n=%d, columns=%lld) must be positive",
+                                                     nt, n, columns);
+    OCX_TRY(need(u, "u"));
+    OCX_TRY(need(dz, "dz"));
+    OCX_TRY(need(nu, "nu"));
+    OCX_TRY(need(out, "out"));
+    if (r_bed && r_layers != 1 && r_layers != n)
+        return fail(OCX_E_ARG, "r_layers = %d: 1 (a bed rate on layer 0) or n = %d (a rate per layer)", r_layers, n);
+    const size_t wsb = ocx_vertical_implicit_workspace_bytes(n, columns);
+    if (!workspace || workspace_bytes < wsb)
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed, %zu given", wsb, workspace ? workspace_bytes : 0);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* cp = static_cast<double*>(workspace);
+    const unsigned gc = (unsigned)std::min<long long>((columns + 255) / 256, 1 << 16);
+    {
+        ProfScope _ps("vertical_coefficients", s);
+        k_thomas_cp<<<gc, 256, 0, s>>>(dz, nu, r_bed, r_layers, n, columns, dt, cp);
+    }
+    OCX_TRY(launch_check("vertical_implicit coefficients"));
+    const unsigned gs = (unsigned)std::min<long long>((columns + 255) / 256, 1 << 16);
+    ProfScope _ps("vertical_solve", s);
+    k_thomas_solve<<<gs, 256, 0, s>>>(u, dz, nu, r_bed, r_layers, tau_top, nt, n, columns, dt, cp, out);
+    return launch_check("vertical_implicit");
 }
 
 int ocx_mix_num_f64(const double* c, const double* c2, const double* V1, int nz, int ny, int nx, double dt,

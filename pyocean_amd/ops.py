@@ -226,6 +226,61 @@ def periodic_pad(a, kind: str, halo: int, px: bool, py: bool, crop: bool = False
     return out
 
 
+def diffuse_h(T, Tw, h, act, cx, cy, okx, oky, V, kh, dt, out=None):
+    """
+    Explicit Laplacian diffusion along the layers, the ALE kernel of ``TracerDiffusion.diffuse_h``
+    (``_diffuse_ale_kernel``, core/dynamics/sigma_baroclinic.py:533-561; core/components/tracers.py:420-446), on a
+    stack: T, Tw (nt, nz, ny, nx) or (nz, ny, nx); h (nz, ny, nx) zeroed on inactive cells; act (ny, nx);
+    cx = L_u/dx_u (ny, nx+1), cy = L_v/dy_v (ny+1, nx); okx, oky face masks; V (nz, ny, nx).  Bit-identical to the
+    reference's arithmetic (``ocx_diffuse_h_f64``).
+    """
+    squeeze = T.dim() == 3
+    T4, W4 = (T[None], Tw[None]) if squeeze else (T, Tw)
+    nt, n, ny, nx = T4.shape
+    _need(T4, (nt, n, ny, nx), "T")
+    _need(W4, (nt, n, ny, nx), "Tw")
+    _need(h, (n, ny, nx), "h")
+    _need(V, (n, ny, nx), "V")
+    _need(cx, (ny, nx + 1), "cx")
+    _need(cy, (ny + 1, nx), "cy")
+    a, ox, oy = _ok(act, (ny, nx)), _ok(okx, (ny, nx + 1)), _ok(oky, (ny + 1, nx))
+    o4 = torch.empty_like(T4) if out is None else _need(out if not squeeze else out[None], T4.shape, "out")
+    check(lib().ocx_diffuse_h_f64(_ptr(T4), _ptr(W4), _ptr(h), _ptr(a), _ptr(cx), _ptr(cy), _ptr(ox), _ptr(oy),
+                                  _ptr(V), nt, n, ny, nx, float(kh), float(dt), _ptr(o4), _stream()),
+          "ocx_diffuse_h_f64")
+    return o4[0] if squeeze else o4
+
+
+def vertical_implicit(u, dz, nu, r_bed, dt, tau_top=None, out=None, workspace: Workspace | None = None):
+    """
+    Backward-Euler vertical diffusion, column by column (``vertical_implicit`` / ``_thomas_columns``,
+    core/dynamics/sigma_wd.py:57-99), same arithmetic: u (n, M) or a stack (nt, n, M); dz (n, M); nu (n+1, M);
+    r_bed None, (M,) on layer 0 or (n, M); tau_top None or (M,).  ``out`` may be ``u`` (in place).
+    """
+    squeeze = u.dim() == 2
+    u3 = u[None] if squeeze else u
+    nt, n, M = u3.shape
+    _need(u3, (nt, n, M), "u")
+    _need(dz, (n, M), "dz")
+    _need(nu, (n + 1, M), "nu")
+    r_layers = 0
+    if r_bed is not None:
+        if r_bed.dim() == 1:
+            _need(r_bed, (M,), "r_bed")
+            r_layers = 1
+        else:
+            _need(r_bed, (n, M), "r_bed")
+            r_layers = n
+    if tau_top is not None:
+        _need(tau_top, (M,), "tau_top")
+    o3 = torch.empty_like(u3) if out is None else _need(out if not squeeze else out[None], u3.shape, "out")
+    L = lib()
+    ws = (workspace or _WS).get(int(L.ocx_vertical_implicit_workspace_bytes(n, M)))
+    check(L.ocx_vertical_implicit_f64(_ptr(u3), _ptr(dz), _ptr(nu), _ptr(r_bed), r_layers, _ptr(tau_top), nt, n, M,
+                                      float(dt), _ptr(o3), _ptr(ws), ws.numel(), _stream()), "ocx_vertical_implicit_f64")
+    return o3[0] if squeeze else o3
+
+
 def numerical_mixing(c, c2, V1, dt, out=None, accumulate=False):
     """Numerical mixing of the advection V1 (<c^2> - c^2) / dt per cell (``ocx_mix_num_f64``): c, c2 (nz, ny, nx)
     the advected c and c^2 (members of one stack); ``accumulate`` adds to ``out`` (the variance budget's +=).

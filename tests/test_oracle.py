@@ -463,3 +463,78 @@ def test_numerical_mixing_vanishes_for_a_uniform_tracer(oracle_c):
     out = oracle_c.advect_fct(C, k["V0"], k["V1"], k["Qx"], k["Qy"], k["ok"], k["dt"])
     mix = R.numerical_mixing(out[0], out[1], k["V1"], k["dt"])
     assert np.abs(mix).max() <= 1e-12 * (k["V1"].max() * 6.25 / k["dt"])
+
+
+# ------------------------------------------------------------------------------------------------ tracer diffusion
+def _diffusion_grid(n=3, ny=20, nx=24, dx=1000.0, h0=10.0, seed=0, land=False):
+    rng = np.random.default_rng(seed)
+    act = np.ones((ny, nx), bool)
+    if land:
+        act[5:9, 7:12] = False
+    h = np.where(act[None], h0 * (1.0 + 0.2 * rng.random((n, ny, nx))), 0.0)
+    cx = np.full((ny, nx + 1), dx / dx)          # L_u / dx_u
+    cy = np.full((ny + 1, nx), dx / dx)
+    okx = np.ones((ny, nx + 1), bool)
+    oky = np.ones((ny + 1, nx), bool)
+    V = h * dx * dx
+    return h, act, cx, cy, okx, oky, V
+
+
+def test_horizontal_diffusion_conserves_and_keeps_uniform_fields():
+    h, act, cx, cy, okx, oky, V = _diffusion_grid(land=True)
+    rng = np.random.default_rng(1)
+    T = np.where(act[None], rng.normal(10, 2, h.shape), 0.0)
+    out = R.diffuse_h_ale(T, T, h, act, cx, cy, okx, oky, V, 50.0, 600.0)
+    assert abs((V * out).sum() - (V * T).sum()) <= 1e-13 * (V * np.abs(T)).sum()
+    U = np.full(h.shape, 3.5)
+    assert np.array_equal(R.diffuse_h_ale(U, U, h, act, cx, cy, okx, oky, V, 50.0, 600.0), U)
+
+
+def test_horizontal_diffusion_spreads_by_two_k_dt_per_step():
+    """Uniform grid and thickness: the discrete Laplacian raises the second moment sum x^2 T by exactly
+    2 K dt / dx^2 sum T per step (in cell units) while the pulse stays away from the edges."""
+    n, ny, nx, dx, K, dt = 1, 1, 41, 1000.0, 30.0, 1000.0
+    h = np.full((n, ny, nx), 10.0)
+    act = np.ones((ny, nx), bool)
+    cx, cy = np.ones((ny, nx + 1)), np.ones((ny + 1, nx))
+    okx, oky = np.ones((ny, nx + 1), bool), np.ones((ny + 1, nx), bool)
+    V = h * dx * dx
+    x = np.arange(nx) - nx // 2
+    T = np.exp(-0.5 * (x / 3.0) ** 2)[None, None, :]
+    out = R.diffuse_h_ale(T, T, h, act, cx, cy, okx, oky, V, K, dt)
+    m2 = lambda c: (x * x * c[0, 0]).sum()                             # noqa: E731
+    assert abs(m2(out) - m2(T) - 2 * K * dt / dx ** 2 * T.sum()) <= 1e-10 * m2(T)
+
+
+def test_horizontal_diffusion_respects_closed_faces():
+    h, act, cx, cy, okx, oky, V = _diffusion_grid()
+    okx[:, 12] = False                                                  # a wall between columns 11 and 12
+    rng = np.random.default_rng(2)
+    T = rng.normal(0, 1, h.shape)
+    out = R.diffuse_h_ale(T, T, h, act, cx, cy, okx, oky, V, 80.0, 900.0)
+    for half in (np.s_[..., :12], np.s_[..., 12:]):
+        assert abs((V * out)[half].sum() - (V * T)[half].sum()) <= 1e-12 * (V * np.abs(T)).sum()
+
+
+def test_vertical_implicit_matches_a_banded_solver_and_conserves():
+    from scipy.linalg import solve_banded
+    rng = np.random.default_rng(3)
+    n, M, dt = 12, 40, 600.0
+    dz = rng.uniform(1e-3, 30.0, (n, M))
+    nu = rng.uniform(1e-6, 1e-1, (n + 1, M))
+    u = rng.normal(10.0, 3.0, (n, M))
+    got = R.vertical_implicit(u, dz, nu, np.zeros(M), dt)
+    for col in range(M):
+        hf = 0.5 * (dz[1:, col] + dz[:-1, col])
+        c = dt * nu[1:-1, col] / hf
+        ab = np.zeros((3, n))
+        ab[0, 1:] = -c
+        ab[2, :-1] = -c
+        ab[1] = dz[:, col]
+        ab[1, 1:] += c
+        ab[1, :-1] += c
+        ref = solve_banded((1, 1), ab, dz[:, col] * u[:, col])
+        np.testing.assert_allclose(got[:, col], ref, rtol=1e-11)
+    np.testing.assert_allclose((dz * got).sum(0), (dz * u).sum(0), rtol=1e-13)
+    U = np.full((n, M), 2.0)
+    np.testing.assert_allclose(R.vertical_implicit(U, dz, nu, np.zeros(M), dt), U, rtol=1e-13)
