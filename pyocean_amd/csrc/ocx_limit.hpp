@@ -43,7 +43,7 @@ struct LimitArgs {
     unsigned flags;
 };
 
-// Work decomposition: a workgroup holds LG consecutive layers x (up to MAXW) tracers of one (strip, chunk); its
+// Work decomposition: a workgroup holds LG consecutive layers x (up to LMAXW) tracers of one (strip, chunk); its
 // waves march the rows in step (one workgroup barrier per row).  Each wave publishes the bound contributions of its
 // newest row in LDS, so a wave reads those of layers k +- 1 from the waves above and below it and only the group's
 // bottom and top layers read their outer neighbour layer from memory; the waves of one layer (its tracers) read the
