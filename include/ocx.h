@@ -37,6 +37,7 @@ extern "C" {
 #define OCX_E_ARG (-1)        /* invalid shape, pointer or flag */
 #define OCX_E_HIP (-2)        /* a HIP runtime call failed */
 #define OCX_E_WORKSPACE (-3)  /* workspace missing or too small */
+#define OCX_E_RANGE (-4)      /* the inputs violate a bound the reference raises on (an emptied layer, CFL) */
 
 const char* ocx_last_error(void);
 int ocx_version(void);
@@ -186,6 +187,72 @@ int ocx_set_tuning(int rows_per_chunk, int waves_per_group);
  * milliseconds, launch counts), then forgets the records. */
 int ocx_profile_enable(int on);
 int ocx_profile_collect(int max_kernels, const char** names, double* ms_total, long long* launches);
+
+/* ---------------------------------------------------------------------------------------- ALE layer transport
+ * The producer of the advection step's inputs (Qx, Qy, V0, V1, ok) in the ALE vertical coordinate:
+ * ALELayerTransport.__call__ (core/components/layer_transport.py:111-156) without open boundaries and freshwater.
+ * Face arrays of one direction in their natural layout: axis 0 the u-faces (n, ny, nx+1) with cells along x,
+ * axis 1 the v-faces (n, ny+1, nx) with cells along y (the reference calls the same functions on swapped axes;
+ * its (n, nx, ny+1) arrays are the transposes of these).  Cells (n, ny, nx); 2-D cell fields (ny, nx); active one
+ * byte 0/1.  q_pos is the transport thickness for flow towards + (from the low-index cell, the reference's q_e /
+ * q_n), q_neg for flow towards - (q_w / q_s). */
+
+/* u^θ and Q_bt of one face direction, columns = the face count of one layer.
+ * Replaces transport_velocity / _transport_velocity_kernel (core/vertical/ale.py:789-829).  theta_bc: pass theta
+ * for the reference's theta_bc=None. */
+int ocx_transport_velocity_f64(const double* u_new, const double* u_old, const uint8_t* last_wet, const double* w,
+                               const double* L, const double* D, int n, long long columns, double theta,
+                               double theta_bc, double* ut, double* Q_bt, void* stream);
+
+/* Transport thickness at the faces of one axis; clip: cut at the shallower bed (H the rest depth, NULL without
+ * clip).  Replaces face_transport_thickness / _ftt_kernel (core/vertical/ale.py:168-230). */
+int ocx_face_transport_thickness_f64(const double* h, const double* H, const uint8_t* active, int n, int ny, int nx,
+                                     int axis, int clip, double* q_pos, double* q_neg, void* stream);
+
+/* PPM edge thicknesses along one axis (e_lo: west / south edge, e_hi: east / north), cells (n, ny, nx).
+ * Replaces ppm_edges_x / _ppm_edges_kernel (core/vertical/ale.py:440-476, 578-623). */
+int ocx_ppm_edges_f64(const double* h, const uint8_t* active, int n, int ny, int nx, int axis, double h_min,
+                      double* e_lo, double* e_hi, void* stream);
+
+/* Layer volume fluxes with PPM thickness and the Newton correction per face column (all columns iterate until all
+ * have converged, at most max_iter passes), then the residual share.  d_t: the cell width along the axis (ny, nx);
+ * L, Q_bt: face arrays of one layer.  Replaces ppm_transport_x / _ppm_transport_kernel (core/vertical/ale.py:626-764).
+ * ocx_ppm_transport_iterations reads back (synchronously) how many Newton passes the last call on this workspace
+ * took. */
+size_t ocx_ppm_transport_workspace_bytes(int n, int ny, int nx, int axis, int max_iter);
+int ocx_ppm_transport_f64(const double* h, const double* ut, const double* L, const double* Q_bt, const double* q_pos,
+                          const double* q_neg, const uint8_t* active, const double* d_t, int n, int ny, int nx,
+                          int axis, double dt, int max_iter, double* out, void* workspace, size_t workspace_bytes,
+                          void* stream);
+int ocx_ppm_transport_iterations(const void* workspace, int n, int ny, int nx, int axis, int max_iter, void* stream);
+
+/* Outflow limiter, in place on Qx and Qy; host-synchronous (the number of passes depends on the data).
+ * Replaces limit_outflow / _give_back / _outflow_exceeds (core/vertical/ale.py:299-437).  check_only: only test
+ * whether any active cell-layer sends out more than it holds.  status: OCX_OUTFLOW_*; n_limited: cell-layers scaled
+ * over all passes.  On OCX_OUTFLOW_EMPTY_DONOR where = (axis, a, f) of the face; on OCX_OUTFLOW_ACCEPTED /
+ * NEGATIVE where = (k, j, i) of the lowest h and value = that h. */
+#define OCX_OUTFLOW_NOOP 0          /* nothing exceeded: Qx, Qy untouched */
+#define OCX_OUTFLOW_OK 1            /* limited; no cell-layer exceeds any more */
+#define OCX_OUTFLOW_ACCEPTED 2      /* still exceeding after max_pass but h >= -h_floor (the reference warns) */
+#define OCX_OUTFLOW_EXCEEDS 3       /* check_only: some cell-layer exceeds */
+#define OCX_OUTFLOW_EMPTY_DONOR 4   /* returned with OCX_E_RANGE: a cut has nowhere to go */
+#define OCX_OUTFLOW_NEGATIVE 5      /* returned with OCX_E_RANGE: h would fall below -h_floor */
+size_t ocx_limit_outflow_workspace_bytes(int n, int ny, int nx);
+int ocx_limit_outflow_f64(const double* h, double* Qx, double* Qy, const double* area, const uint8_t* active, int n,
+                          int ny, int nx, double dt, double h_floor, int max_pass, int periodic_x, int periodic_y,
+                          int check_only, void* workspace, size_t workspace_bytes, long long* n_limited, int* status,
+                          long long* where, double* value, void* stream);
+
+/* Lagrangian thickness h† = max(h - dt div / A, 0) with div from divq (n, ny, nx) or from Qx, Qy (then divq NULL);
+ * optionally V0 = h A, V1 = h† A, ok = active & V1 > 0.  With bad_index non-NULL the call is host-synchronous and
+ * returns OCX_E_RANGE with the first cell-layer below -tol (linear index, else -1).
+ * Replaces lagrangian_thickness (core/vertical/ale.py:278-295) and the volume lines of
+ * ALELayerTransport.__call__ (core/components/layer_transport.py:142-144). */
+size_t ocx_lagrangian_workspace_bytes(void);
+int ocx_lagrangian_thickness_f64(const double* h, const double* divq, const double* Qx, const double* Qy,
+                                 const double* area, const uint8_t* active, int n, int ny, int nx, double dt,
+                                 double tol, double* out, double* V0, double* V1, uint8_t* ok, void* workspace,
+                                 size_t workspace_bytes, long long* bad_index, void* stream);
 
 #ifdef __cplusplus
 }

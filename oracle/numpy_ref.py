@@ -427,3 +427,343 @@ def vertical_implicit(u, dz, nu, r_bed, dt, tau_top=None):
     for k in range(n - 2, -1, -1):
         out[k] = dp[k] - cp[k] * out[k + 1]
     return out
+
+
+# ------------------------------------------------------------------------------------------------ ALE layer transport
+H_PPM_MIN = 2e-10        # positive-definite floor of the PPM thickness (core/vertical/ale.py:298)
+
+
+def face_transport_thickness(h, H, active, clip):
+    """
+    Transport thickness at the u-faces, (q_e, q_w) each (n, ny, nx+1): the upwind layer of the west (q_e) and east
+    (q_w) cell, cut at the shallower bed when ``clip``; 0 on faces without water on both sides.  Follows the
+    compiled ``_ftt_kernel`` (core/vertical/ale.py:168-199): NaN thickness or depth read as 0, running bed-up sums.
+    For the v-faces call it on swapped axes.
+    """
+    h = np.asarray(h, float)
+    n, ny, nx = h.shape
+    act = np.asarray(active, bool)
+    f = np.arange(nx + 1)
+    iw, ie = np.maximum(f - 1, 0), np.minimum(f, nx - 1)
+    open_ = act[:, iw] & act[:, ie]
+    hn = np.where(np.isnan(h), 0.0, h)
+    hw, he = hn[..., iw], hn[..., ie]
+    if clip:
+        Hn = np.asarray(H, float)
+        Hn = np.where(np.isnan(Hn), 0.0, Hn)
+        Hw, He = Hn[:, iw], Hn[:, ie]
+        z_bed = np.maximum(-Hw, -He)
+        cw = np.zeros((ny, nx + 1))
+        ce = np.zeros((ny, nx + 1))
+        qe = np.empty((n, ny, nx + 1))
+        qw = np.empty((n, ny, nx + 1))
+        for k in range(n):
+            cw = cw + hw[k]
+            ce = ce + he[k]
+            qe[k] = np.minimum(hw[k], np.maximum(0.0, (-Hw + cw) - z_bed))
+            qw[k] = np.minimum(he[k], np.maximum(0.0, (-He + ce) - z_bed))
+    else:
+        qe, qw = hw, he
+    return np.where(open_[None], qe, 0.0), np.where(open_[None], qw, 0.0)
+
+
+def transport_velocity(u_new, u_old, last_wet, w, L, D, theta, theta_bc=None):
+    """
+    Transport velocity and free-surface volume flux of the ALE step, (u^θ, Q_bt):
+    u^θ_k = θ_bc u_k' + (1 - θ_bc) u_k + (θ - θ_bc)(Σ w u' - Σ w u), u = 0 on faces dry in the last step,
+    Q_bt = L D Σ_k w_k u^θ_k.  Follows ``_transport_velocity_kernel`` (core/vertical/ale.py:789-808), sums from
+    the bed up.  u (n, a, b); last_wet, L, D (a, b).
+    """
+    tb = theta if theta_bc is None else theta_bc
+    un = np.where(np.asarray(last_wet, bool)[None], u_old, 0.0)
+    n = u_new.shape[0]
+    m1 = np.zeros(u_new.shape[1:])
+    m0 = np.zeros(u_new.shape[1:])
+    for k in range(n):
+        m1 = m1 + w[k] * u_new[k]
+        m0 = m0 + w[k] * un[k]
+    d = (theta - tb) * (m1 - m0)
+    ut = tb * u_new + (1 - tb) * un + d[None]
+    s = np.zeros(u_new.shape[1:])
+    for k in range(n):
+        s = s + w[k] * ut[k]
+    return ut, L * D * s
+
+
+def ppm_edges(h, active, h_min=H_PPM_MIN):
+    """
+    West and east edge thickness (h_w, h_e) of every cell along x: limited slopes (zero next to land and at the
+    domain edge), edges from the neighbour means and slopes, then the positive-definite limiter that scales a
+    parabola whose minimum lies inside the cell, or flattens a cell with h <= h_min.  Follows the compiled
+    ``_ppm_edges_kernel`` (core/vertical/ale.py:578-623) term for term, including its (3 dh) dh; the input is
+    read as ``ppm_transport_x`` passes it (NaN -> 0, inactive cells 0).
+    """
+    act = np.asarray(active, bool)
+    h = np.where(act[None], np.nan_to_num(np.asarray(h, float)), 0.0)
+    n, ny, nx = h.shape
+    m = act.astype(float)
+    mw = np.zeros((ny, nx))
+    me = np.zeros((ny, nx))
+    mw[:, 1:] = m[:, :-1]
+    me[:, :-1] = m[:, 1:]
+    hm1 = h.copy()
+    hp1 = h.copy()
+    hm1[..., 1:] = mw[None, :, 1:] * h[..., :-1] + (1.0 - mw[None, :, 1:]) * h[..., 1:]
+    hp1[..., :-1] = me[None, :, :-1] * h[..., 1:] + (1.0 - me[None, :, :-1]) * h[..., :-1]
+    s = 0.5 * (hp1 - hm1)
+    dmx = np.maximum(np.maximum(hp1, hm1), h) - h
+    dmn = h - np.minimum(np.minimum(hp1, hm1), h)
+    a = np.minimum(np.abs(s), 2.0 * np.minimum(dmx, dmn))
+    s = np.where(s > 0.0, a, np.where(s < 0.0, -a, 0.0))
+    slp = np.where((mw * me * m > 0.0)[None], s, 0.0)
+    sw = np.zeros_like(slp)
+    se = np.zeros_like(slp)
+    sw[..., 1:] = slp[..., :-1]
+    se[..., :-1] = slp[..., 1:]
+    hw = 0.5 * (hm1 + h) + (sw - slp) / 6.0
+    he = 0.5 * (hp1 + h) + (slp - se) / 6.0
+    curv = 3.0 * ((hw + he) - 2.0 * h)
+    dh = he - hw
+    inside = (curv > 0.0) & (np.abs(dh) < curv)
+    flat = inside & (h <= h_min)
+    num = 12.0 * curv * (h - h_min)
+    den = curv * curv + 3.0 * dh * dh
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sc = np.where(inside & (num < den), num / den, 1.0)
+    out_w = np.where(flat, h, h + sc * (hw - h))
+    out_e = np.where(flat, h, h + sc * (he - h))
+    return out_w, out_e
+
+
+def _ppm_face_flux(u, h, h_w, h_e, dx_t, q_e, q_w, L, dt):
+    """(F, dF/du) of every layer at every u-face for face velocities u (n, ny, nx+1): the swept-volume mean
+    thickness of the donor cell's parabola and its derivative factor, cut by the transport thickness where that
+    is below the donor layer.  Follows ``_face_flux`` (core/vertical/ale.py:626-648)."""
+    nx = h.shape[-1]
+    f = np.arange(nx + 1)
+    iw, ie = np.maximum(f - 1, 0), np.minimum(f, nx - 1)
+    pos = u >= 0.0
+    dxd = np.where(pos, dx_t[None][..., iw], dx_t[None][..., ie])
+    c = np.minimum(np.abs(u) * dt / dxd, 1.0)
+    hu = np.where(pos, h[..., iw], h[..., ie])
+    ww = np.where(pos, h_w[..., iw], h_w[..., ie])
+    ee = np.where(pos, h_e[..., iw], h_e[..., ie])
+    curv3 = (ww + ee) - 2.0 * hu
+    dh = np.where(pos, ww - ee, ee - ww)
+    edge = np.where(pos, ee, ww)
+    q = np.where(pos, q_e, q_w)
+    mean = edge + c * (0.5 * dh + curv3 * (c - 1.5))
+    marg = edge + c * (dh + 3.0 * curv3 * (c - 1.0))
+    clip = q < hu
+    mean = np.where(clip, np.minimum(mean, q), mean)
+    marg = np.where(clip, np.minimum(marg, q), marg)
+    return L[None] * u * np.maximum(mean, 0.0), L[None] * np.maximum(marg, 0.0)
+
+
+def ppm_transport_x(h, ut, L, Q_bt, q_e, q_w, active, dt, dx_t, max_iter=20, return_iterations=False):
+    """
+    Layer volume fluxes at the u-faces with PPM thickness and a uniform velocity correction du per face column,
+    found by Newton's method with bracketing bisection so that Σ_k F_k(u_k + du) = Q_bt; every column iterates
+    until all columns have converged (|err| <= 1e-12 max(|Q_bt|, 1)) or ``max_iter`` passes, then the remaining
+    residual R = Q_bt - Σ F is shared over the layers in proportion to the transport thickness on R's side.
+    Follows the compiled ``_ppm_transport_kernel`` (core/vertical/ale.py:651-740) and its wrapper (:743-764); sums
+    run from the bed up.  For the v-faces call it on swapped axes.
+    """
+    act = np.asarray(active, bool)
+    hc = np.where(act[None], np.nan_to_num(np.asarray(h, float)), 0.0)
+    n, ny, nx = hc.shape
+    h_w, h_e = ppm_edges(hc, act)
+    ut = np.asarray(ut, float)
+    L = np.asarray(L, float)
+    Q_bt = np.asarray(Q_bt, float)
+    q_e = np.asarray(q_e, float)
+    q_w = np.asarray(q_w, float)
+    dx_t = np.asarray(dx_t, float)
+    f = np.arange(nx + 1)
+    iw, ie = np.maximum(f - 1, 0), np.minimum(f, nx - 1)
+    open_ = act[:, iw] & act[:, ie]
+    limw, lime = 0.5 * dx_t[:, iw] / dt, 0.5 * dx_t[:, ie] / dt
+    mnw = np.full((ny, nx + 1), np.inf)
+    mxe = np.full((ny, nx + 1), -np.inf)
+    for k in range(n):
+        mnw = np.minimum(mnw, limw - ut[k])
+        mxe = np.maximum(mxe, -(lime + ut[k]))
+    hi = np.maximum(np.minimum(limw, mnw), 0.0)
+    lo = np.minimum(np.maximum(-lime, mxe), 0.0)
+
+    def evaluate(u):
+        F, dF = _ppm_face_flux(u, hc, h_w, h_e, dx_t, q_e, q_w, L, dt)
+        s = np.zeros((ny, nx + 1))
+        sd = np.zeros((ny, nx + 1))
+        for k in range(n):
+            s = s + F[k]
+            sd = sd + dF[k]
+        return F, np.where(open_, s, 0.0) - Q_bt, np.where(open_, sd, 0.0)
+
+    du = np.zeros((ny, nx + 1))
+    F, err, dsum = evaluate(ut)
+    iters = 0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for _ in range(max_iter):
+            hi = np.where(err > 0.0, du, hi)
+            lo = np.where(err < 0.0, du, lo)
+            step = np.where(dsum > 0.0, -err / dsum, 0.0)
+            new = du + step
+            new = np.where((step > 0.0) & (new >= hi), 0.5 * (du + hi), new)
+            new = np.where((step < 0.0) & (new <= lo), 0.5 * (du + lo), new)
+            du = new
+            F, err, dsum = evaluate(ut + du[None])
+            iters += 1
+            if not (np.abs(err) > 1e-12 * np.maximum(np.abs(Q_bt), 1.0)).any():
+                break
+    s = np.zeros((ny, nx + 1))
+    te = np.zeros((ny, nx + 1))
+    tw = np.zeros((ny, nx + 1))
+    for k in range(n):
+        s = s + F[k]
+        te = te + q_e[k]
+        tw = tw + q_w[k]
+    R = Q_bt - s
+    use_e = R >= 0.0
+    tot = np.where(use_e, te, tw)
+    use_e = np.where(tot <= 0.0, ~use_e, use_e)
+    tot = np.where(use_e, te, tw)
+    den = np.where(tot > 0.0, tot, 1.0)
+    out = np.where(open_[None], F + R[None] * np.where(use_e[None], q_e, q_w) / den[None], 0.0)
+    return (out, iters) if return_iterations else out
+
+
+def lagrangian_thickness(h, divq, dt, area, tol=1e-9):
+    """
+    Layer thickness after the Lagrangian step, h - Δt ∇·Q / A (A = 1 where the area is 0); values in [-tol, 0) are
+    set to 0 and a layer emptied beyond that raises with its (k, j, i).  Follows ``lagrangian_thickness``
+    (core/vertical/ale.py:278-295).
+    """
+    out = h - dt * divq / np.where(area > 0, area, 1.0)[None]
+    bad = out < -tol
+    if bad.any():
+        k, j, i = (int(a[0]) for a in np.nonzero(bad))
+        raise RuntimeError(f"layer {k} of cell (j={j}, i={i}) emptied in the Lagrangian step: "
+                           f"h = {out[k, j, i]:.3e} m (reduce dt)")
+    return np.maximum(out, 0.0)
+
+
+def outflow(Qx, Qy):
+    """Outgoing volume flux of every cell-layer over its four faces (core/vertical/ale.py:301-304)."""
+    return (np.maximum(Qx[..., 1:], 0.0) - np.minimum(Qx[..., :-1], 0.0)
+            + np.maximum(Qy[:, 1:, :], 0.0) - np.minimum(Qy[:, :-1, :], 0.0))
+
+
+def _sides(a, ax, wrap, fill):
+    """Cell values on the low and high side of every face along axis ``ax``: (low, high), each one face longer
+    than ``a``; the edge faces take ``fill`` or, on a periodic axis, the cell across the wrap."""
+    first, last = np.take(a, [0], axis=ax), np.take(a, [-1], axis=ax)
+    pad = np.full_like(first, fill)
+    low = np.concatenate([last if wrap else pad, a], axis=ax)
+    high = np.concatenate([a, first if wrap else pad], axis=ax)
+    return low, high
+
+
+def _sum_k(a):
+    s = np.zeros(a.shape[1:])
+    for k in range(a.shape[0]):
+        s = s + a[k]
+    return s
+
+
+def _give_back(Q, delta, room, ax, full, wrap):
+    """The net cut of each face back on its layers, in place: to same-way layers by donor room, then off
+    counter-flowing layers by |Q| (never past 0), then on the donor's layers by content.  Follows ``_give_back``
+    (core/vertical/ale.py:399-437); raises as it does when a donor column holds no water."""
+    room_lo, room_hi = _sides(room, ax, wrap, 0.0)
+    room_lo, room_hi = np.maximum(room_lo, 0.0), np.maximum(room_hi, 0.0)
+    s = np.sign(delta)[None]
+    same = (np.sign(Q) == s) & (Q != 0.0) & (s != 0.0)
+    rm = np.where(same, np.where(s > 0, room_lo, room_hi), 0.0)
+    tot = _sum_k(rm)
+    need = np.abs(delta)
+    give = np.minimum(need, tot)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        Q += s * rm * np.where(tot > 0, give / np.where(tot > 0, tot, 1.0), 0.0)[None]
+        rest = need - give
+        opp = (np.sign(Q) == -s) & (s != 0.0)
+        cap = _sum_k(np.where(opp, np.abs(Q), 0.0))
+        frac = np.where(cap > 0, np.minimum(rest, cap) / np.where(cap > 0, cap, 1.0), 0.0)[None]
+        Q -= np.where(opp, Q * frac, 0.0)
+        rest = rest - np.minimum(rest, cap)
+        full_lo, full_hi = _sides(full, ax, wrap, 0.0)
+        fm = np.where(s > 0, full_lo, full_hi) * (rest > 0)[None]
+        tot = _sum_k(fm)
+        if ((rest > 0) & (tot <= 0)).any():
+            idx = tuple(int(a[0]) for a in np.nonzero((rest > 0) & (tot <= 0)))
+            raise RuntimeError(f"limit_outflow: face {idx} must carry {float(rest[idx]):.3g} m³/s out of a donor "
+                               f"column that holds no water — barotropic CFL")
+        Q += s * fm * np.where(tot > 0, rest / np.where(tot > 0, tot, 1.0), 0.0)[None]
+
+
+def limit_outflow(h, Qx, Qy, area, dt, active, h_floor=H_PPM_MIN, max_pass=50, periodic=(False, False)):
+    """
+    Outflow limiter of the Lagrangian step: a cell-layer whose outgoing fluxes would take more than
+    (h - h_floor)+ A in Δt has them scaled down to that, and each face's net cut goes back to the other layers of
+    the face (``_give_back``), so Σ_k Q_k is unchanged; repeated until no cell-layer exceeds (<= ``max_pass``).
+    Returns (Qx, Qy, n_limited, status) with status "ok", "noop" (inputs returned) or "accepted" (still exceeding
+    after max_pass but h >= -h_floor everywhere); raises where the reference raises.  Follows ``limit_outflow``
+    (core/vertical/ale.py:331-396), sums over the layers from the bed up.
+    """
+    act = np.asarray(active, bool)
+    A = np.where(act, area, 0.0)[None]
+    room0 = np.maximum(h - h_floor, 0.0) * A / dt
+    O = outflow(Qx, Qy)
+    over = act[None] & (O > room0)
+    if not over.any():
+        return Qx, Qy, 0, "noop"
+    Qx, Qy, n_lim = Qx.copy(), Qy.copy(), 0
+    for _ in range(max_pass):
+        n_lim += int(over.sum())
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.where(over, room0 / np.where(O > 0, O, 1.0), 1.0)
+        for Q, ax, wrap in ((Qx, 2, periodic[0]), (Qy, 1, periodic[1])):
+            r_lo, r_hi = _sides(r, ax, wrap, 1.0)
+            new = np.where(Q > 0, Q * r_lo, np.where(Q < 0, Q * r_hi, Q))
+            delta = _sum_k(Q - new)
+            Q[...] = new
+            _give_back(Q, delta, room0 - outflow(Qx, Qy), ax, room0, wrap)
+        O = outflow(Qx, Qy)
+        over = act[None] & (O > room0 * (1.0 + 1e-12))
+        if not over.any():
+            return Qx, Qy, n_lim, "ok"
+    div = Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:] - Qy[:, :-1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        h_dag = np.where(act[None], h - dt * div / np.where(A > 0, A, 1.0), 0.0)
+    if h_dag.min() >= -h_floor:
+        return Qx, Qy, n_lim, "accepted"
+    k, j, i = (int(a[0]) for a in np.nonzero(h_dag == h_dag.min()))
+    raise RuntimeError(f"limit_outflow: layer {k} of cell (j={j}, i={i}) would reach h = {h_dag.min():.3g} m after "
+                       f"{max_pass} passes — barotropic CFL")
+
+
+def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_new, v_old, wet_u, wet_v, wu, wv,
+                        Du, Dv, dt, theta, theta_bc=None, clip=True, max_iter=20):
+    """
+    One ALE layer-transport step with PPM thickness: transport velocities, transport thicknesses (the vertical
+    coordinate's, clipped at the shallower bed for z*), PPM fluxes of both directions (the v-faces on swapped axes),
+    the outflow limiter, the Lagrangian thickness and the layer volumes.  Follows ``ALELayerTransport.__call__``
+    (core/components/layer_transport.py:111-156) without open boundaries and freshwater, with
+    ``transport_thickness`` (core/components/vertical_coordinate.py:125-140) and ``layer_volumes``
+    (core/dynamics/sigma_baroclinic.py:1812-1816).  Returns dict(Qx, Qy, V0, V1, ok, h_dagger, n_limited).
+    """
+    act = np.asarray(active, bool)
+    utx, Qbx = transport_velocity(u_new, u_old, wet_u, wu, Lu, Du, theta, theta_bc)
+    uty, Qby = transport_velocity(v_new, v_old, wet_v, wv, Lv, Dv, theta, theta_bc)
+    qe, qw = face_transport_thickness(h, H, act, clip)
+    qn, qs = face_transport_thickness(h.swapaxes(1, 2), H.T, act.T, clip)
+    Qx = ppm_transport_x(h, utx, Lu, Qbx, qe, qw, act, dt, dx_t, max_iter)
+    Qy = ppm_transport_x(h.swapaxes(1, 2), uty.swapaxes(1, 2), Lv.T, Qby.T, qn, qs, act.T, dt, dy_t.T,
+                         max_iter).swapaxes(1, 2)
+    Qx, Qy, n_lim, _ = limit_outflow(h, Qx, Qy, area, dt, act)
+    divh = Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:, :] - Qy[:, :-1, :]
+    h_lag = lagrangian_thickness(h, divh, dt, area)
+    V0, V1 = h * area[None], h_lag * area[None]
+    return dict(Qx=np.ascontiguousarray(Qx), Qy=np.ascontiguousarray(Qy), V0=V0, V1=V1, ok=act[None] & (V1 > 0),
+                h_dagger=h_lag, n_limited=n_lim, qe=qe, qw=qw, qn=np.ascontiguousarray(qn.swapaxes(1, 2)),
+                qs=np.ascontiguousarray(qs.swapaxes(1, 2)), utx=utx, uty=uty, Qbx=Qbx, Qby=Qby)

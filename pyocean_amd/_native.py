@@ -18,7 +18,8 @@ WENO_JS = 16
 UNLIMITED = 32
 SPLIT = 64
 
-OCX_E_ARG, OCX_E_HIP, OCX_E_WORKSPACE = -1, -2, -3
+OCX_E_ARG, OCX_E_HIP, OCX_E_WORKSPACE, OCX_E_RANGE = -1, -2, -3, -4
+OUTFLOW_NOOP, OUTFLOW_OK, OUTFLOW_ACCEPTED, OUTFLOW_EXCEEDS, OUTFLOW_EMPTY_DONOR, OUTFLOW_NEGATIVE = range(6)
 
 # (name, restype, argtypes) of every exported symbol; the CPU tests check that each one is exported
 P, I, U, D, Z, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_double, ctypes.c_size_t, ctypes.c_longlong
@@ -42,6 +43,16 @@ SIGNATURES = {
     "ocx_vertical_implicit_f64": (I, [P, P, P, P, I, P, I, I, L, D, P, P, Z, P]),
     "ocx_content_workspace_bytes": (Z, [I]),
     "ocx_content_f64": (I, [P, P, I, I, I, I, P, P, Z, P]),
+    "ocx_transport_velocity_f64": (I, [P, P, P, P, P, P, I, L, D, D, P, P, P]),
+    "ocx_face_transport_thickness_f64": (I, [P, P, P, I, I, I, I, I, P, P, P]),
+    "ocx_ppm_edges_f64": (I, [P, P, I, I, I, I, D, P, P, P]),
+    "ocx_ppm_transport_workspace_bytes": (Z, [I, I, I, I, I]),
+    "ocx_ppm_transport_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, I, P, P, Z, P]),
+    "ocx_ppm_transport_iterations": (I, [P, I, I, I, I, I, P]),
+    "ocx_limit_outflow_workspace_bytes": (Z, [I, I, I]),
+    "ocx_limit_outflow_f64": (I, [P, P, P, P, P, I, I, I, D, D, I, I, I, I, P, Z, P, P, P, P, P]),
+    "ocx_lagrangian_workspace_bytes": (Z, []),
+    "ocx_lagrangian_thickness_f64": (I, [P, P, P, P, P, P, I, I, I, D, D, P, P, P, P, P, Z, P, P]),
     "ocx_batch_create": (I, [P, I, I, I, I, P]),
     "ocx_batch_step": (I, [P, I, P]),
     "ocx_batch_graph_step": (I, [P, I, P]),

@@ -16,31 +16,18 @@
 #include "ocx_limit.hpp"
 #include "ocx_batch.hpp"
 #include "ocx_diffuse.hpp"
+#include "ocx_host.hpp"
+
+namespace ocx_host {
+thread_local std::string g_err;
+std::mutex g_prof_mu;
+std::vector<ProfRec> g_prof;
+bool g_prof_on = false;
+}  // namespace ocx_host
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-int hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) return fail(OCX_E_HIP, "%s: %s", what, hipGetErrorString(e));
-    return OCX_OK;
-}
-
-#define OCX_TRY(expr)              \
-    do {                           \
-        int _rc = (expr);          \
-        if (_rc != OCX_OK) return _rc; \
-    } while (0)
+using namespace ocx_host;
 
 constexpr int TX = 64, TY = 4;
 
@@ -54,26 +41,6 @@ int check_shape(int nt, int nz, int ny, int nx) {
         return fail(OCX_E_ARG, "plane ny=%d x nx=%d too large (at most 2^28 cells per layer)", ny, nx);
     return OCX_OK;
 }
-
-int need(const void* p, const char* name) {
-    if (!p) return fail(OCX_E_ARG, "%s is NULL", name);
-    return OCX_OK;
-}
-
-size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
-
-// Carves a workspace into consecutive 256-byte-aligned pieces.
-struct Carver {
-    char* base;
-    size_t used = 0;
-    explicit Carver(void* b) : base(static_cast<char*>(b)) {}
-    template <typename T>
-    T* take(size_t count) {
-        T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
-        used += align256(count * sizeof(T));
-        return p;
-    }
-};
 
 struct Sizes {
     size_t S, SX, SY, P;
@@ -169,43 +136,6 @@ dim3 tile_grid(int nt, int nz, int ny, int nx) {
     const long long planes = (long long)nt * nz;
     return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + TY - 1) / TY), (unsigned)std::min(planes, 65535LL));
 }
-
-int launch_check(const char* what) { return hip_check(hipGetLastError(), what); }
-
-// Optional per-kernel timing: a hipEvent pair on the launch stream around every kernel launch while enabled.
-struct ProfRec {
-    const char* name;
-    hipEvent_t a, b;
-};
-std::mutex g_prof_mu;
-std::vector<ProfRec> g_prof;
-bool g_prof_on = false;
-
-struct ProfScope {
-    hipStream_t s;
-    bool on;
-    ProfRec rec{};
-    ProfScope(const char* name, hipStream_t st) : s(st), on(g_prof_on) {
-        if (!on) return;
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (st && hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
-            on = false;                // no event nodes inside a captured graph
-            return;
-        }
-        rec.name = name;
-        if (hipEventCreate(&rec.a) != hipSuccess || hipEventCreate(&rec.b) != hipSuccess) {
-            on = false;
-            return;
-        }
-        (void)hipEventRecord(rec.a, s);
-    }
-    ~ProfScope() {
-        if (!on) return;
-        (void)hipEventRecord(rec.b, s);
-        std::lock_guard<std::mutex> g(g_prof_mu);
-        g_prof.push_back(rec);
-    }
-};
 
 }  // namespace
 

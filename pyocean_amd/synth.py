@@ -247,3 +247,91 @@ def batch_shapes(count=64, seed=7):
         nz = int(rng.choice([20, 30, 50, 75]))
         out.append((nx, ny, nz))
     return out
+
+
+# ------------------------------------------------------------------------------------------------ ALE producer inputs
+@dataclass
+class AleInputs:
+    """Inputs of one ALE layer-transport step (core/components/layer_transport.py:111-156) on the regional grid."""
+    h: np.ndarray             # (N, ny, nx) layer thickness at step n, bed first
+    H: np.ndarray             # (ny, nx) rest depth
+    active: np.ndarray        # (ny, nx)
+    area: np.ndarray
+    dx_t: np.ndarray
+    dy_t: np.ndarray
+    Lu: np.ndarray            # (ny, nx+1) u-face lengths
+    Lv: np.ndarray            # (ny+1, nx)
+    u_new: np.ndarray         # (N, ny, nx+1)
+    u_old: np.ndarray
+    v_new: np.ndarray         # (N, ny+1, nx)
+    v_old: np.ndarray
+    wet_u: np.ndarray         # (ny, nx+1) faces wet in the last free-surface step
+    wet_v: np.ndarray
+    wu: np.ndarray            # (N, ny, nx+1) face fractions
+    wv: np.ndarray
+    Du: np.ndarray            # (ny, nx+1) face depths
+    Dv: np.ndarray
+    dt: float
+    theta: float = 0.6
+    theta_bc: float = 0.5
+    info: dict = field(default_factory=dict)
+
+
+def _face_fractions(hf):
+    """w_k = h_k,f / Σ_k h_k,f, uniform 1/N where the face holds no water (core/vertical/ale.py:256-261)."""
+    tot = hf.sum(axis=0)
+    wet = tot > 0
+    return np.where(wet[None], hf / np.where(wet, tot, 1.0)[None], 1.0 / hf.shape[0])
+
+
+def ale_inputs(nx=1024, ny=1024, nz=50, courant=0.3, shear=0.1, seed=0) -> AleInputs:
+    """
+    The regional grid's z* layers and a velocity field for the ALE layer transport: the barotropic stream-function
+    velocity of ``regional`` plus a baroclinic shear (``shear`` x the rms, a smooth random field times a profile
+    decaying with depth) and, for the old step, the same field turned by a few per cent; face fractions and depths
+    from the mean face thickness (q_E + q_W)/2 of the z* layers (clipped at the shallower bed).  dt from the face
+    Courant number ``courant`` of the new velocity.
+    """
+    g = _regional_grid(nx, ny, nz, seed)
+    _, _, u, v, dyu, dxv = _regional_transports(g, seed)
+    h, active, H = g["h"], g["active"], g["H"]
+    rng = np.random.default_rng(seed + 11)
+    prof = np.exp(g["zc"] / 500.0)                                     # (nz, ny, nx), 1 at the surface
+    pu = np.concatenate([prof[..., :1], prof], axis=-1)
+    pv = np.concatenate([prof[:, :1], prof], axis=1)
+    wet_u = np.zeros((ny, nx + 1), bool)
+    wet_u[:, 1:-1] = active[:, 1:] & active[:, :-1]
+    wet_v = np.zeros((ny + 1, nx), bool)
+    wet_v[1:-1] = active[1:] & active[:-1]
+    urms = np.sqrt(np.mean(u[wet_u] ** 2))
+    su = _smooth_noise(rng, ny, nx + 1, max(2.0, 16.0 * min(nx, ny) / 1024.0))
+    sv = _smooth_noise(rng, ny + 1, nx, max(2.0, 16.0 * min(nx, ny) / 1024.0))
+    u_new = (u[None] + shear * urms * su[None] * (pu - pu.mean(axis=0))) * wet_u[None]
+    v_new = (v[None] + shear * urms * sv[None] * (pv - pv.mean(axis=0))) * wet_v[None]
+    turn = 0.03
+    u_old = (1.0 - turn) * u_new + turn * np.roll(u_new, 1, axis=-1) * wet_u[None]
+    v_old = (1.0 - turn) * v_new + turn * np.roll(v_new, 1, axis=-2) * wet_v[None]
+    # mean face thickness of the z* layers clipped at the shallower bed (face_transport_thickness, clip=True)
+    hz = np.where(active[None], h, 0.0)
+
+    def mean_face(hc, Hc, act):
+        n_, a_, b_ = hc.shape
+        f = np.arange(b_ + 1)
+        iw, ie = np.maximum(f - 1, 0), np.minimum(f, b_ - 1)
+        op = act[:, iw] & act[:, ie]
+        z_bed = np.maximum(-Hc[:, iw], -Hc[:, ie])
+        qe = np.minimum(hc[..., iw], np.maximum(0.0, (-Hc[:, iw] + np.cumsum(hc[..., iw], axis=0)) - z_bed))
+        qw = np.minimum(hc[..., ie], np.maximum(0.0, (-Hc[:, ie] + np.cumsum(hc[..., ie], axis=0)) - z_bed))
+        return np.where(op[None], 0.5 * (qe + qw), 0.0)
+
+    hfu = mean_face(hz, H, active)
+    hfv = mean_face(hz.swapaxes(1, 2), H.T, active.T).swapaxes(1, 2)
+    dx_t, dy_t = g["dx_t"], g["dy_t"]
+    cx = np.abs(u_new[:, :, 1:-1]).max(axis=0) / (0.5 * (dx_t[:, 1:] + dx_t[:, :-1]))
+    cy = np.abs(v_new[:, 1:-1]).max(axis=0) / (0.5 * (dy_t[1:] + dy_t[:-1]))
+    dt = courant / max(cx.max(), cy.max())
+    info = dict(name=f"ale regional {nx}x{ny}x{nz}", land=float(1.0 - active.mean()), dt=float(dt))
+    return AleInputs(h=h, H=H, active=active, area=g["area"], dx_t=dx_t, dy_t=dy_t, Lu=dyu, Lv=dxv,
+                     u_new=u_new, u_old=u_old, v_new=v_new, v_old=v_old, wet_u=wet_u, wet_v=wet_v,
+                     wu=_face_fractions(hfu), wv=_face_fractions(hfv), Du=hfu.sum(axis=0), Dv=hfv.sum(axis=0),
+                     dt=float(dt), info=info)

@@ -58,3 +58,84 @@ def kat_initial(nx=200):
     """The 1-D known-answer profile (docs/manual/figures/make_tracers.py:45-49): square + Gaussian."""
     x = np.arange(nx) + 0.5
     return np.where((x > 20) & (x < 60), 1.0, 0.0) + np.exp(-((x - 130.0) / 10.0) ** 2)
+
+
+# ------------------------------------------------------------------------------------------------ ALE layer transport
+# Inputs shaped like the reference's tests of core/vertical/ale.py (tests/test_vertical_ale.py:160-165, 361-380,
+# 502-516), built with the oracle's transport thickness.
+from oracle import numpy_ref as R_  # noqa: E402
+
+
+def ppm_case(h, u, dt=10.0, dx=100.0, L=10.0, clip=False, H=None, active=None):
+    n, ny, nx = h.shape
+    active = np.ones((ny, nx), bool) if active is None else active
+    H = h.sum(axis=0) if H is None else H
+    qe, qw = R_.face_transport_thickness(h, H, active, clip)
+    return qe, qw, np.full((ny, nx), dx), np.full((ny, nx + 1), L), active
+
+
+def ppm_random(seed, n=6, ny=4, nx=9, clip=False, land=False, vanish=False):
+    """The reference's random PPM case (tests/test_vertical_ale.py:361-380), built with the oracle."""
+    rng = np.random.default_rng(seed)
+    H = rng.uniform(50.0, 300.0, (ny, nx)) if clip else np.full((ny, nx), 100.0)
+    active = (rng.uniform(size=(ny, nx)) > 0.2) if land else np.ones((ny, nx), bool)
+    frac = rng.uniform(0.05, 1.0, (n, ny, nx))
+    if vanish:
+        frac[rng.uniform(size=frac.shape) < 0.3] = 0.0
+        frac[:, 0, 0] = 0.0
+        frac[2, 0, 0] = 1.0
+    h = frac / frac.sum(0) * H[None]
+    ut = rng.normal(0.0, 0.3, (n, ny, nx + 1))
+    qe, qw = R_.face_transport_thickness(h, H, active, clip)
+    dx_t = rng.uniform(800.0, 1200.0, (ny, nx))
+    L = rng.uniform(800.0, 1200.0, (ny, nx + 1))
+    F0 = uncorrected(h, ut, L, qe, qw, active, 60.0, dx_t)
+    Qbt = F0.sum(0) + rng.normal(0.0, 50.0, (ny, nx + 1))
+    return h, ut, L, Qbt, qe, qw, active, dx_t
+
+
+def uncorrected(h, u, L, qe, qw, active, dt, dx_t):
+    """Layer fluxes without the Newton correction (zero on closed faces)."""
+    hc = np.where(active[None], h, 0.0)
+    hw, he = R_.ppm_edges(hc, active)
+    F, _ = R_._ppm_face_flux(u, hc, hw, he, dx_t, qe, qw, L, dt)
+    nx = h.shape[-1]
+    f = np.arange(nx + 1)
+    open_ = active[:, np.maximum(f - 1, 0)] & active[:, np.minimum(f, nx - 1)]
+    return np.where(open_[None], F, 0.0)
+
+
+def thin_divergent_cell():
+    dx, dt = 5000.0, 300.0
+    h = np.array([[[50.0, 50.0, 50.0]], [[0.08, 0.001, 49.9]], [[50.0, 50.0, 50.0]]])
+    area = np.full((1, 3), dx * dx)
+    Qx = np.zeros((3, 1, 4))
+    Qx[1, 0, 1] = -0.459 * 0.001 * dx * dx / dt
+    Qx[1, 0, 2] = +0.553 * 0.001 * dx * dx / dt
+    Qx[0, 0, 1:3] = [0.2, 0.3]
+    Qx[2, 0, 1:3] = [-0.4, 0.1]
+    return h, Qx, np.zeros((3, 2, 3)), area, dt, np.ones((1, 3), bool)
+
+
+def limiter_case(seed, n=5, ny=9, nx=11, periodic=(False, False)):
+    """Thin layers under strong, partly counter-flowing transports in x and y, some land."""
+    rng = np.random.default_rng(seed)
+    act = rng.uniform(size=(ny, nx)) > 0.1
+    h = rng.uniform(1.0, 20.0, (n, ny, nx))
+    h[rng.uniform(size=h.shape) < 0.25] = rng.uniform(1e-4, 5e-3)
+    h = np.where(act[None], h, 0.0)
+    area = rng.uniform(0.8e6, 1.2e6, (ny, nx))
+    dt = 60.0
+    Qx = rng.normal(0.0, 1.0, (n, ny, nx + 1)) * 0.25 * h.mean() * 1000.0 * 0.5
+    Qy = rng.normal(0.0, 1.0, (n, ny + 1, nx)) * 0.25 * h.mean() * 1000.0 * 0.5
+    f = np.arange(nx + 1)
+    openx = act[:, np.maximum(f - 1, 0)] & act[:, np.minimum(f, nx - 1)]
+    g = np.arange(ny + 1)
+    openy = act[np.maximum(g - 1, 0)] & act[np.minimum(g, ny - 1)]
+    Qx = np.where(openx[None], Qx, 0.0)
+    Qy = np.where(openy[None], Qy, 0.0)
+    if periodic[0]:
+        Qx[..., -1] = Qx[..., 0]
+    if periodic[1]:
+        Qy[:, -1] = Qy[:, 0]
+    return h, Qx, Qy, area, dt, act
