@@ -383,6 +383,104 @@ def bench_diffusion(args) -> int:
     return 0
 
 
+# algorithmic HBM bytes of the ALE layer-transport kernels per cell-layer (faces of one direction ~ cells): each
+# donor cell is read by two faces, so a face reads one cell's worth of the cell fields
+ALE_BYTES = {
+    "face_thickness": 8.0 + 16.0,                     # h read, q_pos and q_neg written
+    "transport_velocity": 3 * 8.0 + 8.0,              # u_new, u_old, w read, u^θ written
+    "ppm_edges": 8.0 + 16.0,                          # h read, two edges written
+    "ppm_init": 8.0 + 24.0 + 8.0,                     # u^θ, the donor's h and edges, the donor-side q
+    "ppm_newton": 8.0 + 24.0 + 8.0,                   # the same per Newton pass
+    "ppm_final": 8.0 + 24.0 + 16.0 + 8.0,             # u^θ, donor h and edges, both q, the flux written
+    "outflow_over": 8.0 + 16.0 + 8.0,                 # h, Qx and Qy read (a face per cell each), r written
+    "lagrangian": 8.0 + 16.0 + 24.0 + 1.0,            # h, Qx, Qy read; h†, V0, V1 and ok written
+}
+
+
+def bench_ale(args) -> int:
+    """One ALE layer-transport step (the producer of the tracer step's Qx, Qy, V0, V1, ok) on the regional grid:
+    transport thicknesses of both directions, transport velocities, PPM fluxes with the Newton correction, the
+    outflow limiter (its check, and its passes where a layer would empty), the Lagrangian thickness and volumes."""
+    import torch
+
+    from pyocean_amd import _native, ale, synth
+
+    _log(f"generating ALE inputs {args.nx}x{args.ny}x{args.nz}")
+    a = synth.ale_inputs(args.nx, args.ny, args.nz, seed=0)
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in vars(a).items() if isinstance(v, np.ndarray)}
+    lt = ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
+                            dy_t=d["dy_t"], dt=a.dt, theta=a.theta)
+    nz, ny, nx = a.h.shape
+    state = {}
+
+    def step(count=False):
+        qe, qw = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 0)
+        qn, qs = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 1)
+        state["out"] = lt(d["h"], d["u_new"], d["u_old"], d["v_new"], d["v_old"], d["wet_u"], d["wet_v"], d["wu"],
+                          d["wv"], d["Du"], d["Dv"], qe, qw, qn, qs, theta_bc=a.theta_bc, count_iterations=count)
+
+    step(count=True)
+    iters = lt.iterations
+    passes = iters[0] + iters[1]
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    if args.copy_calibration:                      # a device copy of known size for the traffic counters
+        d["h"].clone()
+        torch.cuda.synchronize()
+    _native.profile_enable(True)
+    _native.profile_collect()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _native.profile_enable(False)
+    prof = _native.profile_collect()
+    cells = nz * ny * nx
+    faces = nz * ny * (nx + 1)                     # per direction (the v-faces are as many to within 0.1 %)
+    value = cells * args.steps / wall
+    kern = {k: v[0] / args.steps for k, v in prof.items()}
+    # launches that do work per step: every Newton pass launch after the last needed one returns at once
+    work = {k: (passes if k == "ppm_newton" else prof[k][1] / args.steps) for k in prof}
+    name = "ppm_newton"                            # the dominant kernel (its passes together)
+    per_launch_ms = kern[name] / max(work[name], 1)
+    b = ALE_BYTES[name] * faces
+    achieved = b / (per_launch_ms * 1e-3) / 1e9
+    step_bytes_ = sum(ALE_BYTES.get(k, 0.0) * cells * work[k] for k in kern)
+    traffic = read_traffic(name, f"ale_regional{nx}x{ny}x{nz}")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import numpy_ref as R
+        s = synth.ale_inputs(256, 256, nz, seed=0)
+        t1 = time.perf_counter()
+        R.ale_layer_transport(s.h, s.H, s.active, s.area, s.dx_t, s.dy_t, s.Lu, s.Lv, s.u_new, s.u_old, s.v_new,
+                              s.v_old, s.wet_u, s.wet_v, s.wu, s.wv, s.Du, s.Dv, s.dt, s.theta, s.theta_bc)
+        tc = time.perf_counter() - t1
+        cpu = {"value": 256 * 256 * nz / tc, "unit": "cell-layer updates/s", "cores": 1, "kind": "port",
+               "sample": f"one layer-transport step of the oracle (NumPy) on a 256x256x{nz} cut of the same grid "
+                         f"({tc:.1f} s)"}
+    res = {
+        "metric": "ALE layer-transport cell-layer updates/s (transport velocity, PPM flux + Newton, outflow "
+                  "limiter, Lagrangian step; both face directions)",
+        "value": value, "unit": "cell-layer updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded regional grid, z* layers, barotropic + baroclinic velocity)",
+        "config": {"workload": f"ALE layer transport regional {nx}x{ny}x{nz}",
+                   "newton_passes": {"u": iters[0], "v": iters[1]}, "n_limited": state["out"].n_limited},
+        "roofline": {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic, "per_launch_ms": per_launch_ms,
+                     "algorithmic_bytes": b},
+        "step_roofline": {"bytes_per_step": step_bytes_, "achieved": step_bytes_ / (wall / args.steps) / 1e9,
+                          "unit": "GB/s", "frac": step_bytes_ / (wall / args.steps) / HBM_PEAK},
+        "kernels_ms": kern,
+        "working_launches_per_step": work,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -400,11 +498,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion"), default="step",
+    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion", "ale"), default="step",
                     help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "
                          "persistent launch; serve: the Poisson job stream through the continuous-batching loop; "
                          "precision: split against FP64 after --checkpoints steps; diffusion: the along-layer and "
-                         "vertical tracer diffusion of the stack")
+                         "vertical tracer diffusion of the stack; ale: the ALE layer-transport step that "
+                         "produces the advection's inputs")
     ap.add_argument("--checkpoints", default="1,10,200,2000", help="precision form: step counts to compare at")
     ap.add_argument("--mix", choices=("mixed", "split", "f64"), default="mixed",
                     help="batch form: member precisions (mixed: even members FP64, odd split)")
@@ -437,6 +536,8 @@ def main() -> int:
         return bench_precision(args)
     if args.form == "diffusion":
         return bench_diffusion(args)
+    if args.form == "ale":
+        return bench_ale(args)
 
     if args.rch or args.wpb:
         from pyocean_amd import batch as _batch
