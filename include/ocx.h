@@ -254,6 +254,28 @@ int ocx_lagrangian_thickness_f64(const double* h, const double* divq, const doub
                                  double tol, double* out, double* V0, double* V1, uint8_t* ok, void* workspace,
                                  size_t workspace_bytes, long long* bad_index, void* stream);
 
+/* ---------------------------------------------------------------------------------------- vertical regrid + remap
+ * ALECoordinate.regrid_remap (core/components/vertical_coordinate.py:69-79) for the z* and sigma rules.  Columns
+ * are the (ny, nx) cells flattened: level arrays (n, columns), C-contiguous. */
+
+/* Conservative remap of every wet column of the fields f (nf, n0, columns) from thicknesses h_src (n0, columns) to
+ * h_tgt (n1, columns); scheme 0 PCM, 1 PLM, 2 PPM_H4 (lowered to PLM / PCM on columns with <= 3 / 1 active cells).
+ * Dry columns (wet 0): f copied when n0 == n1, else NaN.  out (nf, n1, columns).
+ * Replaces remap_columns / _remap_columns / remap_column (core/vertical/remap.py:259-427). */
+int ocx_remap_columns_f64(const double* h_src, const double* f, const double* h_tgt, const uint8_t* wet, int n0,
+                          int n1, long long columns, int nf, int scheme, double* out, void* stream);
+
+/* New layer thicknesses of the z* (rule 0, dz the nominal thicknesses surface first) or sigma (rule 1, dz the
+ * fractions bed first) rule: wet = active & H + eta > 0 & sum h > 0 (H with NaN read as 0, written to wet if
+ * non-NULL); h_new = the rule's layers where wet, h elsewhere.  With bad_column non-NULL the call is
+ * host-synchronous and returns OCX_E_RANGE naming the first wet column the rule cannot build (the reference raises
+ * ValueError there).  Replaces ALECoordinate.regrid (vertical_coordinate.py:94-109) with regrid_columns and
+ * zstar_column / sigma_column / enforce_min_thickness (core/vertical/regrid.py:41-95, 325-405). */
+size_t ocx_regrid_workspace_bytes(void);
+int ocx_regrid_f64(const double* H, const double* eta, const uint8_t* active, const double* h, int n,
+                   long long columns, int rule, const double* dz, double h_min, double* h_new, uint8_t* wet,
+                   void* workspace, size_t workspace_bytes, long long* bad_column, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

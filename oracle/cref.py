@@ -16,6 +16,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
 SRC = HERE / "loops.c"
+SRCS = [SRC, HERE / "remap.c"]
 
 
 def build(verbose: bool = False) -> Path:
@@ -23,7 +24,7 @@ def build(verbose: bool = False) -> Path:
     LIB.parent.mkdir(exist_ok=True)
     tmp = LIB.with_suffix(f".so.tmp{os.getpid()}")
     cmd = ["gcc", "-O3", "-march=x86-64-v2", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
-           "-shared", "-std=c11", str(SRC), "-o", str(tmp), "-lm"]
+           "-shared", "-std=c11", *map(str, SRCS), "-o", str(tmp), "-lm"]
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
@@ -48,6 +49,12 @@ def lib():
         L.oc_faces_h.argtypes = [P, P, P, P, P, I, I, I, I, I, I, P, P]
         L.oc_fct_coeff.argtypes = [P, P, P, P, P, P, D, I, I, I, I, P, P]
         L.oc_advect_div.argtypes = [P, P, P, P, P, P, I, I, I, I, I, I, I, P]
+        L.oc_reconstruct.argtypes = [P, P, I, I, P]
+        L.oc_remap_column.argtypes = [P, P, I, P, I, I, P]
+        L.oc_remap_columns.argtypes = [P, P, I, P, I, LL, I, I, P, P]
+        L.oc_zstar_column.argtypes = [D, D, P, I, D, P]
+        L.oc_sigma_column.argtypes = [D, D, P, I, D, P]
+        L.oc_regrid.argtypes = [P, P, P, P, I, LL, I, P, D, P, P]
         _lib = L
     return _lib
 
@@ -118,3 +125,65 @@ def fct_coefficients(c0, cL, V1, Ax, Ay, ok, dt):
     lib().oc_fct_coeff(_p(c0), _p(_f64(cL)), _p(_f64(V1)), _p(_f64(Ax)), _p(_f64(Ay)), _p(_u8(ok, (n, ny, nx))),
                        float(dt), nt, n, ny, nx, _p(Cx), _p(Cy))
     return Cx, Cy
+
+
+# ------------------------------------------------------------------------------------------------ vertical remap
+PCM, PLM, PPM_H4 = 0, 1, 2
+SCHEMES = {"pcm": PCM, "plm": PLM, "ppm_h4": PPM_H4}
+
+
+def reconstruct(h, u, scheme):
+    """(n, 3) polynomial coefficients of every cell (``reconstruct``, core/vertical/remap.py:174-245)."""
+    h, u = _f64(h), _f64(u)
+    coef = np.empty((h.size, 3))
+    lib().oc_reconstruct(_p(h), _p(u), h.size, int(scheme), _p(coef))
+    return coef
+
+
+def remap_column(h_src, u_src, h_tgt, scheme):
+    """One column (``remap_column``, core/vertical/remap.py:259-383)."""
+    h_src, u_src, h_tgt = _f64(h_src), _f64(u_src), _f64(h_tgt)
+    out = np.empty(h_tgt.size)
+    lib().oc_remap_column(_p(h_src), _p(u_src), h_src.size, _p(h_tgt), h_tgt.size, int(scheme), _p(out))
+    return out
+
+
+def remap_columns(h_src, f_src, h_tgt, scheme, wet):
+    """Every wet column (``remap_columns``, core/vertical/remap.py:386-427): h_src (n0, ...), f_src (n0, ...) or a
+    stack (nf, n0, ...), h_tgt (n1, ...), wet (...).  Dry columns: copied when n0 == n1, else NaN."""
+    h_src, f, h_tgt = _f64(h_src), _f64(f_src), _f64(h_tgt)
+    n0, n1 = h_src.shape[0], h_tgt.shape[0]
+    M = int(np.prod(h_src.shape[1:]))
+    stack = f.ndim == h_src.ndim + 1
+    nf = f.shape[0] if stack else 1
+    out = np.empty((nf, n1) + h_src.shape[1:])
+    lib().oc_remap_columns(_p(h_src), _p(f), n0, _p(h_tgt), n1, M, nf, int(scheme), _p(_u8(wet, h_src.shape[1:])),
+                           _p(out))
+    return out if stack else out[0]
+
+
+def zstar_column(H, eta, dz_top, h_min=1e-3):
+    dz = _f64(dz_top)
+    zw = np.empty(dz.size + 1)
+    lib().oc_zstar_column(float(H), float(eta), _p(dz), dz.size, float(h_min), _p(zw))
+    return zw
+
+
+def sigma_column(H, eta, dsigma, h_min=1e-3):
+    ds = _f64(dsigma)
+    zw = np.empty(ds.size + 1)
+    lib().oc_sigma_column(float(H), float(eta), _p(ds), ds.size, float(h_min), _p(zw))
+    return zw
+
+
+def regrid(H, eta, active, h, rule, dz, h_min=1e-3):
+    """``ALECoordinate.regrid`` for rule 'zstar' (dz = nominal thicknesses, surface first) or 'sigma' (dz = the
+    fractions, bed first): (h_new, wet) with wet = active & H + eta > 0 & Σ h > 0 (vertical_coordinate.py:69-109)."""
+    h = _f64(h)
+    n = h.shape[0]
+    M = int(np.prod(h.shape[1:]))
+    h_new = np.empty_like(h)
+    wet = np.empty(h.shape[1:], np.uint8)
+    lib().oc_regrid(_p(_f64(H)), _p(_f64(eta)), _p(_u8(active, h.shape[1:])), _p(h), n, M,
+                    {"zstar": 0, "sigma": 1}[rule], _p(_f64(dz)), float(h_min), _p(h_new), _p(wet))
+    return h_new, wet.astype(bool)

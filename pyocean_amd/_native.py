@@ -53,6 +53,9 @@ SIGNATURES = {
     "ocx_limit_outflow_f64": (I, [P, P, P, P, P, I, I, I, D, D, I, I, I, I, P, Z, P, P, P, P, P]),
     "ocx_lagrangian_workspace_bytes": (Z, []),
     "ocx_lagrangian_thickness_f64": (I, [P, P, P, P, P, P, I, I, I, D, D, P, P, P, P, P, Z, P, P]),
+    "ocx_remap_columns_f64": (I, [P, P, P, P, I, I, L, I, I, P, P]),
+    "ocx_regrid_workspace_bytes": (Z, []),
+    "ocx_regrid_f64": (I, [P, P, P, P, I, L, I, P, D, P, P, P, Z, P, P]),
     "ocx_batch_create": (I, [P, I, I, I, I, P]),
     "ocx_batch_step": (I, [P, I, P]),
     "ocx_batch_graph_step": (I, [P, I, P]),

@@ -1,0 +1,75 @@
+// C ABI of the vertical regrid and remap kernels (declarations and contracts: include/ocx.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/ocx.h"
+#include "ocx_host.hpp"
+#include "ocx_remap.hpp"
+
+namespace {
+
+using namespace ocx_host;
+using namespace ocx::rmp;
+
+__global__ void k_set_bad(unsigned long long* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = ~0ull;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocx_remap_columns_f64(const double* h_src, const double* f, const double* h_tgt, const uint8_t* wet, int n0,
+                          int n1, long long columns, int nf, int scheme, double* out, void* stream) {
+    if (n0 < 1 || n1 < 1 || columns < 1 || nf < 1)
+        return fail(OCX_E_ARG, "shape (n0=%d, n1=%d, columns=%lld, nf=%d) must be positive", n0, n1, columns, nf);
+    if (nf > 65535 * 4) return fail(OCX_E_ARG, "nf = %d fields: at most %d per call", nf, 65535 * 4);
+    if (scheme < PCM || scheme > PPM_H4) return fail(OCX_E_ARG, "scheme = %d: 0 (PCM), 1 (PLM) or 2 (PPM_H4)", scheme);
+    const void* p[] = {h_src, f, h_tgt, wet, out};
+    const char* nm[] = {"h_src", "f", "h_tgt", "wet", "out"};
+    for (int q = 0; q < 5; ++q) OCX_TRY(need(p[q], nm[q]));
+    if (out == f || out == h_src || out == h_tgt) return fail(OCX_E_ARG, "out must not alias the inputs");
+    RemapArgs r{h_src, f, h_tgt, wet, out, n0, n1, nf, scheme, columns};
+    const int fw = std::min(nf, 4);
+    const dim3 grid((unsigned)((columns + 63) / 64), (unsigned)((nf + fw - 1) / fw)), block(64, fw);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ProfScope _ps("remap", s);
+    k_remap<<<grid, block, 0, s>>>(r);
+    return launch_check("remap_columns");
+}
+
+size_t ocx_regrid_workspace_bytes(void) { return 256; }
+
+int ocx_regrid_f64(const double* H, const double* eta, const uint8_t* active, const double* h, int n,
+                   long long columns, int rule, const double* dz, double h_min, double* h_new, uint8_t* wet,
+                   void* workspace, size_t workspace_bytes, long long* bad_column, void* stream) {
+    if (n < 1 || columns < 1) return fail(OCX_E_ARG, "shape (n=%d, columns=%lld) must be positive", n, columns);
+    if (rule != 0 && rule != 1) return fail(OCX_E_ARG, "rule = %d: 0 (z*) or 1 (sigma)", rule);
+    const void* p[] = {H, eta, active, h, dz, h_new};
+    const char* nm[] = {"H", "eta", "active", "h", "dz", "h_new"};
+    for (int q = 0; q < 6; ++q) OCX_TRY(need(p[q], nm[q]));
+    if (h_new == h) return fail(OCX_E_ARG, "h_new must not alias h");
+    if (!workspace || workspace_bytes < ocx_regrid_workspace_bytes())
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed", ocx_regrid_workspace_bytes());
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long* bad = static_cast<unsigned long long*>(workspace);
+    k_set_bad<<<1, 64, 0, s>>>(bad);
+    OCX_TRY(launch_check("regrid reset"));
+    {
+        ProfScope _ps("regrid", s);
+        k_regrid<<<(unsigned)((columns + 255) / 256), 256, 0, s>>>(H, eta, active, h, n, columns, rule, dz, h_min,
+                                                                   h_new, wet, bad);
+    }
+    OCX_TRY(launch_check("regrid"));
+    if (!bad_column) return OCX_OK;
+    unsigned long long b = 0;
+    OCX_TRY(hip_check(hipMemcpyAsync(&b, bad, sizeof b, hipMemcpyDeviceToHost, s), "regrid flag"));
+    OCX_TRY(hip_check(hipStreamSynchronize(s), "regrid sync"));
+    *bad_column = b == ~0ull ? -1 : (long long)b;
+    if (b == ~0ull) return OCX_OK;
+    return fail(OCX_E_RANGE, "wet column %llu: %s", b,
+                rule == 0 ? "z* needs finite H > 0 and D = H + eta > 0" : "sigma needs finite H and D = H + eta > 0");
+}
+
+}  // extern "C"

@@ -1,0 +1,595 @@
+// Vertical regrid (z*, sigma) and conservative remap (PCM, PLM, PPM_H4) of the ALE coordinate (gfx950), FP64 in the
+// reference's per-element arithmetic (no contraction, IEEE divisions), so results are bit-identical.
+//
+//   regrid   zstar_column / sigma_column / enforce_min_thickness (core/vertical/regrid.py:41-95) under
+//            ALECoordinate.regrid (core/components/vertical_coordinate.py:69-109)
+//   remap    remap_column and its reconstruction (core/vertical/remap.py:42-383): edges, limiters and the
+//            overlap integration with the thickest-piece residual
+//
+// Columns are independent: one thread per (column, field), columns contiguous across lanes, so every level of a
+// column is one coalesced load for the wavefront; the waves of a workgroup are fields of the same 64 columns and
+// share the thickness loads through the caches.  The PPM_H4 path is one streaming pass up the column: a 5-cell
+// window of (h, u) in registers yields each interface value once, its bounded and continuity-checked edges, the
+// cell's limited parabola, and at once that source cell's share of the target cells, which are written as soon as
+// no later source cell can reach them.  Nothing of the column is stored.
+#pragma once
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace ocx {
+namespace rmp {
+
+constexpr int PCM = 0, PLM = 1, PPM_H4 = 2;
+constexpr double H_NEGLECT = 1e-30;        // negligible thickness of the cell reconstructions
+constexpr double H_NEGLECT_EDGE = 1e-10;   // of the edge values
+constexpr double H_MIN_FRAC = 1e-5;
+constexpr double EPS = DBL_EPSILON;
+
+// Python's min / max of two floats: the first argument unless the second compares smaller / larger
+__device__ __forceinline__ double pmin(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double pmax(double a, double b) { return b > a ? b : a; }
+__device__ __forceinline__ double clampu(double e, double a, double b) {   // max(min(e, max(a, b)), min(a, b))
+    return pmax(pmin(e, pmax(a, b)), pmin(a, b));
+}
+
+// strided view of one column: level k at p[k * M]
+struct Col {
+    const double* p;
+    long long M;
+    __device__ __forceinline__ double operator[](int k) const { return p[(size_t)k * M]; }
+};
+
+// ------------------------------------------------------------------------------------------ PPM_H4 pieces
+// (outer, inner) edge values at one end from the 4 end cells (dz from the end, > 0), closed form of the cubic
+__device__ void end_values_h4(const double dz[4], const double u[4], double& e_out, double& e_in) {
+    const double min_frac = 1.0e-6;
+    const double h1 = dz[0], h2 = dz[1];
+    double h3 = dz[2], h4 = dz[3];
+    if ((h2 + h3) < min_frac * h1) h3 = min_frac * h1 - h2;
+    if ((h3 + h4) < min_frac * h1) h4 = min_frac * h1 - h3;
+    const double h12 = h1 + h2, h23 = h2 + h3, h34 = h3 + h4;
+    const double h123 = h12 + h3, h234 = h2 + h34, h1234 = h12 + h34;
+    const double I_denB3 = 1.0 / (h123 * h12 * h23);
+    const double I_h12 = (h123 * h23) * I_denB3;
+    const double I_h23 = (h12 * h123) * I_denB3;
+    const double I_h123 = (h12 * h23) * I_denB3;
+    const double I_denom = 1.0 / (h1234 * (h234 * h34));
+    const double I_h234 = (h1234 * h34) * I_denom;
+    const double I_h1234 = (h234 * h34) * I_denom;
+    const double w11 = -h1 * (I_h1234 + I_h123 + I_h12);
+    const double w21 = h1 * h12 * (I_h234 * I_h1234 + I_h23 * (I_h234 + I_h123));
+    const double w31 = -h1 * h12 * h123 * I_denom;
+    const double w12 = 2.0 * (I_h12 * (1.0 + (h1 + h12) * (I_h1234 + I_h123)) + h1 * I_h1234 * I_h123);
+    const double w22 = -2.0 * ((h1 * h12 * I_h1234) * (I_h23 * (I_h234 + I_h123)) +
+                               (h1 + h12) * (I_h1234 * I_h234 + I_h23 * (I_h234 + I_h123)));
+    const double w32 = 2.0 * ((h1 + h12) * h123 + h1 * h12) * I_denom;
+    const double w13 = -3.0 * I_h12 * I_h123 * (1.0 + I_h1234 * ((h1 + h12) + h123));
+    const double w23 = 3.0 * I_h23 * (I_h123 + I_h1234 * ((h1 + h12) + h123) * (I_h123 + I_h234));
+    const double w33 = -3.0 * ((h1 + h12) + h123) * I_denom;
+    const double w14 = 4.0 * I_h1234 * I_h123 * I_h12;
+    const double w24 = -4.0 * I_h1234 * (I_h23 * (I_h123 + I_h234));
+    const double w34 = 4.0 * I_denom;
+    const double d1 = u[1] - u[0], d2 = u[2] - u[1], d3 = u[3] - u[2];
+    const double c0 = ((u[0] + (w11 * d1)) + (w21 * d2)) + (w31 * d3);
+    const double c1 = ((w12 * d1) + (w22 * d2)) + (w32 * d3);
+    const double c2 = ((w13 * d1) + (w23 * d2)) + (w33 * d3);
+    const double c3 = ((w14 * d1) + (w24 * d2)) + (w34 * d3);
+    const double d = dz[0];
+    e_out = c0;
+    e_in = c0 + d * (c1 + d * (c2 + d * c3));
+}
+
+// the explicit 4th-order value at the interface between cells 1 and 2 of the 4-cell stencil (h0..h3, u0..u3)
+__device__ double interior_edge(double h0, double h1, double h2, double h3, double u0, double u1, double u2,
+                                double u3) {
+    if (h0 + h1 == 0.0 || h1 + h2 == 0.0 || h2 + h3 == 0.0) {
+        const double h_min = H_MIN_FRAC * pmax(H_NEGLECT_EDGE, (h0 + h1) + (h2 + h3));
+        h0 = pmax(h_min, h0);
+        h1 = pmax(h_min, h1);
+        h2 = pmax(h_min, h2);
+        h3 = pmax(h_min, h3);
+    }
+    const double I_h12 = 1.0 / (h1 + h2);
+    const double I_den_et2 = 1.0 / (((h0 + h1) + h2) * (h0 + h1));
+    const double I_h012 = (h0 + h1) * I_den_et2;
+    const double I_den_et3 = 1.0 / ((h1 + (h2 + h3)) * (h2 + h3));
+    const double I_h123 = (h2 + h3) * I_den_et3;
+    const double et1 = (1.0 + (h1 * I_h012 + (h0 + h1) * I_h123)) * I_h12 * (h2 * (h2 + h3)) * u1 +
+                       (1.0 + (h2 * I_h123 + (h2 + h3) * I_h012)) * I_h12 * (h1 * (h0 + h1)) * u2;
+    const double et2 = (h1 * (h2 * (h2 + h3)) * I_den_et2) * (u1 - u0);
+    const double et3 = (h2 * (h1 * (h0 + h1)) * I_den_et3) * (u2 - u3);
+    return (et1 + (et2 + et3)) / ((h0 + h1) + (h2 + h3));
+}
+
+// the cell's limited slope times h used to bound its edges (cells km1, k, kp1 already clamped at the column ends)
+__device__ __forceinline__ double bound_slope(double hm, double hc, double hp, double um, double uc, double up) {
+    double slope_x_h = 0.0;
+    if ((hm + hp) + 2.0 * hc > 0.0) {
+        const double sl = uc - um;
+        const double sc = (up - um) * (hc / ((hm + hp) + 2.0 * hc));
+        const double sr = up - uc;
+        if (sl * sr > 0.0) {
+            const double m = pmin(pmin(fabs(sl), fabs(sc)), fabs(sr));
+            slope_x_h = sc >= 0.0 ? m : -m;
+        }
+    }
+    return slope_x_h;
+}
+
+// an edge value bounded by its cell and the neighbour on its side (unb = u of that neighbour, or the cell itself at
+// the column end); right: the upper edge
+__device__ __forceinline__ double bound_edge(double e, double slope_x_h, double unb, double uc, bool right) {
+    const double sgn = slope_x_h >= 0.0 ? 1.0 : -1.0;
+    if ((unb - e) * (e - uc) < 0.0) {
+        const double m = pmin(fabs(slope_x_h), fabs(e - uc));
+        e = right ? uc + sgn * m : uc - sgn * m;
+    }
+    return clampu(e, unb, uc);
+}
+
+// the Colella-Woodward limited parabola of an interior cell from its final edges
+__device__ __forceinline__ void cw_coef(double u_l, double u_c, double u_r, double el, double er, double& c0,
+                                        double& c1, double& c2) {
+    if ((u_r - u_c) * (u_c - u_l) <= 0.0) {
+        el = u_c;
+        er = u_c;
+    } else {
+        const double expr1 = 3.0 * (er - el) * ((u_c - el) + (u_c - er));
+        const double expr2 = (er - el) * (er - el);
+        if (expr1 > expr2) {
+            el = u_c + 2.0 * (u_c - er);
+            el = clampu(el, u_l, u_c);
+        } else if (expr1 < -expr2) {
+            er = u_c + 2.0 * (u_c - el);
+            er = clampu(er, u_r, u_c);
+        }
+    }
+    if (fabs(er - el) < pmax(1e-60, EPS * fabs(u_c))) {
+        el = u_c;
+        er = u_c;
+    }
+    c0 = el;
+    c1 = 4.0 * (u_c - el) + 2.0 * (u_c - er);
+    c2 = 3.0 * ((er - u_c) + (el - u_c));
+}
+
+// ------------------------------------------------------------------------------------------ PLM (short columns)
+__device__ double plm_slope_wa(double h_l, double h_c, double h_r, double u_l, double u_c, double u_r) {
+    const double sigma_r = u_r - u_c;
+    const double sigma_l = u_c - u_l;
+    const double sigma_c = 2.0 * (u_r - u_l) * (h_c / (h_l + 2.0 * h_c + h_r + H_NEGLECT));
+    const double u_min = pmin(pmin(u_l, u_c), u_r);
+    const double u_max = pmax(pmax(u_l, u_c), u_r);
+    double slope;
+    if (sigma_l * sigma_r > 0.0) {
+        const double s = pmin(fabs(sigma_c), 2.0 * pmin(u_c - u_min, u_max - u_c));
+        slope = sigma_c >= 0.0 ? s : -s;
+    } else {
+        slope = 0.0;
+    }
+    if (u_c - 0.5 * fabs(slope) < u_min || u_c + 0.5 * fabs(slope) > u_max) slope = slope * (1.0 - EPS);
+    if (fabs(slope) < 1e-140) slope = 0.0;
+    return slope;
+}
+
+__device__ double plm_mono(double u_l, double u_c, double u_r, double s_l, double s_c, double s_r) {
+    const double almost_two = 2.0 * (1.0 - EPS);
+    const double e_r = u_l + 0.5 * s_l;
+    const double e_l = u_r - 0.5 * s_r;
+    double slp = fabs(s_c);
+    double edge = u_c - 0.5 * s_c;
+    if ((edge - e_r) * (u_c - edge) < 0.0) {
+        edge = 0.5 * (edge + e_r);
+        slp = pmin(slp, fabs(edge - u_c) * almost_two);
+    }
+    edge = u_c + 0.5 * s_c;
+    if ((edge - u_c) * (e_l - edge) < 0.0) {
+        edge = 0.5 * (edge + e_l);
+        slp = pmin(slp, fabs(edge - u_c) * almost_two);
+    }
+    return s_c >= 0.0 ? slp : -slp;
+}
+
+// slope of cell k (0 at the column ends)
+__device__ __forceinline__ double plm_slp(const Col& h, const Col& u, int n, int k) {
+    if (k <= 0 || k >= n - 1) return 0.0;
+    return plm_slope_wa(h[k - 1], h[k], h[k + 1], u[k - 1], u[k], u[k + 1]);
+}
+
+__device__ __forceinline__ double plm_mslp(const Col& h, const Col& u, int n, int k) {
+    if (k <= 0 || k >= n - 1) return 0.0;
+    return plm_mono(u[k - 1], u[k], u[k + 1], plm_slp(h, u, n, k - 1), plm_slp(h, u, n, k), plm_slp(h, u, n, k + 1));
+}
+
+// PLM coefficients of cell k (n >= 3), recomputed from the column (short columns only reach this)
+__device__ void plm_coef(const Col& h, const Col& u, int n, int k, double& c0, double& c1, double& c2) {
+    c2 = 0.0;
+    if (k <= 0 || k >= n - 1) {
+        c0 = u[k];
+        c1 = 0.0;
+        return;
+    }
+    const double uk = u[k];
+    const double m = plm_mslp(h, u, n, k);
+    const double u_l = uk - 0.5 * m;
+    const double u_r = uk + 0.5 * m;
+    c0 = u_l;
+    c1 = u_r - u_l;
+    const double edge = c1 + c0;
+    const double m1 = plm_mslp(h, u, n, k + 1);
+    const double s1 = plm_slp(h, u, n, k + 1);
+    const double e_r = u[k + 1] - 0.5 * (s1 >= 0.0 ? fabs(m1) : -fabs(m1));
+    if ((edge - uk) * (e_r - edge) < 0.0) c1 = c1 * (1.0 - EPS);
+}
+
+// ------------------------------------------------------------------------------------------ overlap integration
+__device__ __forceinline__ double average(double c0, double c1, double c2, double xa, double xb) {
+    if (xb - xa <= 1e-14) {
+        const double x = 0.5 * (xa + xb);
+        return c0 + x * (c1 + x * c2);
+    }
+    return c0 + 0.5 * c1 * (xa + xb) + c2 * (xa * xa + xa * xb + xb * xb) / 3.0;
+}
+
+// the target side of one column: the cells the source has reached, their running content and the vanished cells
+// still waiting for a local value
+struct Targets {
+    Col ht;
+    double* out;        // level k at out[k * M]
+    long long M;
+    int n1;
+    int j;              // open target cell of the overlap walk
+    double tj;          // its lower interface t_edge[j]
+    double content;     // its content so far
+    int jv;             // next target whose lower interface has not been located in a source cell
+    double tv;          // t_edge[jv]
+
+    __device__ __forceinline__ void put(int k, double v) { out[(size_t)k * M] = v; }
+    __device__ __forceinline__ double top(int jj, double t_lo, double b) const {   // upper end of target jj
+        return jj < n1 - 1 ? t_lo + ht[jj] : pmax(t_lo + ht[jj], b);
+    }
+    // a target the walk has passed: its mean (vanished targets are set by their local value instead)
+    __device__ __forceinline__ void finish(int jj, double c) {
+        const double h = ht[jj];
+        if (h > 0.0) put(jj, c / h);
+    }
+};
+
+// one active source cell [a, a + hi) with mean us and polynomial (c0, c1, c2): its pieces over the targets, the
+// largest taking the residual u h - Σ others so the cell's content is exact
+__device__ void merge_cell(Targets& T, double a, double hi, double us, double c0, double c1, double c2) {
+    const double b = a + hi;
+    // pass 1: the geometry of the pieces, which one is thickest
+    int npc = 0, ithick = 0, jthick = T.j;
+    double thick = -1.0;
+    {
+        int j = T.j;
+        double t = T.tj;
+        while (true) {
+            const double lo = pmax(a, t);
+            const double hi_t = T.top(j, t, b);
+            const double up = pmin(b, hi_t);
+            const double L = up - lo;
+            if (L > 0.0) {
+                if (L > thick) {
+                    thick = L;
+                    ithick = npc;
+                    jthick = j;
+                }
+                ++npc;
+            }
+            if (hi_t <= b && j < T.n1 - 1) {
+                t = t + T.ht[j];
+                ++j;
+            } else {
+                break;
+            }
+        }
+    }
+    // pass 2: the pieces in order; every target passed here except the thickest piece's is final
+    double s = 0.0, c_thick = 0.0;
+    int p = 0;
+    int j = T.j;
+    double t = T.tj, c = T.content;
+    while (true) {
+        const double lo = pmax(a, t);
+        const double hi_t = T.top(j, t, b);
+        const double up = pmin(b, hi_t);
+        const double L = up - lo;
+        if (L > 0.0) {
+            if (p != ithick) {
+                const double piece = L * average(c0, c1, c2, (lo - a) / hi, (up - a) / hi);
+                s += piece;
+                c = c + piece;
+            } else {
+                c_thick = c;                 // the content before this cell's piece
+            }
+            ++p;
+        }
+        if (hi_t <= b && j < T.n1 - 1) {
+            if (j != jthick || npc == 0) T.finish(j, c);
+            t = t + T.ht[j];
+            ++j;
+            c = 0.0;
+        } else {
+            break;
+        }
+    }
+    if (npc > 0) {
+        const double tp = us * hi - s;
+        if (jthick == j) {
+            c = c_thick + tp;                // the open target holds the thickest piece
+        } else {
+            T.finish(jthick, c_thick + tp);
+        }
+    }
+    T.j = j;
+    T.tj = t;
+    T.content = c;
+}
+
+// vanished targets whose lower interface lies in the source cell ending at b get the cell's polynomial there
+__device__ __forceinline__ void locate_vanished(Targets& T, double a, double hi, double b, double c0, double c1,
+                                                double c2) {
+    while (T.jv < T.n1 && T.tv <= b) {
+        const double h = T.ht[T.jv];
+        if (!(h > 0.0)) {
+            const double xi = (T.tv - a) / hi;
+            T.put(T.jv, c0 + xi * (c1 + xi * c2));
+        }
+        T.tv = T.tv + h;
+        ++T.jv;
+    }
+}
+
+// after the last active source cell: the open target and those beyond are final; vanished ones above the source
+// take the last active cell's mean
+__device__ void finish_targets(Targets& T, double u_last) {
+    T.finish(T.j, T.content);
+    for (int jj = T.j + 1; jj < T.n1; ++jj) T.finish(jj, 0.0);
+    while (T.jv < T.n1) {
+        const double h = T.ht[T.jv];
+        if (!(h > 0.0)) T.put(T.jv, u_last);
+        ++T.jv;
+    }
+}
+
+// ------------------------------------------------------------------------------------------ the remap kernel
+struct RemapArgs {
+    const double *h_src, *f, *h_tgt;
+    const uint8_t* wet;
+    double* out;
+    int n0, n1, nf, scheme;
+    long long M;
+};
+
+// column col of field q
+__device__ void remap_one(const RemapArgs& r, long long col, int q) {
+    const long long M = r.M;
+    const Col hs{r.h_src + col, M};
+    const Col u{r.f + (size_t)q * r.n0 * M + col, M};
+    double* out = r.out + (size_t)q * r.n1 * M + col;
+    const int n0 = r.n0, n1 = r.n1;
+    if (!r.wet[col]) {
+        for (int k = 0; k < n1; ++k) out[(size_t)k * M] = n0 == n1 ? u[k] : __builtin_nan("");
+        return;
+    }
+    int n_active = 0, last = -1;
+    for (int k = 0; k < n0; ++k) {
+        if (hs[k] > 0.0) {
+            ++n_active;
+            last = k;
+        }
+    }
+    if (last < 0) {                                   // no source thickness: keep the values
+        double mean = 0.0;
+        for (int k = 0; k < n0; ++k) mean += u[k] / n0;
+        for (int k = 0; k < n1; ++k) out[(size_t)k * M] = n0 == n1 ? u[k] : mean;
+        return;
+    }
+    int scheme = r.scheme;
+    if (n_active <= 1) scheme = PCM;
+    else if (n_active <= 3 && scheme > PLM) scheme = PLM;
+    if ((scheme == PLM && n0 < 3) || (scheme == PPM_H4 && n0 < 4)) scheme = PCM;
+
+    Targets T;
+    T.ht = Col{r.h_tgt + col, M};
+    T.out = out;
+    T.M = M;
+    T.n1 = n1;
+    T.j = 0;
+    T.tj = 0.0;
+    T.content = 0.0;
+    T.jv = 0;
+    T.tv = 0.0;
+
+    if (scheme != PPM_H4) {
+        double a = 0.0;
+        for (int i = 0; i < n0; ++i) {
+            const double hi = hs[i];
+            if (!(hi > 0.0)) continue;
+            double c0, c1, c2;
+            if (scheme == PLM) {
+                plm_coef(hs, u, n0, i, c0, c1, c2);
+            } else {
+                c0 = u[i];
+                c1 = 0.0;
+                c2 = 0.0;
+            }
+            const double b = a + hi;
+            merge_cell(T, a, hi, u[i], c0, c1, c2);
+            locate_vanished(T, a, hi, b, c0, c1, c2);
+            a = b;
+        }
+        finish_targets(T, u[last]);
+        return;
+    }
+
+    // PPM_H4: the window w[0..4] holds cells i-2 .. i+2 (entries outside the column are never used)
+    double wh[5], wu[5];
+    for (int d = 0; d < 5; ++d) {
+        const int k = d - 2;
+        wh[d] = (k >= 0 && k < n0) ? hs[k] : 0.0;
+        wu[d] = (k >= 0 && k < n0) ? u[k] : 0.0;
+    }
+    // end values: bottom (interfaces 0 and 1) and top (interfaces n0 and n0 - 1)
+    double e_bot_out, e_bot_in, e_top_out, e_top_in;
+    {
+        double dz[4], uu[4];
+        for (int d = 0; d < 4; ++d) {
+            dz[d] = pmax(H_NEGLECT_EDGE, hs[d]);
+            uu[d] = u[d];
+        }
+        end_values_h4(dz, uu, e_bot_out, e_bot_in);
+        for (int d = 0; d < 4; ++d) {
+            dz[d] = pmax(H_NEGLECT_EDGE, hs[n0 - 1 - d]);
+            uu[d] = u[n0 - 1 - d];
+        }
+        end_values_h4(dz, uu, e_top_out, e_top_in);
+    }
+    // raw value of interface m (0 .. n0) from the window, which must hold cells m-2 .. m+1 at positions
+    // (m - i) + 0 .. (m - i) + 3 when called at step i with m = i + 1
+    auto interface_raw = [&](int m) -> double {
+        if (m == 0) return e_bot_out;
+        if (m == 1) return e_bot_in;
+        if (m == n0) return e_top_out;
+        if (m == n0 - 1) return e_top_in;
+        return interior_edge(wh[1], wh[2], wh[3], wh[4], wu[1], wu[2], wu[3], wu[4]);
+    };
+    // bound slope of the cell at window position d (cell i - 2 + d), neighbours clamped at the column ends
+    auto slope_at = [&](int d, int k) -> double {
+        const int dm = k > 0 ? d - 1 : d, dp = k < n0 - 1 ? d + 1 : d;
+        return bound_slope(wh[dm], wh[d], wh[dp], wu[dm], wu[d], wu[dp]);
+    };
+    // cell 0: its lower edge (no continuity check below it)
+    double s_cur = slope_at(2, 0);                                  // slope of cell i (window position 2)
+    double fl = bound_edge(e_bot_out, s_cur, wu[2], wu[2], false);  // final lower edge of cell i
+    double a = 0.0;
+    for (int i = 0; i < n0; ++i) {
+        // interface i + 1: raw value, the upper edge of cell i and the lower edge of cell i + 1, their check
+        double fr, fl_next = 0.0, s_next = 0.0;
+        if (i + 1 < n0) {
+            const double raw = interface_raw(i + 1);
+            s_next = slope_at(3, i + 1);
+            const double br = bound_edge(raw, s_cur, wu[3], wu[2], true);
+            const double bl = bound_edge(raw, s_next, wu[2], wu[3], false);
+            fr = br;
+            fl_next = bl;
+            if ((bl - br) * (wu[3] - wu[2]) < 0.0) {
+                const double av = clampu(0.5 * (br + bl), wu[2], wu[3]);
+                fr = av;
+                fl_next = av;
+            }
+        } else {
+            fr = bound_edge(e_top_out, s_cur, wu[2], wu[2], true);
+        }
+        // the cell's polynomial: PCM at both ends, the limited parabola inside
+        double c0 = wu[2], c1 = 0.0, c2 = 0.0;
+        if (i > 0 && i < n0 - 1) cw_coef(wu[1], wu[2], wu[3], fl, fr, c0, c1, c2);
+        const double hi = wh[2];
+        if (hi > 0.0) {
+            const double b = a + hi;
+            merge_cell(T, a, hi, wu[2], c0, c1, c2);
+            locate_vanished(T, a, hi, b, c0, c1, c2);
+            a = b;
+        }
+        // shift the window up one cell
+        for (int d = 0; d < 4; ++d) {
+            wh[d] = wh[d + 1];
+            wu[d] = wu[d + 1];
+        }
+        const int k_in = i + 3;
+        wh[4] = k_in < n0 ? hs[k_in] : 0.0;
+        wu[4] = k_in < n0 ? u[k_in] : 0.0;
+        fl = fl_next;
+        s_cur = s_next;
+    }
+    finish_targets(T, u[last]);
+}
+
+// block (64, FW): x over columns, y over fields; grid (ceil(M / 64), ceil(nf / FW))
+__global__ __launch_bounds__(256) void k_remap(RemapArgs r) {
+    const long long col = (long long)blockIdx.x * 64 + threadIdx.x;
+    const int q = blockIdx.y * blockDim.y + threadIdx.y;
+    if (col >= r.M || q >= r.nf) return;
+    remap_one(r, col, q);
+}
+
+// ------------------------------------------------------------------------------------------ regrid
+// one thread per column: wet = active & H + eta > 0 & Σ h > 0 (H with NaN read as 0); h_new = the rule's layers where
+// wet, h elsewhere.  rule 0: z* (dz = nominal thicknesses, surface first); 1: sigma (dz = fractions, bed first).
+// A wet column the rule cannot build (non-finite H or D, D <= 0, z* with H <= 0) is recorded in *bad (atomicMin of
+// its index).
+__global__ void k_regrid(const double* __restrict__ H, const double* __restrict__ eta,
+                         const uint8_t* __restrict__ active, const double* __restrict__ h, int n, long long M,
+                         int rule, const double* __restrict__ dz, double h_min, double* __restrict__ h_new,
+                         uint8_t* __restrict__ wet_out, unsigned long long* bad) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= M) return;
+    double Hc = H[c];
+    Hc = Hc != Hc ? 0.0 : (Hc == __builtin_inf() ? DBL_MAX : (Hc == -__builtin_inf() ? -DBL_MAX : Hc));
+    const double et = eta[c];
+    double tot = 0.0;
+    for (int k = 0; k < n; ++k) tot += h[(size_t)k * M + c];
+    const bool wet = active[c] && (Hc + et > 0.0) && (tot > 0.0);
+    if (wet_out) wet_out[c] = wet ? 1 : 0;
+    if (!wet) {
+        for (int k = 0; k < n; ++k) h_new[(size_t)k * M + c] = h[(size_t)k * M + c];
+        return;
+    }
+    const double D = Hc + et;
+    if (!(fabs(Hc) <= DBL_MAX) || !(fabs(D) <= DBL_MAX) || (rule == 0 && !(Hc > 0.0))) {
+        atomicMin(bad, (unsigned long long)c);
+        for (int k = 0; k < n; ++k) h_new[(size_t)k * M + c] = h[(size_t)k * M + c];
+        return;
+    }
+    // interfaces are produced top-down (z*) or bottom-up (sigma) and written as thicknesses in a second sweep
+    if (rule == 0) {
+        // z*: e[0] = eta, e[k+1] = e[k] - s dz[k], e[n] = -H, raised from the bed by hm; h_k = e[n-1-k] - e[n-k]
+        const double hm = pmin(h_min, D / n);
+        const double s = D / Hc;
+        // nominal positions from the top, then the floor from the bed up: walk the interfaces bottom-up, each
+        // one's nominal value recomputed by the same sequential sum (kept in h_new as scratch)
+        double e = et;
+        for (int k = 0; k < n; ++k) {                 // h_new[n-1-k] temporarily holds the nominal e[k+1]
+            e = e - s * dz[k];
+            h_new[(size_t)(n - 1 - k) * M + c] = e;
+        }
+        double below = -Hc;                           // e[n]
+        for (int k = n - 1; k >= 0; --k) {
+            double ek = k == 0 ? et : h_new[(size_t)(n - k) * M + c];     // nominal e[k]
+            if (ek < below + hm) ek = below + hm;
+            // layer n-1-k (bed first) spans e[k+1] .. e[k]: thickness e[k] - e[k+1]
+            h_new[(size_t)(n - 1 - k) * M + c] = ek - below;
+            below = ek;
+        }
+    } else {
+        // sigma: zw[0] = -H, zw[j+1] = zw[j] + D dsigma[j], zw[n] = eta, then the minimum thickness from the top
+        // and from the bed; interfaces kept in h_new (zw[j+1] at level j) until the thicknesses are formed
+        double z = -Hc;
+        for (int j = 0; j < n; ++j) {
+            z = z + D * dz[j];
+            h_new[(size_t)j * M + c] = j == n - 1 ? et : z;              // zw[j + 1]
+        }
+        auto zw = [&](int j) -> double { return j == 0 ? -Hc : h_new[(size_t)(j - 1) * M + c]; };
+        const double hm = pmin(h_min, (zw(n) - zw(0)) / n);
+        for (int j = n - 1; j > 0; --j) {
+            const double up = zw(j + 1);
+            if (zw(j) > up - hm) h_new[(size_t)(j - 1) * M + c] = up - hm;
+        }
+        for (int j = 1; j < n; ++j) {
+            const double dn = zw(j - 1);
+            if (zw(j) < dn + hm) h_new[(size_t)(j - 1) * M + c] = dn + hm;
+        }
+        double prev = -Hc;
+        for (int j = 0; j < n; ++j) {
+            const double zj = h_new[(size_t)j * M + c];
+            h_new[(size_t)j * M + c] = zj - prev;
+            prev = zj;
+        }
+    }
+}
+
+}  // namespace rmp
+}  // namespace ocx

@@ -139,3 +139,42 @@ def limiter_case(seed, n=5, ny=9, nx=11, periodic=(False, False)):
     if periodic[1]:
         Qy[:, -1] = Qy[:, 0]
     return h, Qx, Qy, area, dt, act
+
+
+# ------------------------------------------------------------------------------------------------ vertical remap
+def remap_case(seed, n0=50, n1=50, ny=9, nx=23, nf=3):
+    """Columns of every kind the remap meets: thick, z*-like with vanished bed layers, interleaved vanished layers
+    (0, 1e-10, 1e-3 m), 1-3 active cells (PCM / PLM), all vanished (dry), a dry mask, target totals off by +-1e-9 m
+    and +-1e-13 m, vanished targets.  Returns h_src (n0, ny, nx), f (nf, n0, ny, nx), h_tgt (n1, ny, nx), wet."""
+    rng = np.random.default_rng(seed)
+    M = ny * nx
+    hs = np.empty((n0, M))
+    for c in range(M):
+        kind = c % 8
+        if kind == 0:
+            col = rng.uniform(0.5, 40.0, n0)
+        elif kind == 1:
+            nv = int(rng.integers(1, max(n0 - 4, 2)))
+            col = np.r_[np.zeros(nv), rng.uniform(2.0, 30.0, n0 - nv)]
+        elif kind == 2:
+            col = np.where(rng.random(n0) < 0.7, rng.choice([0.0, 1e-10, 1e-3], n0), rng.uniform(0.5, 50.0, n0))
+        elif kind in (3, 4, 5):
+            col = np.zeros(n0)
+            na = min(kind - 2, n0)
+            col[rng.choice(n0, na, replace=False)] = rng.uniform(0.5, 10.0, na)
+        elif kind == 6:
+            col = np.zeros(n0) if c % 16 == 6 else rng.uniform(0.0, 3.0, n0)
+        else:
+            col = np.where(rng.random(n0) < 0.2, 0.0, rng.uniform(0.1, 20.0, n0))
+        hs[:, c] = col
+    ht = rng.uniform(0.1, 2.0, (n1, M))
+    ht[rng.random((n1, M)) < 0.1] = 0.0                                  # vanished targets
+    tot_s, tot_t = hs.sum(0), ht.sum(0)
+    ht = np.where(tot_t > 0, ht * (tot_s / np.where(tot_t > 0, tot_t, 1.0)), ht)
+    ht[-1, 1::5] += 1e-9
+    ht[-1, 2::5] += 1e-13
+    ht[-1, 3::7] -= np.minimum(1e-9, ht[-1, 3::7])
+    f = np.cumsum(rng.normal(0.0, 1.0, (nf, n0, M)), axis=1) + rng.uniform(5.0, 20.0, (nf, 1, M))
+    f[:, :, 7::11] = 12.5                                                 # uniform columns
+    wet = rng.random(M) > 0.1
+    return hs.reshape(n0, ny, nx), f.reshape(nf, n0, ny, nx), ht.reshape(n1, ny, nx), wet.reshape(ny, nx)

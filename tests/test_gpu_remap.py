@@ -1,0 +1,133 @@
+"""
+GPU tests of the vertical regrid and remap: the HIP kernels are bit-identical to the oracle's C restatement of the
+reference arithmetic on columns of every kind (thick, z*-like, interleaved vanished layers, 1-3 active cells, dry,
+round-off thickness mismatches, vanished targets), for PCM, PLM and PPM_H4, equal and different layer counts, a
+field stack and single fields; the z* and sigma rules on a regional grid; and, after the ALE layer transport, the
+regrid-remap of a ten-tracer stack.  At 1024 x 1024 x 50 the column contents are checked to round-off.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cref
+from oracle import numpy_ref as R
+from pyocean_amd import ale, synth, vertical
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu case]", *a, flush=True)
+
+
+def _d(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _eq(gpu_t, ref, what):
+    g = gpu_t.cpu().numpy()
+    assert g.shape == ref.shape, (what, g.shape, ref.shape)
+    bad = ~((g == ref) | (np.isnan(g) & np.isnan(ref)))
+    if bad.any():
+        w = tuple(np.argwhere(bad)[0])
+        raise AssertionError(f"{what}: {int(bad.sum())} of {g.size} differ, first at {w}: gpu {g[w]!r} ref {ref[w]!r}")
+
+
+@pytest.mark.parametrize("scheme", [vertical.PCM, vertical.PLM, vertical.PPM_H4])
+@pytest.mark.parametrize("n0,n1", [(50, 50), (30, 45), (40, 12), (4, 4)])
+def test_remap_matches_the_oracle_bit_for_bit(gpu, scheme, n0, n1):
+    _say("remap", scheme, n0, n1)
+    hs, f, ht, wet = cases.remap_case(10 + n0 + n1, n0=n0, n1=n1)
+    out = vertical.remap_columns(_d(hs), _d(f), _d(ht), scheme, _d(wet))
+    _eq(out, cref.remap_columns(hs, f, ht, scheme, wet), "stack")
+    one = vertical.remap_columns(_d(hs), _d(f[1]), _d(ht), scheme, _d(wet))
+    _eq(one, cref.remap_columns(hs, f[1], ht, scheme, wet), "single field")
+
+
+def test_remap_known_answers(gpu):
+    _say("remap known answers")
+    h0 = np.array([0, 1, 0, 0, 2, 3, 1], float)[:, None, None]
+    u0 = np.array([5, 10, 11, 12, 13, 15, 16], float)[:, None, None]
+    out = vertical.remap_columns(_d(h0), _d(u0), _d(np.ones((7, 1, 1))), vertical.PPM_H4, _d(np.ones((1, 1), bool)))
+    np.testing.assert_allclose(out.cpu().numpy()[:, 0, 0], [10, 12.642, 13.358, 14.338, 15.052, 15.61, 16], atol=1e-3)
+    u = np.array([20.0, 21.0, 22.0, 23.0])[:, None, None]
+    for scheme in (vertical.PCM, vertical.PLM, vertical.PPM_H4):
+        z = vertical.remap_columns(_d(np.zeros((4, 1, 1))), _d(u), _d(np.zeros((2, 1, 1))), scheme,
+                                   _d(np.ones((1, 1), bool)))
+        np.testing.assert_allclose(z.cpu().numpy()[:, 0, 0], 21.5)
+
+
+@pytest.mark.parametrize("rule", ["zstar", "sigma"])
+def test_regrid_matches_the_oracle_bit_for_bit(gpu, rule):
+    _say("regrid", rule)
+    a = synth.ale_inputs(96, 70, 50, seed=3)
+    rng = np.random.default_rng(4)
+    eta = rng.uniform(-0.8, 0.8, a.H.shape) * a.active
+    h = a.h * rng.uniform(0.9, 1.1, a.h.shape)
+    dz = np.full(50, 40.0) if rule == "zstar" else np.full(50, 1.0 / 50)
+    params = {"dz_top": dz} if rule == "zstar" else {"dsigma": dz}
+    H = a.H.copy()
+    if rule == "sigma":
+        H[~a.active] = np.nan                                          # NaN depth reads as 0
+    h_new, wet = vertical.regrid(_d(H), _d(eta), _d(a.active), _d(h), rule, params)
+    r_new, r_wet = cref.regrid(H, eta, a.active, h, rule, dz)
+    _eq(wet, r_wet.astype(np.uint8), "wet")
+    _eq(h_new, r_new, "h_new")
+
+
+def test_regrid_rejects_a_wet_column_it_cannot_build(gpu):
+    _say("regrid errors")
+    H = np.full((3, 4), 100.0)
+    H[1, 2] = -5.0
+    eta = np.full((3, 4), 10.0)                                         # D > 0 everywhere: column (1, 2) is wet
+    h = np.full((5, 3, 4), 2.0)
+    with pytest.raises(ValueError, match="j=1, i=2"):
+        vertical.regrid(_d(H), _d(eta), _d(np.ones((3, 4), bool)), _d(h), "zstar", {"dz_top": np.full(5, 20.0)})
+    with pytest.raises(NotImplementedError):
+        vertical.regrid(_d(H), _d(eta), _d(np.ones((3, 4), bool)), _d(h), "hybrid", {})
+
+
+def test_regrid_remap_after_the_layer_transport_matches_the_oracle(gpu):
+    _say("layer transport -> regrid_remap")
+    a = synth.ale_inputs(64, 48, 20, seed=5)
+    r = R.ale_layer_transport(a.h, a.H, a.active, a.area, a.dx_t, a.dy_t, a.Lu, a.Lv, a.u_new, a.u_old, a.v_new,
+                              a.v_old, a.wet_u, a.wet_v, a.wu, a.wv, a.Du, a.Dv, a.dt, a.theta, a.theta_bc)
+    h_dag = r["h_dagger"]
+    rng = np.random.default_rng(6)
+    eta = h_dag.sum(0) - a.H
+    C = rng.normal(10.0, 2.0, (10,) + h_dag.shape)
+    dz = np.full(20, 2000.0 / 20)
+    vc = vertical.VerticalCoordinate(H=_d(a.H), active=_d(a.active), rule="zstar", params={"dz_top": dz})
+    h_new, Cn = vc.regrid_remap(_d(eta), _d(h_dag), _d(C))
+    r_new, r_wet = cref.regrid(a.H, eta, a.active, h_dag, "zstar", dz)
+    _eq(h_new, r_new, "h_new")
+    _eq(Cn, cref.remap_columns(h_dag, C, r_new, cref.PPM_H4, r_wet), "C remapped")
+    h_new2, lst = vc.regrid_remap(_d(eta), _d(h_dag), [_d(C[0]), _d(C[3])])
+    _eq(lst[1], cref.remap_columns(h_dag, C[3], r_new, cref.PPM_H4, r_wet), "list field")
+
+
+def test_remap_at_full_size_conserves_every_column(gpu):
+    _say("remap 1024x1024x50")
+    a = synth.ale_inputs(1024, 1024, 50, seed=0)
+    rng = np.random.default_rng(1)
+    hs = _d(a.h)
+    ht = _d(a.h * rng.uniform(0.8, 1.2, a.h.shape))
+    ht = ht * (hs.sum(0) / torch.where(ht.sum(0) > 0, ht.sum(0), torch.ones_like(ht.sum(0))))[None]
+    wet = _d(a.active & (a.h.sum(0) > 0))
+    C = torch.empty((4,) + a.h.shape, dtype=torch.float64, device="cuda")
+    C[0] = 12.5
+    C[1:] = _d(rng.normal(10.0, 2.0, (3,) + a.h.shape))
+    out = vertical.remap_columns(hs, C, ht, vertical.PPM_H4, wet)
+    w = wet.bool()
+    # a uniform field stays uniform to round-off: a piece's length is a difference of interface heights of up to the
+    # column depth D, so a thin target cell carries a relative error of order eps D / h
+    D = ht.sum(0)[None]
+    tol = 64 * 2.3e-16 * 12.5 * (D / torch.where(ht > 0, ht, torch.ones_like(ht)))
+    assert ((out[0] - 12.5).abs() <= tol)[:, w].all()
+    c0 = (hs[None] * C).sum(1)[:, w]
+    c1 = (ht[None] * out).sum(1)[:, w]
+    assert ((c1 - c0).abs() <= 1e-12 * (hs[None] * C.abs()).sum(1)[:, w] + 1e-300).all()
+    assert torch.isfinite(out[:, :, w]).all()
