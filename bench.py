@@ -452,14 +452,14 @@ def bench_ale(args) -> int:
     cpu = None
     if not args.no_cpu:
         from oracle import numpy_ref as R
-        s = synth.ale_inputs(256, 256, nz, seed=0)
+        s = synth.ale_inputs(512, 512, nz, seed=0)
         t1 = time.perf_counter()
         R.ale_layer_transport(s.h, s.H, s.active, s.area, s.dx_t, s.dy_t, s.Lu, s.Lv, s.u_new, s.u_old, s.v_new,
                               s.v_old, s.wet_u, s.wet_v, s.wu, s.wv, s.Du, s.Dv, s.dt, s.theta, s.theta_bc)
         tc = time.perf_counter() - t1
-        cpu = {"value": 256 * 256 * nz / tc, "unit": "cell-layer updates/s", "cores": 1, "kind": "port",
-               "sample": f"one layer-transport step of the oracle (NumPy) on a 256x256x{nz} cut of the same grid "
-                         f"({tc:.1f} s)"}
+        cpu = {"value": 512 * 512 * nz / tc, "unit": "cell-layer updates/s", "cores": 1, "kind": "port",
+               "sample": f"one layer-transport step of the oracle (NumPy) on a 512x512x{nz} regional grid of the "
+                         f"same construction ({tc:.1f} s)"}
     res = {
         "metric": "ALE layer-transport cell-layer updates/s (transport velocity, PPM flux + Newton, outflow "
                   "limiter, Lagrangian step; both face directions)",
@@ -475,6 +475,96 @@ def bench_ale(args) -> int:
                           "unit": "GB/s", "frac": step_bytes_ / (wall / args.steps) / HBM_PEAK},
         "kernels_ms": kern,
         "working_launches_per_step": work,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_remap(args) -> int:
+    """The vertical coordinate's regrid + remap step of the ALE cycle on the regional grid: the layers after one
+    layer-transport step (h†) are regridded to z* (40 m nominal levels) and the tracer stack (--nt fields) is
+    remapped onto them with PPM_H4."""
+    import torch
+
+    from pyocean_amd import _native, ale, synth, vertical
+
+    _log(f"generating ALE inputs {args.nx}x{args.ny}x{args.nz}")
+    a = synth.ale_inputs(args.nx, args.ny, args.nz, seed=0)
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in vars(a).items() if isinstance(v, np.ndarray)}
+    lt = ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
+                            dy_t=d["dy_t"], dt=a.dt, theta=a.theta)
+    qe, qw = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 0)
+    qn, qs = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 1)
+    fl = lt(d["h"], d["u_new"], d["u_old"], d["v_new"], d["v_old"], d["wet_u"], d["wet_v"], d["wu"], d["wv"],
+            d["Du"], d["Dv"], qe, qw, qn, qs, theta_bc=a.theta_bc)
+    h_src = fl.h_dagger
+    eta = h_src.sum(0) - d["H"]
+    nz, ny, nx = a.h.shape
+    rng = np.random.default_rng(2)
+    C = torch.from_numpy(rng.normal(10.0, 2.0, (args.nt, nz, ny, nx))).cuda()
+    dz = np.full(nz, 2000.0 / nz)
+    vc = vertical.VerticalCoordinate(H=d["H"], active=d["active"], rule="zstar", params={"dz_top": dz})
+    state = {}
+
+    def step():
+        state["h"], state["C"] = vc.regrid_remap(eta, h_src, C)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    if args.copy_calibration:
+        h_src.clone()
+        torch.cuda.synchronize()
+    _native.profile_enable(True)
+    _native.profile_collect()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _native.profile_enable(False)
+    prof = _native.profile_collect()
+    cells = nz * ny * nx
+    value = args.nt * cells * args.steps / wall
+    kern = {k: v[0] / args.steps for k, v in prof.items()}
+    # remap: each field's values read and written once (16 B per field-cell), both thickness stacks once per
+    # column shared by the fields (16 B per cell)
+    b = (16.0 * args.nt + 16.0) * cells
+    achieved = b / (kern["remap"] * 1e-3) / 1e9
+    traffic = read_traffic("remap", f"remap_regional{nx}x{ny}x{nz}nt{args.nt}")
+    valu_busy = None
+    cpath = ROOT / "profiles" / f"counters_remap_regional{nx}x{ny}x{nz}nt{args.nt}.txt"
+    if cpath.exists():
+        ctr = dict(l.split()[:2] for l in cpath.read_text().splitlines() if l and not l.startswith("#"))
+        # VALU wavefront instructions x 4 cycles over the SIMD-cycles of the measured launch time
+        valu_busy = float(ctr["SQ_INSTS_VALU"]) * 4 / (1024 * 2.4e9 * kern["remap"] * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import cref
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        hs, hn = h_src.cpu().numpy(), state["h"].cpu().numpy()
+        _, wet = cref.regrid(a.H, eta.cpu().numpy(), a.active, hs, "zstar", dz)
+        sample = np.ascontiguousarray(C.cpu().numpy())
+        reps = 6
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            cref.remap_columns(hs, sample, hn, cref.PPM_H4, wet)
+        tc = time.perf_counter() - t1
+        cpu = {"value": reps * args.nt * cells / tc, "unit": "field-cell remaps/s", "cores": threads, "kind": "port",
+               "sample": f"the same {args.nt}-field remap, {reps} repetitions, PPM_H4, the oracle's C loops over "
+                         f"columns (OpenMP) ({tc:.1f} s)"}
+    res = {
+        "metric": "field-cell remaps/s (z* regrid + PPM_H4 conservative remap of the tracer stack)",
+        "value": value, "unit": "field-cell remaps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded regional grid after one layer-transport step)",
+        "config": {"workload": f"regrid_remap regional {nx}x{ny}x{nz}, {args.nt} fields, z*, PPM_H4"},
+        "roofline": {"kernel": "remap", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic, "algorithmic_bytes": b,
+                     "valu_busy_from_counters": valu_busy},
+        "kernels_ms": kern,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
@@ -498,12 +588,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion", "ale"), default="step",
+    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion", "ale", "remap"),
+                    default="step",
                     help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "
                          "persistent launch; serve: the Poisson job stream through the continuous-batching loop; "
                          "precision: split against FP64 after --checkpoints steps; diffusion: the along-layer and "
                          "vertical tracer diffusion of the stack; ale: the ALE layer-transport step that "
-                         "produces the advection's inputs")
+                         "produces the advection's inputs; remap: the vertical regrid + remap of the stack")
     ap.add_argument("--checkpoints", default="1,10,200,2000", help="precision form: step counts to compare at")
     ap.add_argument("--mix", choices=("mixed", "split", "f64"), default="mixed",
                     help="batch form: member precisions (mixed: even members FP64, odd split)")
@@ -538,6 +629,8 @@ def main() -> int:
         return bench_diffusion(args)
     if args.form == "ale":
         return bench_ale(args)
+    if args.form == "remap":
+        return bench_remap(args)
 
     if args.rch or args.wpb:
         from pyocean_amd import batch as _batch

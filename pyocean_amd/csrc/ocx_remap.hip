@@ -12,6 +12,10 @@ namespace {
 using namespace ocx_host;
 using namespace ocx::rmp;
 
+#ifndef REMAP_G
+#define REMAP_G 2
+#endif
+
 __global__ void k_set_bad(unsigned long long* p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = ~0ull;
 }
@@ -31,11 +35,18 @@ int ocx_remap_columns_f64(const double* h_src, const double* f, const double* h_
     for (int q = 0; q < 5; ++q) OCX_TRY(need(p[q], nm[q]));
     if (out == f || out == h_src || out == h_tgt) return fail(OCX_E_ARG, "out must not alias the inputs");
     RemapArgs r{h_src, f, h_tgt, wet, out, n0, n1, nf, scheme, columns};
-    const int fw = std::min(nf, 4);
-    const dim3 grid((unsigned)((columns + 63) / 64), (unsigned)((nf + fw - 1) / fw)), block(64, fw);
     hipStream_t s = static_cast<hipStream_t>(stream);
     ProfScope _ps("remap", s);
-    k_remap<<<grid, block, 0, s>>>(r);
+    // one field per thread for a single field; otherwise groups of REMAP_G fields per thread, four groups (waves)
+    // of the same 64 columns per workgroup
+    const int g = nf == 1 ? 1 : REMAP_G;
+    const int groups = (nf + g - 1) / g;
+    const int fw = std::min(groups, 4);
+    const dim3 grid((unsigned)((columns + 63) / 64), (unsigned)((groups + fw - 1) / fw)), block(64, fw);
+    if (g == 1)
+        k_remap<<<grid, block, 0, s>>>(r);
+    else
+        k_remap_g<REMAP_G><<<grid, block, 0, s>>>(r);
     return launch_check("remap_columns");
 }
 

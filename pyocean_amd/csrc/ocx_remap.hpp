@@ -17,6 +17,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "ocx_div.hpp"
+
 namespace ocx {
 namespace rmp {
 
@@ -41,8 +43,11 @@ struct Col {
 };
 
 // ------------------------------------------------------------------------------------------ PPM_H4 pieces
-// (outer, inner) edge values at one end from the 4 end cells (dz from the end, > 0), closed form of the cubic
-__device__ void end_values_h4(const double dz[4], const double u[4], double& e_out, double& e_in) {
+// the end cubic's weights, which depend on the 4 end thicknesses only (dz from the end, > 0)
+struct EndW {
+    double w11, w21, w31, w12, w22, w32, w13, w23, w33, w14, w24, w34, d;
+};
+__device__ EndW end_weights(const double dz[4]) {
     const double min_frac = 1.0e-6;
     const double h1 = dz[0], h2 = dz[1];
     double h3 = dz[2], h4 = dz[3];
@@ -50,39 +55,52 @@ __device__ void end_values_h4(const double dz[4], const double u[4], double& e_o
     if ((h3 + h4) < min_frac * h1) h4 = min_frac * h1 - h3;
     const double h12 = h1 + h2, h23 = h2 + h3, h34 = h3 + h4;
     const double h123 = h12 + h3, h234 = h2 + h34, h1234 = h12 + h34;
-    const double I_denB3 = 1.0 / (h123 * h12 * h23);
+    const double I_denB3 = qdiv(1.0, h123 * h12 * h23);
     const double I_h12 = (h123 * h23) * I_denB3;
     const double I_h23 = (h12 * h123) * I_denB3;
     const double I_h123 = (h12 * h23) * I_denB3;
-    const double I_denom = 1.0 / (h1234 * (h234 * h34));
+    const double I_denom = qdiv(1.0, h1234 * (h234 * h34));
     const double I_h234 = (h1234 * h34) * I_denom;
     const double I_h1234 = (h234 * h34) * I_denom;
-    const double w11 = -h1 * (I_h1234 + I_h123 + I_h12);
-    const double w21 = h1 * h12 * (I_h234 * I_h1234 + I_h23 * (I_h234 + I_h123));
-    const double w31 = -h1 * h12 * h123 * I_denom;
-    const double w12 = 2.0 * (I_h12 * (1.0 + (h1 + h12) * (I_h1234 + I_h123)) + h1 * I_h1234 * I_h123);
-    const double w22 = -2.0 * ((h1 * h12 * I_h1234) * (I_h23 * (I_h234 + I_h123)) +
-                               (h1 + h12) * (I_h1234 * I_h234 + I_h23 * (I_h234 + I_h123)));
-    const double w32 = 2.0 * ((h1 + h12) * h123 + h1 * h12) * I_denom;
-    const double w13 = -3.0 * I_h12 * I_h123 * (1.0 + I_h1234 * ((h1 + h12) + h123));
-    const double w23 = 3.0 * I_h23 * (I_h123 + I_h1234 * ((h1 + h12) + h123) * (I_h123 + I_h234));
-    const double w33 = -3.0 * ((h1 + h12) + h123) * I_denom;
-    const double w14 = 4.0 * I_h1234 * I_h123 * I_h12;
-    const double w24 = -4.0 * I_h1234 * (I_h23 * (I_h123 + I_h234));
-    const double w34 = 4.0 * I_denom;
-    const double d1 = u[1] - u[0], d2 = u[2] - u[1], d3 = u[3] - u[2];
-    const double c0 = ((u[0] + (w11 * d1)) + (w21 * d2)) + (w31 * d3);
-    const double c1 = ((w12 * d1) + (w22 * d2)) + (w32 * d3);
-    const double c2 = ((w13 * d1) + (w23 * d2)) + (w33 * d3);
-    const double c3 = ((w14 * d1) + (w24 * d2)) + (w34 * d3);
-    const double d = dz[0];
-    e_out = c0;
-    e_in = c0 + d * (c1 + d * (c2 + d * c3));
+    EndW W;
+    W.w11 = -h1 * (I_h1234 + I_h123 + I_h12);
+    W.w21 = h1 * h12 * (I_h234 * I_h1234 + I_h23 * (I_h234 + I_h123));
+    W.w31 = -h1 * h12 * h123 * I_denom;
+    W.w12 = 2.0 * (I_h12 * (1.0 + (h1 + h12) * (I_h1234 + I_h123)) + h1 * I_h1234 * I_h123);
+    W.w22 = -2.0 * ((h1 * h12 * I_h1234) * (I_h23 * (I_h234 + I_h123)) +
+                    (h1 + h12) * (I_h1234 * I_h234 + I_h23 * (I_h234 + I_h123)));
+    W.w32 = 2.0 * ((h1 + h12) * h123 + h1 * h12) * I_denom;
+    W.w13 = -3.0 * I_h12 * I_h123 * (1.0 + I_h1234 * ((h1 + h12) + h123));
+    W.w23 = 3.0 * I_h23 * (I_h123 + I_h1234 * ((h1 + h12) + h123) * (I_h123 + I_h234));
+    W.w33 = -3.0 * ((h1 + h12) + h123) * I_denom;
+    W.w14 = 4.0 * I_h1234 * I_h123 * I_h12;
+    W.w24 = -4.0 * I_h1234 * (I_h23 * (I_h123 + I_h234));
+    W.w34 = 4.0 * I_denom;
+    W.d = dz[0];                                  // evaluated at the unmodified end thickness
+    return W;
 }
 
-// the explicit 4th-order value at the interface between cells 1 and 2 of the 4-cell stencil (h0..h3, u0..u3)
-__device__ double interior_edge(double h0, double h1, double h2, double h3, double u0, double u1, double u2,
-                                double u3) {
+// (outer, inner) edge values at one end: the cubic whose means over the 4 end cells are u
+__device__ __forceinline__ void end_apply(const EndW& W, double u0, double u1, double u2, double u3, double& e_out,
+                                          double& e_in) {
+    const double d1 = u1 - u0, d2 = u2 - u1, d3 = u3 - u2;
+    const double c0 = ((u0 + (W.w11 * d1)) + (W.w21 * d2)) + (W.w31 * d3);
+    const double c1 = ((W.w12 * d1) + (W.w22 * d2)) + (W.w32 * d3);
+    const double c2 = ((W.w13 * d1) + (W.w23 * d2)) + (W.w33 * d3);
+    const double c3 = ((W.w14 * d1) + (W.w24 * d2)) + (W.w34 * d3);
+    e_out = c0;
+    e_in = c0 + W.d * (c1 + W.d * (c2 + W.d * c3));
+}
+
+__device__ void end_values_h4(const double dz[4], const double u[4], double& e_out, double& e_in) {
+    end_apply(end_weights(dz), u[0], u[1], u[2], u[3], e_out, e_in);
+}
+
+// the explicit 4th-order interface value between cells 1 and 2 of the 4-cell stencil: its thickness factors ...
+struct EdgeW {
+    double A1, B1, C2, C3, Dsum;
+};
+__device__ EdgeW interior_weights(double h0, double h1, double h2, double h3) {
     if (h0 + h1 == 0.0 || h1 + h2 == 0.0 || h2 + h3 == 0.0) {
         const double h_min = H_MIN_FRAC * pmax(H_NEGLECT_EDGE, (h0 + h1) + (h2 + h3));
         h0 = pmax(h_min, h0);
@@ -90,24 +108,44 @@ __device__ double interior_edge(double h0, double h1, double h2, double h3, doub
         h2 = pmax(h_min, h2);
         h3 = pmax(h_min, h3);
     }
-    const double I_h12 = 1.0 / (h1 + h2);
-    const double I_den_et2 = 1.0 / (((h0 + h1) + h2) * (h0 + h1));
+    const double I_h12 = qdiv(1.0, h1 + h2);
+    const double I_den_et2 = qdiv(1.0, ((h0 + h1) + h2) * (h0 + h1));
     const double I_h012 = (h0 + h1) * I_den_et2;
-    const double I_den_et3 = 1.0 / ((h1 + (h2 + h3)) * (h2 + h3));
+    const double I_den_et3 = qdiv(1.0, (h1 + (h2 + h3)) * (h2 + h3));
     const double I_h123 = (h2 + h3) * I_den_et3;
-    const double et1 = (1.0 + (h1 * I_h012 + (h0 + h1) * I_h123)) * I_h12 * (h2 * (h2 + h3)) * u1 +
-                       (1.0 + (h2 * I_h123 + (h2 + h3) * I_h012)) * I_h12 * (h1 * (h0 + h1)) * u2;
-    const double et2 = (h1 * (h2 * (h2 + h3)) * I_den_et2) * (u1 - u0);
-    const double et3 = (h2 * (h1 * (h0 + h1)) * I_den_et3) * (u2 - u3);
-    return (et1 + (et2 + et3)) / ((h0 + h1) + (h2 + h3));
+    EdgeW W;
+    W.A1 = (1.0 + (h1 * I_h012 + (h0 + h1) * I_h123)) * I_h12 * (h2 * (h2 + h3));
+    W.B1 = (1.0 + (h2 * I_h123 + (h2 + h3) * I_h012)) * I_h12 * (h1 * (h0 + h1));
+    W.C2 = h1 * (h2 * (h2 + h3)) * I_den_et2;
+    W.C3 = h2 * (h1 * (h0 + h1)) * I_den_et3;
+    W.Dsum = (h0 + h1) + (h2 + h3);
+    return W;
+}
+// ... applied to the cell means u0..u3
+__device__ __forceinline__ double interior_apply(const EdgeW& W, double u0, double u1, double u2, double u3) {
+    const double et1 = W.A1 * u1 + W.B1 * u2;
+    const double et2 = W.C2 * (u1 - u0);
+    const double et3 = W.C3 * (u2 - u3);
+    return qdiv(et1 + (et2 + et3), W.Dsum);
 }
 
-// the cell's limited slope times h used to bound its edges (cells km1, k, kp1 already clamped at the column ends)
-__device__ __forceinline__ double bound_slope(double hm, double hc, double hp, double um, double uc, double up) {
+__device__ double interior_edge(double h0, double h1, double h2, double h3, double u0, double u1, double u2,
+                                double u3) {
+    return interior_apply(interior_weights(h0, h1, h2, h3), u0, u1, u2, u3);
+}
+
+// the thickness ratio h_k / (h_km1 + h_kp1 + 2 h_k) of the edge-bounding slope, and whether it is defined
+__device__ __forceinline__ double slope_ratio(double hm, double hc, double hp, bool& ok) {
+    const double den = (hm + hp) + 2.0 * hc;
+    ok = den > 0.0;
+    return ok ? qdiv(hc, den) : 0.0;
+}
+// the cell's limited slope times h used to bound its edges, from its ratio (neighbours clamped at the column ends)
+__device__ __forceinline__ double bound_slope_r(bool ok, double ratio, double um, double uc, double up) {
     double slope_x_h = 0.0;
-    if ((hm + hp) + 2.0 * hc > 0.0) {
+    if (ok) {
         const double sl = uc - um;
-        const double sc = (up - um) * (hc / ((hm + hp) + 2.0 * hc));
+        const double sc = (up - um) * ratio;
         const double sr = up - uc;
         if (sl * sr > 0.0) {
             const double m = pmin(pmin(fabs(sl), fabs(sc)), fabs(sr));
@@ -116,7 +154,11 @@ __device__ __forceinline__ double bound_slope(double hm, double hc, double hp, d
     }
     return slope_x_h;
 }
-
+__device__ __forceinline__ double bound_slope(double hm, double hc, double hp, double um, double uc, double up) {
+    bool ok;
+    const double ratio = slope_ratio(hm, hc, hp, ok);
+    return bound_slope_r(ok, ratio, um, uc, up);
+}
 // an edge value bounded by its cell and the neighbour on its side (unb = u of that neighbour, or the cell itself at
 // the column end); right: the upper edge
 __device__ __forceinline__ double bound_edge(double e, double slope_x_h, double unb, double uc, bool right) {
@@ -158,7 +200,7 @@ __device__ __forceinline__ void cw_coef(double u_l, double u_c, double u_r, doub
 __device__ double plm_slope_wa(double h_l, double h_c, double h_r, double u_l, double u_c, double u_r) {
     const double sigma_r = u_r - u_c;
     const double sigma_l = u_c - u_l;
-    const double sigma_c = 2.0 * (u_r - u_l) * (h_c / (h_l + 2.0 * h_c + h_r + H_NEGLECT));
+    const double sigma_c = 2.0 * (u_r - u_l) * qdiv(h_c, h_l + 2.0 * h_c + h_r + H_NEGLECT);
     const double u_min = pmin(pmin(u_l, u_c), u_r);
     const double u_max = pmax(pmax(u_l, u_c), u_r);
     double slope;
@@ -229,7 +271,7 @@ __device__ __forceinline__ double average(double c0, double c1, double c2, doubl
         const double x = 0.5 * (xa + xb);
         return c0 + x * (c1 + x * c2);
     }
-    return c0 + 0.5 * c1 * (xa + xb) + c2 * (xa * xa + xa * xb + xb * xb) / 3.0;
+    return c0 + 0.5 * c1 * (xa + xb) + div_const(c2 * (xa * xa + xa * xb + xb * xb), 3.0, 1.0 / 3.0);
 }
 
 // the target side of one column: the cells the source has reached, their running content and the vanished cells
@@ -252,7 +294,7 @@ struct Targets {
     // a target the walk has passed: its mean (vanished targets are set by their local value instead)
     __device__ __forceinline__ void finish(int jj, double c) {
         const double h = ht[jj];
-        if (h > 0.0) put(jj, c / h);
+        if (h > 0.0) put(jj, qdiv(c, h));
     }
 };
 
@@ -299,7 +341,7 @@ __device__ void merge_cell(Targets& T, double a, double hi, double us, double c0
         const double L = up - lo;
         if (L > 0.0) {
             if (p != ithick) {
-                const double piece = L * average(c0, c1, c2, (lo - a) / hi, (up - a) / hi);
+                const double piece = L * average(c0, c1, c2, qdiv(lo - a, hi), qdiv(up - a, hi));
                 s += piece;
                 c = c + piece;
             } else {
@@ -335,7 +377,7 @@ __device__ __forceinline__ void locate_vanished(Targets& T, double a, double hi,
     while (T.jv < T.n1 && T.tv <= b) {
         const double h = T.ht[T.jv];
         if (!(h > 0.0)) {
-            const double xi = (T.tv - a) / hi;
+            const double xi = qdiv(T.tv - a, hi);
             T.put(T.jv, c0 + xi * (c1 + xi * c2));
         }
         T.tv = T.tv + h;
@@ -351,6 +393,153 @@ __device__ void finish_targets(Targets& T, double u_last) {
     while (T.jv < T.n1) {
         const double h = T.ht[T.jv];
         if (!(h > 0.0)) T.put(T.jv, u_last);
+        ++T.jv;
+    }
+}
+
+
+// ------------------------------------------------------------------------------------------ grouped PPM_H4 remap
+// G fields of one column in one thread: the thickness work - the interface stencils' reciprocals and products, the
+// slope ratios, the end cubics' weights and the whole overlap geometry - is done once for the G fields; per field
+// remain the sums with its values, one division per interface and one per target cell.  Every value is computed
+// by the same operations as the single-field path.
+template <int G>
+struct GTargets {
+    Col ht;
+    double* out[G];
+    long long M;
+    int n1, ng;
+    int j;
+    double tj;
+    double content[G];
+    int jv;
+    double tv;
+
+    __device__ __forceinline__ double top(int jj, double t_lo, double b) const {
+        return jj < n1 - 1 ? t_lo + ht[jj] : pmax(t_lo + ht[jj], b);
+    }
+    __device__ __forceinline__ void put(int g, int k, double v) {
+        if (g < ng) out[g][(size_t)k * M] = v;
+    }
+    __device__ __forceinline__ void finish(int jj, const double* c) {
+        const double h = ht[jj];
+        if (h > 0.0) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) put(g, jj, qdiv(c[g], h));
+        }
+    }
+};
+
+template <int G>
+__device__ void merge_cell_g(GTargets<G>& T, double a, double hi, const double* us, const double* c0,
+                             const double* c1, const double* c2) {
+    const double b = a + hi;
+    int npc = 0, ithick = 0, jthick = T.j;
+    double thick = -1.0;
+    {
+        int j = T.j;
+        double t = T.tj;
+        while (true) {
+            const double lo = pmax(a, t);
+            const double hi_t = T.top(j, t, b);
+            const double up = pmin(b, hi_t);
+            const double L = up - lo;
+            if (L > 0.0) {
+                if (L > thick) {
+                    thick = L;
+                    ithick = npc;
+                    jthick = j;
+                }
+                ++npc;
+            }
+            if (hi_t <= b && j < T.n1 - 1) {
+                t = t + T.ht[j];
+                ++j;
+            } else {
+                break;
+            }
+        }
+    }
+    double s[G], c[G], c_thick[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        s[g] = 0.0;
+        c[g] = T.content[g];
+        c_thick[g] = 0.0;
+    }
+    int p = 0;
+    int j = T.j;
+    double t = T.tj;
+    while (true) {
+        const double lo = pmax(a, t);
+        const double hi_t = T.top(j, t, b);
+        const double up = pmin(b, hi_t);
+        const double L = up - lo;
+        if (L > 0.0) {
+            if (p != ithick) {
+                const double xa = qdiv(lo - a, hi), xb = qdiv(up - a, hi);
+                if (xb - xa <= 1e-14) {
+                    const double x = 0.5 * (xa + xb);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const double piece = L * (c0[g] + x * (c1[g] + x * c2[g]));
+                        s[g] += piece;
+                        c[g] = c[g] + piece;
+                    }
+                } else {
+                    const double sx = xa + xb;
+                    const double s2 = xa * xa + xa * xb + xb * xb;
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const double piece = L * (c0[g] + 0.5 * c1[g] * sx + div_const(c2[g] * s2, 3.0, 1.0 / 3.0));
+                        s[g] += piece;
+                        c[g] = c[g] + piece;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < G; ++g) c_thick[g] = c[g];
+            }
+            ++p;
+        }
+        if (hi_t <= b && j < T.n1 - 1) {
+            if (j != jthick || npc == 0) T.finish(j, c);
+            t = t + T.ht[j];
+            ++j;
+#pragma unroll
+            for (int g = 0; g < G; ++g) c[g] = 0.0;
+        } else {
+            break;
+        }
+    }
+    if (npc > 0) {
+        double tp[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) tp[g] = c_thick[g] + (us[g] * hi - s[g]);
+        if (jthick == j) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) c[g] = tp[g];
+        } else {
+            T.finish(jthick, tp);
+        }
+    }
+    T.j = j;
+    T.tj = t;
+#pragma unroll
+    for (int g = 0; g < G; ++g) T.content[g] = c[g];
+}
+
+template <int G>
+__device__ __forceinline__ void locate_vanished_g(GTargets<G>& T, double a, double hi, double b, const double* c0,
+                                                  const double* c1, const double* c2) {
+    while (T.jv < T.n1 && T.tv <= b) {
+        const double h = T.ht[T.jv];
+        if (!(h > 0.0)) {
+            const double xi = qdiv(T.tv - a, hi);
+#pragma unroll
+            for (int g = 0; g < G; ++g) T.put(g, T.jv, c0[g] + xi * (c1[g] + xi * c2[g]));
+        }
+        T.tv = T.tv + h;
         ++T.jv;
     }
 }
@@ -506,6 +695,172 @@ __device__ void remap_one(const RemapArgs& r, long long col, int q) {
         s_cur = s_next;
     }
     finish_targets(T, u[last]);
+}
+
+
+// fields q0 .. q0 + G - 1 of column col (those beyond nf are computed on field q0 and not stored): the PPM_H4 path
+// shares the thickness work; the other schemes and dry or vanished columns take the single-field path per field
+template <int G>
+__device__ void remap_group(const RemapArgs& r, long long col, int q0) {
+    const long long M = r.M;
+    const int n0 = r.n0, n1 = r.n1;
+    const int ng = r.nf - q0 < G ? r.nf - q0 : G;
+    const Col hs{r.h_src + col, M};
+    bool ppm = false;
+    int last = -1;
+    if (r.wet[col] && r.scheme == PPM_H4 && n0 >= 4) {
+        int n_active = 0;
+        for (int k = 0; k < n0; ++k) {
+            if (hs[k] > 0.0) {
+                ++n_active;
+                last = k;
+            }
+        }
+        ppm = n_active > 3;
+    }
+    if (!ppm) {
+        for (int g = 0; g < ng; ++g) remap_one(r, col, q0 + g);
+        return;
+    }
+    Col u[G];
+    GTargets<G> T;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int q = q0 + (g < ng ? g : 0);
+        u[g] = Col{r.f + (size_t)q * n0 * M + col, M};
+        T.out[g] = r.out + (size_t)q * n1 * M + col;
+        T.content[g] = 0.0;
+    }
+    T.ht = Col{r.h_tgt + col, M};
+    T.M = M;
+    T.n1 = n1;
+    T.ng = ng;
+    T.j = 0;
+    T.tj = 0.0;
+    T.jv = 0;
+    T.tv = 0.0;
+
+    double wh[5], wu[G][5];
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+        const int k = d - 2;
+        const bool in = k >= 0 && k < n0;
+        wh[d] = in ? hs[k] : 0.0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) wu[g][d] = in ? u[g][k] : 0.0;
+    }
+    double e_bot_in[G], e_top_out[G], e_top_in[G], fl[G], s_cur[G];
+    {
+        double dz[4];
+        for (int d = 0; d < 4; ++d) dz[d] = pmax(H_NEGLECT_EDGE, hs[d]);
+        const EndW Wb = end_weights(dz);
+        bool ok;
+        const double ratio = slope_ratio(wh[2], wh[2], wh[3], ok);      // cell 0: km1 = 0
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            double e_out;
+            end_apply(Wb, u[g][0], u[g][1], u[g][2], u[g][3], e_out, e_bot_in[g]);
+            s_cur[g] = bound_slope_r(ok, ratio, wu[g][2], wu[g][2], wu[g][3]);
+            fl[g] = bound_edge(e_out, s_cur[g], wu[g][2], wu[g][2], false);
+            e_top_out[g] = 0.0;
+            e_top_in[g] = 0.0;
+        }
+    }
+    double a = 0.0;
+    for (int i = 0; i < n0; ++i) {
+        if (i == n0 - 2) {
+            // the top end cubic, from cells n0-1 .. n0-4 (window positions 3 - d at step n0 - 2)
+            double dz[4];
+            for (int d = 0; d < 4; ++d) dz[d] = pmax(H_NEGLECT_EDGE, hs[n0 - 1 - d]);
+            const EndW Wt = end_weights(dz);
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                end_apply(Wt, u[g][n0 - 1], u[g][n0 - 2], u[g][n0 - 3], u[g][n0 - 4], e_top_out[g], e_top_in[g]);
+        }
+        double fr[G], fl_next[G], s_next[G];
+        if (i + 1 < n0) {
+            const int m = i + 1;
+            const bool interior = m >= 2 && m <= n0 - 2;
+            EdgeW W;
+            if (interior) W = interior_weights(wh[1], wh[2], wh[3], wh[4]);
+            bool ok;
+            const double ratio = i + 1 < n0 - 1 ? slope_ratio(wh[2], wh[3], wh[4], ok) : slope_ratio(wh[2], wh[3], wh[3], ok);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const double raw = m == 1 ? e_bot_in[g] : (m == n0 - 1 ? e_top_in[g]
+                                                            : interior_apply(W, wu[g][1], wu[g][2], wu[g][3], wu[g][4]));
+                const double up = i + 1 < n0 - 1 ? wu[g][4] : wu[g][3];
+                s_next[g] = bound_slope_r(ok, ratio, wu[g][2], wu[g][3], up);
+                const double br = bound_edge(raw, s_cur[g], wu[g][3], wu[g][2], true);
+                const double bl = bound_edge(raw, s_next[g], wu[g][2], wu[g][3], false);
+                fr[g] = br;
+                fl_next[g] = bl;
+                if ((bl - br) * (wu[g][3] - wu[g][2]) < 0.0) {
+                    const double av = clampu(0.5 * (br + bl), wu[g][2], wu[g][3]);
+                    fr[g] = av;
+                    fl_next[g] = av;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                fr[g] = bound_edge(e_top_out[g], s_cur[g], wu[g][2], wu[g][2], true);
+                fl_next[g] = 0.0;
+                s_next[g] = 0.0;
+            }
+        }
+        const double hi = wh[2];
+        if (hi > 0.0) {
+            double c0[G], c1[G], c2[G], us[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                c0[g] = wu[g][2];
+                c1[g] = 0.0;
+                c2[g] = 0.0;
+                us[g] = wu[g][2];
+                if (i > 0 && i < n0 - 1) cw_coef(wu[g][1], wu[g][2], wu[g][3], fl[g], fr[g], c0[g], c1[g], c2[g]);
+            }
+            const double b = a + hi;
+            merge_cell_g<G>(T, a, hi, us, c0, c1, c2);
+            locate_vanished_g<G>(T, a, hi, b, c0, c1, c2);
+            a = b;
+        }
+        for (int d = 0; d < 4; ++d) {
+            wh[d] = wh[d + 1];
+#pragma unroll
+            for (int g = 0; g < G; ++g) wu[g][d] = wu[g][d + 1];
+        }
+        const int k_in = i + 3;
+        wh[4] = k_in < n0 ? hs[k_in] : 0.0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            wu[g][4] = k_in < n0 ? u[g][k_in] : 0.0;
+            fl[g] = fl_next[g];
+            s_cur[g] = s_next[g];
+        }
+    }
+    // the open target and those beyond; vanished targets above the source take the last active cell's mean
+    T.finish(T.j, T.content);
+    double zero[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) zero[g] = 0.0;
+    for (int jj = T.j + 1; jj < n1; ++jj) T.finish(jj, zero);
+    while (T.jv < n1) {
+        const double h = T.ht[T.jv];
+        if (!(h > 0.0)) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) T.put(g, T.jv, u[g][last]);
+        }
+        ++T.jv;
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_remap_g(RemapArgs r) {
+    const long long col = (long long)blockIdx.x * 64 + threadIdx.x;
+    const int q0 = (blockIdx.y * blockDim.y + threadIdx.y) * G;
+    if (col >= r.M || q0 >= r.nf) return;
+    remap_group<G>(r, col, q0);
 }
 
 // block (64, FW): x over columns, y over fields; grid (ceil(M / 64), ceil(nf / FW))

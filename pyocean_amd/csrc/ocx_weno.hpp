@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ocx_div.hpp"
+
 namespace ocx {
 
 constexpr unsigned OPEN_W = 1u, OPEN_E = 2u, OPEN_S = 4u, OPEN_N = 8u;
@@ -26,31 +28,6 @@ __device__ __forceinline__ long long xcd_logical(long long b, long long nb) {
 // Index of the calling wavefront in its workgroup, as a wave-uniform (scalar) value: everything derived from it
 // (tracer, plane base pointers) stays in scalar registers instead of being recomputed per lane.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
-
-// Correctly rounded x / b for a constant divisor b with rb = RN(1/b): q = RN(x rb) is within an ulp of x/b, the
-// remainder x - b q is exact under an FMA, and RN(q + (x - b q) rb) = RN(x/b) (Markstein's correction; needs rb
-// within half an ulp of 1/b and no underflow, so tiny |x| takes the IEEE division).
-__device__ __forceinline__ double div_const(double x, double b, double rb) {
-    if (fabs(x) < 0x1p-960 && x != 0.0) return x / b;
-    const double q = x * rb;
-    const double r = __fma_rn(-b, q, x);
-    return __fma_rn(r, rb, q);
-}
-
-// x / y for y > 0 by the Newton-Raphson sequence that the compiler emits for an IEEE FP64 division (reciprocal,
-// two refinements, quotient, one residual correction), without the range scaling and special-value fix-up steps.
-// Where those steps would not change anything (neither operand near the overflow or denormal range) the result is
-// the IEEE quotient bit for bit; callers guarantee that range.
-__device__ __forceinline__ double div_nr(double x, double y) {
-    double r = __builtin_amdgcn_rcp(y);
-    double e = __fma_rn(-y, r, 1.0);
-    r = __fma_rn(r, e, r);
-    e = __fma_rn(-y, r, 1.0);
-    r = __fma_rn(r, e, r);
-    const double q = x * r;
-    const double t = __fma_rn(-y, q, x);
-    return __fma_rn(t, r, q);
-}
 
 // WENO5 value from the upwind-ordered stencil v1..v5 (v3 = upwind cell).  Z: WENO5-Z (Borges et al. 2008,
 // tau5 = |b2 - b0|, eps = 1e-40); otherwise WENO5-JS (Jiang & Shu 1996, eps = 1e-6).  d = (0.1, 0.6, 0.3).
