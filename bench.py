@@ -571,6 +571,105 @@ def bench_remap(args) -> int:
     return 0
 
 
+def bench_cycle(args) -> int:
+    """One ALE tracer cycle on the device, every phase on its own kernels and nothing leaving HBM: the layer
+    transport (transport thicknesses, PPM fluxes with the Newton correction, outflow limiter, Lagrangian step), the
+    FCT advection of the --nt tracer stack, its along-layer and implicit vertical diffusion, and the z* regrid with
+    the PPM_H4 remap of the stack."""
+    import torch
+
+    from pyocean_amd import _native, ale, ops, synth, vertical
+
+    _log(f"generating ALE inputs {args.nx}x{args.ny}x{args.nz}")
+    a = synth.ale_inputs(args.nx, args.ny, args.nz, seed=0)
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in vars(a).items() if isinstance(v, np.ndarray)}
+    nz, ny, nx = a.h.shape
+    nt = args.nt
+    rng = np.random.default_rng(3)
+    C0 = torch.from_numpy(10.0 + rng.normal(0.0, 1.0, (nt, nz, ny, nx)) * a.active).cuda()
+    lt = ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
+                            dy_t=d["dy_t"], dt=a.dt, theta=a.theta)
+    vc = vertical.VerticalCoordinate(H=d["H"], active=d["active"], rule="zstar",
+                                     params={"dz_top": np.full(nz, 2000.0 / nz)})
+    act8 = d["active"].to(torch.uint8)
+    cx = (d["Lu"] / torch.cat([d["dx_t"][:, :1], 0.5 * (d["dx_t"][:, 1:] + d["dx_t"][:, :-1]), d["dx_t"][:, -1:]],
+                              1)).contiguous()
+    cy = (d["Lv"] / torch.cat([d["dy_t"][:1], 0.5 * (d["dy_t"][1:] + d["dy_t"][:-1]), d["dy_t"][-1:]], 0)).contiguous()
+    okx = torch.zeros((ny, nx + 1), dtype=torch.uint8, device="cuda")
+    okx[:, 1:-1] = act8[:, 1:] & act8[:, :-1]
+    oky = torch.zeros((ny + 1, nx), dtype=torch.uint8, device="cuda")
+    oky[1:-1] = act8[1:] & act8[:-1]
+    nu = torch.full((nz + 1, ny * nx), 1e-4, dtype=torch.float64, device="cuda")
+    ws_fct, ws_v = ops.Workspace(), ops.Workspace()
+    C1 = torch.empty_like(C0)
+    C2 = torch.empty_like(C0)
+    events = {}
+
+    def mark(name):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        events.setdefault(name, []).append(e)
+
+    def cycle(timed=False):
+        if timed:
+            mark("start")
+        qe, qw = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 0)
+        qn, qs = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 1)
+        fl = lt(d["h"], d["u_new"], d["u_old"], d["v_new"], d["v_old"], d["wet_u"], d["wet_v"], d["wu"], d["wv"],
+                d["Du"], d["Dv"], qe, qw, qn, qs, theta_bc=a.theta_bc)
+        if timed:
+            mark("layer_transport")
+        ops.advect_fct(C0, fl.V0, fl.V1, fl.Qx, fl.Qy, fl.ok, a.dt, out=C1, workspace=ws_fct)
+        if timed:
+            mark("advection")
+        ops.diffuse_h(C1, C1, fl.h_dagger, act8, cx, cy, okx, oky, fl.V1, 100.0, a.dt, out=C2)
+        dz = torch.clamp(fl.h_dagger, min=1e-6).reshape(nz, -1)
+        ops.vertical_implicit(C2.view(nt, nz, -1), dz, nu, None, a.dt, out=C2.view(nt, nz, -1), workspace=ws_v)
+        if timed:
+            mark("diffusion")
+        eta = fl.h_dagger.sum(0) - d["H"]
+        h_new, Cn = vc.regrid_remap(eta, fl.h_dagger, C2)
+        if timed:
+            mark("regrid_remap")
+        return h_new, Cn
+
+    for _ in range(max(args.warmup, 1)):
+        cycle()
+    torch.cuda.synchronize()
+    events.clear()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h_new, Cn = cycle(timed=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    names = ["layer_transport", "advection", "diffusion", "regrid_remap"]
+    prev = events["start"]
+    phase = {}
+    for nm in names:
+        phase[nm] = float(np.mean([p.elapsed_time(q) for p, q in zip(prev, events[nm])]))
+        prev = events[nm]
+    cells = nz * ny * nx
+    # content of the stack before and after (the cycle conserves Σ V c up to the open-edge-free round-off)
+    V_new = h_new * d["area"][None]
+    c0 = ops.content(C0, torch.where(d["active"][None], d["h"], torch.zeros_like(d["h"])) * d["area"][None])
+    c1 = ops.content(Cn.contiguous(), V_new)
+    rel = float(((c1 - c0).abs() / c0.abs()).max())
+    res = {
+        "metric": "ALE tracer cycle tracer-cell updates/s (layer transport, FCT advection, diffusion, z* regrid + "
+                  "PPM_H4 remap)",
+        "value": nt * cells * args.steps / wall, "unit": "tracer-cell updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded regional grid, z* layers, barotropic + baroclinic velocity, 10 tracers)",
+        "config": {"workload": f"ALE tracer cycle regional {nx}x{ny}x{nz}, {nt} tracers"},
+        "phase_ms": phase,
+        "content_rel_change": rel,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -588,13 +687,14 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion", "ale", "remap"),
+    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion", "ale", "remap", "cycle"),
                     default="step",
                     help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "
                          "persistent launch; serve: the Poisson job stream through the continuous-batching loop; "
                          "precision: split against FP64 after --checkpoints steps; diffusion: the along-layer and "
                          "vertical tracer diffusion of the stack; ale: the ALE layer-transport step that "
-                         "produces the advection's inputs; remap: the vertical regrid + remap of the stack")
+                         "produces the advection's inputs; remap: the vertical regrid + remap of the stack; "
+                         "cycle: the whole ALE tracer cycle (transport, advection, diffusion, regrid + remap)")
     ap.add_argument("--checkpoints", default="1,10,200,2000", help="precision form: step counts to compare at")
     ap.add_argument("--mix", choices=("mixed", "split", "f64"), default="mixed",
                     help="batch form: member precisions (mixed: even members FP64, odd split)")
@@ -631,6 +731,8 @@ def main() -> int:
         return bench_ale(args)
     if args.form == "remap":
         return bench_remap(args)
+    if args.form == "cycle":
+        return bench_cycle(args)
 
     if args.rch or args.wpb:
         from pyocean_amd import batch as _batch
