@@ -42,6 +42,21 @@ extern "C" {
 const char* ocx_last_error(void);
 int ocx_version(void);
 
+/* Device context (the survey's proposed ocx_create / ocx_alloc / ocx_h2d boundary; no counterpart in the reference,
+ * whose arrays live in host NumPy): a device and a stream on it (default priority), for a host that does not link
+ * HIP itself.  Pass ocx_ctx_stream(ctx) as the `stream` of the compute calls and pointers from ocx_alloc as their
+ * arrays.  ocx_h2d / ocx_d2h copy on the context's stream and return when the copy is complete; ocx_free waits
+ * for the stream first.  A context is used by one host thread at a time. */
+typedef struct ocx_ctx ocx_ctx;
+int ocx_create(int device, ocx_ctx** out);
+int ocx_destroy(ocx_ctx* ctx);
+void* ocx_ctx_stream(ocx_ctx* ctx);
+int ocx_alloc(ocx_ctx* ctx, size_t bytes, void** dptr);
+int ocx_free(ocx_ctx* ctx, void* dptr);
+int ocx_h2d(ocx_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ocx_d2h(ocx_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ocx_sync(ocx_ctx* ctx);
+
 /* Workspace (device bytes) of ocx_advect_fct_f64 / ocx_advect_fct_split / ocx_advect_rk3_f64 for one call of that
  * shape. */
 size_t ocx_fct_workspace_bytes(int nt, int nz, int ny, int nx);

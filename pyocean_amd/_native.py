@@ -26,6 +26,14 @@ P, I, U, D, Z, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_double
 SIGNATURES = {
     "ocx_last_error": (ctypes.c_char_p, []),
     "ocx_version": (I, []),
+    "ocx_create": (I, [I, P]),
+    "ocx_destroy": (I, [P]),
+    "ocx_ctx_stream": (P, [P]),
+    "ocx_alloc": (I, [P, Z, P]),
+    "ocx_free": (I, [P, P]),
+    "ocx_h2d": (I, [P, P, P, Z]),
+    "ocx_d2h": (I, [P, P, P, Z]),
+    "ocx_sync": (I, [P]),
     "ocx_fct_workspace_bytes": (Z, [I, I, I, I]),
     "ocx_fct_split_workspace_bytes": (Z, [I, I, I, I]),
     "ocx_rk3_workspace_bytes": (Z, [I, I, I, I]),

@@ -15,7 +15,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "lib" / "libocx.so"
-SOURCES = ["ocx_capi.hip", "ocx_ale.hip", "ocx_remap.hip"]
+SOURCES = ["ocx_capi.hip", "ocx_ale.hip", "ocx_remap.hip", "ocx_ctx.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
