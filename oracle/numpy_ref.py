@@ -742,25 +742,58 @@ def limit_outflow(h, Qx, Qy, area, dt, active, h_floor=H_PPM_MIN, max_pass=50, p
                        f"{max_pass} passes — barotropic CFL")
 
 
+def transport_thickness(h, H, active, clip=True, periodic=(False, False)):
+    """
+    The vertical coordinate's transport thicknesses ((q_e, q_w) at the u-faces, (q_n, q_s) at the v-faces in
+    their natural (n, ny+1, nx) layout): ``face_transport_thickness`` on both axes, on periodic axes padded by one
+    wrapped cell and cropped (``ALECoordinate.transport_thickness``, core/components/vertical_coordinate.py:125-140).
+    """
+    px, py = periodic
+    act = np.asarray(active, bool)
+    if px or py:
+        hp, Hp, ap = pad(h, "c", 1, px, py), pad(H, "c", 1, px, py), pad(act, "c", 1, px, py)
+        x = tuple(crop(q, 1, px, py) for q in face_transport_thickness(hp, Hp, ap, clip))
+        y = tuple(crop(np.ascontiguousarray(q.swapaxes(1, 2)), 1, px, py)
+                  for q in face_transport_thickness(hp.swapaxes(1, 2), Hp.T, ap.T, clip))
+    else:
+        x = face_transport_thickness(h, H, act, clip)
+        y = tuple(np.ascontiguousarray(q.swapaxes(1, 2))
+                  for q in face_transport_thickness(h.swapaxes(1, 2), H.T, act.T, clip))
+    return x, y
+
+
 def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_new, v_old, wet_u, wet_v, wu, wv,
-                        Du, Dv, dt, theta, theta_bc=None, clip=True, max_iter=20):
+                        Du, Dv, dt, theta, theta_bc=None, clip=True, max_iter=20, periodic=(False, False)):
     """
     One ALE layer-transport step with PPM thickness: transport velocities, transport thicknesses (the vertical
     coordinate's, clipped at the shallower bed for z*), PPM fluxes of both directions (the v-faces on swapped axes),
     the outflow limiter, the Lagrangian thickness and the layer volumes.  Follows ``ALELayerTransport.__call__``
     (core/components/layer_transport.py:111-156) without open boundaries and freshwater, with
     ``transport_thickness`` (core/components/vertical_coordinate.py:125-140) and ``layer_volumes``
-    (core/dynamics/sigma_baroclinic.py:1812-1816).  Returns dict(Qx, Qy, V0, V1, ok, h_dagger, n_limited).
+    (core/dynamics/sigma_baroclinic.py:1812-1816).  On periodic axes the PPM fluxes run on inputs padded by 4
+    wrapped cells and are cropped (layer_transport.py:122-129, core/model/periodic.py:71-103).  Returns
+    dict(Qx, Qy, V0, V1, ok, h_dagger, n_limited).
     """
     act = np.asarray(active, bool)
+    px, py = periodic
     utx, Qbx = transport_velocity(u_new, u_old, wet_u, wu, Lu, Du, theta, theta_bc)
     uty, Qby = transport_velocity(v_new, v_old, wet_v, wv, Lv, Dv, theta, theta_bc)
-    qe, qw = face_transport_thickness(h, H, act, clip)
-    qn, qs = face_transport_thickness(h.swapaxes(1, 2), H.T, act.T, clip)
-    Qx = ppm_transport_x(h, utx, Lu, Qbx, qe, qw, act, dt, dx_t, max_iter)
-    Qy = ppm_transport_x(h.swapaxes(1, 2), uty.swapaxes(1, 2), Lv.T, Qby.T, qn, qs, act.T, dt, dy_t.T,
-                         max_iter).swapaxes(1, 2)
-    Qx, Qy, n_lim, _ = limit_outflow(h, Qx, Qy, area, dt, act)
+    (qe, qw), (qn_nat, qs_nat) = transport_thickness(h, H, act, clip, periodic)
+    qn, qs = qn_nat.swapaxes(1, 2), qs_nat.swapaxes(1, 2)
+    sw = lambda a: a.swapaxes(1, 2)                                  # noqa: E731
+    if px or py:
+        P = lambda a, k: pad(a, k, 4, px, py)                        # noqa: E731
+        hp, ap = P(h, "c"), P(act, "c")
+        Qx = crop(ppm_transport_x(hp, P(utx, "u"), P(Lu, "u"), P(Qbx, "u"), P(qe, "u"), P(qw, "u"), ap, dt,
+                                  P(dx_t, "c"), max_iter), 4, px, py)
+        Qy = crop(np.ascontiguousarray(sw(ppm_transport_x(sw(hp), sw(P(uty, "v")), P(Lv, "v").T, P(Qby, "v").T,
+                                                          sw(P(qn_nat, "v")), sw(P(qs_nat, "v")), ap.T, dt,
+                                                          P(dy_t, "c").T, max_iter))), 4, px, py)
+    else:
+        Qx = ppm_transport_x(h, utx, Lu, Qbx, qe, qw, act, dt, dx_t, max_iter)
+        Qy = ppm_transport_x(sw(h), sw(uty), Lv.T, Qby.T, qn, qs, act.T, dt, dy_t.T, max_iter).swapaxes(1, 2)
+    Qx, Qy, n_lim, _ = limit_outflow(h, np.ascontiguousarray(Qx), np.ascontiguousarray(Qy), area, dt, act,
+                                     periodic=(px, py))
     divh = Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:, :] - Qy[:, :-1, :]
     h_lag = lagrangian_thickness(h, divh, dt, area)
     V0, V1 = h * area[None], h_lag * area[None]

@@ -28,7 +28,7 @@ import torch
 
 from . import _native
 from ._native import check, lib
-from .ops import Workspace, _ptr, _stream
+from .ops import Workspace, _ptr, _stream, periodic_pad
 
 H_PPM_MIN = 2e-10        # positive-definite floor of the PPM thickness [m] (core/vertical/ale.py:298)
 _F64 = torch.float64
@@ -96,9 +96,18 @@ def transport_velocity(u_new, u_old, last_wet, w, L, D, theta: float, theta_bc: 
 
 
 # ------------------------------------------------------------------------------------------------ transport thickness
-def face_thickness_axis(h, H, active, clip: bool, axis: int):
+def face_thickness_axis(h, H, active, clip: bool, axis: int, periodic: tuple[bool, bool] = (False, False)):
     """(q_pos, q_neg) at the faces of ``axis`` (0: u-faces (n, ny, nx+1), 1: v-faces (n, ny+1, nx)) from the
-    natural-layout h (n, ny, nx), H, active (ny, nx)."""
+    natural-layout h (n, ny, nx), H, active (ny, nx).  On periodic axes the inputs are padded by one wrapped cell
+    and the faces cropped, as ``ALECoordinate.transport_thickness`` does (core/components/vertical_coordinate.py:
+    125-140)."""
+    px, py = periodic
+    if px or py:
+        act = _mask(active, tuple(h.shape[1:]), "active")
+        qp, qn = face_thickness_axis(periodic_pad(h, "c", 1, px, py), periodic_pad(H, "c", 1, px, py) if clip else None,
+                                     periodic_pad(act, "c", 1, px, py), clip, axis)
+        k = "u" if axis == 0 else "v"
+        return (periodic_pad(qp, k, 1, px, py, crop=True), periodic_pad(qn, k, 1, px, py, crop=True))
     n, ny, nx = h.shape
     hc = _f64(h, (n, ny, nx), "h")
     act = _mask(active, (ny, nx), "active")
@@ -278,7 +287,7 @@ class LayerTransport:
     """
 
     def __init__(self, *, Lu, Lv, area, active, dx_t, dy_t, dt: float, theta: float, max_iter: int = 20,
-                 h_floor: float = H_PPM_MIN, max_pass: int = 50):
+                 h_floor: float = H_PPM_MIN, max_pass: int = 50, periodic: tuple[bool, bool] = (False, False)):
         ny, nx = area.shape
         self.Lu, self.Lv = _f64(Lu, (ny, nx + 1), "Lu"), _f64(Lv, (ny + 1, nx), "Lv")
         self.area, self.dx_t, self.dy_t = (_f64(x, (ny, nx), nm) for x, nm in ((area, "area"), (dx_t, "dx_t"),
@@ -286,6 +295,7 @@ class LayerTransport:
         self.active = _mask(active, (ny, nx), "active")
         self.dt, self.theta, self.max_iter = float(dt), float(theta), int(max_iter)
         self.h_floor, self.max_pass = float(h_floor), int(max_pass)
+        self.periodic = (bool(periodic[0]), bool(periodic[1]))
         self.ws_x, self.ws_y = Workspace(), Workspace()
         self.iterations = (0, 0)
 
@@ -296,14 +306,31 @@ class LayerTransport:
         qe, qw (u-faces) and qn, qs (v-faces, natural (n, ny+1, nx) layout) transport thicknesses."""
         utx, Qbx = transport_velocity(u_new, u_old, wet_u, wu, self.Lu, Du, self.theta, theta_bc)
         uty, Qby = transport_velocity(v_new, v_old, wet_v, wv, self.Lv, Dv, self.theta, theta_bc)
-        Qx = ppm_transport(h, utx, self.Lu, Qbx, qe, qw, self.active, self.dt, self.dx_t, 0, self.max_iter,
-                           workspace=self.ws_x, return_iterations=count_iterations)
-        Qy = ppm_transport(h, uty, self.Lv, Qby, qn, qs, self.active, self.dt, self.dy_t, 1, self.max_iter,
-                           workspace=self.ws_y, return_iterations=count_iterations)
+        px, py = self.periodic
+        if px or py:
+            # the PPM stencils on inputs padded by 4 wrapped cells, the fluxes cropped (layer_transport.py:122-129)
+            def P(a, kind):
+                return periodic_pad(a, kind, 4, px, py)
+            hp, ap = P(h, "c"), P(self.active, "c")
+            Qx = ppm_transport(hp, P(utx, "u"), P(self.Lu, "u"), P(Qbx, "u"), P(qe, "u"), P(qw, "u"), ap, self.dt,
+                               P(self.dx_t, "c"), 0, self.max_iter, workspace=self.ws_x,
+                               return_iterations=count_iterations)
+            Qy = ppm_transport(hp, P(uty, "v"), P(self.Lv, "v"), P(Qby, "v"), P(qn, "v"), P(qs, "v"), ap, self.dt,
+                               P(self.dy_t, "c"), 1, self.max_iter, workspace=self.ws_y,
+                               return_iterations=count_iterations)
+        else:
+            Qx = ppm_transport(h, utx, self.Lu, Qbx, qe, qw, self.active, self.dt, self.dx_t, 0, self.max_iter,
+                               workspace=self.ws_x, return_iterations=count_iterations)
+            Qy = ppm_transport(h, uty, self.Lv, Qby, qn, qs, self.active, self.dt, self.dy_t, 1, self.max_iter,
+                               workspace=self.ws_y, return_iterations=count_iterations)
         if count_iterations:
             (Qx, ix), (Qy, iy) = Qx, Qy
             self.iterations = (ix, iy)
-        Qx, Qy, n_lim = limit_outflow(h, Qx, Qy, self.area, self.dt, self.active, self.h_floor, self.max_pass)
+        if px or py:
+            Qx = periodic_pad(Qx, "u", 4, px, py, crop=True)
+            Qy = periodic_pad(Qy, "v", 4, px, py, crop=True)
+        Qx, Qy, n_lim = limit_outflow(h, Qx, Qy, self.area, self.dt, self.active, self.h_floor, self.max_pass,
+                                      periodic=self.periodic)
         n, ny, nx = h.shape
         h = _f64(h, (n,) + tuple(self.area.shape), "h")
         h_lag = torch.empty_like(h)

@@ -278,3 +278,25 @@ def test_layer_transport_at_full_size_keeps_closure_and_bounds(gpu):
     assert out.h_dagger.min().item() >= 0.0
     assert torch.equal(out.V0, d["h"] * d["area"][None]) and torch.equal(out.V1, out.h_dagger * d["area"][None])
     assert torch.equal(out.ok.bool(), act.bool()[None] & (out.V1 > 0))
+
+
+@pytest.mark.parametrize("periodic", [(True, False), (False, True), (True, True)])
+def test_periodic_layer_transport_matches_the_oracle_bit_for_bit(gpu, periodic):
+    _say("layer transport periodic", periodic)
+    a = synth.ale_inputs(64, 48, 10, seed=1)
+    r = R.ale_layer_transport(a.h, a.H, a.active, a.area, a.dx_t, a.dy_t, a.Lu, a.Lv, a.u_new, a.u_old, a.v_new,
+                              a.v_old, a.wet_u, a.wet_v, a.wu, a.wv, a.Du, a.Dv, a.dt, a.theta, a.theta_bc,
+                              periodic=periodic)
+    d = {k: _d(v) for k, v in vars(a).items() if isinstance(v, np.ndarray)}
+    qe, qw = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 0, periodic=periodic)
+    qn, qs = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 1, periodic=periodic)
+    (re_, rw), (rn, rs) = R.transport_thickness(a.h, a.H, a.active, True, periodic)
+    _eq(qe, re_, "q_e")
+    _eq(qs, rs, "q_s")
+    lt = ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
+                            dy_t=d["dy_t"], dt=a.dt, theta=a.theta, periodic=periodic)
+    out = lt(d["h"], d["u_new"], d["u_old"], d["v_new"], d["v_old"], d["wet_u"], d["wet_v"], d["wu"], d["wv"],
+             d["Du"], d["Dv"], qe, qw, qn, qs, theta_bc=a.theta_bc)
+    assert out.n_limited == r["n_limited"]
+    for k in ("Qx", "Qy", "V1", "h_dagger"):
+        _eq(getattr(out, k), r[k], k)

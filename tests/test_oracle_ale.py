@@ -364,3 +364,38 @@ def test_limit_outflow_on_periodic_axes_keeps_the_wrap_face_one_face(periodic):
     np.testing.assert_allclose(Qx2.sum(0), Qx.sum(0), rtol=1e-12, atol=1e-9)
     room0 = np.maximum(h - R.H_PPM_MIN, 0.0) * np.where(act, area, 0.0)[None] / dt
     assert not (act[None] & (R.outflow(Qx2, Qy2) > room0 * (1 + 1e-12))).any()
+
+
+@pytest.mark.parametrize("periodic", [(True, False), (True, True)])
+def test_periodic_layer_transport_is_shift_equivariant_and_keeps_the_wrap_face_one_face(periodic):
+    # on periodic axes the domain has no edge: rolling every input by s cells rolls the fluxes by s (to the
+    # Newton tolerance: the global stop sees the padded copies), and the two copies of the wrap face are equal
+    from pyocean_amd import synth
+    a = synth.ale_inputs(40, 32, 6, seed=2)
+    px, py = periodic
+
+    def run(shift):
+        def rc(x, face_axis=None):                                    # roll cell (or face) arrays along x and y
+            out = x
+            if px:
+                out = np.roll(out, shift, axis=-1) if face_axis != "u" else \
+                    np.concatenate([np.roll(out[..., :-1], shift, axis=-1),
+                                    np.roll(out[..., :-1], shift, axis=-1)[..., :1]], axis=-1)
+            if py:
+                out = np.roll(out, shift, axis=-2) if face_axis != "v" else \
+                    np.concatenate([np.roll(out[..., :-1, :], shift, axis=-2),
+                                    np.roll(out[..., :-1, :], shift, axis=-2)[..., :1, :]], axis=-2)
+            return out
+        return R.ale_layer_transport(
+            rc(a.h), rc(a.H), rc(a.active), rc(a.area), rc(a.dx_t), rc(a.dy_t), rc(a.Lu, "u"), rc(a.Lv, "v"),
+            rc(a.u_new, "u"), rc(a.u_old, "u"), rc(a.v_new, "v"), rc(a.v_old, "v"), rc(a.wet_u, "u"),
+            rc(a.wet_v, "v"), rc(a.wu, "u"), rc(a.wv, "v"), rc(a.Du, "u"), rc(a.Dv, "v"), a.dt, a.theta, a.theta_bc,
+            periodic=periodic), rc
+    r0, _ = run(0)
+    r1, rc = run(5)
+    assert np.array_equal(r0["Qx"][..., 0], r0["Qx"][..., -1])
+    if py:
+        assert np.array_equal(r0["Qy"][:, 0], r0["Qy"][:, -1])
+    scale = np.abs(r0["Qx"]).max()
+    np.testing.assert_allclose(r1["Qx"], rc(r0["Qx"], "u"), rtol=0, atol=1e-9 * scale)
+    np.testing.assert_allclose(r1["h_dagger"], rc(r0["h_dagger"]), rtol=0, atol=1e-9)
