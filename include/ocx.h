@@ -291,6 +291,20 @@ int ocx_regrid_f64(const double* H, const double* eta, const uint8_t* active, co
                    long long columns, int rule, const double* dz, double h_min, double* h_new, uint8_t* wet,
                    void* workspace, size_t workspace_bytes, long long* bad_column, void* stream);
 
+/* The hybrid (HYCOM1) rule: interfaces on the target densities rho_target (n+1, surface first) of the layer
+ * densities rho (n, columns) by P1M_H2 interpolation, isopycnal layers kept in place (keep), the z* floor dz_top
+ * (surface first), the maximum layer thicknesses max_h_top, the shallow sigma window between d_sigma and d_hybrid
+ * (d_hybrid 0: none; dsigma bed first) and the minimum thickness h_min.  wet, h_new and the bad-column report as
+ * ocx_regrid_f64; the workspace holds the columns' working arrays ((6 n + 3) doubles per column).
+ * Replaces regrid_columns('hybrid') / hycom1_column / _p1m_h2_positions / _keep_consistent / window_shallow_sigma
+ * (core/vertical/regrid.py:98-357) under ALECoordinate.regrid (vertical_coordinate.py:94-109). */
+size_t ocx_regrid_hybrid_workspace_bytes(int n, long long columns);
+int ocx_regrid_hybrid_f64(const double* H, const double* eta, const uint8_t* active, const double* h, const double* rho,
+                          int n, long long columns, const double* dz_top, const double* dsigma,
+                          const double* rho_target, const double* max_h_top, double d_sigma, double d_hybrid,
+                          double h_min, int keep, double* h_new, uint8_t* wet, void* workspace,
+                          size_t workspace_bytes, long long* bad_column, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,9 @@ def lib():
         L.oc_zstar_column.argtypes = [D, D, P, I, D, P]
         L.oc_sigma_column.argtypes = [D, D, P, I, D, P]
         L.oc_regrid.argtypes = [P, P, P, P, I, LL, I, P, D, P, P]
+        L.oc_hycom1_column.argtypes = [D, D, P, P, I, P, P, P, D, I, P]
+        L.oc_window_shallow_sigma.argtypes = [P, P, P, D, D, D, D, D, I, P]
+        L.oc_regrid_hybrid.argtypes = [P, P, P, P, P, I, LL, P, P, P, P, D, D, D, I, P, P]
         _lib = L
     return _lib
 
@@ -186,4 +189,41 @@ def regrid(H, eta, active, h, rule, dz, h_min=1e-3):
     wet = np.empty(h.shape[1:], np.uint8)
     lib().oc_regrid(_p(_f64(H)), _p(_f64(eta)), _p(_u8(active, h.shape[1:])), _p(h), n, M,
                     {"zstar": 0, "sigma": 1}[rule], _p(_f64(dz)), float(h_min), _p(h_new), _p(wet))
+    return h_new, wet.astype(bool)
+
+
+def hycom1_column(H, eta, h_bed, rho_bed, rho_target, dz_top, max_h_top=None, h_min=1e-3, keep=False):
+    """Hybrid interfaces of one column, bed first (``hycom1_column``, core/vertical/regrid.py:206-259)."""
+    h, r = _f64(h_bed), _f64(rho_bed)
+    n = h.size
+    mh = _f64(np.full(n, np.inf) if max_h_top is None else max_h_top)
+    zw = np.empty(n + 1)
+    lib().oc_hycom1_column(float(H), float(eta), _p(h), _p(r), n, _p(_f64(rho_target)), _p(_f64(dz_top)), _p(mh),
+                           float(h_min), int(bool(keep)), _p(zw))
+    return zw
+
+
+def window_shallow_sigma(zw_hyb, zw_sig, zw_zs, H, D, d_sigma, d_hybrid, h_min=1e-3):
+    """``window_shallow_sigma`` (core/vertical/regrid.py:271-303)."""
+    a, b, c = _f64(zw_hyb), _f64(zw_sig), _f64(zw_zs)
+    out = np.empty_like(a)
+    lib().oc_window_shallow_sigma(_p(a), _p(b), _p(c), float(H), float(D), float(d_sigma), float(d_hybrid),
+                                  float(h_min), a.size - 1, _p(out))
+    return out
+
+
+def regrid_hybrid(H, eta, active, h, rho, params):
+    """``ALECoordinate.regrid`` with the hybrid rule: (h_new, wet) (regrid.py:325-357, vertical_coordinate.py:94-109)."""
+    h = _f64(h)
+    n = h.shape[0]
+    M = int(np.prod(h.shape[1:]))
+    h_new = np.empty_like(h)
+    wet = np.empty(h.shape[1:], np.uint8)
+    f = lambda k, d: _f64(params.get(k, d))                                      # noqa: E731
+    lib().oc_regrid_hybrid(_p(_f64(H)), _p(_f64(eta)), _p(_u8(active, h.shape[1:])), _p(h), _p(_f64(rho)), n, M,
+                           _p(f("dz_top", np.ones(n))), _p(f("dsigma", np.full(n, 1.0 / n))),
+                           _p(f("rho_target", np.zeros(n + 1))), _p(f("max_h_top", np.full(n, np.inf))),
+                           float(params.get("d_sigma", 0.0)), float(params.get("d_hybrid", 0.0)),
+                           float(params.get("h_min", 1e-3)), int(bool(params.get("keep_isopycnal", True))),
+                           _p(h_new), _p(wet))
     return h_new, wet.astype(bool)

@@ -416,3 +416,215 @@ void oc_regrid(const double* H, const double* eta, const uint8_t* active, const 
         for (int k = 0; k < n; ++k) h_new[(size_t)k * M + c] = zw[k + 1] - zw[k];
     }
 }
+
+/* ------------------------------------------------------------------------------------------------ hybrid rule */
+/* P1M_H2 positions (depths from the surface, 0 .. D) of the target interface densities (surface first, m+1) in the
+ * source column h_top, rho_top (surface first, n layers): _p1m_h2_positions (core/vertical/regrid.py:98-151) */
+static void p1m_h2_positions(const double* h_top, const double* rho_top, int n, const double* targets, int m,
+                             double* x_new) {
+    double* e = malloc(sizeof(double) * 2 * (size_t)n);
+    double* xg = malloc(sizeof(double) * (size_t)(n + 1));
+    e[0] = rho_top[0];
+    e[2 * (n - 1) + 1] = rho_top[n - 1];
+    for (int k = 1; k < n; ++k) {
+        if (h_top[k - 1] + h_top[k] == 0.0)
+            e[2 * k] = 0.5 * (rho_top[k - 1] + rho_top[k]);
+        else
+            e[2 * k] = (rho_top[k - 1] * h_top[k] + rho_top[k] * h_top[k - 1]) / (h_top[k - 1] + h_top[k]);
+        e[2 * (k - 1) + 1] = e[2 * k];
+    }
+    bound_edges(h_top, rho_top, n, e);
+    for (int k = 0; k < n - 1; ++k) {
+        if (e[2 * k + 1] != e[2 * (k + 1)]) {
+            const double a = 0.5 * (e[2 * k + 1] + e[2 * (k + 1)]);
+            e[2 * k + 1] = a;
+            e[2 * (k + 1)] = a;
+        }
+    }
+    xg[0] = 0.0;
+    for (int k = 0; k < n; ++k) xg[k + 1] = xg[k] + h_top[k];
+    for (int k = 0; k <= m; ++k) x_new[k] = m == n ? xg[k] : xg[n] * k / m;
+    for (int kk = 1; kk < m; ++kk) {
+        const double t = targets[kk];
+        if (t <= e[0]) {
+            x_new[kk] = xg[0];
+            continue;
+        }
+        int found = 0;
+        for (int k = 1; k < n; ++k) {
+            if (t >= e[2 * (k - 1) + 1] && t <= e[2 * k]) {
+                x_new[kk] = xg[k];
+                found = 1;
+                break;
+            }
+        }
+        if (found) continue;
+        if (t >= e[2 * (n - 1) + 1]) {
+            x_new[kk] = xg[n];
+            continue;
+        }
+        for (int k = 0; k < n; ++k) {
+            if (e[2 * k] < t && t < e[2 * k + 1]) {
+                const double xi = (t - e[2 * k]) / (e[2 * k + 1] - e[2 * k]);
+                x_new[kk] = xg[k] + pmin(pmax(xi, 0.0), 1.0) * h_top[k];
+                break;
+            }
+        }
+    }
+    free(e);
+    free(xg);
+}
+
+/* isopycnal layer preservation: _keep_consistent (core/vertical/regrid.py:157-203) */
+static void keep_consistent(const double* h_top, const double* rho_top, const double* targets, int n, double h_mass,
+                            double* x) {
+    uint8_t* ok = malloc((size_t)n);
+    uint8_t* kept = calloc((size_t)(n + 1), 1);
+    double* xg = malloc(sizeof(double) * (size_t)(n + 1));
+    for (int j = 0; j < n; ++j)
+        ok[j] = h_top[j] < h_mass || (targets[j] <= rho_top[j] && rho_top[j] <= targets[j + 1]);
+    xg[0] = 0.0;
+    for (int j = 0; j < n; ++j) xg[j + 1] = xg[j] + h_top[j];
+    kept[0] = 1;
+    kept[n] = 1;
+    for (int j = 1; j < n; ++j) {
+        const int straddle = h_top[j - 1] >= h_mass && h_top[j] >= h_mass && rho_top[j - 1] <= targets[j] &&
+                             targets[j] <= rho_top[j];
+        if ((ok[j - 1] && ok[j]) || straddle) {
+            x[j] = xg[j];
+            kept[j] = 1;
+        }
+    }
+    double lo = xg[0];
+    for (int j = 1; j < n; ++j) {
+        if (kept[j])
+            lo = x[j];
+        else if (x[j] < lo)
+            x[j] = lo;
+    }
+    double hi = xg[n];
+    for (int j = n - 1; j > 0; --j) {
+        if (kept[j])
+            hi = x[j];
+        else if (x[j] > hi)
+            x[j] = hi;
+    }
+    free(ok);
+    free(kept);
+    free(xg);
+}
+
+/* hybrid z*-isopycnal interfaces (bed first) of one column with n source layers onto n target layers:
+ * hycom1_column (core/vertical/regrid.py:206-259) */
+void oc_hycom1_column(double H, double eta, const double* h_bed, const double* rho_bed, int n,
+                      const double* rho_target, const double* dz_top, const double* max_h_top, double h_min, int keep,
+                      double* zw) {
+    const int m = n;
+    const double D = H + eta;
+    double* h_top = malloc(sizeof(double) * (size_t)n);
+    double* rho_own = malloc(sizeof(double) * (size_t)n);
+    double* rho = malloc(sizeof(double) * (size_t)n);
+    double* x = malloc(sizeof(double) * (size_t)(m + 1));
+    for (int k = 0; k < n; ++k) {
+        h_top[k] = h_bed[n - 1 - k];
+        rho_own[k] = rho_bed[n - 1 - k];
+        rho[k] = rho_own[k];
+    }
+    const double h_mass = 2.0 * h_min;
+    double last = -1.0;
+    for (int k = 0; k < n; ++k) {
+        if (h_top[k] >= h_mass)
+            last = rho[k];
+        else if (last >= 0.0)
+            rho[k] = last;
+    }
+    int k = 0;
+    for (k = 0; k < n; ++k)
+        if (h_top[k] >= h_mass) break;
+    if (k == n) k = n - 1;                           /* a Python for loop leaves k at its last value */
+    if (k > 0 && h_top[k] >= h_mass)
+        for (int kk = 0; kk < k; ++kk) rho[kk] = rho[k];
+    for (int kk = n - 2; kk >= 0; --kk) rho[kk] = pmin(rho[kk], rho[kk + 1]);
+    p1m_h2_positions(h_top, rho, n, rho_target, m, x);
+    if (keep) keep_consistent(h_top, rho_own, rho_target, n, 2.0 * h_min, x);
+    const double s = D / H;
+    double nominal = 0.0;
+    for (int kk = 1; kk <= m; ++kk) {
+        nominal += dz_top[kk - 1] * s;
+        x[kk] = pmin(pmax(x[kk], nominal), D);
+    }
+    for (int kk = 1; kk < m; ++kk) x[kk] = pmin(x[kk], x[kk - 1] + max_h_top[kk - 1]);
+    for (int kk = 1; kk <= m; ++kk) x[kk] = pmax(x[kk], x[kk - 1]);
+    x[m] = D;
+    for (int j = 0; j <= m; ++j) zw[j] = eta - x[m - j];
+    zw[0] = -H;
+    zw[m] = eta;
+    enforce_min_thickness(zw, m, h_min);
+    free(h_top);
+    free(rho_own);
+    free(rho);
+    free(x);
+}
+
+static double transition_weight(double D, double d_sigma, double d_hybrid) {
+    if (d_hybrid <= d_sigma) return D > d_sigma ? 1.0 : 0.0;
+    return pmin(pmax((D - d_sigma) / (d_hybrid - d_sigma), 0.0), 1.0);
+}
+
+void oc_window_shallow_sigma(const double* zw_hyb, const double* zw_sig, const double* zw_zs, double H, double D,
+                                 double d_sigma, double d_hybrid, double h_min, int n, double* zw) {
+    const double q = transition_weight(D, d_sigma, d_hybrid);
+    if (q >= 1.0) {
+        for (int j = 0; j <= n; ++j) zw[j] = zw_hyb[j];
+        return;
+    }
+    for (int j = 0; j <= n; ++j) {
+        const double lo = q * (-H) + (1.0 - q) * zw_sig[j];
+        const double hi = q * zw_zs[j] + (1.0 - q) * zw_sig[j];
+        zw[j] = pmin(pmax(zw_hyb[j], lo), hi);
+    }
+    zw[0] = -H;
+    zw[n] = zw_sig[n];
+    enforce_min_thickness(zw, n, h_min);
+}
+
+/* ALECoordinate.regrid for the hybrid rule (_regrid_columns code 2, core/vertical/regrid.py:325-357): rho (n, M)
+ * the layer densities; dz_top (z* floor, surface first), dsigma (bed first), rho_target (n+1, surface first),
+ * max_h_top (n); d_sigma, d_hybrid the shallow window (d_hybrid 0: none); keep the isopycnal preservation. */
+void oc_regrid_hybrid(const double* H, const double* eta, const uint8_t* active, const double* h, const double* rho,
+                      int n, i64 M, const double* dz_top, const double* dsigma, const double* rho_target,
+                      const double* max_h_top, double d_sigma, double d_hybrid, double h_min, int keep, double* h_new,
+                      uint8_t* wet_out) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (i64 c = 0; c < M; ++c) {
+        const double Hc = H[c] == H[c] ? (H[c] == INFINITY ? DBL_MAX : (H[c] == -INFINITY ? -DBL_MAX : H[c])) : 0.0;
+        double tot = 0.0;
+        for (int k = 0; k < n; ++k) tot += h[(size_t)k * M + c];
+        const int wet = active[c] && (Hc + eta[c] > 0.0) && (tot > 0.0);
+        if (wet_out) wet_out[c] = (uint8_t)wet;
+        if (!wet) {
+            for (int k = 0; k < n; ++k) h_new[(size_t)k * M + c] = h[(size_t)k * M + c];
+            continue;
+        }
+        double zw[n + 1], hb[n], rb[n];
+        double q = d_hybrid > 0.0 ? transition_weight(Hc + eta[c], d_sigma, d_hybrid) : 1.0;
+        if (d_hybrid > 0.0 && Hc <= 0.0) q = 0.0;
+        if (q <= 0.0) {
+            oc_sigma_column(Hc, eta[c], dsigma, n, h_min, zw);
+        } else {
+            for (int k = 0; k < n; ++k) {
+                hb[k] = h[(size_t)k * M + c];
+                rb[k] = rho[(size_t)k * M + c];
+            }
+            oc_hycom1_column(Hc, eta[c], hb, rb, n, rho_target, dz_top, max_h_top, h_min, keep, zw);
+            if (q < 1.0) {
+                double sig[n + 1], zs[n + 1], hyb[n + 1];
+                oc_sigma_column(Hc, eta[c], dsigma, n, h_min, sig);
+                oc_zstar_column(Hc, eta[c], dz_top, n, h_min, zs);
+                for (int j = 0; j <= n; ++j) hyb[j] = zw[j];
+                oc_window_shallow_sigma(hyb, sig, zs, Hc, Hc + eta[c], d_sigma, d_hybrid, h_min, n, zw);
+            }
+        }
+        for (int k = 0; k < n; ++k) h_new[(size_t)k * M + c] = zw[k + 1] - zw[k];
+    }
+}

@@ -64,6 +64,8 @@ SIGNATURES = {
     "ocx_remap_columns_f64": (I, [P, P, P, P, I, I, L, I, I, P, P]),
     "ocx_regrid_workspace_bytes": (Z, []),
     "ocx_regrid_f64": (I, [P, P, P, P, I, L, I, P, D, P, P, P, Z, P, P]),
+    "ocx_regrid_hybrid_workspace_bytes": (Z, [I, L]),
+    "ocx_regrid_hybrid_f64": (I, [P, P, P, P, P, I, L, P, P, P, P, D, D, D, I, P, P, P, Z, P, P]),
     "ocx_batch_create": (I, [P, I, I, I, I, P]),
     "ocx_batch_step": (I, [P, I, P]),
     "ocx_batch_graph_step": (I, [P, I, P]),

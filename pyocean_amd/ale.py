@@ -339,7 +339,8 @@ class LayerTransport:
         Lb = lib()
         ws = _WS_LAG.get(int(Lb.ocx_lagrangian_workspace_bytes()))
         bad = ctypes.c_longlong(-1)
-        check(Lb.ocx_lagrangian_thickness_f64(_ptr(h), None, _ptr(Qx.contiguous()), _ptr(Qy.contiguous()),
+        Qx, Qy = Qx.contiguous(), Qy.contiguous()
+        check(Lb.ocx_lagrangian_thickness_f64(_ptr(h), None, _ptr(Qx), _ptr(Qy),
                                               _ptr(self.area), _ptr(self.active), n, ny, nx, self.dt, 1e-9,
                                               _ptr(h_lag), _ptr(V0), _ptr(V1), _ptr(ok), _ptr(ws), ws.numel(),
                                               ctypes.byref(bad), _stream()), "lagrangian_thickness")

@@ -83,4 +83,44 @@ int ocx_regrid_f64(const double* H, const double* eta, const uint8_t* active, co
                 rule == 0 ? "z* needs finite H > 0 and D = H + eta > 0" : "sigma needs finite H and D = H + eta > 0");
 }
 
+size_t ocx_regrid_hybrid_workspace_bytes(int n, long long columns) {
+    if (n < 1 || columns < 1) return 0;
+    return align256((size_t)(6 * (long long)n + 3) * (size_t)columns * sizeof(double)) + 256;
+}
+
+int ocx_regrid_hybrid_f64(const double* H, const double* eta, const uint8_t* active, const double* h, const double* rho,
+                          int n, long long columns, const double* dz_top, const double* dsigma,
+                          const double* rho_target, const double* max_h_top, double d_sigma, double d_hybrid,
+                          double h_min, int keep, double* h_new, uint8_t* wet, void* workspace,
+                          size_t workspace_bytes, long long* bad_column, void* stream) {
+    if (n < 1 || columns < 1) return fail(OCX_E_ARG, "shape (n=%d, columns=%lld) must be positive", n, columns);
+    const void* p[] = {H, eta, active, h, rho, dz_top, dsigma, rho_target, max_h_top, h_new};
+    const char* nm[] = {"H", "eta", "active", "h", "rho", "dz_top", "dsigma", "rho_target", "max_h_top", "h_new"};
+    for (int q = 0; q < 10; ++q) OCX_TRY(need(p[q], nm[q]));
+    if (h_new == h) return fail(OCX_E_ARG, "h_new must not alias h");
+    const size_t wsb = ocx_regrid_hybrid_workspace_bytes(n, columns);
+    if (!workspace || workspace_bytes < wsb)
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed, %zu given", wsb, workspace ? workspace_bytes : 0);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* base = static_cast<char*>(workspace);
+    unsigned long long* bad = reinterpret_cast<unsigned long long*>(base);
+    HybridArgs r{H, eta, h, rho, dz_top, dsigma, rho_target, max_h_top, active, d_sigma, d_hybrid, h_min,
+                 keep != 0, n, columns, h_new, wet, reinterpret_cast<double*>(base + 256), bad};
+    k_set_bad<<<1, 64, 0, s>>>(bad);
+    OCX_TRY(launch_check("regrid reset"));
+    {
+        ProfScope _ps("regrid_hybrid", s);
+        k_regrid_hybrid<<<(unsigned)((columns + 127) / 128), 128, 0, s>>>(r);
+    }
+    OCX_TRY(launch_check("regrid_hybrid"));
+    if (!bad_column) return OCX_OK;
+    unsigned long long b = 0;
+    OCX_TRY(hip_check(hipMemcpyAsync(&b, bad, sizeof b, hipMemcpyDeviceToHost, s), "regrid flag"));
+    OCX_TRY(hip_check(hipStreamSynchronize(s), "regrid sync"));
+    *bad_column = b == ~0ull ? -1 : (long long)b;
+    if (b == ~0ull) return OCX_OK;
+    return fail(OCX_E_RANGE, "wet column %llu: the hybrid rule needs finite densities, D = H + eta > 0, and H > 0 "
+                "unless it has a shallow sigma zone (d_hybrid > 0)", b);
+}
+
 }  // extern "C"

@@ -948,3 +948,261 @@ __global__ void k_regrid(const double* __restrict__ H, const double* __restrict_
 
 }  // namespace rmp
 }  // namespace ocx
+
+namespace ocx {
+namespace rmp {
+
+// ------------------------------------------------------------------------------------------ hybrid (HYCOM1) rule
+// hycom1_column / _p1m_h2_positions / _keep_consistent / window_shallow_sigma (core/vertical/regrid.py:98-303) per
+// column, one thread each; the column's working arrays live in a caller-provided scratch laid out level-major
+// (level k of column c at k * M + c), so the lanes of a wavefront stay coalesced.
+
+struct WCol {                     // writable strided column
+    double* p;
+    long long M;
+    __device__ __forceinline__ double& operator[](int k) const { return p[(size_t)k * M]; }
+};
+
+__device__ void enforce_min_thickness_w(const WCol& zw, int n, double h_min) {
+    const double hm = pmin(h_min, (zw[n] - zw[0]) / n);
+    for (int j = n - 1; j > 0; --j)
+        if (zw[j] > zw[j + 1] - hm) zw[j] = zw[j + 1] - hm;
+    for (int j = 1; j < n; ++j)
+        if (zw[j] < zw[j - 1] + hm) zw[j] = zw[j - 1] + hm;
+}
+
+__device__ void sigma_column_w(double H, double eta, const double* dsigma, int n, double h_min, const WCol& zw) {
+    const double D = H + eta;
+    zw[0] = -H;
+    for (int j = 0; j < n; ++j) zw[j + 1] = zw[j] + D * dsigma[j];
+    zw[n] = eta;
+    enforce_min_thickness_w(zw, n, h_min);
+}
+
+// z* interfaces bed first: zw[n - k] is the surface-first e[k]
+__device__ void zstar_column_w(double H, double eta, const double* dz_top, int n, double h_min, const WCol& zw) {
+    const double D = H + eta;
+    const double hm = pmin(h_min, D / n);
+    const double s = D / H;
+    zw[n] = eta;
+    for (int k = 0; k < n; ++k) zw[n - k - 1] = zw[n - k] - s * dz_top[k];
+    zw[0] = -H;
+    for (int j = 1; j <= n; ++j)
+        if (zw[j] < zw[j - 1] + hm) zw[j] = zw[j - 1] + hm;
+}
+
+__device__ __forceinline__ double transition_weight(double D, double d_sigma, double d_hybrid) {
+    if (d_hybrid <= d_sigma) return D > d_sigma ? 1.0 : 0.0;
+    return pmin(pmax((D - d_sigma) / (d_hybrid - d_sigma), 0.0), 1.0);
+}
+
+struct HybridArgs {
+    const double *H, *eta, *h, *rho, *dz_top, *dsigma, *rho_target, *max_h_top;
+    const uint8_t* active;
+    double d_sigma, d_hybrid, h_min;
+    int keep, n;
+    long long M;
+    double* h_new;
+    uint8_t* wet;
+    double* scratch;              // (6 n + 3) levels x M
+    unsigned long long* bad;
+};
+
+__global__ void k_regrid_hybrid(HybridArgs r) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= r.M) return;
+    const long long M = r.M;
+    const int n = r.n, m = n;
+    double Hc = r.H[c];
+    Hc = Hc != Hc ? 0.0 : (Hc == __builtin_inf() ? DBL_MAX : (Hc == -__builtin_inf() ? -DBL_MAX : Hc));
+    const double et = r.eta[c];
+    const Col hb{r.h + c, M}, rb{r.rho + c, M};
+    double tot = 0.0;
+    for (int k = 0; k < n; ++k) tot += hb[k];
+    const bool wet = r.active[c] && (Hc + et > 0.0) && (tot > 0.0);
+    if (r.wet) r.wet[c] = wet ? 1 : 0;
+    auto keep_h = [&]() {
+        for (int k = 0; k < n; ++k) r.h_new[(size_t)k * M + c] = hb[k];
+    };
+    if (!wet) {
+        keep_h();
+        return;
+    }
+    const double D = Hc + et;
+    double q = r.d_hybrid > 0.0 ? transition_weight(D, r.d_sigma, r.d_hybrid) : 1.0;
+    if (r.d_hybrid > 0.0 && Hc <= 0.0) q = 0.0;
+    bool rho_ok = true;                                       // every wet column's densities must be finite
+    for (int k = 0; k < n; ++k) rho_ok = rho_ok && fabs(rb[k]) <= DBL_MAX;
+    if (!(fabs(D) <= DBL_MAX) || (r.d_hybrid <= 0.0 && !(Hc > 0.0)) || !rho_ok) {
+        atomicMin(r.bad, (unsigned long long)c);
+        keep_h();
+        return;
+    }
+    double* base = r.scratch + c;
+    const WCol rho{base, M};                                  // n: the densities the rule follows (surface first)
+    const WCol e{base + (size_t)n * M, M};                    // 2n: P1M edges (left, right) per layer
+    const WCol zw{base + (size_t)3 * n * M, M};               // n + 1: interfaces (first the positions x)
+    const WCol sig{base + (size_t)(4 * n + 1) * M, M};        // n + 1
+    const WCol zs{base + (size_t)(5 * n + 2) * M, M};         // n + 1
+    if (q <= 0.0) {
+        sigma_column_w(Hc, et, r.dsigma, n, r.h_min, zw);
+    } else {
+        // layer densities, surface first: massless layers take the density of the nearest layer with mass above
+        // (below for a top run without mass), then the running minimum from the bed up
+        const double h_mass = 2.0 * r.h_min;
+        auto h_top = [&](int k) -> double { return hb[n - 1 - k]; };
+        auto rho_own = [&](int k) -> double { return rb[n - 1 - k]; };
+        double last = -1.0;
+        for (int k = 0; k < n; ++k) {
+            const double rk = rho_own(k);
+            rho[k] = rk;
+            if (h_top(k) >= h_mass)
+                last = rk;
+            else if (last >= 0.0)
+                rho[k] = last;
+        }
+        int k0 = 0;
+        for (k0 = 0; k0 < n; ++k0)
+            if (h_top(k0) >= h_mass) break;
+        if (k0 == n) k0 = n - 1;
+        if (k0 > 0 && h_top(k0) >= h_mass)
+            for (int kk = 0; kk < k0; ++kk) rho[kk] = rho[k0];
+        for (int kk = n - 2; kk >= 0; --kk) rho[kk] = pmin(rho[kk], rho[kk + 1]);
+        // P1M with H2 edges: edges from the thickness-weighted neighbours, bounded, averaged where discontinuous
+        e[0] = rho[0];
+        e[2 * (n - 1) + 1] = rho[n - 1];
+        for (int k = 1; k < n; ++k) {
+            const double ha = h_top(k - 1), hc = h_top(k);
+            double v;
+            if (ha + hc == 0.0)
+                v = 0.5 * (rho[k - 1] + rho[k]);
+            else
+                v = (rho[k - 1] * hc + rho[k] * ha) / (ha + hc);
+            e[2 * k] = v;
+            e[2 * (k - 1) + 1] = v;
+        }
+        for (int k = 0; k < n; ++k) {
+            const int km1 = k - 1 > 0 ? k - 1 : 0, kp1 = k + 1 < n - 1 ? k + 1 : n - 1;
+            const double s = bound_slope(h_top(km1), h_top(k), h_top(kp1), rho[km1], rho[k], rho[kp1]);
+            e[2 * k] = bound_edge(e[2 * k], s, rho[km1], rho[k], false);
+            e[2 * k + 1] = bound_edge(e[2 * k + 1], s, rho[kp1], rho[k], true);
+        }
+        for (int k = 0; k < n - 1; ++k) {
+            if (e[2 * k + 1] != e[2 * (k + 1)]) {
+                const double a = 0.5 * (e[2 * k + 1] + e[2 * (k + 1)]);
+                e[2 * k + 1] = a;
+                e[2 * (k + 1)] = a;
+            }
+        }
+        // positions x of the target densities, depths from the surface (x[k] in zw[k])
+        double xgn = 0.0;
+        for (int k = 0; k < n; ++k) xgn += h_top(k);
+        {
+            double xg = 0.0;
+            for (int k = 0; k <= m; ++k) {
+                zw[k] = xg;                                   // m == n: the input grid
+                if (k < n) xg += h_top(k);
+            }
+        }
+        for (int kk = 1; kk < m; ++kk) {
+            const double t = r.rho_target[kk];
+            if (t <= e[0]) {
+                zw[kk] = 0.0;
+                continue;
+            }
+            bool found = false;
+            double xg = h_top(0);
+            for (int k = 1; k < n; ++k) {
+                if (t >= e[2 * (k - 1) + 1] && t <= e[2 * k]) {
+                    zw[kk] = xg;
+                    found = true;
+                    break;
+                }
+                xg += h_top(k);
+            }
+            if (found) continue;
+            if (t >= e[2 * (n - 1) + 1]) {
+                zw[kk] = xgn;
+                continue;
+            }
+            xg = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const double el = e[2 * k], er = e[2 * k + 1];
+                if (el < t && t < er) {
+                    const double xi = (t - el) / (er - el);
+                    zw[kk] = xg + pmin(pmax(xi, 0.0), 1.0) * h_top(k);
+                    break;
+                }
+                xg += h_top(k);
+            }
+        }
+        if (r.keep) {
+            // isopycnal preservation: interfaces between consistent layers, or straddling their own target, stay
+            auto ok = [&](int j) -> bool {
+                const double rj = rho_own(j);
+                return h_top(j) < h_mass || (r.rho_target[j] <= rj && rj <= r.rho_target[j + 1]);
+            };
+            auto kept = [&](int j) -> bool {
+                if (j == 0 || j == n) return true;
+                const bool straddle = h_top(j - 1) >= h_mass && h_top(j) >= h_mass &&
+                                      rho_own(j - 1) <= r.rho_target[j] && r.rho_target[j] <= rho_own(j);
+                return (ok(j - 1) && ok(j)) || straddle;
+            };
+            double xg = 0.0;
+            for (int j = 1; j < n; ++j) {
+                xg += h_top(j - 1);
+                if (kept(j)) zw[j] = xg;
+            }
+            double lo = 0.0;
+            for (int j = 1; j < n; ++j) {
+                if (kept(j))
+                    lo = zw[j];
+                else if (zw[j] < lo)
+                    zw[j] = lo;
+            }
+            double hi = xgn;
+            for (int j = n - 1; j > 0; --j) {
+                if (kept(j))
+                    hi = zw[j];
+                else if (zw[j] > hi)
+                    zw[j] = hi;
+            }
+        }
+        // z* floor, maximum thickness, order, then the interfaces bed first
+        const double s = D / Hc;
+        double nominal = 0.0;
+        for (int kk = 1; kk <= m; ++kk) {
+            nominal += r.dz_top[kk - 1] * s;
+            zw[kk] = pmin(pmax(zw[kk], nominal), D);
+        }
+        for (int kk = 1; kk < m; ++kk) zw[kk] = pmin(zw[kk], zw[kk - 1] + r.max_h_top[kk - 1]);
+        for (int kk = 1; kk <= m; ++kk) zw[kk] = pmax(zw[kk], zw[kk - 1]);
+        zw[m] = D;
+        // zw[j] = eta - x[m - j]: reverse in place
+        for (int j = 0; j < (m + 1) / 2; ++j) {
+            const double a = zw[j], b = zw[m - j];
+            zw[j] = et - b;
+            zw[m - j] = et - a;
+        }
+        if ((m + 1) % 2 == 1) zw[m / 2] = et - zw[m / 2];
+        zw[0] = -Hc;
+        zw[m] = et;
+        enforce_min_thickness_w(zw, m, r.h_min);
+        if (q < 1.0) {
+            sigma_column_w(Hc, et, r.dsigma, n, r.h_min, sig);
+            zstar_column_w(Hc, et, r.dz_top, n, r.h_min, zs);
+            for (int j = 0; j <= n; ++j) {
+                const double lo = q * (-Hc) + (1.0 - q) * sig[j];
+                const double hi = q * zs[j] + (1.0 - q) * sig[j];
+                zw[j] = pmin(pmax(zw[j], lo), hi);
+            }
+            zw[0] = -Hc;
+            zw[n] = sig[n];
+            enforce_min_thickness_w(zw, n, r.h_min);
+        }
+    }
+    for (int k = 0; k < n; ++k) r.h_new[(size_t)k * M + c] = zw[k + 1] - zw[k];
+}
+
+}  // namespace rmp
+}  // namespace ocx

@@ -24,7 +24,13 @@ def require_gpu() -> torch.device:
 
 
 def _ptr(t):
-    return None if t is None else ctypes.c_void_p(t.data_ptr())
+    """The device address of ``t`` as a ctypes argument that keeps ``t`` alive while the call runs (a temporary
+    tensor in an argument list would otherwise be freed, and its memory reused, before the kernel reads it)."""
+    if t is None:
+        return None
+    p = ctypes.c_void_p(t.data_ptr())
+    p._tensor = t
+    return p
 
 
 def _stream():

@@ -4,8 +4,8 @@ Vertical regrid and conservative remap of the ALE coordinate on the device (HIP 
 Mirrors ``core/vertical/remap.py`` (``remap_columns``, schemes PCM / PLM / PPM_H4), the z* and sigma rules of
 ``core/vertical/regrid.py`` and ``ALECoordinate.regrid`` / ``regrid_remap``
 (core/components/vertical_coordinate.py:69-109), with the same argument meaning and errors; torch tensors in HBM
-replace NumPy arrays.  Bit-identical to the reference's FP64 arithmetic.  The hybrid (HYCOM1) rule is not offered
-and raises ``NotImplementedError``.
+replace NumPy arrays.  Bit-identical to the reference's FP64 arithmetic.  The hybrid (HYCOM1) rule takes the layer
+densities ``rho`` from the caller (the reference computes them from T and S with its equation of state).
 
 A field stack (nf, n, ny, nx) - the tracer stack C, say - is remapped in one launch without copies: the thickness
 reads are shared by the fields of a column.
@@ -24,6 +24,7 @@ SCHEMES = {"pcm": PCM, "plm": PLM, "ppm_h4": PPM_H4}
 H_MIN = 1e-3            # minimum layer thickness after regridding [m] (core/vertical/regrid.py:40)
 _F64 = torch.float64
 _WS = Workspace()
+_WS_HYB = Workspace()
 
 
 def _f64(t, shape, name):
@@ -68,40 +69,61 @@ def remap_columns(h_src, f_src, h_tgt, scheme: int, wet, out=None):
     return o
 
 
-def regrid(H, eta, active, h, rule: str, params: dict, check_columns: bool = True):
+def regrid(H, eta, active, h, rule: str, params: dict, check_columns: bool = True, rho=None):
     """
     New layer thicknesses of the rule at surface eta from layers h (n, ny, nx): (h_new, wet) with
     wet = active & H + eta > 0 & Σ h > 0; h kept where dry.  rule 'zstar' (params['dz_top'], nominal thicknesses
-    surface first) or 'sigma' (params['dsigma'], fractions bed first); params['h_min'] (default 1e-3 m).
+    surface first), 'sigma' (params['dsigma'], fractions bed first) or 'hybrid' (params['dz_top'], ['rho_target']
+    (n+1, surface first), optional ['max_h_top'], ['d_sigma'], ['d_hybrid'] with ['dsigma'], ['keep_isopycnal']
+    (default True); ``rho`` (n, ny, nx) the layer densities); params['h_min'] (default 1e-3 m).
     ``check_columns`` raises ValueError, as the reference does, for a wet column the rule cannot build (a host
-    synchronisation).  core/components/vertical_coordinate.py:94-109, core/vertical/regrid.py:41-95, 360-405.
+    synchronisation).  core/components/vertical_coordinate.py:94-109, core/vertical/regrid.py:41-405.
     """
-    if rule == "hybrid":
-        raise NotImplementedError("the hybrid (HYCOM1) rule is not offered on the device")
-    if rule not in ("zstar", "sigma"):
+    if rule not in ("zstar", "sigma", "hybrid"):
         raise ValueError(f"unknown rule {rule!r}; use 'zstar', 'sigma' or 'hybrid'")
-    key = "dz_top" if rule == "zstar" else "dsigma"
-    if key not in params:
-        raise ValueError(f"regrid_columns({rule!r}) needs params[{key!r}]")
+    required = {"zstar": ("dz_top",), "sigma": ("dsigma",), "hybrid": ("dz_top", "rho_target")}[rule]
+    if rule == "hybrid" and float(params.get("d_hybrid", 0.0)) > 0.0:
+        required += ("dsigma",)
+    for key in required:
+        if key not in params:
+            raise ValueError(f"regrid_columns({rule!r}) needs params[{key!r}]")
+    if rule == "hybrid" and rho is None:
+        raise ValueError("regrid_columns('hybrid') needs the layer densities rho")
     n, ny, nx = h.shape
     hc = _f64(h, (n, ny, nx), "h")
     Hc, ec = _f64(H, (ny, nx), "H"), _f64(eta, (ny, nx), "eta")
     act = _mask(active, (ny, nx), "active")
-    dz = torch.as_tensor(params[key], dtype=_F64, device=hc.device).contiguous()
-    if dz.shape != (n,):
-        raise ValueError(f"params[{key!r}]: {n} values expected, got {tuple(dz.shape)}")
+    dev = hc.device
+
+    def vec(key, default, size):
+        v = torch.as_tensor(params.get(key, default), dtype=_F64, device=dev).contiguous()
+        if v.shape != (size,):
+            raise ValueError(f"params[{key!r}]: {size} values expected, got {tuple(v.shape)}")
+        return v
     h_new = torch.empty_like(hc)
-    wet = torch.empty((ny, nx), dtype=torch.uint8, device=hc.device)
+    wet = torch.empty((ny, nx), dtype=torch.uint8, device=dev)
     L = lib()
-    ws = _WS.get(int(L.ocx_regrid_workspace_bytes()))
     bad = ctypes.c_longlong(-1)
-    rc = L.ocx_regrid_f64(_ptr(Hc), _ptr(ec), _ptr(act), _ptr(hc), n, ny * nx, 0 if rule == "zstar" else 1, _ptr(dz),
-                          float(params.get("h_min", H_MIN)), _ptr(h_new), _ptr(wet), _ptr(ws), ws.numel(),
-                          ctypes.byref(bad) if check_columns else None, _stream())
+    badp = ctypes.byref(bad) if check_columns else None
+    h_min = float(params.get("h_min", H_MIN))
+    if rule == "hybrid":
+        rc_ = _f64(rho, (n, ny, nx), "rho")
+        dz_top, dsigma = vec("dz_top", None, n), vec("dsigma", [1.0 / n] * n, n)
+        targets, max_h = vec("rho_target", None, n + 1), vec("max_h_top", [float("inf")] * n, n)
+        ws = _WS_HYB.get(int(L.ocx_regrid_hybrid_workspace_bytes(n, ny * nx)))
+        rc = L.ocx_regrid_hybrid_f64(_ptr(Hc), _ptr(ec), _ptr(act), _ptr(hc), _ptr(rc_), n, ny * nx, _ptr(dz_top),
+                                     _ptr(dsigma), _ptr(targets), _ptr(max_h), float(params.get("d_sigma", 0.0)),
+                                     float(params.get("d_hybrid", 0.0)), h_min,
+                                     int(bool(params.get("keep_isopycnal", True))), _ptr(h_new), _ptr(wet), _ptr(ws),
+                                     ws.numel(), badp, _stream())
+    else:
+        dz = vec("dz_top" if rule == "zstar" else "dsigma", None, n)
+        ws = _WS.get(int(L.ocx_regrid_workspace_bytes()))
+        rc = L.ocx_regrid_f64(_ptr(Hc), _ptr(ec), _ptr(act), _ptr(hc), n, ny * nx, 0 if rule == "zstar" else 1,
+                              _ptr(dz), h_min, _ptr(h_new), _ptr(wet), _ptr(ws), ws.numel(), badp, _stream())
     if rc == -4:
         b = bad.value
-        raise ValueError(f"wet columns need finite H and D = H + eta > 0{' and H > 0 for z*' if rule == 'zstar' else ''}"
-                         f" (column j={b // nx}, i={b % nx})")
+        raise ValueError(f"{L.ocx_last_error().decode()} (column j={b // nx}, i={b % nx})")
     check(rc, "regrid")
     return h_new, wet
 
@@ -109,19 +131,20 @@ def regrid(H, eta, active, h, rule: str, params: dict, check_columns: bool = Tru
 class VerticalCoordinate:
     """
     The ALE vertical coordinate's regrid + remap step (``ALECoordinate.regrid_remap``,
-    core/components/vertical_coordinate.py:69-79) for the z* and sigma rules: H, active (ny, nx); rule; params as
-    the reference's (dz_top / dsigma, h_min); remap_code PCM / PLM / PPM_H4.
+    core/components/vertical_coordinate.py:69-79) for the z*, sigma and hybrid rules: H, active (ny, nx); rule;
+    params as the reference's (see ``regrid``); remap_code PCM / PLM / PPM_H4.
     """
 
     def __init__(self, *, H, active, rule: str, params: dict, remap_code: int = PPM_H4):
-        if rule == "hybrid":
-            raise NotImplementedError("the hybrid (HYCOM1) rule is not offered on the device")
+        if rule not in ("zstar", "sigma", "hybrid"):
+            raise ValueError(f"unknown rule {rule!r}; use 'zstar', 'sigma' or 'hybrid'")
         self.H, self.active, self.rule, self.params, self.remap_code = H, active, rule, params, int(remap_code)
 
-    def regrid_remap(self, eta, h_src, fields, T=None, S=None):
+    def regrid_remap(self, eta, h_src, fields, T=None, S=None, rho=None):
         """(h_new, remapped): ``fields`` a list of (n, ny, nx) tensors (a list comes back) or one stack
-        (nf, n, ny, nx) (a stack comes back, one launch).  T and S are read by the hybrid rule only."""
-        h_new, wet = regrid(self.H, eta, self.active, h_src, self.rule, self.params)
+        (nf, n, ny, nx) (a stack comes back, one launch).  The hybrid rule follows the layer densities ``rho``
+        (the reference's regrid_density(T, S) of its equation of state, computed by the caller)."""
+        h_new, wet = regrid(self.H, eta, self.active, h_src, self.rule, self.params, rho=rho)
         if torch.is_tensor(fields):
             return h_new, remap_columns(h_src, fields, h_new, self.remap_code, wet)
         return h_new, [remap_columns(h_src, f, h_new, self.remap_code, wet) for f in fields]

@@ -86,8 +86,16 @@ def test_regrid_rejects_a_wet_column_it_cannot_build(gpu):
     h = np.full((5, 3, 4), 2.0)
     with pytest.raises(ValueError, match="j=1, i=2"):
         vertical.regrid(_d(H), _d(eta), _d(np.ones((3, 4), bool)), _d(h), "zstar", {"dz_top": np.full(5, 20.0)})
-    with pytest.raises(NotImplementedError):
-        vertical.regrid(_d(H), _d(eta), _d(np.ones((3, 4), bool)), _d(h), "hybrid", {})
+    with pytest.raises(ValueError, match="rho_target"):
+        vertical.regrid(_d(H), _d(eta), _d(np.ones((3, 4), bool)), _d(h), "hybrid", {"dz_top": np.ones(5)})
+    rho = np.full((5, 3, 4), 1025.0)
+    params = {"dz_top": np.full(5, 20.0), "rho_target": np.linspace(1024.0, 1026.0, 6)}
+    with pytest.raises(ValueError, match="j=1, i=2"):                 # H <= 0 without a sigma zone
+        vertical.regrid(_d(H), _d(eta), _d(np.ones((3, 4), bool)), _d(h), "hybrid", params, rho=_d(rho))
+    H[1, 2] = 100.0
+    rho[3, 2, 1] = np.nan
+    with pytest.raises(ValueError, match="j=2, i=1"):                 # a non-finite density in a wet column
+        vertical.regrid(_d(H), _d(eta), _d(np.ones((3, 4), bool)), _d(h), "hybrid", params, rho=_d(rho))
 
 
 def test_regrid_remap_after_the_layer_transport_matches_the_oracle(gpu):
@@ -131,3 +139,54 @@ def test_remap_at_full_size_conserves_every_column(gpu):
     c1 = (ht[None] * out).sum(1)[:, w]
     assert ((c1 - c0).abs() <= 1e-12 * (hs[None] * C.abs()).sum(1)[:, w] + 1e-300).all()
     assert torch.isfinite(out[:, :, w]).all()
+
+
+def _hybrid_case(seed, n):
+    rng = np.random.default_rng(seed)
+    ny, nx = 13, 17
+    H = rng.uniform(3.0, 400.0, (ny, nx))
+    H[0, :3] = [-0.5, 0.0, 1.0]
+    eta = rng.uniform(-0.3, 0.3, (ny, nx))
+    eta[0, 0] = 0.8
+    act = rng.random((ny, nx)) > 0.1
+    frac = rng.uniform(0.0, 1.0, (n, ny, nx))
+    frac[rng.random(frac.shape) < 0.3] = 0.0
+    tot = frac.sum(0)
+    tot[tot == 0] = 1.0
+    h = np.where((H + eta)[None] > 0, frac / tot * (H + eta)[None], 0.0)
+    h = np.where(h < 2e-3, rng.choice([0.0, 1e-3, 1.5e-3], h.shape), h)
+    zc = -np.cumsum(h[::-1], axis=0)[::-1] + 0.5 * h
+    rho = 1024.0 - 0.01 * zc + rng.normal(0, 0.2, h.shape)
+    targets = np.sort(rng.uniform(1023.0, 1030.0, n + 1))
+    return H, eta, act, h, rho, targets, rng
+
+
+@pytest.mark.parametrize("keep", [True, False])
+@pytest.mark.parametrize("window", [False, True])
+@pytest.mark.parametrize("n", [10, 50])
+def test_hybrid_regrid_matches_the_oracle_bit_for_bit(gpu, keep, window, n):
+    _say("regrid hybrid", keep, window, n)
+    H, eta, act, h, rho, targets, rng = _hybrid_case(5 + n + 2 * keep + window, n)
+    params = dict(dz_top=rng.uniform(1.0, 10.0, n), dsigma=np.full(n, 1.0 / n), rho_target=targets,
+                  max_h_top=np.where(rng.random(n) < 0.3, 30.0, np.inf), d_sigma=20.0,
+                  d_hybrid=100.0 if window else 0.0, keep_isopycnal=keep)
+    if not window:
+        H = np.where(H > 0, H, 5.0)
+    h_new, wet = vertical.regrid(_d(H), _d(eta), _d(act), _d(h), "hybrid", params, rho=_d(rho))
+    r_new, r_wet = cref.regrid_hybrid(H, eta, act, h, rho, params)
+    _eq(wet, r_wet.astype(np.uint8), "wet")
+    _eq(h_new, r_new, "h_new")
+
+
+def test_hybrid_regrid_remap_of_a_stack_matches_the_oracle(gpu):
+    _say("regrid_remap hybrid")
+    n = 30
+    H, eta, act, h, rho, targets, rng = _hybrid_case(9, n)
+    H = np.where(H > 0, H, 5.0)
+    params = dict(dz_top=np.full(n, 2.0), rho_target=targets)
+    C = rng.normal(10.0, 1.0, (4, n) + H.shape)
+    vc = vertical.VerticalCoordinate(H=_d(H), active=_d(act), rule="hybrid", params=params)
+    h_new, Cn = vc.regrid_remap(_d(eta), _d(h), _d(C), rho=_d(rho))
+    r_new, r_wet = cref.regrid_hybrid(H, eta, act, h, rho, params)
+    _eq(h_new, r_new, "h_new")
+    _eq(Cn, cref.remap_columns(h, C, r_new, cref.PPM_H4, r_wet), "C remapped")

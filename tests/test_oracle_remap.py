@@ -246,3 +246,153 @@ def test_regrid_keeps_dry_columns_and_preserves_the_column_thickness():
     np.testing.assert_allclose(hz.sum(0)[wet], (H + eta)[wet], rtol=1e-13)
     hs, _ = cref.regrid(H, eta, active, h, "sigma", np.full(n, 1.0 / n))
     np.testing.assert_allclose(hs[:, wet], np.broadcast_to((H + eta)[wet] / n, (n, wet.sum())), rtol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------------ hybrid rule
+def _linear_column(H=201.0, n=10, rho_top=1024.0, drho_dz=0.02):
+    zw = np.linspace(-H, 0.0, n + 1)
+    zc = 0.5 * (zw[1:] + zw[:-1])
+    return np.diff(zw), rho_top - drho_dz * zc
+
+
+def _hyb(H, eta, h, rho, targets, dz=DELR, max_h=None, keep=False):
+    return cref.hycom1_column(H, eta, h, rho, targets, dz, max_h, H_MIN, keep)
+
+
+def test_hybrid_interfaces_sit_on_the_target_densities_and_respect_the_zstar_floor():
+    h, rho = _linear_column()
+    z_t = -np.array([0.0, 30, 60, 80, 100, 120, 140, 160, 170, 178, 201.0])
+    zw = _hyb(201.0, 0.0, h, rho, 1024.0 - 0.02 * z_t)
+    np.testing.assert_allclose(zw[::-1][1:-1], z_t[1:-1], atol=1e-9)
+    zw = _hyb(201.0, 0.0, h, rho, np.full(11, 1024.0))
+    np.testing.assert_allclose(zw, cref.zstar_column(201.0, 0.0, DELR, H_MIN), atol=1e-9)
+
+
+def test_hybrid_unstratified_out_of_range_and_nonmonotonic_columns():
+    zw = _hyb(201.0, 0.0, np.full(10, 20.1), np.full(10, 1025.0), np.linspace(1024.0, 1027.0, 11))
+    assert np.isfinite(zw).all() and (np.diff(zw) > 0).all()
+    h, rho = _linear_column()
+    zw = _hyb(201.0, 0.0, h, rho, np.linspace(1000.0, 1100.0, 11))
+    assert (np.diff(zw) >= H_MIN * (1 - 1e-12)).all()
+    np.testing.assert_allclose(np.diff(zw).sum(), 201.0, rtol=1e-14)
+    rho2 = rho.copy()
+    rho2[3] += 0.5
+    targets = np.linspace(1024.0, 1028.0, 11)
+    zw = _hyb(201.0, 0.0, h, rho2, targets)
+    assert (np.diff(zw) > 0).all()
+    rho_m = rho2.copy()
+    for k in range(1, rho_m.size):
+        rho_m[k] = min(rho_m[k], rho_m[k - 1])
+    np.testing.assert_allclose(zw, _hyb(201.0, 0.0, h, rho_m, targets), rtol=0, atol=1e-12)
+
+
+def test_hybrid_massless_bed_layer_and_maximum_thickness():
+    n = 10
+    targets = np.linspace(1019.0, 1026.0, n + 1)
+    h = np.full(n, H_MIN)
+    h[1], h[8], h[9] = 0.7, 19.3 - 7 * H_MIN - 0.01, 0.01
+    rho = np.full(n, 1020.0)
+    rho[1:8] = 1025.0
+    zw = _hyb(20.0, 0.0, h, rho, targets, dz=np.full(n, 0.01))
+    assert (-20.0 + H_MIN + 0.7) - zw[n - 2] < 0.1 * 0.7
+    h, rho = _linear_column()
+    zw = _hyb(201.0, 0.0, h, rho, np.r_[1024.0, np.full(9, 1028.0), 1028.1], max_h=np.full(10, 25.0))
+    assert (np.diff(zw) <= 25.0 + 1e-9).all()
+    np.testing.assert_allclose(np.diff(zw).sum(), 201.0, rtol=1e-14)
+
+
+def _window_case(H, d_sigma=50.0, d_hybrid=100.0, n=10):
+    zw_t = np.linspace(-H, 0.0, n + 1)
+    zc = 0.5 * (zw_t[1:] + zw_t[:-1])
+    h, rho = np.diff(zw_t), 1024.0 - 0.02 * zc
+    dz = np.full(n, 2.0)
+    targets = 1024.0 + 0.02 * H * np.linspace(0.0, 1.0, n + 1) ** 2
+    hyb = cref.hycom1_column(H, 0.0, h, rho, targets, dz, None, H_MIN, False)
+    sig = cref.sigma_column(H, 0.0, np.full(n, 1.0 / n), H_MIN)
+    zs = cref.zstar_column(H, 0.0, dz, H_MIN)
+    return hyb, sig, zs, cref.window_shallow_sigma(hyb, sig, zs, H, H, d_sigma, d_hybrid, H_MIN)
+
+
+def test_window_is_sigma_shallow_hybrid_deep_and_keeps_fitting_isopycnals():
+    hyb, sig, _, out = _window_case(40.0)
+    np.testing.assert_array_equal(out, sig)
+    hyb, sig, _, out = _window_case(120.0)
+    np.testing.assert_array_equal(out, hyb)
+    H, q = 75.0, 0.5
+    hyb, sig, zs, out = _window_case(H)
+    lo, hi = q * (-H) + (1 - q) * sig, q * zs + (1 - q) * sig
+    assert ((out >= lo - 1e-12) & (out <= hi + 1e-12)).all()
+    inside = (hyb > lo) & (hyb < hi)
+    inside[[0, -1]] = False
+    assert inside.sum() == 3 and (~inside[1:-1]).sum() == 6
+    np.testing.assert_allclose(out[inside], hyb[inside], atol=1e-12)
+    for Hc, other in ((50.0 + 1e-6, 0), (100.0 - 1e-6, 1)):
+        hyb, sig, _, out = _window_case(Hc)
+        assert np.abs(out - (sig if other == 0 else hyb)).max() < 1e-4
+
+
+def test_regrid_hybrid_columns_match_single_columns_and_the_sigma_zone():
+    hyb, sig, zs, out = _window_case(75.0)
+    n = 10
+    zw_t = np.linspace(-75.0, 0.0, n + 1)
+    zc = 0.5 * (zw_t[1:] + zw_t[:-1])
+    h3 = np.diff(zw_t)[:, None, None].repeat(2, 1)
+    r3 = (1024.0 - 0.02 * zc)[:, None, None].repeat(2, 1)
+    params = dict(dz_top=np.full(n, 2.0), rho_target=1024.0 + 0.02 * 75.0 * np.linspace(0.0, 1.0, n + 1) ** 2,
+                  dsigma=np.full(n, 1.0 / n), d_sigma=50.0, d_hybrid=100.0, keep_isopycnal=False)
+    h_new, wet = cref.regrid_hybrid(np.full((2, 1), 75.0), np.zeros((2, 1)), np.ones((2, 1), bool), h3, r3, params)
+    np.testing.assert_allclose(h_new[:, 0, 0], np.diff(out), atol=1e-12)
+    # flooded land (H < 0) in the sigma zone: pure sigma at any depth
+    H = np.array([[-0.5, 10.0]])
+    eta = np.array([[0.7, 0.0]])
+    base = dict(dz_top=np.ones(10), rho_target=np.linspace(1024, 1026, 11), dsigma=np.full(10, 0.1), d_sigma=5.0,
+                d_hybrid=20.0)
+    h_new, _ = cref.regrid_hybrid(H, eta, np.ones((1, 2), bool), np.ones((10, 1, 2)), np.full((10, 1, 2), 1025.0),
+                                  base)
+    np.testing.assert_allclose(h_new[:, 0, 0], 0.02, rtol=1e-12)
+
+
+def _consistent_column(n=10, H=200.0, seed=0):
+    rng = np.random.default_rng(seed)
+    targets = np.linspace(1020.0, 1030.0, n + 1)
+    mid_top = 0.5 * (targets[:-1] + targets[1:])
+    h = rng.uniform(0.5, 1.5, n)
+    h *= H / h.sum()
+    return h, mid_top[::-1].copy(), targets
+
+
+def test_keep_isopycnal_leaves_consistent_layers_and_regrids_only_inconsistent_interfaces():
+    h, rho, targets = _consistent_column()
+    dz = np.full(10, 1e-6)
+    zin = -200.0 + np.concatenate([[0.0], np.cumsum(h)])
+    zw = _hyb(200.0, 0.0, h, rho, targets, dz=dz, keep=True)
+    np.testing.assert_allclose(zw, zin, rtol=0, atol=1e-11)
+    assert np.abs(_hyb(200.0, 0.0, h, rho, targets, dz=dz) - zw).max() > 1.0
+    rho[4] = 1021.2
+    p1m = _hyb(200.0, 0.0, h, rho, targets, dz=dz)
+    zw = _hyb(200.0, 0.0, h, rho, targets, dz=dz, keep=True)
+    other = [k for k in range(11) if k != 5]
+    np.testing.assert_allclose(zw[other], zin[other], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(zw[5], np.clip(p1m[5], zin[4], zin[6]), rtol=0, atol=1e-11)
+
+
+def test_keep_isopycnal_massless_layers_floor_and_jump():
+    h, rho, targets = _consistent_column()
+    h[3] = H_MIN
+    rho[3] = 1029.9
+    h *= 200.0 / h.sum()
+    zw = _hyb(200.0, 0.0, h, rho, targets, dz=np.full(10, 1e-6), keep=True)
+    np.testing.assert_allclose(zw, -200.0 + np.concatenate([[0.0], np.cumsum(h)]), rtol=0, atol=1e-11)
+    h, rho, targets = _consistent_column()
+    h[-3:] = [0.2, 0.2, 0.2]
+    h *= 200.0 / h.sum()
+    dz = np.full(10, 5.0)
+    zw = _hyb(200.0, 0.0, h, rho, targets, dz=dz, keep=True)
+    assert (np.diff(zw) > 0).all() and zw[0] == -200.0 and zw[-1] == 0.0
+    np.testing.assert_allclose(zw[::-1][1:4], -np.cumsum(dz)[:3], atol=1e-11)
+    h_top = np.array([10.0, 90.0, 10.0, 30.0, 30.0, 30.0])
+    rho_top = np.array([1001.2, 1001.2, 1002.0, 1002.0, 1002.0, 1002.0])
+    targets = np.array([999.5, 1001.0, 1001.5, 1001.8, 1001.9, 1001.95, 1001.99])
+    dz = np.array([10.0, 40.0, 60.0, 30.0, 30.0, 30.0])
+    zw = cref.hycom1_column(200.0, 0.0, h_top[::-1].copy(), rho_top[::-1].copy(), targets, dz, None, H_MIN, True)
+    np.testing.assert_allclose(-zw[::-1], [0.0, 10.0, 100.0, 110.0, 140.0, 170.0, 200.0], atol=1e-9)
