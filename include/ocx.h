@@ -305,6 +305,15 @@ int ocx_regrid_hybrid_f64(const double* H, const double* eta, const uint8_t* act
                           double h_min, int keep, double* h_new, uint8_t* wet, void* workspace,
                           size_t workspace_bytes, long long* bad_column, void* stream);
 
+/* Layer densities of the hybrid rule, ALECoordinate.regrid_density (core/components/vertical_coordinate.py:80-100),
+ * elementwise over n cells: eos 0 = the linear EOS, rho0 - t_coef (T - t_ref) [+ s_coef (S - s_ref)]
+ * (LinearEOS.density, core/components/eos.py:62-69; linear = host array {rho0, t_coef, t_ref, s_coef, s_ref},
+ * S may be NULL); eos 1 = Wright's full-range density at sea pressure p [Pa] (WrightFullEOS.density,
+ * physics/eos/wright_full.py:80-89; S required; p a device array of n values or NULL for p_scalar at every cell,
+ * e.g. the reference pressure 2e7 Pa of the potential density).  T, S, p, rho are device arrays. */
+int ocx_regrid_density_f64(int eos, const double* T, const double* S, const double* p, double p_scalar,
+                           const double* linear, long long n, double* rho, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

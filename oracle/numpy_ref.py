@@ -800,3 +800,29 @@ def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_
     return dict(Qx=np.ascontiguousarray(Qx), Qy=np.ascontiguousarray(Qy), V0=V0, V1=V1, ok=act[None] & (V1 > 0),
                 h_dagger=h_lag, n_limited=n_lim, qe=qe, qw=qw, qn=np.ascontiguousarray(qn.swapaxes(1, 2)),
                 qs=np.ascontiguousarray(qs.swapaxes(1, 2)), utx=utx, uty=uty, Qbx=Qbx, Qby=Qby)
+
+
+# ------------------------------------------------------------------------------------------------ layer density
+# Wright (1997), Table 1, second column (full-range fit), the coefficients MOM6's WRIGHT_FULL uses; the same
+# decimal values as physics/eos/wright_full.py:63-77.
+WRIGHT = dict(a0=7.133718e-4, a1=2.724670e-7, a2=-1.646582e-7, b0=5.613770e8, b1=3.600337e6, b2=-3.727194e4,
+              b3=1.660557e2, b4=6.844158e5, b5=-8.389457e3, c0=1.609893e5, c1=8.427815e2, c2=-6.931554,
+              c3=3.869318e-2, c4=-1.664201e2, c5=-2.765195)
+
+
+def linear_density(T, S=None, rho0=1000.0, t_coef=0.2, t_ref=0.0, s_coef=0.0, s_ref=35.0):
+    """Linear EOS rho0 - t_coef (T - t_ref) [+ s_coef (S - s_ref)] [kg m^-3] (LinearEOS.density,
+    core/components/eos.py:62-69; Vallis 2006 eq. 1.156)."""
+    if S is None:
+        return rho0 - t_coef * (T - t_ref)
+    return rho0 - t_coef * (T - t_ref) + s_coef * (S - s_ref)
+
+
+def wright_density(T, S, p):
+    """Wright (1997) full-range in situ density rho(theta, S, p) [kg m^-3], p the sea pressure [Pa]
+    (density_elem, physics/eos/wright_full.py:80-89, MOM6 WRIGHT_FULL parenthesization)."""
+    w = WRIGHT
+    al0 = w["a0"] + (w["a1"] * T + w["a2"] * S)
+    p0 = w["b0"] + (w["b4"] * S + T * (w["b1"] + (T * (w["b2"] + w["b3"] * T) + w["b5"] * S)))
+    lam = w["c0"] + (w["c4"] * S + T * (w["c1"] + (T * (w["c2"] + w["c3"] * T) + w["c5"] * S)))
+    return (p + p0) / (lam + al0 * (p + p0))

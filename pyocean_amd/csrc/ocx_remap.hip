@@ -123,4 +123,22 @@ int ocx_regrid_hybrid_f64(const double* H, const double* eta, const uint8_t* act
                 "unless it has a shallow sigma zone (d_hybrid > 0)", b);
 }
 
+int ocx_regrid_density_f64(int eos, const double* T, const double* S, const double* p, double p_scalar,
+                           const double* linear, long long n, double* rho, void* stream) {
+    if (n < 1) return fail(OCX_E_ARG, "n = %lld cells must be positive", n);
+    if (eos != EOS_LINEAR && eos != EOS_WRIGHT_FULL) return fail(OCX_E_ARG, "eos = %d: 0 (linear) or 1 (wright_full)", eos);
+    OCX_TRY(need(T, "T"));
+    OCX_TRY(need(rho, "rho"));
+    if (eos == EOS_WRIGHT_FULL && !S) return fail(OCX_E_ARG, "wright_full EOS needs salinity");
+    if (eos == EOS_LINEAR && !linear) return fail(OCX_E_ARG, "the linear EOS needs its 5 coefficients");
+    DensityArgs a{T, S, eos == EOS_WRIGHT_FULL ? p : nullptr, p_scalar, {0, 0, 0, 0, 0}, eos, n, rho};
+    if (eos == EOS_LINEAR)
+        for (int q = 0; q < 5; ++q) a.lin[q] = linear[q];
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ProfScope _ps("regrid_density", s);
+    const long long blocks = std::min<long long>((n + 255) / 256, 65536);
+    k_regrid_density<<<(unsigned)blocks, 256, 0, s>>>(a);
+    return launch_check("regrid_density");
+}
+
 }  // extern "C"

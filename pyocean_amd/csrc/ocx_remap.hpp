@@ -1204,5 +1204,50 @@ __global__ void k_regrid_hybrid(HybridArgs r) {
     for (int k = 0; k < n; ++k) r.h_new[(size_t)k * M + c] = zw[k + 1] - zw[k];
 }
 
+// ---------------------------------------------------------------------------------------- layer density
+// The densities the hybrid rule follows (ALECoordinate.regrid_density, core/components/vertical_coordinate.py:80-100):
+// the linear EOS rho0 - t_coef (T - t_ref) [+ s_coef (S - s_ref)] (core/components/eos.py:62-69), or Wright's (1997)
+// full-range density at sea pressure p (physics/eos/wright_full.py:80-89; the MOM6 WRIGHT_FULL parenthesization and
+// the coefficients of Wright's Table 1, second column).  Elementwise over n cells, every operation rounded on its own.
+constexpr int EOS_LINEAR = 0, EOS_WRIGHT_FULL = 1;
+
+struct DensityArgs {
+    const double* T;
+    const double* S;     // null: the linear EOS without salinity
+    const double* p;     // null: the scalar pressure ps for every cell
+    double ps;
+    double lin[5];       // rho0, t_coef, t_ref, s_coef, s_ref
+    int kind;
+    long long n;
+    double* rho;
+};
+
+__device__ __forceinline__ double wright_density(double T, double S, double p) {
+    constexpr double a0 = 7.133718e-4, a1 = 2.724670e-7, a2 = -1.646582e-7;
+    constexpr double b0 = 5.613770e8, b1 = 3.600337e6, b2 = -3.727194e4, b3 = 1.660557e2, b4 = 6.844158e5,
+                     b5 = -8.389457e3;
+    constexpr double c0 = 1.609893e5, c1 = 8.427815e2, c2 = -6.931554, c3 = 3.869318e-2, c4 = -1.664201e2,
+                     c5 = -2.765195;
+    const double al0 = a0 + (a1 * T + a2 * S);
+    const double p0 = b0 + (b4 * S + T * (b1 + (T * (b2 + b3 * T) + b5 * S)));
+    const double lam = c0 + (c4 * S + T * (c1 + (T * (c2 + c3 * T) + c5 * S)));
+    return (p + p0) / (lam + al0 * (p + p0));
+}
+
+__global__ void k_regrid_density(DensityArgs a) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < a.n; c += stride) {
+        const double T = a.T[c];
+        double r;
+        if (a.kind == EOS_WRIGHT_FULL) {
+            r = wright_density(T, a.S[c], a.p ? a.p[c] : a.ps);
+        } else {
+            r = a.lin[0] - a.lin[1] * (T - a.lin[2]);
+            if (a.S) r = r + a.lin[3] * (a.S[c] - a.lin[4]);
+        }
+        a.rho[c] = r;
+    }
+}
+
 }  // namespace rmp
 }  // namespace ocx

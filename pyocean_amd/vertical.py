@@ -130,21 +130,46 @@ def regrid(H, eta, active, h, rule: str, params: dict, check_columns: bool = Tru
 
 class VerticalCoordinate:
     """
-    The ALE vertical coordinate's regrid + remap step (``ALECoordinate.regrid_remap``,
-    core/components/vertical_coordinate.py:69-79) for the z*, sigma and hybrid rules: H, active (ny, nx); rule;
-    params as the reference's (see ``regrid``); remap_code PCM / PLM / PPM_H4.
+    The ALE vertical coordinate's regrid + remap step (``ALECoordinate``, core/components/vertical_coordinate.py:
+    27-109) for the z*, sigma and hybrid rules: H, active (ny, nx); rule; params as the reference's (see
+    ``regrid``); remap_code PCM / PLM / PPM_H4; ``eos`` (``pyocean_amd.eos``) and ``p_ref`` [Pa] give the layer
+    densities of the hybrid rule.
     """
 
-    def __init__(self, *, H, active, rule: str, params: dict, remap_code: int = PPM_H4):
+    def __init__(self, *, H, active, rule: str, params: dict, remap_code: int = PPM_H4, eos=None,
+                 p_ref: float = 2e7):
         if rule not in ("zstar", "sigma", "hybrid"):
             raise ValueError(f"unknown rule {rule!r}; use 'zstar', 'sigma' or 'hybrid'")
         self.H, self.active, self.rule, self.params, self.remap_code = H, active, rule, params, int(remap_code)
+        self.eos, self.p_ref = eos, float(p_ref)
+
+    def regrid_density(self, T, S=None):
+        """Layer densities the hybrid rule follows [kg m^-3]: the linear EOS's density, else the potential density
+        at p_ref (core/components/vertical_coordinate.py:80-100), unmodified."""
+        if self.eos is None:
+            raise ValueError("regrid_density needs an equation of state (eos=...)")
+        if self.eos.linear:
+            return self.eos.density(T, S)
+        return self.eos.potential_density(T, S, self.p_ref)
+
+    def _rho(self, T, S, rho):
+        if self.rule != "hybrid" or rho is not None:
+            return rho
+        if T is None:
+            raise ValueError("the hybrid rule needs T (and S) or the layer densities rho")
+        return self.regrid_density(T, S)
+
+    def regrid(self, eta, h, T=None, S=None, rho=None):
+        """New layer thicknesses of the rule at surface eta from layers h, h kept where dry
+        (core/components/vertical_coordinate.py:102-117); the hybrid rule follows regrid_density(T, S) or
+        the given densities ``rho``."""
+        return regrid(self.H, eta, self.active, h, self.rule, self.params, rho=self._rho(T, S, rho))[0]
 
     def regrid_remap(self, eta, h_src, fields, T=None, S=None, rho=None):
         """(h_new, remapped): ``fields`` a list of (n, ny, nx) tensors (a list comes back) or one stack
-        (nf, n, ny, nx) (a stack comes back, one launch).  The hybrid rule follows the layer densities ``rho``
-        (the reference's regrid_density(T, S) of its equation of state, computed by the caller)."""
-        h_new, wet = regrid(self.H, eta, self.active, h_src, self.rule, self.params, rho=rho)
+        (nf, n, ny, nx) (a stack comes back, one launch).  The hybrid rule follows regrid_density(T, S), or the
+        given layer densities ``rho`` (core/components/vertical_coordinate.py:69-78)."""
+        h_new, wet = regrid(self.H, eta, self.active, h_src, self.rule, self.params, rho=self._rho(T, S, rho))
         if torch.is_tensor(fields):
             return h_new, remap_columns(h_src, fields, h_new, self.remap_code, wet)
         return h_new, [remap_columns(h_src, f, h_new, self.remap_code, wet) for f in fields]
