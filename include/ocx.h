@@ -136,7 +136,8 @@ int ocx_diffuse_h_f64(const double* T, const double* Tw, const double* h, const 
  * core/dynamics/sigma_wd.py:57-99), same arithmetic: u, out (nt, n, columns) (out may be u); dz (n, columns);
  * nu (n+1, columns) interface diffusivities; r_bed NULL, (columns) on layer 0 (r_layers = 1) or (n, columns)
  * (r_layers = n); tau_top NULL or (columns) surface flux into the top layer.  workspace:
- * ocx_vertical_implicit_workspace_bytes(n, columns) bytes (the forward-sweep coefficients, shared by the fields). */
+ * ocx_vertical_implicit_workspace_bytes(n, columns) bytes (the forward-sweep coefficients, and for n <= 64 each
+ * level's lower coefficient and pivot, shared by the fields). */
 size_t ocx_vertical_implicit_workspace_bytes(int n, long long columns);
 int ocx_vertical_implicit_f64(const double* u, const double* dz, const double* nu, const double* r_bed, int r_layers,
                               const double* tau_top, int nt, int n, long long columns, double dt, double* out,

@@ -809,8 +809,11 @@ int ocx_diffuse_h_f64(const double* T, const double* Tw, const double* h, const 
     return launch_check("diffuse_h");
 }
 
+// cp, then (columns of at most 64 levels) the lower coefficients and pivots of the register-resident solve
 size_t ocx_vertical_implicit_workspace_bytes(int n, long long columns) {
-    return n > 0 && columns > 0 ? (size_t)n * (size_t)columns * sizeof(double) : 0;
+    if (n < 1 || columns < 1) return 0;
+    const size_t level = align256((size_t)n * (size_t)columns * sizeof(double));
+    return n <= 64 ? 3 * level : level;
 }
 
 int ocx_vertical_implicit_f64(const double* u, const double* dz, const double* nu, const double* r_bed, int r_layers,
@@ -829,16 +832,40 @@ int ocx_vertical_implicit_f64(const double* u, const double* dz, const double* n
     if (!workspace || workspace_bytes < wsb)
         return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed, %zu given", wsb, workspace ? workspace_bytes : 0);
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t level = align256((size_t)n * (size_t)columns * sizeof(double));
     double* cp = static_cast<double*>(workspace);
+    // the workspace holds lw and mm whenever n <= 64; the register solve's 1-D grid needs fewer than 2^32 blocks
+    const bool reg = n <= 64 && ((columns + 63) / 64) * ((nt + 3) / 4) < (1ll << 32);
+    double* lw = reg ? reinterpret_cast<double*>(static_cast<char*>(workspace) + level) : nullptr;
+    double* mm = reg ? reinterpret_cast<double*>(static_cast<char*>(workspace) + 2 * level) : nullptr;
     const unsigned gc = (unsigned)std::min<long long>((columns + 255) / 256, 1 << 16);
     {
         ProfScope _ps("vertical_coefficients", s);
-        k_thomas_cp<<<gc, 256, 0, s>>>(dz, nu, r_bed, r_layers, n, columns, dt, cp);
+        k_thomas_cp<<<gc, 256, 0, s>>>(dz, nu, r_bed, r_layers, n, columns, dt, cp, lw, mm);
     }
     OCX_TRY(launch_check("vertical_implicit coefficients"));
-    const unsigned gs = (unsigned)std::min<long long>((columns + 255) / 256, 1 << 16);
     ProfScope _ps("vertical_solve", s);
-    k_thomas_solve<<<gs, 256, 0, s>>>(u, dz, nu, r_bed, r_layers, tau_top, nt, n, columns, dt, cp, out);
+    if (reg) {
+        const int fy = std::min(nt, 4), groups = (nt + fy - 1) / fy;
+        const unsigned grid = (unsigned)(((columns + 63) / 64) * groups);
+        const dim3 block(64, fy);
+#define OCX_SOLVE_R(NB) \
+    k_thomas_solve_r<NB><<<grid, block, 0, s>>>(u, dz, tau_top, nt, n, columns, dt, cp, lw, mm, out, groups)
+        switch ((n + 7) / 8) {     // the smallest multiple of 8 levels that holds the column
+            case 1: OCX_SOLVE_R(8); break;
+            case 2: OCX_SOLVE_R(16); break;
+            case 3: OCX_SOLVE_R(24); break;
+            case 4: OCX_SOLVE_R(32); break;
+            case 5: OCX_SOLVE_R(40); break;
+            case 6: OCX_SOLVE_R(48); break;
+            case 7: OCX_SOLVE_R(56); break;
+            default: OCX_SOLVE_R(64); break;
+        }
+#undef OCX_SOLVE_R
+    } else {
+        const unsigned gs = (unsigned)std::min<long long>((columns + 255) / 256, 1 << 16);
+        k_thomas_solve<<<gs, 256, 0, s>>>(u, dz, nu, r_bed, r_layers, tau_top, nt, n, columns, dt, cp, out);
+    }
     return launch_check("vertical_implicit");
 }
 

@@ -8,8 +8,10 @@
 //
 // Vertical (_thomas_columns, core/dynamics/sigma_wd.py:57-88): backward Euler with interface diffusivity nu and the
 // layer thicknesses dz, a tridiagonal system per column solved by the Thomas algorithm.  The forward-sweep
-// coefficients cp_k depend on dz, nu and the rates only, so one pass stores them (k_thomas_cp) and a second solves
-// the fields of each column with them (k_thomas_solve); columns are contiguous across threads.
+// coefficients cp_k depend on dz, nu and the rates only, so one pass stores them (k_thomas_cp) with each level's
+// lower coefficient and pivot, and a second solves the fields of each column with them: k_thomas_solve_r for columns
+// of up to 64 levels (the forward values in registers), k_thomas_solve beyond; columns are contiguous across
+// threads.
 #pragma once
 #include "ocx_weno.hpp"
 
@@ -83,17 +85,88 @@ __device__ __forceinline__ TriRow tri_row(const double* __restrict__ dz, const d
     return t;
 }
 
-// cp (n, M): forward-sweep coefficients, shared by every tracer
+// cp (n, M): forward-sweep coefficients, shared by every tracer.  With lw and mm (not null) it also stores each
+// level's lower coefficient and pivot m_k = diag_k - lower_k cp_{k-1} (m_0 = diag_0), the values the solve of
+// every field needs, so that k_thomas_solve_r reads them instead of recomputing the row
 __global__ void k_thomas_cp(const double* __restrict__ dz, const double* __restrict__ nu, const double* __restrict__ r,
-                            int r_layers, int n, long long M, double dt, double* __restrict__ cp) {
+                            int r_layers, int n, long long M, double dt, double* __restrict__ cp,
+                            double* __restrict__ lw, double* __restrict__ mm) {
     for (long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x; col < M;
          col += (long long)gridDim.x * blockDim.x) {
         double prev = 0.0;
         for (int k = 0; k < n; ++k) {
             const TriRow t = tri_row(dz, nu, r, r_layers, n, M, k, col, dt);
-            const double c = k == 0 ? t.upper / t.diag : t.upper / (t.diag - t.lower * prev);
+            const double m = k == 0 ? t.diag : t.diag - t.lower * prev;
+            const double c = t.upper / m;
             cp[(size_t)k * M + col] = c;
+            if (lw) {
+                lw[(size_t)k * M + col] = t.lower;
+                mm[(size_t)k * M + col] = m;
+            }
             prev = c;
+        }
+    }
+}
+
+// The same solve for columns of n <= NB levels, one thread per (column, field): the forward sweep's values stay
+// in registers (d[NB], every index static after unrolling), so each field is read once and written once; the
+// per-level lower, pivot and cp come from k_thomas_cp.  Block (64, fy): 64 columns of fy fields.  A 1-D grid of
+// column blocks x field groups, remapped XCD-contiguously with the field groups of a column block adjacent, so
+// that the groups sharing a column's coefficients run together on one XCD and find them in its L2.
+template <int NB>
+__global__ __launch_bounds__(256) void k_thomas_solve_r(const double* u, const double* __restrict__ dz,
+                                                        const double* __restrict__ tau_top, int nt, int n, long long M,
+                                                        double dt, const double* __restrict__ cp,
+                                                        const double* __restrict__ lw, const double* __restrict__ mm,
+                                                        double* out, int groups) {
+    const long long b = xcd_logical(blockIdx.x, gridDim.x);
+    const long long col = (b / groups) * 64 + threadIdx.x;
+    const int t = (int)(b % groups) * blockDim.y + threadIdx.y;
+    if (col >= M || t >= nt) return;
+    const double tau = tau_top ? tau_top[col] : 0.0;     // the reference adds a zero stress too
+    const double* uc = u + (size_t)t * n * M + col;
+    double* oc = out + (size_t)t * n * M + col;
+    const double* dzc = dz + col;
+    const double* mmc = mm + col;
+    const double* lwc = lw + col;
+    const double* cpc = cp + col;
+    double d[NB];
+    double dprev = 0.0;
+    // the inputs of level k are loaded PF levels ahead of its division (a ring of PF slots); levels k >= n repeat
+    // level n-1 (in-bounds reads whose results are discarded), so the loop has no branch to hold the loads back
+    constexpr int PF = 4;
+    double ru[PF], rz[PF], rm[PF], rl[PF];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+        const size_t x = (size_t)(q < n ? q : n - 1) * M;
+        ru[q] = uc[x];
+        rz[q] = dzc[x];
+        rm[q] = mmc[x];
+        rl[q] = lwc[x];
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int q = k % PF;
+        const double uu = ru[q], zz = rz[q], m = rm[q], l = rl[q];
+        if (k + PF < NB) {
+            const size_t x = (size_t)(k + PF < n ? k + PF : n - 1) * M;
+            ru[q] = uc[x];
+            rz[q] = dzc[x];
+            rm[q] = mmc[x];
+            rl[q] = lwc[x];
+        }
+        double rhs = zz * uu;
+        rhs = k == n - 1 ? rhs + dt * tau : rhs;
+        const double v = k == 0 ? rhs / m : (rhs - l * dprev) / m;
+        d[k] = v;
+        dprev = k < n ? v : dprev;
+    }
+#pragma unroll
+    for (int k = NB - 1; k >= 0; --k) {
+        if (k < n) {
+            const size_t x = (size_t)k * M;
+            if (k < n - 1) dprev = d[k] - cpc[x] * dprev;
+            oc[x] = dprev;
         }
     }
 }

@@ -1,7 +1,7 @@
 """
 GPU tests of the tracer diffusion kernels: the along-layer Laplacian and the implicit vertical solve are
 bit-identical to the oracle's restatement of the reference arithmetic, on a regional grid with land, closed faces,
-thin layers and a stack of tracers; the vertical solve also in place.
+thin layers and a stack of tracers; the vertical solve also in place, for 1 to 90 levels.
 """
 from __future__ import annotations
 
@@ -50,10 +50,12 @@ def test_horizontal_diffusion_matches_the_oracle_bit_for_bit(gpu, seed):
 
 
 @pytest.mark.parametrize("rates", ["none", "bed", "layers"])
-def test_vertical_implicit_matches_the_oracle_bit_for_bit(gpu, rates):
-    _say("vertical_implicit", rates)
-    rng = np.random.default_rng(4)
-    nt, n, M, dt = 3, 17, 5000, 450.0
+@pytest.mark.parametrize("n,nt", [(17, 3), (1, 2), (16, 5), (50, 10), (64, 4), (65, 3), (90, 2)])
+def test_vertical_implicit_matches_the_oracle_bit_for_bit(gpu, rates, n, nt):
+    # n <= 64: the register-resident solve (16, 32 and 64 levels); n > 64: the solve through the output array
+    _say("vertical_implicit", rates, n, nt)
+    rng = np.random.default_rng(4 + n)
+    M, dt = 5000, 450.0
     u = rng.normal(5.0, 2.0, (nt, n, M))
     dz = np.maximum(rng.uniform(-1.0, 40.0, (n, M)), 1e-6)
     nu = rng.uniform(1e-6, 5e-2, (n + 1, M))
