@@ -366,18 +366,41 @@ def bench_diffusion(args) -> int:
     updates = nt * nz * ny * nx
     value = updates * args.steps / wall
     # compulsory bytes per tracer-cell: horizontal T, Tw read and out written (24 B) + h, V (16 B) / nt; vertical
-    # u read and out written twice (the back substitution revisits it: 32 B) + dz, nu, cp (40 B) / nt
-    b_h, b_v = 24.0 + 16.0 / nt, 32.0 + 40.0 / nt
+    # u read and out written (16 B) + dz, nu (16 B) / nt
+    b_h, b_v = 24.0 + 16.0 / nt, 16.0 + 16.0 / nt
+    # the dominant kernel, the vertical solve: u read and out written + dz, cp, the lower coefficients and the
+    # pivots (32 B) / nt per tracer-cell
+    solve_ms = prof["vertical_solve"][0] / prof["vertical_solve"][1]
+    solve_bytes = (16.0 + 32.0 / nt) * updates
+    cpu = None
+    if not args.no_cpu:
+        from oracle import numpy_ref as R
+        sy, sx, st = ny, nx, min(nt, 4)
+        Cs = np.ascontiguousarray(case.C[:st, :, :sy, :sx])
+        hs, acts, Vs = h[:, :sy, :sx], act[:sy, :sx], case.V1[:, :sy, :sx]
+        Ms = sy * sx
+        t1 = time.perf_counter()
+        o = R.diffuse_h_ale(Cs, Cs - 10.0, hs, acts, np.ones((sy, sx + 1)), np.ones((sy + 1, sx)),
+                            np.ones((sy, sx + 1), bool), np.ones((sy + 1, sx), bool), Vs, 100.0, case.dt)
+        R.vertical_implicit(o.reshape(st, nz, Ms), np.maximum(hs, 1e-6).reshape(nz, Ms), np.full((nz + 1, Ms), 1e-4),
+                            np.zeros(Ms), case.dt)
+        tc = time.perf_counter() - t1
+        cpu = {"value": st * nz * Ms / tc, "unit": "tracer-cell updates/s", "cores": 1, "kind": "port",
+               "sample": f"{st} tracers of the same {sx}x{sy}x{nz} grid through the oracle's NumPy "
+                         f"diffusion, along-layer then vertical ({tc:.1f} s)"}
     res = {
         "metric": "tracer-cell diffusion updates/s (along-layer Laplacian + implicit vertical, whole stack)",
         "value": value, "unit": "tracer-cell updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded regional grid)",
         "config": {"workload": f"regional {nx}x{ny}x{nz}, {nt} tracers, K_h 100 m2/s, K_v 1e-4 m2/s"},
+        "roofline": {"kernel": "vertical_solve", "bound": "hbm", "achieved": solve_bytes / solve_ms / 1e6,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": solve_bytes / (solve_ms * 1e-3) / HBM_PEAK,
+                     "traffic": None, "algorithmic_bytes": solve_bytes},
         "step_roofline": {"bytes_per_update": b_h + b_v, "achieved": value * (b_h + b_v) / 1e9,
                           "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": value * (b_h + b_v) / HBM_PEAK},
         "kernels_ms": {k: v[0] / v[1] for k, v in prof.items()},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
     return 0
@@ -571,6 +594,45 @@ def bench_remap(args) -> int:
     return 0
 
 
+def cycle_cpu_baseline(nx, ny, nz, nt):
+    """The same ALE tracer cycle through the oracle on one host core, on an nx x ny x nz grid of the same
+    construction: the NumPy layer transport, the C loops of the FCT advection, the NumPy diffusion, and the C z*
+    regrid and PPM_H4 remap."""
+    from oracle import cref
+    from oracle import numpy_ref as R
+    from pyocean_amd import synth
+
+    a = synth.ale_inputs(nx, ny, nz, seed=0)
+    rng = np.random.default_rng(3)
+    C0 = 10.0 + rng.normal(0.0, 1.0, (nt, nz, ny, nx)) * a.active
+    act = a.active.astype(bool)
+    cx = a.Lu / np.concatenate([a.dx_t[:, :1], 0.5 * (a.dx_t[:, 1:] + a.dx_t[:, :-1]), a.dx_t[:, -1:]], 1)
+    cy = a.Lv / np.concatenate([a.dy_t[:1], 0.5 * (a.dy_t[1:] + a.dy_t[:-1]), a.dy_t[-1:]], 0)
+    okx = np.zeros((ny, nx + 1), bool)
+    okx[:, 1:-1] = act[:, 1:] & act[:, :-1]
+    oky = np.zeros((ny + 1, nx), bool)
+    oky[1:-1] = act[1:] & act[:-1]
+    M = ny * nx
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    t0 = time.perf_counter()
+    fl = R.ale_layer_transport(a.h, a.H, a.active, a.area, a.dx_t, a.dy_t, a.Lu, a.Lv, a.u_new, a.u_old, a.v_new,
+                               a.v_old, a.wet_u, a.wet_v, a.wu, a.wv, a.Du, a.Dv, a.dt, a.theta, a.theta_bc)
+    C1 = cref.advect_fct(C0, fl["V0"], fl["V1"], fl["Qx"], fl["Qy"], fl["ok"], a.dt)
+    hd = fl["h_dagger"]
+    C2 = R.diffuse_h_ale(C1, C1, hd, act, cx, cy, okx, oky, fl["V1"], 100.0, a.dt)
+    C2 = R.vertical_implicit(C2.reshape(nt, nz, M), np.maximum(hd, 1e-6).reshape(nz, M), np.full((nz + 1, M), 1e-4),
+                             np.zeros(M), a.dt).reshape(nt, nz, ny, nx)
+    eta = hd.sum(0) - a.H
+    h_new, wet = cref.regrid(a.H, eta, a.active, hd, "zstar", np.full(nz, 2000.0 / nz))
+    cref.remap_columns(hd, C2, h_new, cref.PPM_H4, wet)
+    tc = time.perf_counter() - t0
+    return {"value": nt * nz * ny * nx / tc, "unit": "tracer-cell updates/s", "cores": threads, "kind": "port",
+            "sample": f"one cycle of the oracle on a {nx}x{ny}x{nz} grid of the same construction with {nt} tracers "
+                      f"({tc:.1f} s): NumPy layer transport and diffusion on one core, C advection, regrid and "
+                      f"remap with OpenMP on {threads} threads"}
+
+
 def bench_cycle(args) -> int:
     """One ALE tracer cycle on the device, every phase on its own kernels and nothing leaving HBM: the layer
     transport (transport thicknesses, PPM fluxes with the Newton correction, outflow limiter, Lagrangian step), the
@@ -654,6 +716,9 @@ def bench_cycle(args) -> int:
     c0 = ops.content(C0, torch.where(d["active"][None], d["h"], torch.zeros_like(d["h"])) * d["area"][None])
     c1 = ops.content(Cn.contiguous(), V_new)
     rel = float(((c1 - c0).abs() / c0.abs()).max())
+    cpu = None
+    if not args.no_cpu:
+        cpu = cycle_cpu_baseline(min(nx, 384), min(ny, 384), nz, nt)
     res = {
         "metric": "ALE tracer cycle tracer-cell updates/s (layer transport, FCT advection, diffusion, z* regrid + "
                   "PPM_H4 remap)",
@@ -664,7 +729,7 @@ def bench_cycle(args) -> int:
         "config": {"workload": f"ALE tracer cycle regional {nx}x{ny}x{nz}, {nt} tracers"},
         "phase_ms": phase,
         "content_rel_change": rel,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
     return 0
