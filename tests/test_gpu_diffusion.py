@@ -69,3 +69,45 @@ def test_vertical_implicit_matches_the_oracle_bit_for_bit(gpu, rates, n, nt):
     ops.vertical_implicit(inplace, _d(dz), _d(nu), None if r is None else _d(r), dt,
                           None if tau is None else _d(tau), out=inplace)
     assert np.array_equal(inplace.cpu().numpy(), ref[0])
+
+
+def test_diffusion_at_full_size_conserves_and_keeps_bounds(gpu):
+    # the bench workload's size, 1024 x 1024 x 50 with 10 tracers, through size-independent properties: the
+    # along-layer fluxes telescope (the content Σ V c is unchanged to round-off) and a uniform field is left exactly
+    # as it is; the implicit vertical solve keeps each column's Σ dz c and stays inside the column's range
+    _say("diffusion 1024x1024x50 nt=10")
+    nt, n, ny, nx = 10, 50, 1024, 1024
+    g = torch.Generator(device="cuda").manual_seed(11)
+    f64 = dict(dtype=torch.float64, device="cuda")
+    act = torch.rand((ny, nx), generator=g, device="cuda") > 0.2
+    act8 = act.to(torch.uint8)
+    h = torch.where(act[None], 0.01 + 40.0 * torch.rand((n, ny, nx), generator=g, **f64), torch.zeros((), **f64))
+    V = h * 1.0e6
+    okx = torch.zeros((ny, nx + 1), dtype=torch.uint8, device="cuda")
+    okx[:, 1:-1] = act8[:, 1:] & act8[:, :-1]
+    oky = torch.zeros((ny + 1, nx), dtype=torch.uint8, device="cuda")
+    oky[1:-1] = act8[1:] & act8[:-1]
+    cx = 0.9 + 0.2 * torch.rand((ny, nx + 1), generator=g, **f64)
+    cy = 0.9 + 0.2 * torch.rand((ny + 1, nx), generator=g, **f64)
+    T = 10.0 + torch.randn((nt, n, ny, nx), generator=g, **f64) * act
+    dt = 300.0
+    out = ops.diffuse_h(T, T, h, act8, cx, cy, okx, oky, V, 100.0, dt)
+    c0, c1 = ops.content(T, V), ops.content(out, V)
+    scale = ops.content(T.abs(), V)
+    assert ((c1 - c0).abs() <= 1e-12 * scale).all(), ((c1 - c0).abs() / scale).max().item()
+    uni = torch.full((1, n, ny, nx), 7.25, **f64)
+    assert torch.equal(ops.diffuse_h(uni, uni, h, act8, cx, cy, okx, oky, V, 100.0, dt), uni)
+    del out, uni
+    M = ny * nx
+    u = T.view(nt, n, M)
+    dz = torch.clamp(h, min=1e-6).view(n, M)
+    nu = 1e-4 + 1e-2 * torch.rand((n + 1, M), generator=g, **f64)
+    o = ops.vertical_implicit(u, dz, nu, None, dt)
+    # wet columns only: a dry column's 1e-6 m layers under the same diffusivity make the tridiagonal system as
+    # ill-conditioned as dt nu / dz^2 ~ 1e12 (the reference's arithmetic, reproduced bit for bit, loses 1e-4 there)
+    w = act.view(M)
+    s0, s1 = (dz[None] * u).sum(1)[:, w], (dz[None] * o).sum(1)[:, w]
+    assert ((s1 - s0).abs() <= 1e-12 * (dz[None] * u.abs()).sum(1)[:, w]).all()
+    lo, hi = u.min(1, keepdim=True).values, u.max(1, keepdim=True).values
+    tol = 1e-12 * ((hi - lo) + hi.abs())
+    assert ((o >= lo - tol) & (o <= hi + tol))[:, :, w].all()
