@@ -81,7 +81,7 @@ size_t fct_layout(void* base, int nt, int nz, int ny, int nx, FctWs* w) {
 // launch shape of the single-domain calls (ocx_set_tuning): rows per chunk, waves per workgroup (0: default)
 std::atomic<int> g_rch{0}, g_wpb{0};
 constexpr int DEFAULT_RCH = 64;
-constexpr int DEFAULT_WPB = 2;    // two tracers of a layer per workgroup (measured best at 1024 x 1024 x 50 x 10)
+constexpr int DEFAULT_WPB = 1;    // one tracer of a layer per workgroup (measured best at 1024 x 1024 x 50 x 10)
 
 int rows_per_chunk() {
     const int r = g_rch.load();
