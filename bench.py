@@ -527,11 +527,25 @@ def bench_remap(args) -> int:
     rng = np.random.default_rng(2)
     C = torch.from_numpy(rng.normal(10.0, 2.0, (args.nt, nz, ny, nx))).cuda()
     dz = np.full(nz, 2000.0 / nz)
-    vc = vertical.VerticalCoordinate(H=d["H"], active=d["active"], rule="zstar", params={"dz_top": dz})
+    ts = {}
+    if args.rule == "hybrid":
+        # a stratified column: T warm at the surface and cold below (e-folding 500 m), S = 35; the layer densities
+        # are Wright's at 2e7 Pa, the targets span their range; the z* floor is a tenth of the nominal levels and
+        # a sigma window covers 10 .. 40 m of water
+        from pyocean_amd import eos
+        zc = torch.cumsum(h_src.flip(0), 0).flip(0) - 0.5 * h_src
+        ts = {"T": (4.0 + 16.0 * torch.exp(-zc / 500.0)).contiguous(), "S": torch.full_like(h_src, 35.0)}
+        e = eos.WrightFullEOS()
+        rho = e.potential_density(ts["T"], ts["S"], 2e7)
+        params = {"dz_top": dz / 10.0, "rho_target": np.linspace(float(rho.min()), float(rho.max()), nz + 1),
+                  "dsigma": np.full(nz, 1.0 / nz), "d_sigma": 10.0, "d_hybrid": 40.0, "keep_isopycnal": True}
+        vc = vertical.VerticalCoordinate(H=d["H"], active=d["active"], rule="hybrid", params=params, eos=e)
+    else:
+        vc = vertical.VerticalCoordinate(H=d["H"], active=d["active"], rule="zstar", params={"dz_top": dz})
     state = {}
 
     def step():
-        state["h"], state["C"] = vc.regrid_remap(eta, h_src, C)
+        state["h"], state["C"] = vc.regrid_remap(eta, h_src, C, **ts)
 
     for _ in range(max(args.warmup, 1)):
         step()
@@ -555,10 +569,11 @@ def bench_remap(args) -> int:
     # column shared by the fields (16 B per cell)
     b = (16.0 * args.nt + 16.0) * cells
     achieved = b / (kern["remap"] * 1e-3) / 1e9
-    traffic = read_traffic("remap", f"remap_regional{nx}x{ny}x{nz}nt{args.nt}")
+    # the traffic and instruction counters on file were taken with the z* rule
+    traffic = read_traffic("remap", f"remap_regional{nx}x{ny}x{nz}nt{args.nt}") if args.rule == "zstar" else None
     valu_busy = None
     cpath = ROOT / "profiles" / f"counters_remap_regional{nx}x{ny}x{nz}nt{args.nt}.txt"
-    if cpath.exists():
+    if cpath.exists() and args.rule == "zstar":
         ctr = dict(l.split()[:2] for l in cpath.read_text().splitlines() if l and not l.startswith("#"))
         # VALU wavefront instructions x 4 cycles over the SIMD-cycles of the measured launch time
         valu_busy = float(ctr["SQ_INSTS_VALU"]) * 4 / (1024 * 2.4e9 * kern["remap"] * 1e-3)
@@ -568,22 +583,29 @@ def bench_remap(args) -> int:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
         os.environ["OMP_NUM_THREADS"] = str(threads)
         hs, hn = h_src.cpu().numpy(), state["h"].cpu().numpy()
-        _, wet = cref.regrid(a.H, eta.cpu().numpy(), a.active, hs, "zstar", dz)
+        etan = eta.cpu().numpy()
         sample = np.ascontiguousarray(C.cpu().numpy())
         reps = 6
         t1 = time.perf_counter()
         for _ in range(reps):
+            if args.rule == "hybrid":
+                from oracle import numpy_ref as R
+                rho_np = R.wright_density(ts["T"].cpu().numpy(), ts["S"].cpu().numpy(), 2e7)
+                _, wet = cref.regrid_hybrid(a.H, etan, a.active, hs, rho_np, vc.params)
+            else:
+                _, wet = cref.regrid(a.H, etan, a.active, hs, "zstar", dz)
             cref.remap_columns(hs, sample, hn, cref.PPM_H4, wet)
         tc = time.perf_counter() - t1
         cpu = {"value": reps * args.nt * cells / tc, "unit": "field-cell remaps/s", "cores": threads, "kind": "port",
-               "sample": f"the same {args.nt}-field remap, {reps} repetitions, PPM_H4, the oracle's C loops over "
-                         f"columns (OpenMP) ({tc:.1f} s)"}
+               "sample": f"the same {args.rule} regrid and {args.nt}-field remap, {reps} repetitions, PPM_H4, the "
+                         f"oracle's C loops over columns (OpenMP) ({tc:.1f} s)"}
     res = {
-        "metric": "field-cell remaps/s (z* regrid + PPM_H4 conservative remap of the tracer stack)",
+        "metric": f"field-cell remaps/s ({'z*' if args.rule == 'zstar' else 'hybrid'} regrid + PPM_H4 conservative "
+                  f"remap of the tracer stack)",
         "value": value, "unit": "field-cell remaps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded regional grid after one layer-transport step)",
-        "config": {"workload": f"regrid_remap regional {nx}x{ny}x{nz}, {args.nt} fields, z*, PPM_H4"},
+        "config": {"workload": f"regrid_remap regional {nx}x{ny}x{nz}, {args.nt} fields, {args.rule}, PPM_H4"},
         "roofline": {"kernel": "remap", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic, "algorithmic_bytes": b,
                      "valu_busy_from_counters": valu_busy},
@@ -752,6 +774,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-tracers", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--rule", choices=("zstar", "hybrid"), default="zstar",
+                    help="--form remap: the regrid rule (hybrid: Wright densities of a stratified column)")
     ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion", "ale", "remap", "cycle"),
                     default="step",
                     help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "

@@ -1104,36 +1104,83 @@ __global__ void k_regrid_hybrid(HybridArgs r) {
                 if (k < n) xg += h_top(k);
             }
         }
-        for (int kk = 1; kk < m; ++kk) {
-            const double t = r.rho_target[kk];
-            if (t <= e[0]) {
-                zw[kk] = 0.0;
-                continue;
-            }
-            bool found = false;
-            double xg = h_top(0);
-            for (int k = 1; k < n; ++k) {
-                if (t >= e[2 * (k - 1) + 1] && t <= e[2 * k]) {
-                    zw[kk] = xg;
-                    found = true;
-                    break;
+        // The edges E_0 .. E_n (E_k = e[2k], E_n = e[2n-1]; the averaging made e[2k-1] = e[2k]) of a column are
+        // monotone after the running minimum (each averaged edge lies between its two layers' densities).  Then
+        // the first match of each of the reference's scans is found by bisection: t equal to an inner edge, or
+        // strictly inside one layer.  A column whose edges are not monotone takes the scans themselves.
+        auto E = [&](int k) -> double { return k < n ? e[2 * k] : e[2 * n - 1]; };
+        bool mono = true;
+        for (int k = 0; k < n; ++k) mono = mono && E(k) <= E(k + 1);
+        if (mono) {
+            const WCol xgv = sig;                             // the input grid's positions (sig is rebuilt later)
+            {
+                double xg = 0.0;
+                for (int k = 0; k <= n; ++k) {
+                    xgv[k] = xg;
+                    if (k < n) xg += h_top(k);
                 }
-                xg += h_top(k);
             }
-            if (found) continue;
-            if (t >= e[2 * (n - 1) + 1]) {
-                zw[kk] = xgn;
-                continue;
-            }
-            xg = 0.0;
-            for (int k = 0; k < n; ++k) {
-                const double el = e[2 * k], er = e[2 * k + 1];
-                if (el < t && t < er) {
-                    const double xi = (t - el) / (er - el);
-                    zw[kk] = xg + pmin(pmax(xi, 0.0), 1.0) * h_top(k);
-                    break;
+            const double E0 = E(0), En = E(n);
+            for (int kk = 1; kk < m; ++kk) {
+                const double t = r.rho_target[kk];
+                if (!(t == t)) continue;                      // NaN matches no case: the input grid stays
+                if (t <= E0) {
+                    zw[kk] = 0.0;
+                    continue;
                 }
-                xg += h_top(k);
+                int lo = 1, hi = n;                           // the first inner edge >= t (n: none)
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (E(mid) < t)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (lo < n && E(lo) == t) {
+                    zw[kk] = xgv[lo];
+                    continue;
+                }
+                if (t >= En) {
+                    zw[kk] = xgn;
+                    continue;
+                }
+                const int k = lo - 1;                         // E_k < t < E_k+1
+                const double el = E(k), er = E(k + 1);
+                const double xi = (t - el) / (er - el);
+                zw[kk] = xgv[k] + pmin(pmax(xi, 0.0), 1.0) * h_top(k);
+            }
+        } else {
+            for (int kk = 1; kk < m; ++kk) {
+                const double t = r.rho_target[kk];
+                if (t <= e[0]) {
+                    zw[kk] = 0.0;
+                    continue;
+                }
+                bool found = false;
+                double xg = h_top(0);
+                for (int k = 1; k < n; ++k) {
+                    if (t >= e[2 * (k - 1) + 1] && t <= e[2 * k]) {
+                        zw[kk] = xg;
+                        found = true;
+                        break;
+                    }
+                    xg += h_top(k);
+                }
+                if (found) continue;
+                if (t >= e[2 * (n - 1) + 1]) {
+                    zw[kk] = xgn;
+                    continue;
+                }
+                xg = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    const double el = e[2 * k], er = e[2 * k + 1];
+                    if (el < t && t < er) {
+                        const double xi = (t - el) / (er - el);
+                        zw[kk] = xg + pmin(pmax(xi, 0.0), 1.0) * h_top(k);
+                        break;
+                    }
+                    xg += h_top(k);
+                }
             }
         }
         if (r.keep) {
