@@ -234,3 +234,41 @@ def test_hybrid_regrid_remap_follows_the_equation_of_state(gpu):
         _eq(h_new, r_new, f"{e.name} h_new")
         _eq(vc.regrid(_d(eta), _d(h), T=_d(T), S=_d(S)), r_new, f"{e.name} regrid")
         _eq(Cn, cref.remap_columns(h, C, r_new, cref.PPM_H4, r_wet), f"{e.name} C remapped")
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_hybrid_positions_match_the_oracle_on_exact_hits_and_odd_targets(gpu, seed):
+    # the kernel finds the target positions by bisection on monotone edges, where the reference scans: densities in
+    # constant blocks make the edges equal to the block values, which the targets then hit exactly; some targets are
+    # unsorted, NaN or outside the range, some sources noisy (non-monotone before the running minimum) or unstratified
+    _say("regrid hybrid positions", seed)
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.choice([2, 3, 7, 20, 50]))
+    ny, nx = 11, 13
+    H = rng.uniform(5.0, 300.0, (ny, nx))
+    eta = rng.uniform(-0.2, 0.2, (ny, nx))
+    act = rng.random((ny, nx)) > 0.1
+    frac = rng.uniform(0, 1, (n, ny, nx))
+    frac[rng.random(frac.shape) < 0.3] = 0.0
+    tot = frac.sum(0)
+    tot[tot == 0] = 1.0
+    h = frac / tot * (H + eta)[None]
+    h = np.where(h < 2e-3, rng.choice([0.0, 1e-3, 1.5e-3], h.shape), h)
+    vals = np.sort(rng.choice([1024.0, 1025.0, 1025.5, 1026.0, 1027.25], 4))[::-1]
+    rho = vals[np.sort(rng.integers(0, 4, n))][:, None, None] * np.ones((1, ny, nx))
+    if seed % 3 == 0:
+        rho = rho + rng.normal(0, 0.3, rho.shape)
+    rho[:, 0, 0] = 1025.0
+    cand = np.concatenate([vals, 0.5 * (vals[1:] + vals[:-1]), rng.uniform(1023, 1028, 8), [1020.0, 1030.0]])
+    targets = np.sort(rng.choice(cand, n + 1))
+    if seed % 4 == 1:
+        targets = rng.permutation(targets)
+    if seed % 4 == 2:
+        targets[n // 2] = np.nan
+    params = dict(dz_top=rng.uniform(0.5, 5.0, n), dsigma=np.full(n, 1.0 / n), rho_target=targets,
+                  max_h_top=np.where(rng.random(n) < 0.3, 30.0, np.inf), d_sigma=10.0,
+                  d_hybrid=60.0 if seed % 2 else 0.0, keep_isopycnal=bool(seed % 3))
+    h_new, wet = vertical.regrid(_d(H), _d(eta), _d(act), _d(h), "hybrid", params, rho=_d(rho))
+    r_new, r_wet = cref.regrid_hybrid(H, eta, act, h, rho, params)
+    _eq(wet, r_wet.astype(np.uint8), "wet")
+    _eq(h_new, r_new, "h_new")
