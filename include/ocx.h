@@ -242,6 +242,12 @@ int ocx_ppm_transport_f64(const double* h, const double* ut, const double* L, co
                           void* stream);
 int ocx_ppm_transport_iterations(const void* workspace, int n, int ny, int nx, int axis, int max_iter, void* stream);
 
+/* Layer volume fluxes with the donor-cell (upwind) thickness, corrected to the free-surface flux Q_bt: the
+ * thickness_flux "upwind" option of the layer transport.  ut, q_pos, q_neg, out: face stacks of the axis in their
+ * natural layout; L, Q_bt: one layer of faces.  Replaces upwind_transport_x (core/vertical/ale.py:233-251). */
+int ocx_upwind_transport_f64(const double* ut, const double* L, const double* Q_bt, const double* q_pos,
+                             const double* q_neg, int n, int ny, int nx, int axis, double* out, void* stream);
+
 /* Outflow limiter, in place on Qx and Qy; host-synchronous (the number of passes depends on the data).
  * Replaces limit_outflow / _give_back / _outflow_exceeds (core/vertical/ale.py:299-437).  check_only: only test
  * whether any active cell-layer sends out more than it holds.  status: OCX_OUTFLOW_*; n_limited: cell-layers scaled

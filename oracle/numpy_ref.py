@@ -762,8 +762,25 @@ def transport_thickness(h, H, active, clip=True, periodic=(False, False)):
     return x, y
 
 
+def upwind_transport_x(q_e, q_w, ut, L, Q_bt):
+    """
+    Layer volume fluxes through the u-faces with the donor-cell thickness, corrected to the free-surface flux:
+    Q_k = L u_k q_k(u_k) + R q_k(R) / sum_j q_j(R) with R = Q_bt - sum_k L u_k q_k(u_k), q(s) the transport
+    thickness on the side the water comes from (the other side's where that column is empty).  Follows
+    ``upwind_transport_x`` (core/vertical/ale.py:233-251) term for term.  For the v-faces call it on swapped axes.
+    """
+    F = L[None] * ut * np.where(ut >= 0, q_e, q_w)
+    R = Q_bt - F.sum(axis=0)
+    qR = np.where((R >= 0)[None], q_e, q_w)
+    alt = np.where((R >= 0)[None], q_w, q_e)
+    qR = np.where((qR.sum(axis=0) > 0)[None], qR, alt)
+    tot = qR.sum(axis=0)
+    return F + R[None] * qR / np.where(tot > 0, tot, 1.0)[None]
+
+
 def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_new, v_old, wet_u, wet_v, wu, wv,
-                        Du, Dv, dt, theta, theta_bc=None, clip=True, max_iter=20, periodic=(False, False)):
+                        Du, Dv, dt, theta, theta_bc=None, clip=True, max_iter=20, periodic=(False, False),
+                        thickness_flux="ppm"):
     """
     One ALE layer-transport step with PPM thickness: transport velocities, transport thicknesses (the vertical
     coordinate's, clipped at the shallower bed for z*), PPM fluxes of both directions (the v-faces on swapped axes),
@@ -772,7 +789,8 @@ def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_
     ``transport_thickness`` (core/components/vertical_coordinate.py:125-140) and ``layer_volumes``
     (core/dynamics/sigma_baroclinic.py:1812-1816).  On periodic axes the PPM fluxes run on inputs padded by 4
     wrapped cells and are cropped (layer_transport.py:122-129, core/model/periodic.py:71-103).  Returns
-    dict(Qx, Qy, V0, V1, ok, h_dagger, n_limited).
+    dict(Qx, Qy, V0, V1, ok, h_dagger, n_limited).  ``thickness_flux`` "upwind": the donor-cell fluxes of
+    ``upwind_transport_x`` instead of PPM (layer_transport.py:131-133), with no padding.
     """
     act = np.asarray(active, bool)
     px, py = periodic
@@ -781,7 +799,10 @@ def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_
     (qe, qw), (qn_nat, qs_nat) = transport_thickness(h, H, act, clip, periodic)
     qn, qs = qn_nat.swapaxes(1, 2), qs_nat.swapaxes(1, 2)
     sw = lambda a: a.swapaxes(1, 2)                                  # noqa: E731
-    if px or py:
+    if thickness_flux == "upwind":
+        Qx = upwind_transport_x(qe, qw, utx, Lu, Qbx)
+        Qy = upwind_transport_x(qn, qs, sw(uty), Lv.T, Qby.T).swapaxes(1, 2)
+    elif px or py:
         P = lambda a, k: pad(a, k, 4, px, py)                        # noqa: E731
         hp, ap = P(h, "c"), P(act, "c")
         Qx = crop(ppm_transport_x(hp, P(utx, "u"), P(Lu, "u"), P(Qbx, "u"), P(qe, "u"), P(qw, "u"), ap, dt,

@@ -67,6 +67,7 @@ SIGNATURES = {
     "ocx_regrid_hybrid_workspace_bytes": (Z, [I, L]),
     "ocx_regrid_hybrid_f64": (I, [P, P, P, P, P, I, L, P, P, P, P, D, D, D, I, P, P, P, Z, P, P]),
     "ocx_regrid_density_f64": (I, [I, P, P, P, D, P, L, P, P]),
+    "ocx_upwind_transport_f64": (I, [P, P, P, P, P, I, I, I, I, P, P]),
     "ocx_batch_create": (I, [P, I, I, I, I, P]),
     "ocx_batch_step": (I, [P, I, P]),
     "ocx_batch_graph_step": (I, [P, I, P]),

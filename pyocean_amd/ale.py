@@ -200,6 +200,39 @@ def ppm_transport_x(h, ut, L, Q_bt, q_e, q_w, active, dt: float, dx_t, max_iter:
     return ppm_transport(h, ut, L, Q_bt, q_e, q_w, active, dt, dx_t, 0, max_iter)
 
 
+def upwind_transport(ut, L, Q_bt, q_pos, q_neg, axis: int, out=None):
+    """
+    Layer volume fluxes through the faces of ``axis`` with the donor-cell thickness, corrected to the free-surface
+    flux Q_bt (the ``thickness_flux="upwind"`` option), in natural layouts: ut, q_pos, q_neg, out (n, ny, nx+1) for
+    axis 0 or (n, ny+1, nx) for axis 1; L, Q_bt one layer of faces.  core/vertical/ale.py:233-251.
+    """
+    n = ut.shape[0]
+    fy, fx = ut.shape[1:]
+    ny, nx = (fy, fx - 1) if axis == 0 else (fy - 1, fx)
+    fshape = (n, fy, fx)
+    u = _f64(ut, fshape, "ut")
+    Lc, Qb = _f64(L, fshape[1:], "L"), _f64(Q_bt, fshape[1:], "Q_bt")
+    qp, qn = _f64(q_pos, fshape, "q_pos"), _f64(q_neg, fshape, "q_neg")
+    o = torch.empty(fshape, dtype=_F64, device=u.device) if out is None else out
+    if tuple(o.shape) != fshape or o.dtype != _F64 or not o.is_contiguous():
+        raise ValueError(f"out: a contiguous float64 tensor of shape {fshape}")
+    check(lib().ocx_upwind_transport_f64(_ptr(u), _ptr(Lc), _ptr(Qb), _ptr(qp), _ptr(qn), n, ny, nx, int(axis),
+                                         _ptr(o), _stream()), "ocx_upwind_transport_f64")
+    return o
+
+
+def upwind_transport_x(q_e, q_w, ut, L, Q_bt):
+    """
+    Layer volume fluxes through the u-faces with the donor-cell thickness (the reference's signature and result,
+    core/vertical/ale.py:233-251).  Called as the reference calls it for the v-faces - q_n, q_s, ut in the
+    (n, nx, ny+1) orientation, Lv.T, Qby.T - it runs on the natural v-face arrays and returns the swapped view.
+    """
+    if _swapped(ut):
+        out = upwind_transport(_natural(ut), _natural(L), _natural(Q_bt), _natural(q_e), _natural(q_w), 1)
+        return out.transpose(1, 2)
+    return upwind_transport(ut, L, Q_bt, q_e, q_w, 0)
+
+
 # ------------------------------------------------------------------------------------------------ outflow limiter
 def _outflow_call(h, Qx, Qy, area, dt, active, h_floor, max_pass, periodic, check_only):
     n, ny, nx = h.shape
@@ -280,14 +313,18 @@ class LayerFluxes:
 class LayerTransport:
     """
     The ALE layer-transport step on the device (``ALELayerTransport.__call__``, core/components/layer_transport.py:
-    111-156, PPM thickness flux, no open boundaries or freshwater): per call the transport velocities of both face
-    directions, the PPM fluxes with their Newton correction, the outflow limiter, the Lagrangian thickness and the
-    layer volumes.  Static geometry is given once: Lu (ny, nx+1), Lv (ny+1, nx) face lengths; area, active,
+    111-156, no open boundaries or freshwater): per call the transport velocities of both face directions, the PPM
+    fluxes with their Newton correction (or, with ``thickness_flux="upwind"``, the donor-cell fluxes), the outflow
+    limiter, the Lagrangian thickness and the layer volumes.  Static geometry is given once: Lu (ny, nx+1), Lv (ny+1, nx) face lengths; area, active,
     dx_t, dy_t (ny, nx); dt; theta (free-surface time weight).
     """
 
     def __init__(self, *, Lu, Lv, area, active, dx_t, dy_t, dt: float, theta: float, max_iter: int = 20,
-                 h_floor: float = H_PPM_MIN, max_pass: int = 50, periodic: tuple[bool, bool] = (False, False)):
+                 h_floor: float = H_PPM_MIN, max_pass: int = 50, periodic: tuple[bool, bool] = (False, False),
+                 thickness_flux: str = "ppm"):
+        if thickness_flux not in ("ppm", "upwind"):
+            raise ValueError(f"thickness_flux {thickness_flux!r}: 'ppm' or 'upwind'")
+        self.thickness_flux = thickness_flux
         ny, nx = area.shape
         self.Lu, self.Lv = _f64(Lu, (ny, nx + 1), "Lu"), _f64(Lv, (ny + 1, nx), "Lv")
         self.area, self.dx_t, self.dy_t = (_f64(x, (ny, nx), nm) for x, nm in ((area, "area"), (dx_t, "dx_t"),
@@ -307,7 +344,13 @@ class LayerTransport:
         utx, Qbx = transport_velocity(u_new, u_old, wet_u, wu, self.Lu, Du, self.theta, theta_bc)
         uty, Qby = transport_velocity(v_new, v_old, wet_v, wv, self.Lv, Dv, self.theta, theta_bc)
         px, py = self.periodic
-        if px or py:
+        if self.thickness_flux == "upwind":
+            # donor-cell thickness, face by face: no padding (layer_transport.py:131-133)
+            Qx = upwind_transport(utx, self.Lu, Qbx, qe, qw, 0)
+            Qy = upwind_transport(uty, self.Lv, Qby, qn, qs, 1)
+            if count_iterations:
+                Qx, Qy = (Qx, 0), (Qy, 0)
+        elif px or py:
             # the PPM stencils on inputs padded by 4 wrapped cells, the fluxes cropped (layer_transport.py:122-129)
             def P(a, kind):
                 return periodic_pad(a, kind, 4, px, py)
@@ -326,7 +369,7 @@ class LayerTransport:
         if count_iterations:
             (Qx, ix), (Qy, iy) = Qx, Qy
             self.iterations = (ix, iy)
-        if px or py:
+        if (px or py) and self.thickness_flux == "ppm":
             Qx = periodic_pad(Qx, "u", 4, px, py, crop=True)
             Qy = periodic_pad(Qy, "v", 4, px, py, crop=True)
         Qx, Qy, n_lim = limit_outflow(h, Qx, Qy, self.area, self.dt, self.active, self.h_floor, self.max_pass,

@@ -285,6 +285,24 @@ int ocx_ppm_transport_f64(const double* h, const double* ut, const double* L, co
     return ppm_impl<1>(h, ut, L, Q_bt, q_pos, q_neg, active, d_t, n, ny, nx, dt, max_iter, out, w, s);
 }
 
+int ocx_upwind_transport_f64(const double* ut, const double* L, const double* Q_bt, const double* q_pos,
+                             const double* q_neg, int n, int ny, int nx, int axis, double* out, void* stream) {
+    OCX_TRY(check_ale_shape(n, ny, nx));
+    OCX_TRY(check_axis(axis));
+    const void* p[] = {ut, L, Q_bt, q_pos, q_neg, out};
+    const char* nm[] = {"ut", "L", "Q_bt", "q_pos", "q_neg", "out"};
+    for (int q = 0; q < 6; ++q) OCX_TRY(need(p[q], nm[q]));
+    if (out == ut || out == q_pos || out == q_neg) return fail(OCX_E_ARG, "out must not alias ut, q_pos or q_neg");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ProfScope _ps("upwind_transport", s);
+    const unsigned gb = blocks_for(face_cols(axis, ny, nx));
+    if (axis == 0)
+        k_upwind_transport<0><<<gb, BS, 0, s>>>(ut, L, Q_bt, q_pos, q_neg, Geo<0>{n, ny, nx}, out);
+    else
+        k_upwind_transport<1><<<gb, BS, 0, s>>>(ut, L, Q_bt, q_pos, q_neg, Geo<1>{n, ny, nx}, out);
+    return launch_check("upwind_transport");
+}
+
 int ocx_ppm_transport_iterations(const void* workspace, int n, int ny, int nx, int axis, int max_iter,
                                  void* stream) {
     if (check_ale_shape(n, ny, nx) != OCX_OK || check_axis(axis) != OCX_OK) return OCX_E_ARG;

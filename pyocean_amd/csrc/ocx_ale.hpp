@@ -408,6 +408,42 @@ __global__ void k_ppm_final(PpmArgs<AX> p) {
     }
 }
 
+// ------------------------------------------------------------------------------------------ upwind thickness flux
+// upwind_transport_x (core/vertical/ale.py:233-251), one thread per face column: F_k = L u_k q_k with the transport
+// thickness of the side the water comes from, R = Q_bt - sum_k F_k, and R shared in proportion to the transport
+// thicknesses on R's side (the other side where that column's sum is not positive).  The sums run from layer 0
+// up and start from the first term, as NumPy's sum over the first axis does.  Face arrays are indexed by the face
+// column t directly (t is the face's index in its layer plane for both axes).
+template <int AX>
+__global__ void k_upwind_transport(const double* __restrict__ ut, const double* __restrict__ L,
+                                   const double* __restrict__ Qb, const double* __restrict__ qe,
+                                   const double* __restrict__ qw, Geo<AX> g, double* __restrict__ out) {
+    const long long t = gtid();
+    if (t >= g.cols()) return;
+    const size_t PF = g.pf();
+    const double Lf = L[t];
+    double s = 0.0, se = 0.0, sw = 0.0;
+    for (int k = 0; k < g.n; ++k) {
+        const size_t x = k * PF + t;
+        const double u = ut[x], e = qe[x], w = qw[x];
+        const double F = Lf * u * (u >= 0.0 ? e : w);
+        out[x] = F;
+        s = k == 0 ? F : s + F;
+        se = k == 0 ? e : se + e;
+        sw = k == 0 ? w : sw + w;
+    }
+    const double R = Qb[t] - s;
+    bool use_e = R >= 0.0;
+    if (!((use_e ? se : sw) > 0.0)) use_e = !use_e;
+    const double tot = use_e ? se : sw;
+    const double den = tot > 0.0 ? tot : 1.0;
+    const double* q = use_e ? qe : qw;
+    for (int k = 0; k < g.n; ++k) {
+        const size_t x = k * PF + t;
+        out[x] = out[x] + R * q[x] / den;
+    }
+}
+
 // ------------------------------------------------------------------------------------------ outflow limiter
 struct OutArgs {
     const double *h, *area;

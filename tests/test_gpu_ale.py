@@ -300,3 +300,51 @@ def test_periodic_layer_transport_matches_the_oracle_bit_for_bit(gpu, periodic):
     assert out.n_limited == r["n_limited"]
     for k in ("Qx", "Qy", "V1", "h_dagger"):
         _eq(getattr(out, k), r[k], k)
+
+
+# ------------------------------------------------------------------------------------------------ upwind thickness flux
+def test_upwind_transport_matches_the_oracle_in_both_directions(gpu, regional):
+    _say("upwind transport")
+    a, r = regional
+    utx, Qbx = R.transport_velocity(a.u_new, a.u_old, a.wet_u, a.wu, a.Lu, a.Du, a.theta, a.theta_bc)
+    uty, Qby = R.transport_velocity(a.v_new, a.v_old, a.wet_v, a.wv, a.Lv, a.Dv, a.theta, a.theta_bc)
+    (qe, qw), (qn, qs) = R.transport_thickness(a.h, a.H, a.active, True)
+    refx = R.upwind_transport_x(qe, qw, utx, a.Lu, Qbx)
+    refy = R.upwind_transport_x(qn.swapaxes(1, 2), qs.swapaxes(1, 2), uty.swapaxes(1, 2), a.Lv.T, Qby.T)
+    _eq(ale.upwind_transport(_d(utx), _d(a.Lu), _d(Qbx), _d(qe), _d(qw), 0), refx, "Qx upwind")
+    _eq(ale.upwind_transport(_d(uty), _d(a.Lv), _d(Qby), _d(qn), _d(qs), 1), np.ascontiguousarray(refy.swapaxes(1, 2)),
+        "Qy upwind")
+    # the reference's calling convention for the v-faces (swapped views in and out)
+    got = ale.upwind_transport_x(_d(qn).transpose(1, 2), _d(qs).transpose(1, 2), _d(uty).transpose(1, 2),
+                                 _d(a.Lv).T, _d(Qby).T)
+    _eq(got, refy, "Qy upwind (swapped)")
+    # the reference's lock exchange (test_vertical_ale.py:116-127)
+    h = np.array([[[8.0, 0.0]], [[0.0, 8.0]]])
+    le, lw = R.face_transport_thickness(h, h.sum(0), np.ones((1, 2), bool), True)
+    ut = np.zeros((2, 1, 3))
+    ut[:, 0, 1] = [0.2, -0.2]
+    _eq(ale.upwind_transport_x(_d(le), _d(lw), _d(ut), _d(np.full((1, 3), 100.0)), _d(np.zeros((1, 3)))),
+        R.upwind_transport_x(le, lw, ut, np.full((1, 3), 100.0), np.zeros((1, 3))), "lock exchange")
+
+
+@pytest.mark.parametrize("periodic", [(False, False), (True, True)])
+def test_upwind_layer_transport_matches_the_oracle_bit_for_bit(gpu, periodic):
+    _say("layer transport upwind", periodic)
+    a = synth.ale_inputs(130, 90, 20, seed=2)
+    r = R.ale_layer_transport(a.h, a.H, a.active, a.area, a.dx_t, a.dy_t, a.Lu, a.Lv, a.u_new, a.u_old, a.v_new,
+                              a.v_old, a.wet_u, a.wet_v, a.wu, a.wv, a.Du, a.Dv, a.dt, a.theta, a.theta_bc,
+                              periodic=periodic, thickness_flux="upwind")
+    d = {k: _d(v) for k, v in vars(a).items() if isinstance(v, np.ndarray)}
+    qe, qw = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 0, periodic=periodic)
+    qn, qs = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 1, periodic=periodic)
+    lt = ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
+                            dy_t=d["dy_t"], dt=a.dt, theta=a.theta, periodic=periodic, thickness_flux="upwind")
+    out = lt(d["h"], d["u_new"], d["u_old"], d["v_new"], d["v_old"], d["wet_u"], d["wet_v"], d["wu"], d["wv"],
+             d["Du"], d["Dv"], qe, qw, qn, qs, theta_bc=a.theta_bc, count_iterations=True)
+    assert out.n_limited == r["n_limited"] and lt.iterations == (0, 0)
+    for k in ("Qx", "Qy", "V0", "V1", "h_dagger"):
+        _eq(getattr(out, k), r[k], k)
+    _eq(out.ok, r["ok"].astype(np.uint8), "ok")
+    with pytest.raises(ValueError, match="thickness_flux"):
+        ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
+                           dy_t=d["dy_t"], dt=a.dt, theta=a.theta, thickness_flux="weno")

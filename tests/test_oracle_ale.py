@@ -399,3 +399,78 @@ def test_periodic_layer_transport_is_shift_equivariant_and_keeps_the_wrap_face_o
     scale = np.abs(r0["Qx"]).max()
     np.testing.assert_allclose(r1["Qx"], rc(r0["Qx"], "u"), rtol=0, atol=1e-9 * scale)
     np.testing.assert_allclose(r1["h_dagger"], rc(r0["h_dagger"]), rtol=0, atol=1e-9)
+
+
+# ------------------------------------------------------------------------------------------ upwind thickness flux
+def _qt(h, H=None, clip=False, active=None):
+    H = h.sum(axis=0) if H is None else H
+    active = np.ones(H.shape, bool) if active is None else active
+    return R.face_transport_thickness(h, H, active, clip)
+
+
+def test_upwind_transport_takes_the_thickness_of_the_cell_the_water_comes_from():
+    # lock exchange (test_vertical_ale.py:116-127): the dense layer of the left cell flows right, the light layer of
+    # the right cell flows left, with no net flux; min(h_L, h_R) would give no transport at all
+    h = np.array([[[8.0, 0.0]], [[0.0, 8.0]]])
+    qe, qw = _qt(h, clip=True)
+    ut = np.zeros((2, 1, 3))
+    ut[:, 0, 1] = [0.2, -0.2]
+    Q = R.upwind_transport_x(qe, qw, ut, np.full((1, 3), 100.0), np.zeros((1, 3)))
+    np.testing.assert_allclose(Q[:, 0, 1], [100.0 * 0.2 * 8.0, -100.0 * 0.2 * 8.0])
+    assert (Q[:, 0, [0, 2]] == 0).all()
+
+
+def test_upwind_transport_adds_up_to_the_free_surface_flux_without_emptying_a_vanished_layer():
+    # test_vertical_ale.py:130-146: the residual goes to the layers of its upwind cell in proportion, so a layer
+    # empty there receives none, and the layers add up to Q_bt; where all velocities vanish the whole flux is the
+    # residual, split exactly as the thicknesses
+    rng = np.random.default_rng(3)
+    h = rng.uniform(0.5, 5.0, (5, 2, 4))
+    h[1, :, 1] = 0.0
+    qe, qw = _qt(h)
+    ut = rng.normal(0.0, 0.1, (5, 2, 5))
+    ut[:, :, 2] = 0.0
+    Qbt = rng.normal(0.0, 20.0, (2, 5))
+    Qbt[:, 2] = np.abs(Qbt[:, 2]) + 1.0
+    Q = R.upwind_transport_x(qe, qw, ut, np.full((2, 5), 50.0), Qbt)
+    np.testing.assert_allclose(Q.sum(0), Qbt, rtol=1e-12, atol=1e-12)
+    assert (Q[1, :, 2] == 0).all()
+    np.testing.assert_allclose(Q[:, :, 2] / Qbt[None, :, 2], h[:, :, 1] / h[:, :, 1].sum(0), rtol=1e-14)
+    # a westward residual through a face whose western column is empty goes to the eastern layers instead
+    qe2, qw2 = qe.copy(), qw.copy()
+    qw2[:, 0, 3] = 0.0
+    Qb2 = Qbt.copy()
+    Qb2[0, 3] = -abs(Qb2[0, 3]) - 5.0
+    u2 = np.zeros_like(ut)
+    Q2 = R.upwind_transport_x(qe2, qw2, u2, np.full((2, 5), 50.0), Qb2)
+    np.testing.assert_allclose(Q2[:, 0, 3], Qb2[0, 3] * qe2[:, 0, 3] / qe2[:, 0, 3].sum(), rtol=1e-14)
+
+
+def test_upwind_transport_is_zero_next_to_land():
+    # test_vertical_ale.py:149-154
+    h = np.full((2, 1, 3), 4.0)
+    qe, qw = _qt(h, active=np.array([[True, False, True]]))
+    Q = R.upwind_transport_x(qe, qw, np.full((2, 1, 4), 0.1), np.full((1, 4), 10.0), np.full((1, 4), 8.0))
+    assert (Q[:, 0, 1:3] == 0).all()
+
+
+def test_upwind_layer_transport_closes_each_cells_volume_budget():
+    # the whole layer transport with the upwind option (layer_transport.py:131-133): every cell-layer's new volume
+    # is the old one minus dt times its flux divergence, the layers of a face add up to the free-surface flux, and
+    # the fluxes differ from the PPM option's
+    from pyocean_amd import synth
+    a = synth.ale_inputs(24, 20, 6, seed=4)
+    args = (a.h, a.H, a.active, a.area, a.dx_t, a.dy_t, a.Lu, a.Lv, a.u_new, a.u_old, a.v_new, a.v_old, a.wet_u,
+            a.wet_v, a.wu, a.wv, a.Du, a.Dv, a.dt, a.theta, a.theta_bc)
+    up = R.ale_layer_transport(*args, thickness_flux="upwind")
+    pp = R.ale_layer_transport(*args)
+    divQ = up["Qx"][..., 1:] - up["Qx"][..., :-1] + up["Qy"][:, 1:] - up["Qy"][:, :-1]
+    act = a.active.astype(bool)
+    resid = (up["V1"] - up["V0"] + a.dt * divQ)[:, act]
+    assert np.abs(resid).max() <= 1e-9 * np.abs(up["V0"]).max()
+    utx, Qbx = R.transport_velocity(a.u_new, a.u_old, a.wet_u, a.wu, a.Lu, a.Du, a.theta, a.theta_bc)
+    (qe, qw), _ = R.transport_thickness(a.h, a.H, act)
+    Qx = R.upwind_transport_x(qe, qw, utx, a.Lu, Qbx)
+    both = (qe.sum(0) > 0) | (qw.sum(0) > 0)
+    np.testing.assert_allclose(Qx.sum(0)[both], Qbx[both], rtol=1e-12, atol=1e-9 * np.abs(Qbx).max())
+    assert not np.array_equal(up["Qx"], pp["Qx"])
