@@ -248,6 +248,26 @@ int ocx_ppm_transport_iterations(const void* workspace, int n, int ny, int nx, i
 int ocx_upwind_transport_f64(const double* ut, const double* L, const double* Q_bt, const double* q_pos,
                              const double* q_neg, int n, int ny, int nx, int axis, double* out, void* stream);
 
+/* Freshwater volumes: dV (ny, nx) joins the top layer (k = n-1) of V (n, ny, nx) into out; a removal that would
+ * take the top layer below the floor (floor_cols (ny, nx), or NULL for the scalar floor) is taken from the layers
+ * below, top-down, each first kept at the floor, then down to zero.  n_limited (host, optional: a host
+ * synchronisation) counts those columns.  Replaces freshwater_volumes (core/components/freshwater.py:31-58).
+ * workspace: ocx_freshwater_workspace_bytes(). */
+size_t ocx_freshwater_workspace_bytes(void);
+int ocx_freshwater_volumes_f64(const double* V, const double* dV, const double* floor_cols, double floor, int n,
+                               int ny, int nx, double* out, void* workspace, size_t workspace_bytes,
+                               long long* n_limited, void* stream);
+
+/* Freshwater substep on the tracer stack (nt, n, ny, nx), in place: every member but T (t_index) and T^2 (t2_index,
+ * -1 if absent) keeps its content, c V_adv / V_fw, where fw_area_dt (ny, nx) is nonzero; columns whose volume
+ * changed below the top layer share one concentration over the touched layers.  var_surf (n, ny, nx, optional)
+ * the salinity-variance source of member s_index.  n_limited (host, optional) counts the columns with a change
+ * below the top layer.  Replaces apply_freshwater (core/components/freshwater.py:61-122). */
+int ocx_apply_freshwater_f64(double* stack, int nt, int n, int ny, int nx, int t_index, int t2_index, int s_index,
+                             const double* V_adv, const double* V_fw, const double* fw_area_dt, double dt,
+                             double* var_surf, void* workspace, size_t workspace_bytes, long long* n_limited,
+                             void* stream);
+
 /* Outflow limiter, in place on Qx and Qy; host-synchronous (the number of passes depends on the data).
  * Replaces limit_outflow / _give_back / _outflow_exceeds (core/vertical/ale.py:299-437).  check_only: only test
  * whether any active cell-layer sends out more than it holds.  status: OCX_OUTFLOW_*; n_limited: cell-layers scaled

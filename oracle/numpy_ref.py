@@ -847,3 +847,65 @@ def wright_density(T, S, p):
     p0 = w["b0"] + (w["b4"] * S + T * (w["b1"] + (T * (w["b2"] + w["b3"] * T) + w["b5"] * S)))
     lam = w["c0"] + (w["c4"] * S + T * (w["c1"] + (T * (w["c2"] + w["c3"] * T) + w["c5"] * S)))
     return (p + p0) / (lam + al0 * (p + p0))
+
+
+# ------------------------------------------------------------------------------------------------ freshwater
+def freshwater_volumes(V, dV, floor):
+    """
+    The freshwater volume dV (ny, nx) added to the top layer (k = N-1) of V (N, ny, nx); a removal that would take
+    the top layer below ``floor`` is taken from the layers below, top-down, each first kept at the floor, then
+    down to zero.  Returns (V_new, n_limited).  Follows ``freshwater_volumes`` (core/components/freshwater.py:31-58).
+    """
+    out = V.copy()
+    out[-1] += dV
+    floor = np.broadcast_to(np.asarray(floor, float), dV.shape)
+    spill = (dV < 0) & (out[-1] < floor)
+    if not spill.any():
+        return out, 0
+    js, is_ = np.nonzero(spill)
+    col = V[:, js, is_].copy()
+    deficit = -dV[js, is_]
+    fl = floor[js, is_]
+    for keep in (fl, np.zeros_like(fl)):
+        for k in range(V.shape[0] - 1, -1, -1):
+            take = np.minimum(deficit, np.maximum(col[k] - keep, 0.0))
+            col[k] -= take
+            deficit = deficit - take
+    out[:, js, is_] = col
+    return out, int(spill.sum())
+
+
+def apply_freshwater(stack, layout, V_adv, V_fw, fw_area_dt, dt):
+    """
+    The freshwater substep on the tracer stack (in place, also returned) with (n_limited, var_surf): the members
+    other than T and T2 keep their content (c V_adv / V_fw where the column has freshwater); a column whose volume
+    changed below the top layer shares sum V_adv c / sum V_fw over the touched layers.  Follows
+    ``apply_freshwater`` (core/components/freshwater.py:61-122).
+    """
+    cols = fw_area_dt != 0
+    ratio = np.ones_like(V_adv)
+    np.divide(V_adv, V_fw, out=ratio, where=cols[None] & (V_fw > 0))
+    var_surf = None
+    if "S" in layout:
+        S_old = stack[layout["S"]].copy()
+    salt_like = [i for i in range(layout["n"]) if i not in (layout["T"], layout.get("T2"))]
+    spill = (V_adv[:-1] != V_fw[:-1]).any(axis=0)
+    if spill.any():
+        changed = V_adv != V_fw
+        kmin = np.argmax(changed, axis=0)
+        below = np.arange(V_adv.shape[0])[:, None, None] >= kmin[None]
+        touched = below & spill[None]
+        Vf_sum = np.where(touched, V_fw, 0.0).sum(axis=0)
+        has_water = Vf_sum > 0
+    n_limited = int(np.count_nonzero(spill))
+    for i in salt_like:
+        c = np.where(cols[None], stack[i] * ratio, stack[i])
+        if spill.any():
+            cbar = np.divide(np.where(touched, V_adv * stack[i], 0.0).sum(axis=0), Vf_sum,
+                             out=np.zeros_like(Vf_sum), where=has_water)
+            c = np.where(touched & has_water[None], cbar[None], c)
+        stack[i] = c
+    if "S" in layout:
+        S_new = stack[layout["S"]]
+        var_surf = np.where(cols[None], (V_fw * S_new**2 - V_adv * S_old**2) / dt, 0.0)
+    return stack, n_limited, var_surf

@@ -470,4 +470,71 @@ int ocx_lagrangian_thickness_f64(const double* h, const double* divq, const doub
     return fail(OCX_E_RANGE, "layer %lld of cell (j=%lld, i=%lld) emptied in the Lagrangian step (reduce dt)", k, j, i);
 }
 
+size_t ocx_freshwater_workspace_bytes(void) { return 256; }
+
+int ocx_freshwater_volumes_f64(const double* V, const double* dV, const double* floor_cols, double floor, int n,
+                               int ny, int nx, double* out, void* workspace, size_t workspace_bytes,
+                               long long* n_limited, void* stream) {
+    OCX_TRY(check_ale_shape(n, ny, nx));
+    OCX_TRY(need(V, "V"));
+    OCX_TRY(need(dV, "dV"));
+    OCX_TRY(need(out, "out"));
+    if (out == V) return fail(OCX_E_ARG, "out must not alias V");
+    if (!workspace || workspace_bytes < ocx_freshwater_workspace_bytes())
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed", ocx_freshwater_workspace_bytes());
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long* cnt = static_cast<unsigned long long*>(workspace);
+    k_set_u64<<<1, 64, 0, s>>>(cnt, 0ull);
+    OCX_TRY(launch_check("freshwater reset"));
+    const long long M = (long long)ny * nx;
+    {
+        ProfScope _ps("freshwater_volumes", s);
+        k_freshwater_volumes<<<blocks_for(M), BS, 0, s>>>(V, dV, floor_cols, floor, n, M, out, cnt);
+    }
+    OCX_TRY(launch_check("freshwater_volumes"));
+    if (!n_limited) return OCX_OK;
+    unsigned long long b = 0;
+    OCX_TRY(hip_check(hipMemcpyAsync(&b, cnt, sizeof b, hipMemcpyDeviceToHost, s), "freshwater count"));
+    OCX_TRY(hip_check(hipStreamSynchronize(s), "freshwater sync"));
+    *n_limited = (long long)b;
+    return OCX_OK;
+}
+
+int ocx_apply_freshwater_f64(double* stack, int nt, int n, int ny, int nx, int t_index, int t2_index, int s_index,
+                             const double* V_adv, const double* V_fw, const double* fw_area_dt, double dt,
+                             double* var_surf, void* workspace, size_t workspace_bytes, long long* n_limited,
+                             void* stream) {
+    OCX_TRY(check_ale_shape(n, ny, nx));
+    if (nt < 1) return fail(OCX_E_ARG, "nt = %d must be positive", nt);
+    const void* p[] = {stack, V_adv, V_fw, fw_area_dt};
+    const char* nm[] = {"stack", "V_adv", "V_fw", "fw_area_dt"};
+    for (int q = 0; q < 4; ++q) OCX_TRY(need(p[q], nm[q]));
+    if (t_index >= nt || t2_index >= nt || s_index >= nt)
+        return fail(OCX_E_ARG, "member indices (T %d, T2 %d, S %d) must lie below nt = %d", t_index, t2_index,
+                    s_index, nt);
+    if (s_index >= 0 && (s_index == t_index || s_index == t2_index))
+        return fail(OCX_E_ARG, "S (%d) cannot be T or T2", s_index);
+    if (var_surf && s_index < 0) return fail(OCX_E_ARG, "var_surf needs the index of S");
+    if (!(dt > 0)) return fail(OCX_E_ARG, "dt = %g must be positive", dt);
+    if (!workspace || workspace_bytes < ocx_freshwater_workspace_bytes())
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed", ocx_freshwater_workspace_bytes());
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long* cnt = static_cast<unsigned long long*>(workspace);
+    k_set_u64<<<1, 64, 0, s>>>(cnt, 0ull);
+    OCX_TRY(launch_check("freshwater reset"));
+    const long long M = (long long)ny * nx;
+    FreshArgs a{stack, V_adv, V_fw, fw_area_dt, var_surf, nt, n, t_index, t2_index, s_index, M, dt, cnt};
+    {
+        ProfScope _ps("apply_freshwater", s);
+        k_apply_freshwater<<<blocks_for(M), BS, 0, s>>>(a);
+    }
+    OCX_TRY(launch_check("apply_freshwater"));
+    if (!n_limited) return OCX_OK;
+    unsigned long long b = 0;
+    OCX_TRY(hip_check(hipMemcpyAsync(&b, cnt, sizeof b, hipMemcpyDeviceToHost, s), "freshwater count"));
+    OCX_TRY(hip_check(hipStreamSynchronize(s), "freshwater sync"));
+    *n_limited = (long long)b;
+    return OCX_OK;
+}
+
 }  // extern "C"

@@ -42,6 +42,13 @@ __device__ __forceinline__ double npmin(double a, double b) {
     return b < a ? b : a;
 }
 
+// numpy.maximum: the larger, NaN if either is NaN
+__device__ __forceinline__ double npmax(double a, double b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    return b > a ? b : a;
+}
+
 // numpy.sign: -1, 0 (of the zero's sign) or 1; NaN stays NaN
 __device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x * 0.0); }
 
@@ -442,6 +449,111 @@ __global__ void k_upwind_transport(const double* __restrict__ ut, const double* 
         const size_t x = k * PF + t;
         out[x] = out[x] + R * q[x] / den;
     }
+}
+
+// ------------------------------------------------------------------------------------------ freshwater
+// One lane per wavefront with `hit` set adds the wavefront's count of set lanes to *count (a vector atomic).
+__device__ __forceinline__ void wave_count(bool hit, unsigned long long* count) {
+    const unsigned long long b = __ballot(hit);
+    if (b && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)b) - 1))
+        atomicAdd(count, (unsigned long long)__popcll(b));
+}
+
+// freshwater_volumes (core/components/freshwater.py:31-58), one thread per column of M: the volume dV joins the top
+// layer; a removal that would take it below the floor is taken from the original layers top-down, first keeping
+// each at the floor, then down to zero.  Columns where that happens are counted.
+__global__ void k_freshwater_volumes(const double* __restrict__ V, const double* __restrict__ dV,
+                                     const double* __restrict__ floor_c, double floor_s, int n, long long M,
+                                     double* __restrict__ out, unsigned long long* count) {
+    const long long c = gtid();
+    bool spill = false;
+    if (c < M) {
+        const double d = dV[c];
+        const double fl = floor_c ? floor_c[c] : floor_s;
+        for (int k = 0; k < n - 1; ++k) out[(size_t)k * M + c] = V[(size_t)k * M + c];
+        const size_t top = (size_t)(n - 1) * M + c;
+        const double vt = V[top] + d;
+        out[top] = vt;
+        spill = d < 0.0 && vt < fl;
+        if (spill) {
+            double deficit = -d;
+            out[top] = V[top];
+            for (int pass = 0; pass < 2; ++pass) {
+                const double keep = pass == 0 ? fl : 0.0;
+                for (int k = n - 1; k >= 0; --k) {
+                    const size_t x = (size_t)k * M + c;
+                    const double col = out[x];
+                    const double take = npmin(deficit, npmax(col - keep, 0.0));
+                    out[x] = col - take;
+                    deficit = deficit - take;
+                }
+            }
+        }
+    }
+    wave_count(spill, count);
+}
+
+// apply_freshwater (core/components/freshwater.py:61-122), one thread per column, in place on the stack (nt, n, M):
+// every member but T (t_index) and T^2 (t2_index) takes c V_adv / V_fw where the column has freshwater; in a
+// column whose volume changed below the top layer the touched layers (from the lowest changed one up) share
+// sum V_adv c / sum V_fw.  var_surf (n, M), for member s_index: (V_fw S_new^2 - V_adv S_old^2) / dt where the
+// column has freshwater, else 0.  The sums run from layer 0 up and start from the first term (NumPy's sum over
+// the first axis).  Columns with a change below the top layer are counted.
+struct FreshArgs {
+    double* stack;
+    const double *Va, *Vf, *fwa;
+    double* var_surf;
+    int nt, n, t_index, t2_index, s_index;
+    long long M;
+    double dt;
+    unsigned long long* count;
+};
+
+__global__ void k_apply_freshwater(FreshArgs a) {
+    const long long c = gtid();
+    bool spill = false;
+    if (c < a.M) {
+        const int n = a.n;
+        const long long M = a.M;
+        const bool cols = a.fwa[c] != 0.0;
+        int kmin = -1;
+        for (int k = 0; k < n; ++k) {
+            const bool ch = a.Va[(size_t)k * M + c] != a.Vf[(size_t)k * M + c];
+            if (ch && kmin < 0) kmin = k;
+            if (ch && k < n - 1) spill = true;
+        }
+        if (kmin < 0) kmin = 0;
+        double vsum = 0.0;
+        if (spill)
+            for (int k = 0; k < n; ++k) {
+                const double v = k >= kmin ? a.Vf[(size_t)k * M + c] : 0.0;
+                vsum = k == 0 ? v : vsum + v;
+            }
+        const bool has_water = vsum > 0.0;
+        for (int i = 0; i < a.nt; ++i) {
+            if (i == a.t_index || i == a.t2_index) continue;
+            double* st = a.stack + (size_t)i * n * M + c;
+            double cbar = 0.0;
+            if (spill) {
+                double num = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    const double v = k >= kmin ? a.Va[(size_t)k * M + c] * st[(size_t)k * M] : 0.0;
+                    num = k == 0 ? v : num + v;
+                }
+                cbar = has_water ? num / vsum : 0.0;
+            }
+            for (int k = 0; k < n; ++k) {
+                const size_t x = (size_t)k * M;
+                const double va = a.Va[x + c], vf = a.Vf[x + c], s = st[x];
+                const double ratio = (cols && vf > 0.0) ? va / vf : 1.0;
+                double cv = cols ? s * ratio : s;
+                if (spill && k >= kmin && has_water) cv = cbar;
+                if (i == a.s_index && a.var_surf) a.var_surf[x + c] = cols ? (vf * (cv * cv) - va * (s * s)) / a.dt : 0.0;
+                st[x] = cv;
+            }
+        }
+    }
+    wave_count(spill, a.count);
 }
 
 // ------------------------------------------------------------------------------------------ outflow limiter
