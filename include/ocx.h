@@ -248,15 +248,17 @@ int ocx_ppm_transport_iterations(const void* workspace, int n, int ny, int nx, i
 int ocx_upwind_transport_f64(const double* ut, const double* L, const double* Q_bt, const double* q_pos,
                              const double* q_neg, int n, int ny, int nx, int axis, double* out, void* stream);
 
-/* Freshwater volumes: dV (ny, nx) joins the top layer (k = n-1) of V (n, ny, nx) into out; a removal that would
- * take the top layer below the floor (floor_cols (ny, nx), or NULL for the scalar floor) is taken from the layers
- * below, top-down, each first kept at the floor, then down to zero.  n_limited (host, optional: a host
- * synchronisation) counts those columns.  Replaces freshwater_volumes (core/components/freshwater.py:31-58).
+/* Freshwater volumes: dv_scale dV (ny, nx) joins the top layer (k = n-1) of V (n, ny, nx) into out (dv_scale = dt
+ * with dV = fw gives the layer transport's dt fw); a removal that would take the top layer below the floor
+ * (floor_cols (ny, nx), or NULL for the scalar floor) is taken from the layers below, top-down, each first kept at
+ * the floor, then down to zero.  With area (ny, nx), V_out = out A as well.  n_limited (host, optional: a host
+ * synchronisation) counts those columns.  Replaces freshwater_volumes (core/components/freshwater.py:31-58) and
+ * the layer transport's h_fw, V_fw (core/components/layer_transport.py:148-155).
  * workspace: ocx_freshwater_workspace_bytes(). */
 size_t ocx_freshwater_workspace_bytes(void);
-int ocx_freshwater_volumes_f64(const double* V, const double* dV, const double* floor_cols, double floor, int n,
-                               int ny, int nx, double* out, void* workspace, size_t workspace_bytes,
-                               long long* n_limited, void* stream);
+int ocx_freshwater_volumes_f64(const double* V, const double* dV, double dv_scale, const double* floor_cols,
+                               double floor, const double* area, int n, int ny, int nx, double* out, double* V_out,
+                               void* workspace, size_t workspace_bytes, long long* n_limited, void* stream);
 
 /* Freshwater substep on the tracer stack (nt, n, ny, nx), in place: every member but T (t_index) and T^2 (t2_index,
  * -1 if absent) keeps its content, c V_adv / V_fw, where fw_area_dt (ny, nx) is nonzero; columns whose volume

@@ -780,7 +780,7 @@ def upwind_transport_x(q_e, q_w, ut, L, Q_bt):
 
 def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_new, v_old, wet_u, wet_v, wu, wv,
                         Du, Dv, dt, theta, theta_bc=None, clip=True, max_iter=20, periodic=(False, False),
-                        thickness_flux="ppm"):
+                        thickness_flux="ppm", fw=None):
     """
     One ALE layer-transport step with PPM thickness: transport velocities, transport thicknesses (the vertical
     coordinate's, clipped at the shallower bed for z*), PPM fluxes of both directions (the v-faces on swapped axes),
@@ -790,7 +790,9 @@ def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_
     (core/dynamics/sigma_baroclinic.py:1812-1816).  On periodic axes the PPM fluxes run on inputs padded by 4
     wrapped cells and are cropped (layer_transport.py:122-129, core/model/periodic.py:71-103).  Returns
     dict(Qx, Qy, V0, V1, ok, h_dagger, n_limited).  ``thickness_flux`` "upwind": the donor-cell fluxes of
-    ``upwind_transport_x`` instead of PPM (layer_transport.py:131-133), with no padding.
+    ``upwind_transport_x`` instead of PPM (layer_transport.py:131-133), with no padding.  With the freshwater flux
+    ``fw`` (ny, nx) [m/s] also h_fw (h† with dt fw on the top layer, ``freshwater_volumes`` with the floor 1e-6 m)
+    and V_fw = h_fw A (layer_transport.py:148-155).
     """
     act = np.asarray(active, bool)
     px, py = periodic
@@ -818,9 +820,13 @@ def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_
     divh = Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:, :] - Qy[:, :-1, :]
     h_lag = lagrangian_thickness(h, divh, dt, area)
     V0, V1 = h * area[None], h_lag * area[None]
+    h_fw = V_fw = None
+    if fw is not None:
+        h_fw, _ = freshwater_volumes(h_lag, dt * fw, 1e-6)
+        V_fw = h_fw * area[None]
     return dict(Qx=np.ascontiguousarray(Qx), Qy=np.ascontiguousarray(Qy), V0=V0, V1=V1, ok=act[None] & (V1 > 0),
                 h_dagger=h_lag, n_limited=n_lim, qe=qe, qw=qw, qn=np.ascontiguousarray(qn.swapaxes(1, 2)),
-                qs=np.ascontiguousarray(qs.swapaxes(1, 2)), utx=utx, uty=uty, Qbx=Qbx, Qby=Qby)
+                qs=np.ascontiguousarray(qs.swapaxes(1, 2)), utx=utx, uty=uty, Qbx=Qbx, Qby=Qby, h_fw=h_fw, V_fw=V_fw)
 
 
 # ------------------------------------------------------------------------------------------------ layer density

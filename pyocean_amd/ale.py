@@ -31,6 +31,7 @@ from ._native import check, lib
 from .ops import Workspace, _ptr, _stream, periodic_pad
 
 H_PPM_MIN = 2e-10        # positive-definite floor of the PPM thickness [m] (core/vertical/ale.py:298)
+DZ_FLOOR = 1e-6          # smallest layer thickness an evaporation deficit leaves [m] (core/vertical/ale.py:63)
 _F64 = torch.float64
 
 _WS_PPM = Workspace()
@@ -308,6 +309,8 @@ class LayerFluxes:
     ok: torch.Tensor          # wet cell-layers after the step (uint8)
     h_dagger: torch.Tensor    # Lagrangian thickness [m]
     n_limited: int = 0
+    V_fw: torch.Tensor | None = None    # with freshwater: h_fw A [m³]
+    h_fw: torch.Tensor | None = None    # h† with dt fw on the top layer [m]
 
 
 class LayerTransport:
@@ -337,7 +340,7 @@ class LayerTransport:
         self.iterations = (0, 0)
 
     def __call__(self, h, u_new, u_old, v_new, v_old, wet_u, wet_v, wu, wv, Du, Dv, qe, qw, qn, qs,
-                 theta_bc: float | None = None, count_iterations: bool = False) -> LayerFluxes:
+                 theta_bc: float | None = None, count_iterations: bool = False, fw=None) -> LayerFluxes:
         """h (n, ny, nx) layer thickness at step n; u (n, ny, nx+1), v (n, ny+1, nx) new and old velocities;
         wet_u / wet_v faces wet in the last free-surface step; wu, wv face fractions; Du, Dv face depths;
         qe, qw (u-faces) and qn, qs (v-faces, natural (n, ny+1, nx) layout) transport thicknesses."""
@@ -387,4 +390,10 @@ class LayerTransport:
                                               _ptr(self.area), _ptr(self.active), n, ny, nx, self.dt, 1e-9,
                                               _ptr(h_lag), _ptr(V0), _ptr(V1), _ptr(ok), _ptr(ws), ws.numel(),
                                               ctypes.byref(bad), _stream()), "lagrangian_thickness")
-        return LayerFluxes(Qx, Qy, V0, V1, ok, h_lag, n_lim)
+        h_fw = V_fw = None
+        if fw is not None:
+            # real freshwater: the top layer of h† gains dt fw; a deficit below the floor comes from the layers
+            # below (layer_transport.py:148-155); h† and V1 stay without it
+            from .freshwater import freshwater_volumes
+            h_fw, V_fw, _ = freshwater_volumes(h_lag, fw, DZ_FLOOR, scale=self.dt, area=self.area)
+        return LayerFluxes(Qx, Qy, V0, V1, ok, h_lag, n_lim, V_fw, h_fw)

@@ -472,14 +472,16 @@ int ocx_lagrangian_thickness_f64(const double* h, const double* divq, const doub
 
 size_t ocx_freshwater_workspace_bytes(void) { return 256; }
 
-int ocx_freshwater_volumes_f64(const double* V, const double* dV, const double* floor_cols, double floor, int n,
-                               int ny, int nx, double* out, void* workspace, size_t workspace_bytes,
-                               long long* n_limited, void* stream) {
+int ocx_freshwater_volumes_f64(const double* V, const double* dV, double dv_scale, const double* floor_cols,
+                               double floor, const double* area, int n, int ny, int nx, double* out, double* V_out,
+                               void* workspace, size_t workspace_bytes, long long* n_limited, void* stream) {
     OCX_TRY(check_ale_shape(n, ny, nx));
     OCX_TRY(need(V, "V"));
     OCX_TRY(need(dV, "dV"));
     OCX_TRY(need(out, "out"));
     if (out == V) return fail(OCX_E_ARG, "out must not alias V");
+    if ((area == nullptr) != (V_out == nullptr)) return fail(OCX_E_ARG, "area and V_out go together");
+    if (V_out && (V_out == V || V_out == out)) return fail(OCX_E_ARG, "V_out must not alias V or out");
     if (!workspace || workspace_bytes < ocx_freshwater_workspace_bytes())
         return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed", ocx_freshwater_workspace_bytes());
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -489,7 +491,8 @@ int ocx_freshwater_volumes_f64(const double* V, const double* dV, const double* 
     const long long M = (long long)ny * nx;
     {
         ProfScope _ps("freshwater_volumes", s);
-        k_freshwater_volumes<<<blocks_for(M), BS, 0, s>>>(V, dV, floor_cols, floor, n, M, out, cnt);
+        k_freshwater_volumes<<<blocks_for(M), BS, 0, s>>>(V, dV, dv_scale, floor_cols, floor, area, n, M, out, V_out,
+                                                          cnt);
     }
     OCX_TRY(launch_check("freshwater_volumes"));
     if (!n_limited) return OCX_OK;

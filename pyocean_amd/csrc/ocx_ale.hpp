@@ -459,16 +459,18 @@ __device__ __forceinline__ void wave_count(bool hit, unsigned long long* count) 
         atomicAdd(count, (unsigned long long)__popcll(b));
 }
 
-// freshwater_volumes (core/components/freshwater.py:31-58), one thread per column of M: the volume dV joins the top
-// layer; a removal that would take it below the floor is taken from the original layers top-down, first keeping
-// each at the floor, then down to zero.  Columns where that happens are counted.
-__global__ void k_freshwater_volumes(const double* __restrict__ V, const double* __restrict__ dV,
-                                     const double* __restrict__ floor_c, double floor_s, int n, long long M,
-                                     double* __restrict__ out, unsigned long long* count) {
+// freshwater_volumes (core/components/freshwater.py:31-58), one thread per column of M: the volume scale dV joins
+// the top layer; a removal that would take it below the floor is taken from the original layers top-down, first
+// keeping each at the floor, then down to zero.  Columns where that happens are counted.  With area, the volumes
+// out A are written to Vout as well (layer_volumes of the thicknesses).
+__global__ void k_freshwater_volumes(const double* __restrict__ V, const double* __restrict__ dV, double scale,
+                                     const double* __restrict__ floor_c, double floor_s,
+                                     const double* __restrict__ area, int n, long long M, double* __restrict__ out,
+                                     double* __restrict__ Vout, unsigned long long* count) {
     const long long c = gtid();
     bool spill = false;
     if (c < M) {
-        const double d = dV[c];
+        const double d = scale * dV[c];
         const double fl = floor_c ? floor_c[c] : floor_s;
         for (int k = 0; k < n - 1; ++k) out[(size_t)k * M + c] = V[(size_t)k * M + c];
         const size_t top = (size_t)(n - 1) * M + c;
@@ -488,6 +490,10 @@ __global__ void k_freshwater_volumes(const double* __restrict__ V, const double*
                     deficit = deficit - take;
                 }
             }
+        }
+        if (Vout) {
+            const double A = area[c];
+            for (int k = 0; k < n; ++k) Vout[(size_t)k * M + c] = out[(size_t)k * M + c] * A;
         }
     }
     wave_count(spill, count);

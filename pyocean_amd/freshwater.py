@@ -28,23 +28,29 @@ def _f64(t, shape, name):
     return t.contiguous()
 
 
-def freshwater_volumes(V, dV, floor):
+def freshwater_volumes(V, dV, floor, *, scale: float = 1.0, area=None):
     """
     (V_new, n_limited): the freshwater volume dV (ny, nx) added to the top layer (k = N-1) of V (N, ny, nx); a
     removal that would take the top layer below ``floor`` (a number or (ny, nx)) is taken from the layers below,
     top-down, each first kept at the floor, then down to zero.  n_limited counts those columns (a host
-    synchronisation).  core/components/freshwater.py:31-58.
+    synchronisation).  core/components/freshwater.py:31-58.  ``scale`` multiplies dV first (the layer transport
+    passes dt and fw); with ``area`` the result is (V_new, V_new A, n_limited).
     """
     n, ny, nx = V.shape
     Vc, d = _f64(V, (n, ny, nx), "V"), _f64(dV, (ny, nx), "dV")
     fc = _f64(floor, (ny, nx), "floor") if torch.is_tensor(floor) else None
+    A = _f64(area, (ny, nx), "area") if area is not None else None
     out = torch.empty_like(Vc)
+    vout = torch.empty_like(Vc) if A is not None else None
     L = lib()
     ws = _WS.get(int(L.ocx_freshwater_workspace_bytes()))
     cnt = ctypes.c_longlong(0)
-    check(L.ocx_freshwater_volumes_f64(_ptr(Vc), _ptr(d), _ptr(fc), 0.0 if fc is not None else float(floor), n, ny,
-                                       nx, _ptr(out), _ptr(ws), ws.numel(), ctypes.byref(cnt), _stream()),
+    check(L.ocx_freshwater_volumes_f64(_ptr(Vc), _ptr(d), float(scale), _ptr(fc),
+                                       0.0 if fc is not None else float(floor), _ptr(A), n, ny, nx, _ptr(out),
+                                       _ptr(vout), _ptr(ws), ws.numel(), ctypes.byref(cnt), _stream()),
           "freshwater_volumes")
+    if A is not None:
+        return out, vout, int(cnt.value)
     return out, int(cnt.value)
 
 

@@ -103,3 +103,27 @@ def test_reference_unit_case_on_the_device(gpu):
     assert n2 == n2r == 1
     _eq(out, ref, "stack")
     _eq(var, var_ref, "var_surf")
+
+
+def test_layer_transport_with_freshwater_matches_the_oracle(gpu):
+    # the layer transport's h_fw and V_fw (layer_transport.py:148-155): rain and evaporation, with deficits that
+    # reach below the top layer, on the regional ALE inputs
+    _say("layer transport with freshwater")
+    from pyocean_amd import ale, synth
+    a = synth.ale_inputs(64, 48, 10, seed=1)
+    rng = np.random.default_rng(5)
+    fw = rng.normal(0.0, 2e-3, a.H.shape)
+    fw[:8] = -5e-2                                          # strong evaporation: deficits below the top layer
+    r = R.ale_layer_transport(a.h, a.H, a.active, a.area, a.dx_t, a.dy_t, a.Lu, a.Lv, a.u_new, a.u_old, a.v_new,
+                              a.v_old, a.wet_u, a.wet_v, a.wu, a.wv, a.Du, a.Dv, a.dt, a.theta, a.theta_bc, fw=fw)
+    d = {k: _d(v) for k, v in vars(a).items() if isinstance(v, np.ndarray)}
+    qe, qw = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 0)
+    qn, qs = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 1)
+    lt = ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
+                            dy_t=d["dy_t"], dt=a.dt, theta=a.theta)
+    out = lt(d["h"], d["u_new"], d["u_old"], d["v_new"], d["v_old"], d["wet_u"], d["wet_v"], d["wu"], d["wv"],
+             d["Du"], d["Dv"], qe, qw, qn, qs, theta_bc=a.theta_bc, fw=_d(fw))
+    _eq(out.h_fw, r["h_fw"], "h_fw")
+    _eq(out.V_fw, r["V_fw"], "V_fw")
+    _eq(out.V1, r["V1"], "V1")
+    assert (r["h_fw"][:-1] != r["h_dagger"][:-1]).any()       # some deficit reached below the top layer
