@@ -348,3 +348,22 @@ def test_upwind_layer_transport_matches_the_oracle_bit_for_bit(gpu, periodic):
     with pytest.raises(ValueError, match="thickness_flux"):
         ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
                            dy_t=d["dy_t"], dt=a.dt, theta=a.theta, thickness_flux="weno")
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 3, 2), (4, 1, 7), (2, 5, 1)])
+def test_upwind_transport_on_tiny_and_ragged_shapes(gpu, shape):
+    _say("upwind shapes", shape)
+    n, ny, nx = shape
+    rng = np.random.default_rng(sum(shape))
+    h = rng.uniform(0.0, 3.0, shape)
+    act = rng.random((ny, nx)) > 0.2
+    qe, qw = R.face_transport_thickness(h, h.sum(0) + 1.0, act, True)
+    ut = rng.normal(0.0, 0.2, (n, ny, nx + 1))
+    L, Qb = rng.uniform(1.0, 2.0, (ny, nx + 1)), rng.normal(0.0, 3.0, (ny, nx + 1))
+    _eq(ale.upwind_transport(_d(ut), _d(L), _d(Qb), _d(qe), _d(qw), 0), R.upwind_transport_x(qe, qw, ut, L, Qb), "Qx")
+    (qn, qs) = (np.ascontiguousarray(q.swapaxes(1, 2)) for q in
+                R.face_transport_thickness(h.swapaxes(1, 2), (h.sum(0) + 1.0).T, act.T, True))
+    vt = rng.normal(0.0, 0.2, (n, ny + 1, nx))
+    Lv, Qv = rng.uniform(1.0, 2.0, (ny + 1, nx)), rng.normal(0.0, 3.0, (ny + 1, nx))
+    ref = R.upwind_transport_x(qn.swapaxes(1, 2), qs.swapaxes(1, 2), vt.swapaxes(1, 2), Lv.T, Qv.T).swapaxes(1, 2)
+    _eq(ale.upwind_transport(_d(vt), _d(Lv), _d(Qv), _d(qn), _d(qs), 1), np.ascontiguousarray(ref), "Qy")

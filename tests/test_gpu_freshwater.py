@@ -127,3 +127,23 @@ def test_layer_transport_with_freshwater_matches_the_oracle(gpu):
     _eq(out.V_fw, r["V_fw"], "V_fw")
     _eq(out.V1, r["V1"], "V1")
     assert (r["h_fw"][:-1] != r["h_dagger"][:-1]).any()       # some deficit reached below the top layer
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 3, 2), (4, 1, 7), (2, 5, 1)])
+def test_tiny_and_ragged_shapes(gpu, shape):
+    _say("freshwater shapes", shape)
+    n, ny, nx = shape
+    rng = np.random.default_rng(sum(shape))
+    V = rng.uniform(0.0, 1.0, shape)
+    dV = rng.uniform(-2.0, 0.5, (ny, nx))
+    ref, n_ref = R.freshwater_volumes(V, dV, 1e-6)
+    got, n_got = freshwater.freshwater_volumes(_d(V), _d(dV), 1e-6)
+    _eq(got, ref, "V_fw")
+    assert n_got == n_ref
+    lay = {"T": 0, "S": 1, "n": 2}
+    stack = rng.normal(10.0, 1.0, (2,) + shape)
+    r, nr, vr = R.apply_freshwater(stack.copy(), lay, V, ref, dV, dt=1.0)
+    g, ng, vg = freshwater.apply_freshwater(_d(stack), lay, _d(V), _d(ref), _d(dV), dt=1.0)
+    _eq(g, r, "stack")
+    _eq(vg, vr, "var_surf")
+    assert ng == nr
