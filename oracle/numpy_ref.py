@@ -387,6 +387,28 @@ def diffuse_h_ale(T, Tw, h, act, cx, cy, okx, oky, V, kh, dt):
         return np.where(V > 0, T - dt * div / np.where(V > 0, V, 1.0), T)
 
 
+def diffuse_h_sigma(T, Tw, ds, gx, gy, okx, oky, V, kh, dt):
+    """
+    Explicit Laplacian diffusion along sigma surfaces: the sigma branch of ``TracerDiffusion.diffuse_h_numpy``
+    (core/components/tracers.py:450-481, h is None): F_x = -K_h G_u ds_k (Tw_i - Tw_i-1) on the interior faces with
+    G_u = L_u nan_to_num(D_u) / dx_u (gx, (ny, nx+1)) and likewise G_v (gy), masked by okx / oky.  T, Tw
+    (..., n, ny, nx); ds (n,); V (n, ny, nx).
+    """
+    n = ds.shape[0]
+    ny, nx = gx.shape[0], gx.shape[1] - 1
+    d = np.asarray(ds, float)[:, None, None]
+    lead = T.shape[:-3]
+    Fx = np.zeros(lead + (n, ny, nx + 1))
+    Fy = np.zeros(lead + (n, ny + 1, nx))
+    Fx[..., 1:-1] = -kh * gx[None, :, 1:-1] * d * (Tw[..., 1:] - Tw[..., :-1])
+    Fy[..., 1:-1, :] = -kh * gy[None, 1:-1, :] * d * (Tw[..., 1:, :] - Tw[..., :-1, :])
+    Fx = np.where(okx[None], Fx, 0.0)
+    Fy = np.where(oky[None], Fy, 0.0)
+    div = Fx[..., 1:] - Fx[..., :-1] + Fy[..., 1:, :] - Fy[..., :-1, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(V > 0, T - dt * div / np.where(V > 0, V, 1.0), T)
+
+
 def vertical_implicit(u, dz, nu, r_bed, dt, tau_top=None):
     """
     Backward-Euler vertical diffusion per column, Thomas algorithm: ``_vertical_implicit_numpy`` and its compiled

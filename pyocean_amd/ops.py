@@ -257,6 +257,21 @@ def diffuse_h(T, Tw, h, act, cx, cy, okx, oky, V, kh, dt, out=None):
     return o4[0] if squeeze else o4
 
 
+def diffuse_h_sigma(T, Tw, ds, gx, gy, okx, oky, V, kh, dt, out=None):
+    """
+    Explicit Laplacian diffusion along sigma surfaces, the sigma branch of ``TracerDiffusion.diffuse_h``
+    (core/components/tracers.py:421-428, 450-481): F_x = -K_h G_u ds_k (Tw_i - Tw_i-1) with G_u = L_u D_u / dx_u
+    (gx (ny, nx+1), D_u with NaN read as 0; gy likewise), masked by okx / oky; ds (n,) the layer fractions.  The
+    kernel of ``diffuse_h`` with every face thickness ds_k: its arithmetic ((-K_h G) h) dTw is the sigma formula's
+    term for term.  T, Tw (nt, n, ny, nx) or (n, ny, nx); V (n, ny, nx).
+    """
+    n, ny, nx = V.shape
+    d = torch.as_tensor(ds, dtype=_F64, device=V.device).reshape(n, 1, 1)
+    h = d.expand(n, ny, nx).contiguous()
+    act = torch.ones((ny, nx), dtype=torch.uint8, device=V.device)
+    return diffuse_h(T, Tw, h, act, gx, gy, okx, oky, V, kh, dt, out=out)
+
+
 def vertical_implicit(u, dz, nu, r_bed, dt, tau_top=None, out=None, workspace: Workspace | None = None):
     """
     Backward-Euler vertical diffusion, column by column (``vertical_implicit`` / ``_thomas_columns``,

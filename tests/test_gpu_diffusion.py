@@ -111,3 +111,28 @@ def test_diffusion_at_full_size_conserves_and_keeps_bounds(gpu):
     lo, hi = u.min(1, keepdim=True).values, u.max(1, keepdim=True).values
     tol = 1e-12 * ((hi - lo) + hi.abs())
     assert ((o >= lo - tol) & (o <= hi + tol))[:, :, w].all()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_sigma_diffusion_matches_the_oracle_bit_for_bit(gpu, seed):
+    # the sigma branch of diffuse_h: face factors L D / dx with NaN depths read as 0, the layer fractions ds as the
+    # face thickness of every face, faces masked where the face depth is below d_min
+    _say("diffuse_h sigma", seed)
+    rng = np.random.default_rng(20 + seed)
+    nt, n, ny, nx = 3, 7, 33, 45
+    ds = rng.uniform(0.5, 1.5, n)
+    ds /= ds.sum()
+    H = rng.uniform(-5.0, 200.0, (ny, nx))
+    Du = rng.uniform(-1.0, 150.0, (ny, nx + 1))
+    Dv = rng.uniform(-1.0, 150.0, (ny + 1, nx))
+    Du[rng.random(Du.shape) < 0.05] = np.nan
+    Lu, Lv = rng.uniform(800, 1200, (ny, nx + 1)), rng.uniform(800, 1200, (ny + 1, nx))
+    du, dv = rng.uniform(900, 1100, (ny, nx + 1)), rng.uniform(900, 1100, (ny + 1, nx))
+    gx, gy = Lu * np.nan_to_num(Du) / du, Lv * np.nan_to_num(Dv) / dv
+    okx, oky = np.isfinite(Du) & (Du > 0.1), np.isfinite(Dv) & (Dv > 0.1)
+    V = np.where(H > 0, H, 0.0)[None] * ds[:, None, None] * 1e6
+    T = rng.normal(10.0, 2.0, (nt, n, ny, nx))
+    Tw = T - 8.0
+    ref = R.diffuse_h_sigma(T, Tw, ds, gx, gy, okx, oky, V, 50.0, 300.0)
+    got = ops.diffuse_h_sigma(_d(T), _d(Tw), ds, _d(gx), _d(gy), _d(okx), _d(oky), _d(V), 50.0, 300.0)
+    assert np.array_equal(got.cpu().numpy(), ref)

@@ -516,6 +516,35 @@ def test_horizontal_diffusion_respects_closed_faces():
         assert abs((V * out)[half].sum() - (V * T)[half].sum()) <= 1e-12 * (V * np.abs(T)).sum()
 
 
+def test_sigma_diffusion_conserves_spreads_and_equals_the_layer_form_with_equal_thicknesses():
+    """The sigma branch (tracers.py:470-481): conservation with closed outer faces, a uniform field kept, the
+    second-moment growth 2 K dt / dx^2 of a pulse on a uniform column, and, with every layer thickness equal to its
+    fraction, the same numbers as the layer (ALE) form term for term."""
+    rng = np.random.default_rng(4)
+    n, ny, nx = 4, 9, 13
+    ds = np.array([0.1, 0.2, 0.3, 0.4])
+    gx, gy = rng.uniform(50, 150, (ny, nx + 1)), rng.uniform(50, 150, (ny + 1, nx))
+    okx, oky = rng.random((ny, nx + 1)) > 0.1, rng.random((ny + 1, nx)) > 0.1
+    V = rng.uniform(0.5, 2.0, (n, ny, nx)) * 1e6
+    T = rng.normal(10.0, 2.0, (2, n, ny, nx))
+    out = R.diffuse_h_sigma(T, T - 3.0, ds, gx, gy, okx, oky, V, 40.0, 500.0)
+    assert np.abs((V * out).sum((1, 2, 3)) - (V * T).sum((1, 2, 3))).max() <= 1e-13 * (V * np.abs(T)).sum()
+    U = np.full(T.shape, 2.5)
+    assert np.array_equal(R.diffuse_h_sigma(U, U, ds, gx, gy, okx, oky, V, 40.0, 500.0), U)
+    h = np.broadcast_to(ds[:, None, None], (n, ny, nx)).copy()
+    alt = R.diffuse_h_ale(T, T - 3.0, h, np.ones((ny, nx), bool), gx, gy, okx, oky, V, 40.0, 500.0)
+    assert np.array_equal(out, alt)
+    nx1, dx, K, dt = 41, 1000.0, 30.0, 1000.0
+    x = np.arange(nx1) - nx1 // 2
+    P = np.exp(-0.5 * (x / 3.0) ** 2)[None, None, :]
+    one = np.ones(1)
+    res = R.diffuse_h_sigma(P, P, one, np.full((1, nx1 + 1), 10.0), np.full((2, nx1), 10.0),
+                            np.ones((1, nx1 + 1), bool), np.ones((2, nx1), bool), np.full((1, 1, nx1), 10.0 * dx * dx),
+                            K, dt)
+    m2 = lambda c: (x * x * c[0, 0]).sum()                             # noqa: E731
+    assert abs(m2(res) - m2(P) - 2 * K * dt / dx ** 2 * P.sum()) <= 1e-10 * m2(P)
+
+
 def test_vertical_implicit_matches_a_banded_solver_and_conserves():
     from scipy.linalg import solve_banded
     rng = np.random.default_rng(3)
