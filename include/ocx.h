@@ -248,6 +248,18 @@ int ocx_ppm_transport_iterations(const void* workspace, int n, int ny, int nx, i
 int ocx_upwind_transport_f64(const double* ut, const double* L, const double* Q_bt, const double* q_pos,
                              const double* q_neg, int n, int ny, int nx, int axis, double* out, void* stream);
 
+/* Layer fluxes through the open edges' faces, in place on Qx (n, ny, nx+1) and Qy (n, ny+1, nx): on every edge of
+ * open_mask (1 west, 2 east, 4 south, 8 north) and face with free-surface face depth open_D > 0, Q_k = (h_k / sum h)
+ * Q_bt + L h_k (theta_b u'_new + (1 - theta_b) u'_old) of the interior cell's thicknesses h (n, ny, nx), with
+ * Q_bt = open_Q (the Flather flux); theta_b the baroclinic time weight (theta when the model gives none).
+ * Replaces OpenBoundary.open_face_transports (core/components/open_boundary.py:334-351) and open_face_transport
+ * (core/vertical/ale.py:841-861). */
+int ocx_open_face_transports_f64(double* Qx, double* Qy, const double* h, const double* u_old, const double* v_old,
+                                 const double* u_new, const double* v_new, const double* open_Du,
+                                 const double* open_Qu, const double* open_Dv, const double* open_Qv, const double* Lu,
+                                 const double* Lv, int n, int ny, int nx, unsigned open_mask, double theta_b,
+                                 void* stream);
+
 /* Freshwater volumes: dv_scale dV (ny, nx) joins the top layer (k = n-1) of V (n, ny, nx) into out (dv_scale = dt
  * with dV = fw gives the layer transport's dt fw); a removal that would take the top layer below the floor
  * (floor_cols (ny, nx), or NULL for the scalar floor) is taken from the layers below, top-down, each first kept at

@@ -800,21 +800,58 @@ def upwind_transport_x(q_e, q_w, ut, L, Q_bt):
     return F + R[None] * qR / np.where(tot > 0, tot, 1.0)[None]
 
 
+def open_face_transport(h_int, u_new, u_old, Q_bt, L, theta, theta_bc=None):
+    """
+    Layer volume fluxes through the faces of one open edge: Q_k = (h_k / sum h) Q_bt + L h_k (tb u'_new + (1 - tb)
+    u'_old) with the baroclinic parts u' = u - sum (h/sum h) u, the interior cell's thicknesses h_int (N, m); 0 on a
+    column without water.  Follows ``open_face_transport`` (core/vertical/ale.py:841-861) term for term.
+    """
+    tot = h_int.sum(axis=0)
+    wet = tot > 0
+    w = np.where(wet[None], h_int / np.where(wet, tot, 1.0)[None], 0.0)
+    pn = u_new - (w * u_new).sum(axis=0)[None]
+    po = u_old - (w * u_old).sum(axis=0)[None]
+    tb = theta if theta_bc is None else theta_bc
+    Q = w * Q_bt[None] + L[None] * h_int * (tb * pn + (1.0 - tb) * po)
+    return np.where(wet[None], Q, 0.0)
+
+
+def open_face_transports(Qx, Qy, h, un, vn, u_new, v_new, open_edges, open_Du, open_Qu, open_Dv, open_Qv, Lu, Lv,
+                         theta, theta_bc=None):
+    """
+    In place on Qx, Qy: the open edges' faces where the free-surface face depth is positive take
+    ``open_face_transport`` of the interior cells (``OpenBoundary.open_face_transports``,
+    core/components/open_boundary.py:334-351); un, vn the old velocities, u_new, v_new the new ones.
+    """
+    for edge, Q, D, Qo, L, hc, vo, vn1 in (
+            ("west", Qx[..., 0], open_Du[:, 0], open_Qu[:, 0], Lu[:, 0], h[..., 0], un[..., 0], u_new[..., 0]),
+            ("east", Qx[..., -1], open_Du[:, -1], open_Qu[:, -1], Lu[:, -1], h[..., -1], un[..., -1],
+             u_new[..., -1]),
+            ("south", Qy[:, 0, :], open_Dv[0, :], open_Qv[0, :], Lv[0, :], h[:, 0, :], vn[:, 0, :], v_new[:, 0, :]),
+            ("north", Qy[:, -1, :], open_Dv[-1, :], open_Qv[-1, :], Lv[-1, :], h[:, -1, :], vn[:, -1, :],
+             v_new[:, -1, :])):
+        if edge in open_edges:
+            Q[...] = np.where((D > 0)[None], open_face_transport(hc, vn1, vo, Qo, L, theta, theta_bc), Q)
+    return Qx, Qy
+
+
 def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_new, v_old, wet_u, wet_v, wu, wv,
                         Du, Dv, dt, theta, theta_bc=None, clip=True, max_iter=20, periodic=(False, False),
-                        thickness_flux="ppm", fw=None):
+                        thickness_flux="ppm", fw=None, open_=(), open_Du=None, open_Qu=None, open_Dv=None,
+                        open_Qv=None):
     """
     One ALE layer-transport step with PPM thickness: transport velocities, transport thicknesses (the vertical
     coordinate's, clipped at the shallower bed for z*), PPM fluxes of both directions (the v-faces on swapped axes),
     the outflow limiter, the Lagrangian thickness and the layer volumes.  Follows ``ALELayerTransport.__call__``
-    (core/components/layer_transport.py:111-156) without open boundaries and freshwater, with
+    (core/components/layer_transport.py:111-156), with
     ``transport_thickness`` (core/components/vertical_coordinate.py:125-140) and ``layer_volumes``
     (core/dynamics/sigma_baroclinic.py:1812-1816).  On periodic axes the PPM fluxes run on inputs padded by 4
     wrapped cells and are cropped (layer_transport.py:122-129, core/model/periodic.py:71-103).  Returns
     dict(Qx, Qy, V0, V1, ok, h_dagger, n_limited).  ``thickness_flux`` "upwind": the donor-cell fluxes of
     ``upwind_transport_x`` instead of PPM (layer_transport.py:131-133), with no padding.  With the freshwater flux
     ``fw`` (ny, nx) [m/s] also h_fw (h† with dt fw on the top layer, ``freshwater_volumes`` with the floor 1e-6 m)
-    and V_fw = h_fw A (layer_transport.py:148-155).
+    and V_fw = h_fw A (layer_transport.py:148-155).  ``open_`` edges take ``open_face_transports`` of the open
+    flux fields before the limiter (layer_transport.py:134-135).
     """
     act = np.asarray(active, bool)
     px, py = periodic
@@ -837,6 +874,10 @@ def ale_layer_transport(h, H, active, area, dx_t, dy_t, Lu, Lv, u_new, u_old, v_
     else:
         Qx = ppm_transport_x(h, utx, Lu, Qbx, qe, qw, act, dt, dx_t, max_iter)
         Qy = ppm_transport_x(sw(h), sw(uty), Lv.T, Qby.T, qn, qs, act.T, dt, dy_t.T, max_iter).swapaxes(1, 2)
+    if open_:
+        Qx, Qy = np.ascontiguousarray(Qx), np.ascontiguousarray(Qy)
+        open_face_transports(Qx, Qy, h, u_old, v_old, u_new, v_new, open_, open_Du, open_Qu, open_Dv, open_Qv, Lu,
+                             Lv, theta, theta_bc)
     Qx, Qy, n_lim, _ = limit_outflow(h, np.ascontiguousarray(Qx), np.ascontiguousarray(Qy), area, dt, act,
                                      periodic=(px, py))
     divh = Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:, :] - Qy[:, :-1, :]

@@ -68,6 +68,7 @@ SIGNATURES = {
     "ocx_regrid_hybrid_f64": (I, [P, P, P, P, P, I, L, P, P, P, P, D, D, D, I, P, P, P, Z, P, P]),
     "ocx_regrid_density_f64": (I, [I, P, P, P, D, P, L, P, P]),
     "ocx_upwind_transport_f64": (I, [P, P, P, P, P, I, I, I, I, P, P]),
+    "ocx_open_face_transports_f64": (I, [P, P, P, P, P, P, P, P, P, P, P, P, P, I, I, I, U, D, P]),
     "ocx_freshwater_workspace_bytes": (Z, []),
     "ocx_freshwater_volumes_f64": (I, [P, P, D, P, D, P, I, I, I, P, P, P, Z, P, P]),
     "ocx_apply_freshwater_f64": (I, [P, I, I, I, I, I, I, I, P, P, P, D, P, P, Z, P, P]),

@@ -4,10 +4,11 @@ vertical coordinate, as HIP kernels of ``libocx.so``.
 
 The functions keep the names, argument meaning and errors of ``core/vertical/ale.py`` (torch tensors in HBM in
 place of NumPy arrays) and ``LayerTransport`` follows ``ALELayerTransport.__call__``
-(core/components/layer_transport.py:111-156) without open boundaries and freshwater:
+(core/components/layer_transport.py:111-156):
 
-    transport_velocity  (u and v)  ->  ppm_transport_x (u-faces, v-faces)  ->  limit_outflow
-        ->  lagrangian_thickness + layer volumes and wet mask
+    transport_velocity  (u and v)  ->  ppm_transport_x or upwind_transport_x (u-faces, v-faces)
+        ->  open_face_transports (open edges)  ->  limit_outflow
+        ->  lagrangian_thickness + layer volumes and wet mask  ->  freshwater_volumes (h_fw, V_fw)
 
 Results are bit-identical to the reference's FP64 arithmetic.  Two calls synchronise with the host, as they must
 to behave as the reference does: ``limit_outflow`` (whether to limit, how many passes, its warning or error) and
@@ -234,6 +235,43 @@ def upwind_transport_x(q_e, q_w, ut, L, Q_bt):
     return upwind_transport(ut, L, Q_bt, q_e, q_w, 0)
 
 
+_EDGE_BITS = {"west": 1, "east": 2, "south": 4, "north": 8}
+
+
+def open_face_transports(Qx, Qy, h, un, vn, u_new, v_new, open_edges, open_Du, open_Qu, open_Dv, open_Qv, Lu, Lv,
+                         theta: float, theta_bc: float | None = None):
+    """
+    In place on Qx (n, ny, nx+1) and Qy (n, ny+1, nx): on the open edges (names 'west', 'east', 'south', 'north'),
+    every face whose free-surface face depth open_D > 0 takes the layer fluxes of the interior cell's thicknesses
+    adding up to the Flather flux open_Q (``OpenBoundary.open_face_transports``, core/components/open_boundary.py:
+    334-351; ``open_face_transport``, core/vertical/ale.py:841-861); un, vn the old velocities.  Returns (Qx, Qy).
+    """
+    n, ny, nx = h.shape
+    mask = 0
+    for e in open_edges:
+        if e not in _EDGE_BITS:
+            raise ValueError(f"unknown open edge {e!r}")
+        mask |= _EDGE_BITS[e]
+    if not mask:
+        return Qx, Qy
+    for t, nm in ((Qx, "Qx"), (Qy, "Qy")):
+        if t.dtype != _F64 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"{nm}: a contiguous float64 GPU tensor (written in place)")
+    _f64(Qx, (n, ny, nx + 1), "Qx")
+    _f64(Qy, (n, ny + 1, nx), "Qy")
+    hc = _f64(h, (n, ny, nx), "h")
+    uo, vo = _f64(un, (n, ny, nx + 1), "un"), _f64(vn, (n, ny + 1, nx), "vn")
+    u1, v1 = _f64(u_new, (n, ny, nx + 1), "u_new"), _f64(v_new, (n, ny + 1, nx), "v_new")
+    Du, Qu = _f64(open_Du, (ny, nx + 1), "open_Du"), _f64(open_Qu, (ny, nx + 1), "open_Qu")
+    Dv, Qv = _f64(open_Dv, (ny + 1, nx), "open_Dv"), _f64(open_Qv, (ny + 1, nx), "open_Qv")
+    Luc, Lvc = _f64(Lu, (ny, nx + 1), "Lu"), _f64(Lv, (ny + 1, nx), "Lv")
+    tb = float(theta if theta_bc is None else theta_bc)
+    check(lib().ocx_open_face_transports_f64(_ptr(Qx), _ptr(Qy), _ptr(hc), _ptr(uo), _ptr(vo), _ptr(u1), _ptr(v1),
+                                             _ptr(Du), _ptr(Qu), _ptr(Dv), _ptr(Qv), _ptr(Luc), _ptr(Lvc), n, ny, nx,
+                                             mask, tb, _stream()), "ocx_open_face_transports_f64")
+    return Qx, Qy
+
+
 # ------------------------------------------------------------------------------------------------ outflow limiter
 def _outflow_call(h, Qx, Qy, area, dt, active, h_floor, max_pass, periodic, check_only):
     n, ny, nx = h.shape
@@ -316,7 +354,7 @@ class LayerFluxes:
 class LayerTransport:
     """
     The ALE layer-transport step on the device (``ALELayerTransport.__call__``, core/components/layer_transport.py:
-    111-156, no open boundaries or freshwater): per call the transport velocities of both face directions, the PPM
+    111-156): per call the transport velocities of both face directions, the PPM
     fluxes with their Newton correction (or, with ``thickness_flux="upwind"``, the donor-cell fluxes), the outflow
     limiter, the Lagrangian thickness and the layer volumes.  Static geometry is given once: Lu (ny, nx+1), Lv (ny+1, nx) face lengths; area, active,
     dx_t, dy_t (ny, nx); dt; theta (free-surface time weight).
@@ -340,7 +378,8 @@ class LayerTransport:
         self.iterations = (0, 0)
 
     def __call__(self, h, u_new, u_old, v_new, v_old, wet_u, wet_v, wu, wv, Du, Dv, qe, qw, qn, qs,
-                 theta_bc: float | None = None, count_iterations: bool = False, fw=None) -> LayerFluxes:
+                 theta_bc: float | None = None, count_iterations: bool = False, fw=None, open_=(),
+                 open_Du=None, open_Qu=None, open_Dv=None, open_Qv=None) -> LayerFluxes:
         """h (n, ny, nx) layer thickness at step n; u (n, ny, nx+1), v (n, ny+1, nx) new and old velocities;
         wet_u / wet_v faces wet in the last free-surface step; wu, wv face fractions; Du, Dv face depths;
         qe, qw (u-faces) and qn, qs (v-faces, natural (n, ny+1, nx) layout) transport thicknesses."""
@@ -375,6 +414,11 @@ class LayerTransport:
         if (px or py) and self.thickness_flux == "ppm":
             Qx = periodic_pad(Qx, "u", 4, px, py, crop=True)
             Qy = periodic_pad(Qy, "v", 4, px, py, crop=True)
+        if open_:
+            # the open edges' faces from the interior cells and the Flather flux (layer_transport.py:134-135)
+            Qx, Qy = Qx.contiguous(), Qy.contiguous()
+            open_face_transports(Qx, Qy, h, u_old, v_old, u_new, v_new, open_, open_Du, open_Qu, open_Dv, open_Qv,
+                                 self.Lu, self.Lv, self.theta, theta_bc)
         Qx, Qy, n_lim = limit_outflow(h, Qx, Qy, self.area, self.dt, self.active, self.h_floor, self.max_pass,
                                       periodic=self.periodic)
         n, ny, nx = h.shape

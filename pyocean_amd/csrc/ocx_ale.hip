@@ -470,6 +470,55 @@ int ocx_lagrangian_thickness_f64(const double* h, const double* divq, const doub
     return fail(OCX_E_RANGE, "layer %lld of cell (j=%lld, i=%lld) emptied in the Lagrangian step (reduce dt)", k, j, i);
 }
 
+int ocx_open_face_transports_f64(double* Qx, double* Qy, const double* h, const double* u_old, const double* v_old,
+                                 const double* u_new, const double* v_new, const double* open_Du,
+                                 const double* open_Qu, const double* open_Dv, const double* open_Qv, const double* Lu,
+                                 const double* Lv, int n, int ny, int nx, unsigned open_mask, double theta_b,
+                                 void* stream) {
+    OCX_TRY(check_ale_shape(n, ny, nx));
+    const void* p[] = {Qx, Qy, h, u_old, v_old, u_new, v_new, open_Du, open_Qu, open_Dv, open_Qv, Lu, Lv};
+    const char* nm[] = {"Qx", "Qy", "h", "u_old", "v_old", "u_new", "v_new", "open_Du", "open_Qu", "open_Dv",
+                        "open_Qv", "Lu", "Lv"};
+    for (int q = 0; q < 13; ++q) OCX_TRY(need(p[q], nm[q]));
+    if (open_mask & ~15u) return fail(OCX_E_ARG, "open_mask %u: bits 1 (west), 2 (east), 4 (south), 8 (north)", open_mask);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long NX1 = nx + 1, NY1 = ny + 1;
+    const long long pu = (long long)ny * NX1, pv = NY1 * nx, pc = (long long)ny * nx;
+    ProfScope _ps("open_face_transport", s);
+    for (int edge = 0; edge < 4; ++edge) {
+        if (!(open_mask & (1u << edge))) continue;
+        EdgeArgs e{};
+        e.n = n;
+        e.tb = theta_b;
+        if (edge < 2) {                                  // west / east: the u-faces of column 0 / nx
+            const long long fc = edge == 0 ? 0 : nx, cc = edge == 0 ? 0 : nx - 1;
+            e.Q = Qx;
+            e.un = u_new;
+            e.uo = u_old;
+            e.q0 = fc; e.qk = pu; e.qf = NX1;
+            e.h = h;
+            e.h0 = cc; e.hk = pc; e.hf = nx;
+            e.D = open_Du; e.Qo = open_Qu; e.L = Lu;
+            e.f0 = fc; e.ff = NX1;
+            e.m = ny;
+        } else {                                         // south / north: the v-faces of row 0 / ny
+            const long long fr = edge == 2 ? 0 : ny, cr = edge == 2 ? 0 : ny - 1;
+            e.Q = Qy;
+            e.un = v_new;
+            e.uo = v_old;
+            e.q0 = fr * nx; e.qk = pv; e.qf = 1;
+            e.h = h;
+            e.h0 = cr * nx; e.hk = pc; e.hf = 1;
+            e.D = open_Dv; e.Qo = open_Qv; e.L = Lv;
+            e.f0 = fr * nx; e.ff = 1;
+            e.m = nx;
+        }
+        k_open_face_transport<<<blocks_for(e.m), BS, 0, s>>>(e);
+        OCX_TRY(launch_check("open_face_transport"));
+    }
+    return OCX_OK;
+}
+
 size_t ocx_freshwater_workspace_bytes(void) { return 256; }
 
 int ocx_freshwater_volumes_f64(const double* V, const double* dV, double dv_scale, const double* floor_cols,

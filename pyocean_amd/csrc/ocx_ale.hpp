@@ -451,6 +451,51 @@ __global__ void k_upwind_transport(const double* __restrict__ ut, const double* 
     }
 }
 
+// ------------------------------------------------------------------------------------------ open-edge faces
+// open_face_transport (core/vertical/ale.py:841-861) on the m faces of one open edge, one thread per face: the
+// layer fluxes (h_k / sum h) Q_bt + L h_k (tb u'_new + (1 - tb) u'_old) of the interior cell's thicknesses,
+// written where the face depth D > 0 (open_boundary.py:350-351).  Every array is addressed by an offset, a layer
+// stride and a face stride, so the four edges of the u- and v-face arrays share the kernel.
+struct EdgeArgs {
+    double* Q;                       // face fluxes
+    const double *h, *un, *uo;       // interior cells; new and old velocities at the faces
+    const double *D, *Qo, *L;        // one layer of faces
+    long long q0, qk, qf;            // Q, un, uo: offset, layer stride, face stride
+    long long h0, hk, hf;            // h
+    long long f0, ff;                // D, Qo, L
+    int n, m;
+    double tb;
+};
+
+__global__ void k_open_face_transport(EdgeArgs e) {
+    const long long f = gtid();
+    if (f >= e.m) return;
+    const long long fq = e.q0 + f * e.qf, fh = e.h0 + f * e.hf, fo = e.f0 + f * e.ff;
+    if (!(e.D[fo] > 0.0)) return;
+    double tot = 0.0;
+    for (int k = 0; k < e.n; ++k) {
+        const double hk = e.h[fh + k * e.hk];
+        tot = k == 0 ? hk : tot + hk;
+    }
+    const bool wet = tot > 0.0;
+    const double den = wet ? tot : 1.0;
+    double sn = 0.0, so = 0.0;
+    for (int k = 0; k < e.n; ++k) {
+        const double w = wet ? e.h[fh + k * e.hk] / den : 0.0;
+        const double an = w * e.un[fq + k * e.qk], ao = w * e.uo[fq + k * e.qk];
+        sn = k == 0 ? an : sn + an;
+        so = k == 0 ? ao : so + ao;
+    }
+    const double Qb = e.Qo[fo], Lf = e.L[fo], tb = e.tb;
+    for (int k = 0; k < e.n; ++k) {
+        const double hk = e.h[fh + k * e.hk];
+        const double w = wet ? hk / den : 0.0;
+        const double pn = e.un[fq + k * e.qk] - sn, po = e.uo[fq + k * e.qk] - so;
+        const double q = w * Qb + Lf * hk * (tb * pn + (1.0 - tb) * po);
+        e.Q[fq + k * e.qk] = wet ? q : 0.0;
+    }
+}
+
 // ------------------------------------------------------------------------------------------ freshwater
 // One lane per wavefront with `hit` set adds the wavefront's count of set lanes to *count (a vector atomic).
 __device__ __forceinline__ void wave_count(bool hit, unsigned long long* count) {

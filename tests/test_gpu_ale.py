@@ -367,3 +367,54 @@ def test_upwind_transport_on_tiny_and_ragged_shapes(gpu, shape):
     Lv, Qv = rng.uniform(1.0, 2.0, (ny + 1, nx)), rng.normal(0.0, 3.0, (ny + 1, nx))
     ref = R.upwind_transport_x(qn.swapaxes(1, 2), qs.swapaxes(1, 2), vt.swapaxes(1, 2), Lv.T, Qv.T).swapaxes(1, 2)
     _eq(ale.upwind_transport(_d(vt), _d(Lv), _d(Qv), _d(qn), _d(qs), 1), np.ascontiguousarray(ref), "Qy")
+
+
+# ------------------------------------------------------------------------------------------------ open-edge faces
+def _open_fields(a, rng):
+    ny, nx = a.H.shape
+    Du = np.where(rng.random((ny, nx + 1)) < 0.9, rng.uniform(1.0, 50.0, (ny, nx + 1)), 0.0)
+    Dv = np.where(rng.random((ny + 1, nx)) < 0.9, rng.uniform(1.0, 50.0, (ny + 1, nx)), 0.0)
+    Du[0, 0] = np.nan
+    Qu, Qv = rng.normal(0.0, 5e4, (ny, nx + 1)), rng.normal(0.0, 5e4, (ny + 1, nx))
+    return Du, Qu, Dv, Qv
+
+
+@pytest.mark.parametrize("edges", [("west",), ("east", "south"), ("west", "east", "south", "north")])
+def test_open_face_transports_match_the_oracle(gpu, regional, edges):
+    _say("open face transports", edges)
+    a, _ = regional
+    rng = np.random.default_rng(len(edges))
+    Du, Qu, Dv, Qv = _open_fields(a, rng)
+    n, ny, nx = a.h.shape
+    Qx0, Qy0 = rng.normal(size=(n, ny, nx + 1)), rng.normal(size=(n, ny + 1, nx))
+    rx, ry = R.open_face_transports(Qx0.copy(), Qy0.copy(), a.h, a.u_old, a.v_old, a.u_new, a.v_new, edges, Du, Qu,
+                                    Dv, Qv, a.Lu, a.Lv, a.theta, a.theta_bc)
+    gx, gy = _d(Qx0), _d(Qy0)
+    ale.open_face_transports(gx, gy, _d(a.h), _d(a.u_old), _d(a.v_old), _d(a.u_new), _d(a.v_new), edges, _d(Du),
+                             _d(Qu), _d(Dv), _d(Qv), _d(a.Lu), _d(a.Lv), a.theta, a.theta_bc)
+    _eq(gx, rx, "Qx")
+    _eq(gy, ry, "Qy")
+
+
+def test_layer_transport_with_open_edges_matches_the_oracle_bit_for_bit(gpu):
+    _say("layer transport with open edges")
+    a = synth.ale_inputs(64, 48, 10, seed=1)
+    rng = np.random.default_rng(9)
+    Du, Qu, Dv, Qv = _open_fields(a, rng)
+    Qu, Qv = Qu * 1e-2, Qv * 1e-2
+    edges = ("west", "north")
+    r = R.ale_layer_transport(a.h, a.H, a.active, a.area, a.dx_t, a.dy_t, a.Lu, a.Lv, a.u_new, a.u_old, a.v_new,
+                              a.v_old, a.wet_u, a.wet_v, a.wu, a.wv, a.Du, a.Dv, a.dt, a.theta, a.theta_bc,
+                              open_=edges, open_Du=Du, open_Qu=Qu, open_Dv=Dv, open_Qv=Qv)
+    d = {k: _d(v) for k, v in vars(a).items() if isinstance(v, np.ndarray)}
+    qe, qw = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 0)
+    qn, qs = ale.face_thickness_axis(d["h"], d["H"], d["active"], True, 1)
+    lt = ale.LayerTransport(Lu=d["Lu"], Lv=d["Lv"], area=d["area"], active=d["active"], dx_t=d["dx_t"],
+                            dy_t=d["dy_t"], dt=a.dt, theta=a.theta)
+    out = lt(d["h"], d["u_new"], d["u_old"], d["v_new"], d["v_old"], d["wet_u"], d["wet_v"], d["wu"], d["wv"],
+             d["Du"], d["Dv"], qe, qw, qn, qs, theta_bc=a.theta_bc, open_=edges, open_Du=_d(Du), open_Qu=_d(Qu),
+             open_Dv=_d(Dv), open_Qv=_d(Qv))
+    assert out.n_limited == r["n_limited"]
+    for k in ("Qx", "Qy", "V0", "V1", "h_dagger"):
+        _eq(getattr(out, k), r[k], k)
+    assert (r["Qx"][:, :, 0] != 0).any() and (r["Qy"][:, -1, :] != 0).any()

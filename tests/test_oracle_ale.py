@@ -474,3 +474,46 @@ def test_upwind_layer_transport_closes_each_cells_volume_budget():
     both = (qe.sum(0) > 0) | (qw.sum(0) > 0)
     np.testing.assert_allclose(Qx.sum(0)[both], Qbx[both], rtol=1e-12, atol=1e-9 * np.abs(Qbx).max())
     assert not np.array_equal(up["Qx"], pp["Qx"])
+
+
+# ------------------------------------------------------------------------------------------------ open-edge faces
+def test_open_face_transport_adds_up_to_the_flather_flux():
+    # test_vertical_ale.py:281-287
+    rng = np.random.default_rng(13)
+    h = rng.uniform(0.1, 8.0, (5, 6))
+    un, uo = rng.normal(0.0, 0.4, (5, 6)), rng.normal(0.0, 0.4, (5, 6))
+    Qbt, L = rng.normal(0.0, 50.0, 6), np.full(6, 1000.0)
+    Q = R.open_face_transport(h, un, uo, Qbt, L, 0.6)
+    np.testing.assert_allclose(Q.sum(axis=0), Qbt, rtol=1e-12, atol=1e-9)
+
+
+def test_open_face_transport_vanished_layers_carry_nothing():
+    # test_vertical_ale.py:290-299
+    h = np.array([[0.0, 0.0], [3.0, 0.0], [0.0, 0.0], [2.0, 0.0]])
+    Q = R.open_face_transport(h, np.full((4, 2), 0.3), np.full((4, 2), -0.1), np.array([10.0, 0.0]),
+                              np.array([100.0, 100.0]), 0.6)
+    assert np.isfinite(Q).all()
+    assert (Q[[0, 2], 0] == 0).all() and (Q[:, 1] == 0).all()
+    np.testing.assert_allclose(Q[:, 0].sum(), 10.0, rtol=1e-14)
+
+
+def test_open_face_transports_touch_only_open_wet_edge_faces():
+    # a uniform velocity has no baroclinic part: each layer carries its thickness share of the Flather flux; faces
+    # with no free-surface depth and closed edges keep their fluxes
+    rng = np.random.default_rng(2)
+    n, ny, nx = 3, 5, 6
+    h = rng.uniform(1.0, 4.0, (n, ny, nx))
+    Qx0, Qy0 = rng.normal(size=(n, ny, nx + 1)), rng.normal(size=(n, ny + 1, nx))
+    u, v = np.full((n, ny, nx + 1), 0.2), np.full((n, ny + 1, nx), -0.1)
+    Du, Dv = np.full((ny, nx + 1), 10.0), np.full((ny + 1, nx), 10.0)
+    Du[2, 0] = 0.0
+    Qu, Qv = rng.normal(0, 30.0, (ny, nx + 1)), rng.normal(0, 30.0, (ny + 1, nx))
+    Lu, Lv = np.full((ny, nx + 1), 100.0), np.full((ny + 1, nx), 100.0)
+    Qx, Qy = R.open_face_transports(Qx0.copy(), Qy0.copy(), h, u, v, u, v, ("west", "north"), Du, Qu, Dv, Qv, Lu, Lv,
+                                    0.5)
+    w = h[..., 0] / h[..., 0].sum(0)
+    keep = np.arange(ny) != 2
+    np.testing.assert_allclose(Qx[:, keep, 0], (w * Qu[:, 0])[:, keep], rtol=1e-13, atol=1e-12)
+    assert np.array_equal(Qx[:, 2, 0], Qx0[:, 2, 0])
+    np.testing.assert_allclose(Qy[..., -1, :].sum(0), Qv[-1], rtol=1e-12, atol=1e-9)
+    assert np.array_equal(Qx[..., 1:], Qx0[..., 1:]) and np.array_equal(Qy[:, :-1], Qy0[:, :-1])
