@@ -1195,21 +1195,36 @@ __global__ void k_regrid_hybrid(HybridArgs r) {
                                       rho_own(j - 1) <= r.rho_target[j] && r.rho_target[j] <= rho_own(j);
                 return (ok(j - 1) && ok(j)) || straddle;
             };
+            // the kept interfaces once, one bit each, for columns of up to 63 layers (each layer's test reads its
+            // thickness, density and targets, and the three passes below ask for every interface)
+            const bool bits = n <= 63;
+            unsigned long long kb = 0ull;
+            if (bits) {
+                bool ok_prev = ok(0);
+                for (int j = 1; j < n; ++j) {
+                    const bool ok_j = ok(j);
+                    const bool straddle = h_top(j - 1) >= h_mass && h_top(j) >= h_mass &&
+                                          rho_own(j - 1) <= r.rho_target[j] && r.rho_target[j] <= rho_own(j);
+                    if ((ok_prev && ok_j) || straddle) kb |= 1ull << j;
+                    ok_prev = ok_j;
+                }
+            }
+            auto K = [&](int j) -> bool { return bits ? ((kb >> j) & 1ull) != 0ull : kept(j); };
             double xg = 0.0;
             for (int j = 1; j < n; ++j) {
                 xg += h_top(j - 1);
-                if (kept(j)) zw[j] = xg;
+                if (K(j)) zw[j] = xg;
             }
             double lo = 0.0;
             for (int j = 1; j < n; ++j) {
-                if (kept(j))
+                if (K(j))
                     lo = zw[j];
                 else if (zw[j] < lo)
                     zw[j] = lo;
             }
             double hi = xgn;
             for (int j = n - 1; j > 0; --j) {
-                if (kept(j))
+                if (K(j))
                     hi = zw[j];
                 else if (zw[j] > hi)
                     zw[j] = hi;
