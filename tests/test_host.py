@@ -104,3 +104,31 @@ def test_poisson_job_stream_follows_the_serving_case():
     assert 0.01 < np.mean(grids == 64) < 0.03 and set(grids[grids != 64]) == set(range(64))
     again = serve.poisson_jobs(4000, rate=20.0, n_grids=65, big_grid=64, big_prob=0.02, seed=7)
     assert [(j.arrival, j.steps, j.grid) for j in again] == [(j.arrival, j.steps, j.grid) for j in jobs]
+
+
+def test_next_row_mirrors_reject_bad_arguments_before_any_kernel():
+    # the ALE, freshwater and vertical-coordinate mirrors check names, shapes and devices on the host and raise
+    # the reference's ValueError, with no GPU call (CPU tensors are refused as such)
+    import torch
+
+    from pyocean_amd import ale, freshwater, vertical
+    cpu = torch.zeros((2, 3, 4), dtype=torch.float64)
+    with pytest.raises(ValueError, match="thickness_flux"):
+        ale.LayerTransport(Lu=None, Lv=None, area=None, active=None, dx_t=None, dy_t=None, dt=1.0, theta=0.5,
+                           thickness_flux="weno")
+    with pytest.raises(ValueError, match="unknown open edge"):
+        ale.open_face_transports(None, None, cpu, None, None, None, None, ("up",), None, None, None, None, None,
+                                 None, 0.5)
+    assert ale.open_face_transports("Qx", "Qy", cpu, None, None, None, None, (), None, None, None, None, None,
+                                    None, 0.5) == ("Qx", "Qy")
+    with pytest.raises(ValueError, match="GPU tensor"):
+        freshwater.freshwater_volumes(cpu, torch.zeros((3, 4), dtype=torch.float64), 1e-6)
+    with pytest.raises(ValueError, match="contiguous float64 GPU tensor"):
+        freshwater.apply_freshwater(torch.zeros((2, 2, 3, 4), dtype=torch.float64), {"T": 0, "n": 2}, cpu, cpu,
+                                    torch.zeros((3, 4), dtype=torch.float64), dt=1.0)
+    with pytest.raises(ValueError, match="unknown rule"):
+        vertical.VerticalCoordinate(H=None, active=None, rule="isopycnal", params={})
+    with pytest.raises(ValueError, match="needs params"):
+        vertical.regrid(None, None, None, cpu, "hybrid", {"dz_top": [1.0]})
+    with pytest.raises(ValueError, match="layer densities rho"):
+        vertical.regrid(None, None, None, cpu, "hybrid", {"dz_top": [1.0], "rho_target": [1.0]})
