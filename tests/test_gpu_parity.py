@@ -204,6 +204,28 @@ def test_solid_body_steps_match_the_oracle(gpu, oracle_c):
     assert np.array_equal(C.cpu().numpy(), ref)
 
 
+def test_solid_body_full_run_of_100_steps_matches_the_oracle(gpu, oracle_c):
+    # the CPU-runnable benchmark case as specified (128 x 128 x 20, Gaussian and slotted cylinder, Courant 0.4,
+    # 100 steps): bit for bit with the oracle after every 25 steps, the content of each tracer kept, the cylinder's
+    # values inside [0, 1] (no new extrema under FCT)
+    _say("solid body 100 steps")
+    case = synth.solid_body()
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()        # noqa: E731
+    C, V0, V1, Qx, Qy, ok = (d(case.C), d(case.V0), d(case.V1), d(case.Qx), d(case.Qy),
+                             d(case.ok.astype(np.uint8)))
+    ref = case.C
+    c0 = (case.V0[None] * case.C).sum(axis=(1, 2, 3))
+    lo, hi = case.C.min(axis=(1, 2, 3)), case.C.max(axis=(1, 2, 3))
+    for s in range(1, 101):
+        C = ops.advect_fct(C, V0, V1, Qx, Qy, ok, case.dt)
+        ref = oracle_c.advect_fct(ref, case.V0, case.V1, case.Qx, case.Qy, case.ok, case.dt)
+        if s % 25 == 0:
+            assert np.array_equal(C.cpu().numpy(), ref), f"step {s}"
+    c1 = (case.V1[None] * ref).sum(axis=(1, 2, 3))
+    assert np.all(np.abs(c1 - c0) <= 1e-12 * np.abs(c0))
+    assert np.all(ref.min(axis=(1, 2, 3)) >= lo - 1e-12) and np.all(ref.max(axis=(1, 2, 3)) <= hi + 1e-12)
+
+
 def test_regional_case_matches_the_oracle(gpu, oracle_c):
     # land mask, z* layers with 1 mm sub-bed layers, curvilinear metrics: 2 steps, bit for bit
     _say("regional 160x144x12")
