@@ -31,6 +31,8 @@ extern "C" {
 #define OCX_OPEN_N 8u
 #define OCX_WENO_JS 16u    /* WENO5-JS instead of WENO5-Z (tracer_scheme == "weno5js") */
 #define OCX_UNLIMITED 32u  /* FCT coefficients = 1: the unlimited RK3 result (advect_fct(unlimited=True)) */
+#define OCX_UP3 64u        /* UP3 faces (tracer_scheme == "upwind3"): RK3 step, RHS and faces_h only */
+#define OCX_UP1 128u       /* first-order upwind faces (tracer_scheme == "upwind1"): as OCX_UP3 */
 
 /* error codes */
 #define OCX_OK 0
@@ -89,14 +91,17 @@ int ocx_advect_fct_split(const double* C, const double* V0, const double* V1, co
 /* One SSP-RK3 step without limiter, x, y and (if vert) z sweeps.  Replaces TracerAdvection.advect
  * (core/components/tracers.py:342-375) with _advect_div and _rk_stage (sigma_baroclinic.py:324-364, 856-875).
  * Fz (nz+1, ny, nx) may be NULL when vert == 0; vert as the reference's any(Fz != 0).  fx_i as above
- * ((F(c0) + F(c1))/6 + 2 F(c2)/3 of column flux_face_i). */
+ * ((F(c0) + F(c1))/6 + 2 F(c2)/3 of column flux_face_i).  flags may carry OCX_UP3 or OCX_UP1 (not with
+ * OCX_WENO_JS): the faces of rhs_numpy (tracers.py:139-151, _up3_faces sigma_baroclinic.py:164-187) instead of
+ * WENO, whose z faces are always formed, so Fz is then required and vert ignored. */
 int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
                        const double* Fz, const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx,
                        double dt, unsigned flags, int vert, int flux_face_i, double* C_out, double* fx_i,
                        void* workspace, size_t workspace_bytes, void* stream);
 
 /* WENO horizontal face fluxes (fx, fy) of the stack c.  Replaces _advect_faces_h (sigma_baroclinic.py:265-321)
- * as called by TracerAdvection.faces_h (tracers.py:166-176).  ext: exterior values or NULL (= c). */
+ * as called by TracerAdvection.faces_h (tracers.py:166-176).  ext: exterior values or NULL (= c).  With OCX_UP3 /
+ * OCX_UP1: the UP face fluxes that rhs_numpy differences (flux_x for those schemes, tracers.py:194-206). */
 int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const uint8_t* ok, const double* ext,
                     int nt, int nz, int ny, int nx, unsigned flags, double* fx, double* fy, void* stream);
 
@@ -105,7 +110,8 @@ int ocx_upwind_faces_h_f64(const double* c, const double* Qx, const double* Qy, 
                            int ny, int nx, unsigned flags, double* fx, double* fy, void* stream);
 
 /* Advection RHS -div(Q c) with x, y and (vert) z WENO faces.  Replaces _advect_div (sigma_baroclinic.py:324-364)
- * as called by TracerAdvection.rhs (tracers.py:153-164). */
+ * as called by TracerAdvection.rhs (tracers.py:153-164); with OCX_UP3 / OCX_UP1 replaces rhs_numpy
+ * (tracers.py:139-151): x, y and z UP faces, Fz required. */
 int ocx_rhs_f64(const double* c, const double* Qx, const double* Qy, const double* Fz, const uint8_t* ok,
                 const double* ext, int nt, int nz, int ny, int nx, unsigned flags, int vert, double* out,
                 void* stream);

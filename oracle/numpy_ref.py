@@ -190,6 +190,71 @@ def rhs(c, Qx, Qy, Fz, ok, ext, open_mask=0, z_variant=True, vert=None):
     return -(fx[..., 1:] - fx[..., :-1] + fy[..., 1:, :] - fy[..., :-1, :])
 
 
+def up_faces(c, flux, ok, axis, order=3):
+    """
+    UP3 (``order`` 3) or donor-cell (``order`` 1) face values - not yet times the flux - on the faces along
+    ``axis``: face f (1 <= f <= n-1, between cells f-1 and f) takes, for flux >= 0, (-c[f-2] + 5 c[f-1] + 2 c[f]) / 6
+    where cells f-2, f-1, f are all ok, else c[f-1]; for flux < 0 (or NaN), (2 c[f-1] + 5 c[f] - c[f+1]) / 6 where
+    cells f-1, f, f+1 are ok, else c[f]; cells outside 0..n-1 count as not ok.  Faces 0 and the last are 0.
+    ``flux`` has n+1 faces (x, y) or n (z: Fz without the surface).  Follows ``_up3_faces``
+    (sigma_baroclinic.py:164-187).
+    """
+    cm = np.moveaxis(np.asarray(c, float), axis, -1)
+    om = np.moveaxis(np.asarray(ok).astype(bool), axis, -1)
+    qm = np.moveaxis(np.asarray(flux, float), axis, -1)
+    n = cm.shape[-1]
+    out = np.zeros(qm.shape)
+    if n >= 2:
+        lo, hi = cm[..., :-1], cm[..., 1:]
+        qf = qm[..., 1:n]
+        if order == 1:
+            val = np.where(qf >= 0, lo, hi)
+        else:
+            pad_c = np.zeros(cm.shape[:-1] + (1,))
+            pad_o = np.zeros(om.shape[:-1] + (1,), bool)
+            ce = np.concatenate([pad_c, cm, pad_c], axis=-1)            # ce[..., m + 1] = cell m
+            oe = np.concatenate([pad_o, om, pad_o], axis=-1)
+            f = np.arange(1, n)
+            olo, ohi = om[..., :-1], om[..., 1:]
+            with np.errstate(invalid="ignore", over="ignore"):
+                pos = np.where(oe[..., f - 1] & olo & ohi, (-ce[..., f - 1] + 5.0 * lo + 2.0 * hi) / 6.0, lo)
+                neg = np.where(olo & ohi & oe[..., f + 2], (2.0 * lo + 5.0 * hi - ce[..., f + 2]) / 6.0, hi)
+            val = np.where(qf >= 0, pos, neg)
+        out[..., 1:n] = val
+    return np.moveaxis(out, -1, axis)
+
+
+def faces_up_h(c, Qx, Qy, ok, ext, open_mask=0, order=3):
+    """
+    Horizontal face fluxes (fx, fy) = Q * face of the UP3 / UP1 schemes: every face, interior and closed edge, is
+    Q times its ``up_faces`` value (a closed edge Q * 0); open edges as ``faces_h``.  Follows ``rhs_numpy`` and
+    ``open_faces`` (tracers.py:117-151) and ``flux_x`` (tracers.py:194-206).
+    """
+    c = np.asarray(c, float)
+    nt = c.shape[0]
+    okb = np.broadcast_to(ok, c.shape)
+    Qxb = np.broadcast_to(Qx, (nt,) + Qx.shape)
+    Qyb = np.broadcast_to(Qy, (nt,) + Qy.shape)
+    fx = Qxb * up_faces(c, Qxb, okb, 3, order)
+    fy = Qyb * up_faces(c, Qyb, okb, 2, order)
+    _open_edges(fx, fy, c, Qx, Qy, ext, open_mask, donor=False)
+    return fx, fy
+
+
+def rhs_up(c, Qx, Qy, Fz, ok, ext, open_mask=0, order=3):
+    """
+    Advection RHS -div(Q c) of the stack with UP3 / UP1 faces on x, y and z, the z faces always formed:
+    fz[k] = Fz[k] * face_k for k < N (face_0 = 0), fz[N] = 0.  Follows ``rhs_numpy`` (tracers.py:139-151).
+    """
+    c = np.asarray(c, float)
+    nt, n, ny, nx = c.shape
+    fx, fy = faces_up_h(c, Qx, Qy, ok, ext, open_mask, order)
+    Fzb = np.broadcast_to(np.asarray(Fz, float)[:-1], c.shape)
+    fz = np.zeros((nt, n + 1, ny, nx))
+    fz[:, :-1] = Fzb * up_faces(c, Fzb, np.broadcast_to(ok, c.shape), 1, order)
+    return -(fx[..., 1:] - fx[..., :-1] + fy[..., 1:, :] - fy[..., :-1, :] + fz[:, 1:] - fz[:, :-1])
+
+
 # ------------------------------------------------------------------------------------------------ FCT
 def fct_coefficients(c0, cL, V1, Ax, Ay, ok, dt):
     """
@@ -294,12 +359,14 @@ def open_inflow(Fx, Fy, open_mask):
     return out
 
 
-def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, open_mask=0, z_variant=True, flux_face_i=None):
+def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, open_mask=0, z_variant=True, flux_face_i=None,
+               up_order=0):
     """
     One SSP-RK3 step without limiter, x, y and z sweeps.  Follows ``advect`` (tracers.py:342-375) with
     ``_rk_stage`` (sigma_baroclinic.py:856-875): q1 = q0 + dt r0, q2 = 0.75 q0 + 0.25 (q1 + dt r1),
     q3 = q0/3 + 2/3 (q2 + dt r2), each with the ``safe`` division by Va, Vb, V1.  The exterior values are
-    ``inflow`` in every stage, else each stage's own values.  Returns ``out`` or (out, fx_i).
+    ``inflow`` in every stage, else each stage's own values.  ``up_order`` 3 or 1: the UP3 / UP1 faces of
+    ``rhs_up`` instead of WENO.  Returns ``out`` or (out, fx_i).
     """
     C = np.ascontiguousarray(C, dtype=float)
     Qx, Qy, Fz = np.asarray(Qx, float), np.asarray(Qy, float), np.asarray(Fz, float)
@@ -310,18 +377,21 @@ def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, open_mask=0, z_varian
     dV = V1 - V0
     Va, Vb = V0 + dV, V0 + 0.5 * dV
     q0 = V0 * C
-    r0 = rhs(C, Qx, Qy, Fz, okc, ext(C), open_mask, z_variant, vert)
-    q1 = q0 + dt * r0
+    if up_order:
+        r = lambda c: rhs_up(c, Qx, Qy, Fz, okc, ext(c), open_mask, up_order)                   # noqa: E731
+        fx = lambda c: faces_up_h(c, Qx, Qy, okc, ext(c), open_mask, up_order)[0]              # noqa: E731
+    else:
+        r = lambda c: rhs(c, Qx, Qy, Fz, okc, ext(c), open_mask, z_variant, vert)               # noqa: E731
+        fx = lambda c: faces_h(c, Qx, Qy, okc, ext(c), open_mask, z_variant)[0]                # noqa: E731
+    q1 = q0 + dt * r(C)
     c1 = _safe(q1, Va, C)
-    r1 = rhs(c1, Qx, Qy, Fz, okc, ext(c1), open_mask, z_variant, vert)
-    q2 = 0.75 * q0 + 0.25 * (q1 + dt * r1)
+    q2 = 0.75 * q0 + 0.25 * (q1 + dt * r(c1))
     c2 = _safe(q2, Vb, C)
-    r2 = rhs(c2, Qx, Qy, Fz, okc, ext(c2), open_mask, z_variant, vert)
-    q3 = q0 / 3.0 + 2.0 / 3.0 * (q2 + dt * r2)
+    q3 = q0 / 3.0 + 2.0 / 3.0 * (q2 + dt * r(c2))
     out = _safe(q3, V1, C)
     if flux_face_i is None:
         return out
-    fxi = lambda c: faces_h(c, Qx, Qy, okc, ext(c), open_mask, z_variant)[0][..., flux_face_i]   # noqa: E731
+    fxi = lambda c: fx(c)[..., flux_face_i]                                                     # noqa: E731
     return out, (fxi(C) + fxi(c1)) / 6.0 + 2.0 * fxi(c2) / 3.0
 
 

@@ -16,7 +16,8 @@ LIB_PATH = Path(os.environ.get("OCX_LIB", Path(__file__).resolve().parent / "lib
 OPEN_W, OPEN_E, OPEN_S, OPEN_N = 1, 2, 4, 8
 WENO_JS = 16
 UNLIMITED = 32
-SPLIT = 64
+UP3 = 64
+UP1 = 128
 
 OCX_E_ARG, OCX_E_HIP, OCX_E_WORKSPACE, OCX_E_RANGE = -1, -2, -3, -4
 OUTFLOW_NOOP, OUTFLOW_OK, OUTFLOW_ACCEPTED, OUTFLOW_EXCEEDS, OUTFLOW_EMPTY_DONOR, OUTFLOW_NEGATIVE = range(6)

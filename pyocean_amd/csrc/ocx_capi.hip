@@ -138,6 +138,50 @@ dim3 tile_grid(int nt, int nz, int ny, int nx) {
     return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + TY - 1) / TY), (unsigned)std::min(planes, 65535LL));
 }
 
+// Face scheme of the flags.  The caller has rejected flags with more than one of UP3, UP1, WENO_JS.
+int scheme_of(unsigned flags) {
+    using namespace ocx;
+    return (flags & UP1) ? SCH_UP1 : (flags & UP3) ? SCH_UP3 : (flags & WENO_JS) ? SCH_JS : SCH_Z;
+}
+
+// Arguments of one k_rk_stage launch.
+struct RkArgs {
+    const double *cin, *C, *ext, *V0, *V1, *Qx, *Qy, *Fz;
+    const uint8_t* ok;
+    double *q, *cout, *fxs;
+    int fi, nt, nz, ny, nx;
+    double dt;
+    unsigned flags;
+};
+
+template <int STAGE, int SCH, bool VERT>
+void rk_launch(const RkArgs& a, dim3 grid, hipStream_t s) {
+    ocx::k_rk_stage<STAGE, SCH, VERT, TX, TY><<<grid, dim3(TX, TY), 0, s>>>(
+        a.cin, a.C, a.ext, a.V0, a.V1, a.Qx, a.Qy, a.Fz, a.ok, a.q, a.cout, a.fxs, a.fi, a.nt, a.nz, a.ny, a.nx, a.dt,
+        a.flags);
+}
+
+// One RK3 stage (STAGE 0: the RHS only) with the scheme's faces; UP3 / UP1 always form the z faces.
+template <int STAGE>
+void rk_stage(const RkArgs& a, int sch, bool vert, dim3 grid, hipStream_t s) {
+    using namespace ocx;
+    switch (sch) {
+        case SCH_UP3: rk_launch<STAGE, SCH_UP3, true>(a, grid, s); break;
+        case SCH_UP1: rk_launch<STAGE, SCH_UP1, true>(a, grid, s); break;
+        case SCH_JS: vert ? rk_launch<STAGE, SCH_JS, true>(a, grid, s) : rk_launch<STAGE, SCH_JS, false>(a, grid, s); break;
+        default: vert ? rk_launch<STAGE, SCH_Z, true>(a, grid, s) : rk_launch<STAGE, SCH_Z, false>(a, grid, s);
+    }
+}
+
+// Flags of the RK3 step, the RHS and faces_h: open edges plus at most one of WENO_JS, UP3, UP1.
+int check_rk_flags(unsigned flags, const char* what) {
+    using namespace ocx;
+    if (flags & ~(31u | UP3 | UP1)) return fail(OCX_E_ARG, "flags 0x%x not valid for %s", flags, what);
+    const int schemes = !!(flags & WENO_JS) + !!(flags & UP3) + !!(flags & UP1);
+    if (schemes > 1) return fail(OCX_E_ARG, "flags 0x%x name more than one face scheme", flags);
+    return OCX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -335,9 +379,10 @@ int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, cons
     OCX_TRY(need(Qy, "Qy"));
     OCX_TRY(need(ok, "ok"));
     OCX_TRY(need(C_out, "C_out"));
-    if (vert) OCX_TRY(need(Fz, "Fz"));
+    const bool up = flags & (UP3 | UP1);
+    if (vert || up) OCX_TRY(need(Fz, "Fz"));
     if (C_out == C) return fail(OCX_E_ARG, "C_out must not alias C");
-    if (flags & ~31u) return fail(OCX_E_ARG, "flags 0x%x not valid for the RK3 step", flags);
+    OCX_TRY(check_rk_flags(flags, "the RK3 step"));
     if (flux_face_i != -1 && (flux_face_i < 0 || flux_face_i > nx))
         return fail(OCX_E_ARG, "flux_face_i %d outside 0..%d", flux_face_i, nx);
     if (flux_face_i >= 0 && !fx_i) return fail(OCX_E_ARG, "fx_i is NULL with flux_face_i = %d", flux_face_i);
@@ -347,35 +392,32 @@ int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, cons
     Rk3Ws w;
     rk3_layout(workspace, nt, nz, ny, nx, &w);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
-    const bool js = flags & WENO_JS;
+    const dim3 grid = tile_grid(nt, nz, ny, nx);
+    const int sch = scheme_of(flags);
     const int fi = fx_i ? flux_face_i : -1;
     const size_t col = (size_t)nt * nz * ny;
     double* f0 = fx_i ? w.fxs : nullptr;
     double* f1 = fx_i ? w.fxs + col : nullptr;
     double* f2 = fx_i ? w.fxs + 2 * col : nullptr;
-#define OCX_RK(STAGE, CIN, COUT, FS)                                                                                  \
-    do {                                                                                                             \
-        ProfScope _ps("rk3_stage" #STAGE, s);                                                                        \
-        if (js && vert)                                                                                              \
-            k_rk_stage<STAGE, false, true, TX, TY><<<grid, block, 0, s>>>(CIN, C, inflow, V0, V1, Qx, Qy, Fz, ok, w.q, \
-                                                                          COUT, FS, fi, nt, nz, ny, nx, dt, flags);   \
-        else if (js)                                                                                                 \
-            k_rk_stage<STAGE, false, false, TX, TY><<<grid, block, 0, s>>>(CIN, C, inflow, V0, V1, Qx, Qy, Fz, ok,    \
-                                                                           w.q, COUT, FS, fi, nt, nz, ny, nx, dt,     \
-                                                                           flags);                                    \
-        else if (vert)                                                                                               \
-            k_rk_stage<STAGE, true, true, TX, TY><<<grid, block, 0, s>>>(CIN, C, inflow, V0, V1, Qx, Qy, Fz, ok, w.q,  \
-                                                                         COUT, FS, fi, nt, nz, ny, nx, dt, flags);    \
-        else                                                                                                         \
-            k_rk_stage<STAGE, true, false, TX, TY><<<grid, block, 0, s>>>(CIN, C, inflow, V0, V1, Qx, Qy, Fz, ok, w.q, \
-                                                                          COUT, FS, fi, nt, nz, ny, nx, dt, flags);   \
-        OCX_TRY(launch_check("rk3 stage " #STAGE));                                                                  \
-    } while (0)
-    OCX_RK(1, C, w.c1, f0);
-    OCX_RK(2, w.c1, w.c2, f1);
-    OCX_RK(3, w.c2, C_out, f2);
-#undef OCX_RK
+    RkArgs a{nullptr, C, inflow, V0, V1, Qx, Qy, Fz, ok, w.q, nullptr, nullptr, fi, nt, nz, ny, nx, dt, flags};
+    {
+        ProfScope _ps("rk3_stage1", s);
+        a.cin = C, a.cout = w.c1, a.fxs = f0;
+        rk_stage<1>(a, sch, vert, grid, s);
+        OCX_TRY(launch_check("rk3 stage 1"));
+    }
+    {
+        ProfScope _ps("rk3_stage2", s);
+        a.cin = w.c1, a.cout = w.c2, a.fxs = f1;
+        rk_stage<2>(a, sch, vert, grid, s);
+        OCX_TRY(launch_check("rk3 stage 2"));
+    }
+    {
+        ProfScope _ps("rk3_stage3", s);
+        a.cin = w.c2, a.cout = C_out, a.fxs = f2;
+        rk_stage<3>(a, sch, vert, grid, s);
+        OCX_TRY(launch_check("rk3 stage 3"));
+    }
     if (fx_i) {
         k_rk_face_column<<<(unsigned)((col + 255) / 256), 256, 0, s>>>(f0, f1, f2, fx_i, col);
         OCX_TRY(launch_check("rk3 face column"));
@@ -393,13 +435,15 @@ int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const u
     OCX_TRY(need(ok, "ok"));
     OCX_TRY(need(fx, "fx"));
     OCX_TRY(need(fy, "fy"));
-    if (flags & ~31u) return fail(OCX_E_ARG, "flags 0x%x not valid for faces_h", flags);
+    OCX_TRY(check_rk_flags(flags, "faces_h"));
     hipStream_t s = static_cast<hipStream_t>(stream);
     const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
-    if (flags & WENO_JS)
-        k_faces<false, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags);
-    else
-        k_faces<true, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags);
+    switch (scheme_of(flags)) {
+        case SCH_UP3: k_faces<SCH_UP3, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags); break;
+        case SCH_UP1: k_faces<SCH_UP1, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags); break;
+        case SCH_JS: k_faces<SCH_JS, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags); break;
+        default: k_faces<SCH_Z, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags);
+    }
     return launch_check("faces_h");
 }
 
@@ -429,23 +473,12 @@ int ocx_rhs_f64(const double* c, const double* Qx, const double* Qy, const doubl
     OCX_TRY(need(Qy, "Qy"));
     OCX_TRY(need(ok, "ok"));
     OCX_TRY(need(out, "out"));
-    if (vert) OCX_TRY(need(Fz, "Fz"));
+    const bool up = flags & (UP3 | UP1);
+    if (vert || up) OCX_TRY(need(Fz, "Fz"));
+    OCX_TRY(check_rk_flags(flags, "the RHS"));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
-    const bool js = flags & WENO_JS;
-    if (js && vert)
-        k_rk_stage<0, false, true, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr,
-                                                                  out, nullptr, -1, nt, nz, ny, nx, 0.0, flags);
-    else if (js)
-        k_rk_stage<0, false, false, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok,
-                                                                   nullptr, out, nullptr, -1, nt, nz, ny, nx, 0.0,
-                                                                   flags);
-    else if (vert)
-        k_rk_stage<0, true, true, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr,
-                                                                 out, nullptr, -1, nt, nz, ny, nx, 0.0, flags);
-    else
-        k_rk_stage<0, true, false, TX, TY><<<grid, block, 0, s>>>(c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr,
-                                                                  out, nullptr, -1, nt, nz, ny, nx, 0.0, flags);
+    const RkArgs a{c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr, out, nullptr, -1, nt, nz, ny, nx, 0.0, flags};
+    rk_stage<0>(a, scheme_of(flags), vert, tile_grid(nt, nz, ny, nx), s);
     return launch_check("rhs");
 }
 

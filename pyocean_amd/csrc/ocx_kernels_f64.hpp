@@ -31,12 +31,13 @@ __device__ __forceinline__ void load_tile(double* sc, uint8_t* so, const double*
     }
 }
 
-// WENO face fluxes q * c_face of the tile's faces into LDS (sfx: TY rows x TX+1 faces; sfy: TY+1 faces x TX
-// columns), from the input tile sc/so (halo 3).  Rules of _advect_faces_h (sigma_baroclinic.py:265-321):
-// interior faces with q == 0 stay 0; open edges take ext on inflow, the edge cell on outflow; closed edges 0.
-// ext == nullptr: the input itself is the exterior.
-template <bool Z, int TX, int TY>
-__device__ __forceinline__ void weno_tile_faces(double* sfx, double* sfy, const double* sc, const uint8_t* so,
+// Face fluxes q * c_face of the tile's faces into LDS (sfx: TY rows x TX+1 faces; sfy: TY+1 faces x TX columns),
+// from the input tile sc/so (halo 3).  WENO (SCH_JS, SCH_Z), rules of _advect_faces_h (sigma_baroclinic.py:
+// 265-321): interior faces with q == 0 stay 0; open edges take ext on inflow, the edge cell on outflow; closed
+// edges 0.  UP3 / UP1 (SCH_UP3, SCH_UP1), rules of rhs_numpy (tracers.py:139-151): every interior face is q times
+// the _up3_faces value, a closed edge is q * 0, open edges as WENO.  ext == nullptr: the input is the exterior.
+template <int SCH, int TX, int TY>
+__device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const double* sc, const uint8_t* so,
                                                 const double* __restrict__ Qxk, const double* __restrict__ Qyk,
                                                 const double* __restrict__ extp, int i0, int j0, int ny, int nx,
                                                 unsigned flags) {
@@ -51,17 +52,24 @@ __device__ __forceinline__ void weno_tile_faces(double* sfx, double* sfy, const 
             const double* row = sc + (ly + H) * LX;
             const uint8_t* orow = so + (ly + H) * LX;
             if (f > 0 && f < nx) {
-                if (q != 0.0) val = q * lds_face<Z>(row, orow, 1, lf + H, q);
+                if constexpr (SCH >= SCH_UP3)
+                    val = q * up_face<SCH == SCH_UP3>(row, orow, 1, lf + H, q);
+                else if (q != 0.0)
+                    val = q * lds_face<SCH == SCH_Z>(row, orow, 1, lf + H, q);
             } else if (f == 0) {
                 if (flags & OPEN_W) {
                     const double cc = row[lf + H];
                     const double e = extp ? extp[(size_t)j * nx] : cc;
                     val = q * (q > 0.0 ? e : cc);
+                } else if constexpr (SCH >= SCH_UP3) {
+                    val = q * 0.0;
                 }
             } else if (flags & OPEN_E) {
                 const double cc = row[lf + H - 1];
                 const double e = extp ? extp[(size_t)j * nx + nx - 1] : cc;
                 val = q * (q < 0.0 ? e : cc);
+            } else if constexpr (SCH >= SCH_UP3) {
+                val = q * 0.0;
             }
         }
         sfx[idx] = val;
@@ -75,17 +83,24 @@ __device__ __forceinline__ void weno_tile_faces(double* sfx, double* sfy, const 
             const double* col = sc + lx + H;
             const uint8_t* ocol = so + lx + H;
             if (f > 0 && f < ny) {
-                if (q != 0.0) val = q * lds_face<Z>(col, ocol, LX, lf + H, q);
+                if constexpr (SCH >= SCH_UP3)
+                    val = q * up_face<SCH == SCH_UP3>(col, ocol, LX, lf + H, q);
+                else if (q != 0.0)
+                    val = q * lds_face<SCH == SCH_Z>(col, ocol, LX, lf + H, q);
             } else if (f == 0) {
                 if (flags & OPEN_S) {
                     const double cc = col[(lf + H) * LX];
                     const double e = extp ? extp[i] : cc;
                     val = q * (q > 0.0 ? e : cc);
+                } else if constexpr (SCH >= SCH_UP3) {
+                    val = q * 0.0;
                 }
             } else if (flags & OPEN_N) {
                 const double cc = col[(lf + H - 1) * LX];
                 const double e = extp ? extp[(size_t)(ny - 1) * nx + i] : cc;
                 val = q * (q < 0.0 ? e : cc);
+            } else if constexpr (SCH >= SCH_UP3) {
+                val = q * 0.0;
             }
         }
         sfy[idx] = val;
@@ -114,9 +129,9 @@ __device__ __forceinline__ double dmax(double a, double b) { return b > a ? b : 
 __device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
 
 // ------------------------------------------------------------------------------------------------ faces
-// WENO face fluxes fx, fy of cin on every owned face (the faces_h helper; _advect_faces_h,
-// sigma_baroclinic.py:265-321).
-template <bool Z, int TX, int TY>
+// Face fluxes fx, fy of cin on every owned face: WENO (the faces_h helper; _advect_faces_h,
+// sigma_baroclinic.py:265-321) or UP3 / UP1 (the x faces of flux_x, tracers.py:194-206).
+template <int SCH, int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_faces(const double* __restrict__ cin, const double* __restrict__ ext,
                                                   const double* __restrict__ Qx, const double* __restrict__ Qy,
                                                   const uint8_t* __restrict__ ok, double* __restrict__ fx,
@@ -136,8 +151,8 @@ __global__ __launch_bounds__(TX* TY) void k_faces(const double* __restrict__ cin
         __syncthreads();
         load_tile<TX, TY, H>(sc, so, cin + pc, ok + kc, i0, j0, ny, nx);
         __syncthreads();
-        weno_tile_faces<Z, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
-                                   ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
+        tile_faces<SCH, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
+                                ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
         __syncthreads();
         for (int idx = tid; idx < TY * (TX + 1); idx += TX * TY) {
             const int ly = idx / (TX + 1), lf = idx - ly * (TX + 1);
@@ -229,11 +244,30 @@ __device__ __forceinline__ double zface(const double* __restrict__ ccol, const u
     return q * face_value<Z>(v[0], v[1], v[2], v[3], v[4], b[0], b[1], b[2], b[3], b[4]);
 }
 
+// UP3 / UP1 face value (not yet times Fz) of interface kf (1 <= kf <= n-1) of one column: _up3_faces along the
+// layer index (rhs_numpy, tracers.py:148-150), layers outside 0..n-1 invalid.
+template <bool O3>
+__device__ __forceinline__ double zface_up(const double* __restrict__ ccol, const uint8_t* __restrict__ ocol,
+                                           size_t plane, int n, int kf, double q) {
+    const double lo = ccol[(size_t)(kf - 1) * plane], hi = ccol[(size_t)kf * plane];
+    if (!O3) return q >= 0.0 ? lo : hi;
+    const bool olo = ocol[(size_t)(kf - 1) * plane] != 0, ohi = ocol[(size_t)kf * plane] != 0;
+    if (q >= 0.0) {
+        if (kf >= 2 && ocol[(size_t)(kf - 2) * plane] != 0 && olo && ohi)
+            return (-ccol[(size_t)(kf - 2) * plane] + 5.0 * lo + 2.0 * hi) / 6.0;
+        return lo;
+    }
+    if (kf + 1 < n && olo && ohi && ocol[(size_t)(kf + 1) * plane] != 0)
+        return (2.0 * lo + 5.0 * hi - ccol[(size_t)(kf + 1) * plane]) / 6.0;
+    return hi;
+}
+
 // SSP-RK3 stage of TracerAdvection.advect (tracers.py:342-375, _rk_stage sigma_baroclinic.py:856-875) with the
-// fused RHS r = -div(Q c) of _advect_div (x, y and, with VERT, z faces).
+// fused RHS r = -div(Q c): WENO faces of _advect_div (x, y and, with VERT, z), or UP3 / UP1 faces of rhs_numpy
+// (tracers.py:139-151), whose z faces are always formed (fz = Fz * face, the bed face Fz * 0, the surface 0).
 // STAGE 0: r only.  1: q1 = q0 + dt r, c1 = q1/Va.  2: q2 = 0.75 q0 + 0.25 (q1 + dt r), c2 = q2/Vb (q in place).
 // 3: q3 = q0/3 + 2/3 (q2 + dt r), out = q3/V1.  fxs (optional): the x-face column flux_face_i of this stage.
-template <int STAGE, bool Z, bool VERT, int TX, int TY>
+template <int STAGE, int SCH, bool VERT, int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ cin, const double* __restrict__ C,
                                                      const double* __restrict__ ext, const double* __restrict__ V0,
                                                      const double* __restrict__ V1, const double* __restrict__ Qx,
@@ -256,25 +290,40 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
         __syncthreads();
         load_tile<TX, TY, H>(sc, so, cin + pc, ok + kc, i0, j0, ny, nx);
         __syncthreads();
-        weno_tile_faces<Z, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
-                                   ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
+        tile_faces<SCH, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
+                                ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
         __syncthreads();
         const int j = j0 + ty, i = i0 + tx;
         if (j < ny && i < nx) {
             const size_t xc = (size_t)j * nx + i, x = pc + xc, xv = kc + xc;
             double d = sfx[ty * (TX + 1) + tx + 1] - sfx[ty * (TX + 1) + tx] + sfy[(ty + 1) * TX + tx] -
                        sfy[ty * TX + tx];
-            if (VERT) {
+            if constexpr (SCH >= SCH_UP3) {
+                const double* ccol = cin + (size_t)t * n * plane + xc;
+                const uint8_t* ocol = ok + xc;
+                double ft = 0.0, fb;
+                if (k + 1 < n) {
+                    const double qz = Fz[(size_t)(k + 1) * plane + xc];
+                    ft = qz * zface_up<SCH == SCH_UP3>(ccol, ocol, plane, n, k + 1, qz);
+                }
+                if (k > 0) {
+                    const double qz = Fz[(size_t)k * plane + xc];
+                    fb = qz * zface_up<SCH == SCH_UP3>(ccol, ocol, plane, n, k, qz);
+                } else {
+                    fb = Fz[xc] * 0.0;
+                }
+                d = d + ft - fb;
+            } else if (VERT) {
                 const double* ccol = cin + (size_t)t * n * plane + xc;
                 const uint8_t* ocol = ok + xc;
                 double ft = 0.0, fb = 0.0;
                 if (k + 1 < n) {
                     const double qz = Fz[(size_t)(k + 1) * plane + xc];
-                    if (qz != 0.0) ft = zface<Z>(ccol, ocol, plane, n, k + 1, qz);
+                    if (qz != 0.0) ft = zface<SCH == SCH_Z>(ccol, ocol, plane, n, k + 1, qz);
                 }
                 if (k > 0) {
                     const double qz = Fz[(size_t)k * plane + xc];
-                    if (qz != 0.0) fb = zface<Z>(ccol, ocol, plane, n, k, qz);
+                    if (qz != 0.0) fb = zface<SCH == SCH_Z>(ccol, ocol, plane, n, k, qz);
                 }
                 d = d + ft - fb;
             }

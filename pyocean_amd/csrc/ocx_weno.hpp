@@ -14,6 +14,11 @@ namespace ocx {
 constexpr unsigned OPEN_W = 1u, OPEN_E = 2u, OPEN_S = 4u, OPEN_N = 8u;
 constexpr unsigned WENO_JS = 16u;      // WENO5-JS instead of WENO5-Z
 constexpr unsigned UNLIMITED = 32u;    // FCT coefficients = 1 (the RK3 result)
+constexpr unsigned UP3 = 64u;          // third-order upwind faces (tracer_scheme == "upwind3"), RK3 step only
+constexpr unsigned UP1 = 128u;         // first-order upwind faces (tracer_scheme == "upwind1"), RK3 step only
+
+// Face schemes of the tile kernels: WENO5-JS, WENO5-Z, UP3, UP1.
+constexpr int SCH_JS = 0, SCH_Z = 1, SCH_UP3 = 2, SCH_UP1 = 3;
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
@@ -96,6 +101,19 @@ __device__ __forceinline__ double lds_face(const double* s, const uint8_t* o, in
     const int i4 = pos ? c : c - 1, i5 = pos ? c + 1 : c - 2;
     return face_value<Z>(s[i1 * st], s[i2 * st], s[i3 * st], s[i4 * st], s[i5 * st],
                          o[i1 * st] != 0, o[i2 * st] != 0, o[i3 * st] != 0, o[i4 * st] != 0, o[i5 * st] != 0);
+}
+
+// UP3 / UP1 face value between cells c-1 and c of a line (values s, validity o, stride st), _up3_faces
+// (sigma_baroclinic.py:164-187): q >= 0 takes (-c[i-2] + 5 c[i-1] + 2 c[i]) / 6 where cells i-2 .. i are ok, else
+// c[i-1]; q < 0 (or NaN) takes (2 c[i-1] + 5 c[i] - c[i+1]) / 6 where cells i-1 .. i+1 are ok, else c[i].  UP1 is
+// the donor value.  The line must hold valid indices c-2 .. c+1 (validity 0 outside the domain).
+template <bool O3>
+__device__ __forceinline__ double up_face(const double* s, const uint8_t* o, int st, int c, double q) {
+    const double lo = s[(c - 1) * st], hi = s[c * st];
+    if (!O3) return q >= 0.0 ? lo : hi;
+    if (q >= 0.0)
+        return (o[(c - 2) * st] && o[(c - 1) * st] && o[c * st]) ? (-s[(c - 2) * st] + 5.0 * lo + 2.0 * hi) / 6.0 : lo;
+    return (o[(c - 1) * st] && o[c * st] && o[(c + 1) * st]) ? (2.0 * lo + 5.0 * hi - s[(c + 1) * st]) / 6.0 : hi;
 }
 
 }  // namespace ocx

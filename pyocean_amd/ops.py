@@ -72,8 +72,13 @@ _WS = Workspace()
 _WS_CONTENT = Workspace()
 
 
-def flags_of(open_mask: int = 0, js: bool = False, unlimited: bool = False) -> int:
-    return int(open_mask) | (_native.WENO_JS if js else 0) | (_native.UNLIMITED if unlimited else 0)
+def flags_of(open_mask: int = 0, js: bool = False, unlimited: bool = False, up: int = 0) -> int:
+    """Flag word of the C-ABI: open edges, WENO5-JS faces, the unlimited FCT result, or ``up`` = 3 / 1 for the
+    UP3 / donor-cell faces of the RK3 step."""
+    if up not in (0, 1, 3):
+        raise ValueError(f"up = {up}: 0, 1 or 3")
+    return (int(open_mask) | (_native.WENO_JS if js else 0) | (_native.UNLIMITED if unlimited else 0)
+            | {0: 0, 1: _native.UP1, 3: _native.UP3}[up])
 
 
 PRECISIONS = ("f64", "split")
@@ -113,6 +118,14 @@ def advect_fct(C, V0, V1, Qx, Qy, ok, dt, inflow=None, flags=0, flux_face_i=None
     return out, fxi, of
 
 
+def _up_faces(flags, Fz) -> bool:
+    """True for the UP3 / UP1 faces, whose RHS always forms the z faces and so needs Fz."""
+    up = bool(int(flags) & (_native.UP3 | _native.UP1))
+    if up and Fz is None:
+        raise ValueError("the upwind3 / upwind1 faces need Fz (their RHS always forms the z faces)")
+    return up
+
+
 def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, flags=0, vert=None, flux_face_i=None, out=None,
                workspace: Workspace | None = None):
     """One SSP-RK3 step (x, y and z sweeps) on device tensors (``ocx_advect_rk3_f64``)."""
@@ -123,8 +136,9 @@ def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, flags=0, vert=None, f
         _need(t, s, nm)
     if Fz is not None:
         _need(Fz, (n + 1, ny, nx), "Fz")
+    up = _up_faces(flags, Fz)
     if vert is None:
-        vert = Fz is not None and bool(torch.any(Fz != 0.0).item())
+        vert = not up and Fz is not None and bool(torch.any(Fz != 0.0).item())
     ok = _ok(ok, (n, ny, nx))
     if inflow is not None:
         _need(inflow, C.shape, "inflow")
@@ -133,7 +147,8 @@ def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, flags=0, vert=None, f
     L = lib()
     nb = L.ocx_rk3_workspace_bytes(nt, n, ny, nx)
     ws = (workspace or _WS).get(nb)
-    rc = L.ocx_advect_rk3_f64(_ptr(C), _ptr(V0), _ptr(V1), _ptr(Qx), _ptr(Qy), _ptr(Fz if vert else None), _ptr(ok),
+    fz = Fz if vert or up else None
+    rc = L.ocx_advect_rk3_f64(_ptr(C), _ptr(V0), _ptr(V1), _ptr(Qx), _ptr(Qy), _ptr(fz), _ptr(ok),
                               _ptr(inflow), nt, n, ny, nx, float(dt), int(flags), int(bool(vert)),
                               -1 if flux_face_i is None else int(flux_face_i), _ptr(out), _ptr(fxi), _ptr(ws),
                               ws.numel(), _stream())
@@ -142,7 +157,7 @@ def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, flags=0, vert=None, f
 
 
 def faces_h(c, Qx, Qy, ok, ext=None, flags=0):
-    """WENO horizontal face fluxes (fx, fy) of the stack c (``ocx_faces_h_f64``)."""
+    """Horizontal face fluxes (fx, fy) of the stack c (``ocx_faces_h_f64``): WENO, or UP3 / UP1 by the flags."""
     nt, n, ny, nx = c.shape
     _need(c, (nt, n, ny, nx), "c")
     _need(Qx, (n, ny, nx + 1), "Qx")
@@ -180,14 +195,16 @@ def rhs(c, Qx, Qy, Fz, ok, ext=None, flags=0, vert=None):
     _need(Qy, (n, ny + 1, nx), "Qy")
     if Fz is not None:
         _need(Fz, (n + 1, ny, nx), "Fz")
+    up = _up_faces(flags, Fz)
     if vert is None:
-        vert = Fz is not None and bool(torch.any(Fz != 0.0).item())
+        vert = not up and Fz is not None and bool(torch.any(Fz != 0.0).item())
     ok = _ok(ok, (n, ny, nx))
     if ext is not None:
         _need(ext, c.shape, "ext")
     out = torch.empty_like(c)
-    check(lib().ocx_rhs_f64(_ptr(c), _ptr(Qx), _ptr(Qy), _ptr(Fz if vert else None), _ptr(ok), _ptr(ext), nt, n, ny,
-                            nx, int(flags), int(bool(vert)), _ptr(out), _stream()), "ocx_rhs_f64")
+    fz = Fz if vert or up else None
+    check(lib().ocx_rhs_f64(_ptr(c), _ptr(Qx), _ptr(Qy), _ptr(fz), _ptr(ok), _ptr(ext), nt, n, ny, nx, int(flags),
+                            int(bool(vert)), _ptr(out), _stream()), "ocx_rhs_f64")
     return out
 
 

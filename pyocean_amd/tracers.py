@@ -10,9 +10,12 @@ their state resident.
 
 Index order (tracer, k, j, i), k = 0 at the bed; faces Qx (N, ny, nx+1), Qy (N, ny+1, nx), Fz (N+1, ny, nx).
 
-Scope: the WENO5 schemes ('weno5' = WENO5-Z, 'weno5js') with SSP-RK3, with or without the FCT limiter, open or
-periodic edges.  The reference's other options (upwind3/upwind1 faces, centred faces, the 'pc' predictor-corrector)
-are outside this library and raise NotImplementedError rather than run anywhere else.
+Scope: SSP-RK3 with the WENO5 schemes ('weno5' = WENO5-Z, 'weno5js'), with or without the FCT limiter, and the
+'upwind3' / 'upwind1' faces of the unlimited step, open or periodic edges.  The face choice follows the reference's
+dispatch: the RK3 step, its RHS and flux_x take UP3 / donor faces for a scheme that does not start with 'weno5'
+(tracers.py:110-115, 139-164, 194-206); the FCT step and faces_h run WENO5-Z for 'weno5' and WENO5-JS for every other
+scheme (tracers.py:166-176).  The reference's centred faces and 'pc' predictor-corrector are outside this library
+and raise NotImplementedError rather than run anywhere else.
 """
 from __future__ import annotations
 
@@ -66,14 +69,17 @@ class TracerAdvection:
         return {}
 
     # ------------------------------------------------------------------ helpers
-    def _flags(self, unlimited=False) -> int:
-        if self.tracer_scheme not in ("weno5", "weno5js"):
-            raise NotImplementedError(f"tracer_scheme {self.tracer_scheme!r}: this GPU library implements the "
-                                      "WENO5 schemes ('weno5', 'weno5js') only")
+    def _flags(self, unlimited=False, rk=False) -> int:
+        """Flag word of the open edges and the faces: ``rk`` (the RK3 step, its RHS, flux_x) gives a scheme that
+        does not start with 'weno5' the UP1 ('upwind1') or UP3 faces (tracers.py:110-115); otherwise the WENO5
+        faces of faces_h, Z for 'weno5' and JS for every other scheme (tracers.py:166-176)."""
         m = 0
         for e in getattr(self.ob, "open", ()) or ():
             m |= _EDGE_BITS[e]
-        return ops.flags_of(m, js=self.tracer_scheme == "weno5js", unlimited=unlimited)
+        sch = self.tracer_scheme
+        if rk and not sch.startswith("weno5"):
+            return ops.flags_of(m, up=1 if sch == "upwind1" else 3)
+        return ops.flags_of(m, js=sch != "weno5", unlimited=unlimited)
 
     def _pad(self, t, kind):
         per = self.per
@@ -142,7 +148,7 @@ class TracerAdvection:
         if np.shape(V0) != C.shape[1:] or np.shape(V1) != C.shape[1:]:
             raise ValueError(f"_advect: volume shape {np.shape(V0)}/{np.shape(V1)} vs tracer layers {C.shape[1:]}")
         self._shapes(C, V0, V1, Qx, Qy, Fz)
-        flags = self._flags()
+        flags = self._flags(rk=True)
         vert = bool(np.any(np.asarray(Fz) != 0.0))
         okc = np.broadcast_to(ok, C.shape[1:])
         d = {k: _dev(v) for k, v in (("C", C), ("V0", V0), ("V1", V1), ("Qx", Qx), ("Qy", Qy), ("Fz", Fz))}
@@ -184,13 +190,14 @@ class TracerAdvection:
 
     def rhs(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, Fz: np.ndarray, ok: np.ndarray,
             centred: bool = False, inflow: np.ndarray | None = None) -> np.ndarray:
-        """Advection RHS -div(Q c) with x, y and (any Fz != 0) z WENO faces."""
+        """Advection RHS -div(Q c): WENO faces on x, y and (any Fz != 0) z, or the UP3 / UP1 faces of rhs_numpy
+        on x, y and z (tracers.py:139-164)."""
         if centred:
             raise NotImplementedError("centred faces ('pc' predictor) are not part of this GPU library")
         c = np.ascontiguousarray(c, dtype=float)
         vert = bool(np.any(np.asarray(Fz) != 0.0))
         r = ops.rhs(_dev(c), _dev(Qx), _dev(Qy), _dev(Fz), _dev(np.broadcast_to(ok, c.shape[1:]), np.uint8),
-                    None if inflow is None else _dev(inflow), self._flags(), vert=vert)
+                    None if inflow is None else _dev(inflow), self._flags(rk=True), vert=vert)
         return _host(r)
 
     def flux_x(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, ok: np.ndarray, centred: bool = False,
@@ -198,7 +205,12 @@ class TracerAdvection:
         """x-face fluxes Q c_face (ntr, N, ny, nx+1) that ``rhs`` differences (the TEF transect flux)."""
         if centred:
             raise NotImplementedError("centred faces ('pc' predictor) are not part of this GPU library")
-        return self.faces_h(c, Qx, Qy, np.broadcast_to(ok, np.shape(c)[1:]), inflow=inflow)[0]
+        if self.tracer_scheme.startswith("weno5"):
+            return self.faces_h(c, Qx, Qy, np.broadcast_to(ok, np.shape(c)[1:]), inflow=inflow)[0]
+        c = np.ascontiguousarray(c, dtype=float)
+        fx, _ = ops.faces_h(_dev(c), _dev(Qx), _dev(Qy), _dev(np.broadcast_to(ok, c.shape[1:]), np.uint8),
+                            None if inflow is None else _dev(inflow), self._flags(rk=True))
+        return _host(fx)
 
     def content(self, C: np.ndarray, V: np.ndarray) -> np.ndarray:
         """Per-tracer content sum V c in FP64 (deterministic order)."""

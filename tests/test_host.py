@@ -34,13 +34,25 @@ def test_volume_shape_mismatch_raises_value_error_before_the_gpu():
                        np.ones((3, 4, 5), bool))
 
 
-@pytest.mark.parametrize("scheme", ["upwind3", "upwind1"])
-def test_schemes_outside_the_library_fail_loudly(scheme):
-    adv = _adv(scheme=scheme)
-    C = np.zeros((1, 2, 3, 4))
-    with pytest.raises(NotImplementedError, match="WENO5"):
-        adv.advect_fct(C, np.ones((2, 3, 4)), np.ones((2, 3, 4)), np.zeros((2, 3, 5)), np.zeros((2, 4, 4)),
-                       np.ones((2, 3, 4), bool))
+@pytest.mark.parametrize("scheme,rk_bits,fct_bits", [("weno5", 0, 0), ("weno5js", 16, 16), ("upwind3", 64, 16),
+                                                     ("upwind1", 128, 16)])
+def test_face_flags_follow_the_reference_dispatch(scheme, rk_bits, fct_bits):
+    """The RK3 step takes UP3 / donor faces for the upwind schemes (tracers.py:110-115); the FCT step and faces_h
+    run WENO5-JS for every scheme but 'weno5' (tracers.py:166-176)."""
+    adv = _adv(scheme=scheme, open_=("west", "north"))
+    assert adv._flags(rk=True) == rk_bits | 1 | 8
+    assert adv._flags() == fct_bits | 1 | 8
+    assert adv._flags(unlimited=True) == fct_bits | 1 | 8 | 32
+
+
+def test_flag_word_rejects_unknown_up_orders_and_a_missing_fz():
+    from pyocean_amd import ops
+    assert ops.flags_of(2, up=3) == 66 and ops.flags_of(0, up=1) == 128
+    with pytest.raises(ValueError, match="up = 2"):
+        ops.flags_of(0, up=2)
+    with pytest.raises(ValueError, match="need Fz"):
+        ops._up_faces(64, None)
+    assert ops._up_faces(16, None) is False
 
 
 def test_predictor_corrector_fails_loudly():
