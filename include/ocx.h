@@ -33,6 +33,7 @@ extern "C" {
 #define OCX_UNLIMITED 32u  /* FCT coefficients = 1: the unlimited RK3 result (advect_fct(unlimited=True)) */
 #define OCX_UP3 64u        /* UP3 faces (tracer_scheme == "upwind3"): RK3 step, RHS and faces_h only */
 #define OCX_UP1 128u       /* first-order upwind faces (tracer_scheme == "upwind1"): as OCX_UP3 */
+#define OCX_CENTRED 256u   /* centred 6th/4th/2nd-order faces (rhs/flux_x with centred=True): RHS and faces_h only */
 
 /* error codes */
 #define OCX_OK 0
@@ -99,9 +100,21 @@ int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, cons
                        double dt, unsigned flags, int vert, int flux_face_i, double* C_out, double* fx_i,
                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* The 'pc' step of TracerAdvection.advect with tracer_time == "pc" (advect_numpy, tracers.py:303-330): a half step
+ * with centred faces on x, y and z, ch = (V0 C + dt/2 r_c(C)) / (V0 + dV/2), then the full step from C with the
+ * faces of the flags (WENO5-Z, OCX_WENO_JS, OCX_UP3 or OCX_UP1) of ch, out = (V0 C + dt r(ch)) / V1; both
+ * divisions fall back to C where the volume is not > 0.  Fz is required (the predictor always forms the z faces);
+ * vert as in ocx_advect_rk3_f64 for the WENO corrector.  fx_i: the corrector's x-face column flux_face_i, F(ch).
+ * Arguments, workspace (ocx_rk3_workspace_bytes) and errors as ocx_advect_rk3_f64. */
+int ocx_advect_pc_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                      const double* Fz, const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx,
+                      double dt, unsigned flags, int vert, int flux_face_i, double* C_out, double* fx_i,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
 /* WENO horizontal face fluxes (fx, fy) of the stack c.  Replaces _advect_faces_h (sigma_baroclinic.py:265-321)
  * as called by TracerAdvection.faces_h (tracers.py:166-176).  ext: exterior values or NULL (= c).  With OCX_UP3 /
- * OCX_UP1: the UP face fluxes that rhs_numpy differences (flux_x for those schemes, tracers.py:194-206). */
+ * OCX_UP1 / OCX_CENTRED: the face fluxes that rhs_numpy differences (flux_x for those schemes or centred=True,
+ * tracers.py:194-206; _centred_faces sigma_baroclinic.py:632-652). */
 int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const uint8_t* ok, const double* ext,
                     int nt, int nz, int ny, int nx, unsigned flags, double* fx, double* fy, void* stream);
 
@@ -111,7 +124,8 @@ int ocx_upwind_faces_h_f64(const double* c, const double* Qx, const double* Qy, 
 
 /* Advection RHS -div(Q c) with x, y and (vert) z WENO faces.  Replaces _advect_div (sigma_baroclinic.py:324-364)
  * as called by TracerAdvection.rhs (tracers.py:153-164); with OCX_UP3 / OCX_UP1 replaces rhs_numpy
- * (tracers.py:139-151): x, y and z UP faces, Fz required. */
+ * (tracers.py:139-151): x, y and z UP faces, Fz required; OCX_CENTRED the same with centred faces
+ * (rhs(centred=True)). */
 int ocx_rhs_f64(const double* c, const double* Qx, const double* Qy, const double* Fz, const uint8_t* ok,
                 const double* ext, int nt, int nz, int ny, int nx, unsigned flags, int vert, double* out,
                 void* stream);

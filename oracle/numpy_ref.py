@@ -224,34 +224,69 @@ def up_faces(c, flux, ok, axis, order=3):
     return np.moveaxis(out, -1, axis)
 
 
+def centred_faces(c, ok, axis, n_out=None):
+    """
+    Centred face values on the faces along ``axis``, whatever the flux: face f (1 <= f <= n-1) takes
+    (37 (c[f-1] + c[f]) - 8 (c[f-2] + c[f+1]) + (c[f-3] + c[f+2])) / 60 where cells f-3 .. f+2 are all ok, else
+    (7 (c[f-1] + c[f]) - (c[f-2] + c[f+1])) / 12 where cells f-2 .. f+1 are ok, else (c[f-1] + c[f]) / 2; cells
+    outside 0..n-1 count as not ok; faces 0 and the last are 0.  ``n_out``: number of faces (default n+1).
+    Follows ``_centred_faces`` (sigma_baroclinic.py:632-652).
+    """
+    cm = np.moveaxis(np.asarray(c, float), axis, -1)
+    om = np.moveaxis(np.asarray(ok).astype(bool), axis, -1)
+    n = cm.shape[-1]
+    out = np.zeros(cm.shape[:-1] + (n + 1 if n_out is None else n_out,))
+    if n >= 2:
+        ce = np.concatenate([np.zeros(cm.shape[:-1] + (3,)), cm, np.zeros(cm.shape[:-1] + (3,))], axis=-1)
+        oe = np.concatenate([np.zeros(om.shape[:-1] + (3,), bool), om, np.zeros(om.shape[:-1] + (3,), bool)],
+                            axis=-1)
+        f = np.arange(1, n) + 3                                          # ce[..., m + 3] = cell m
+        a = ce[..., f - 1] + ce[..., f]
+        b = ce[..., f - 2] + ce[..., f + 1]
+        m4 = oe[..., f - 2] & oe[..., f - 1] & oe[..., f] & oe[..., f + 1]
+        m6 = m4 & oe[..., f - 3] & oe[..., f + 2]
+        f6 = (37.0 * a - 8.0 * b + (ce[..., f - 3] + ce[..., f + 2])) / 60.0
+        f4 = (7.0 * a - b) / 12.0
+        out[..., 1:n] = np.where(m6, f6, np.where(m4, f4, 0.5 * a))
+    return np.moveaxis(out, -1, axis)
+
+
+def _np_faces(c, flux, ok, axis, order):
+    """Face values of rhs_numpy's ``faces`` (tracers.py:110-115): ``order`` 'c' centred, else UP3 / UP1."""
+    if order == "c":
+        return centred_faces(c, ok, axis, n_out=flux.shape[axis])
+    return up_faces(c, flux, ok, axis, order)
+
+
 def faces_up_h(c, Qx, Qy, ok, ext, open_mask=0, order=3):
     """
-    Horizontal face fluxes (fx, fy) = Q * face of the UP3 / UP1 schemes: every face, interior and closed edge, is
-    Q times its ``up_faces`` value (a closed edge Q * 0); open edges as ``faces_h``.  Follows ``rhs_numpy`` and
-    ``open_faces`` (tracers.py:117-151) and ``flux_x`` (tracers.py:194-206).
+    Horizontal face fluxes (fx, fy) = Q * face of the UP3 / UP1 (``order`` 3 / 1) or centred (``order`` 'c')
+    faces: every face, interior and closed edge, is Q times its face value (a closed edge Q * 0); open edges as
+    ``faces_h``.  Follows ``rhs_numpy`` and ``open_faces`` (tracers.py:117-151) and ``flux_x`` (tracers.py:194-206).
     """
     c = np.asarray(c, float)
     nt = c.shape[0]
     okb = np.broadcast_to(ok, c.shape)
     Qxb = np.broadcast_to(Qx, (nt,) + Qx.shape)
     Qyb = np.broadcast_to(Qy, (nt,) + Qy.shape)
-    fx = Qxb * up_faces(c, Qxb, okb, 3, order)
-    fy = Qyb * up_faces(c, Qyb, okb, 2, order)
+    fx = Qxb * _np_faces(c, Qxb, okb, 3, order)
+    fy = Qyb * _np_faces(c, Qyb, okb, 2, order)
     _open_edges(fx, fy, c, Qx, Qy, ext, open_mask, donor=False)
     return fx, fy
 
 
 def rhs_up(c, Qx, Qy, Fz, ok, ext, open_mask=0, order=3):
     """
-    Advection RHS -div(Q c) of the stack with UP3 / UP1 faces on x, y and z, the z faces always formed:
-    fz[k] = Fz[k] * face_k for k < N (face_0 = 0), fz[N] = 0.  Follows ``rhs_numpy`` (tracers.py:139-151).
+    Advection RHS -div(Q c) of the stack with UP3 / UP1 / centred (``order`` 3, 1, 'c') faces on x, y and z, the
+    z faces always formed: fz[k] = Fz[k] * face_k for k < N (face_0 = 0), fz[N] = 0.  Follows ``rhs_numpy``
+    (tracers.py:139-151).
     """
     c = np.asarray(c, float)
     nt, n, ny, nx = c.shape
     fx, fy = faces_up_h(c, Qx, Qy, ok, ext, open_mask, order)
     Fzb = np.broadcast_to(np.asarray(Fz, float)[:-1], c.shape)
     fz = np.zeros((nt, n + 1, ny, nx))
-    fz[:, :-1] = Fzb * up_faces(c, Fzb, np.broadcast_to(ok, c.shape), 1, order)
+    fz[:, :-1] = Fzb * _np_faces(c, Fzb, np.broadcast_to(ok, c.shape), 1, order)
     return -(fx[..., 1:] - fx[..., :-1] + fy[..., 1:, :] - fy[..., :-1, :] + fz[:, 1:] - fz[:, :-1])
 
 
@@ -393,6 +428,32 @@ def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, open_mask=0, z_varian
         return out
     fxi = lambda c: fx(c)[..., flux_face_i]                                                     # noqa: E731
     return out, (fxi(C) + fxi(c1)) / 6.0 + 2.0 * fxi(c2) / 3.0
+
+
+def advect_pc(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, open_mask=0, z_variant=True, flux_face_i=None,
+              up_order=0):
+    """
+    One 'pc' step: the predictor ch = safe(V0 C + 0.5 dt r_c(C), V0 + 0.5 dV) with centred faces on x, y and z,
+    then out = safe(V0 C + dt r(ch), V1) with the WENO (``z_variant``) or, with ``up_order`` 3 / 1, the UP faces;
+    ``safe`` falls back to C.  Follows ``advect_numpy`` (tracers.py:303-330).  Returns out or (out, F(ch) at u-face
+    column flux_face_i).
+    """
+    C = np.ascontiguousarray(C, dtype=float)
+    Qx, Qy, Fz = np.asarray(Qx, float), np.asarray(Qy, float), np.asarray(Fz, float)
+    V0, V1 = np.asarray(V0, float), np.asarray(V1, float)
+    okc = np.broadcast_to(ok, C.shape[1:])
+    ext = lambda c: c if inflow is None else np.asarray(inflow, float)          # noqa: E731
+    dV = V1 - V0
+    q0 = V0 * C
+    ch = _safe(q0 + 0.5 * dt * rhs_up(C, Qx, Qy, Fz, okc, ext(C), open_mask, "c"), V0 + 0.5 * dV, C)
+    if up_order:
+        r = rhs_up(ch, Qx, Qy, Fz, okc, ext(ch), open_mask, up_order)
+        fx = lambda: faces_up_h(ch, Qx, Qy, okc, ext(ch), open_mask, up_order)[0]           # noqa: E731
+    else:
+        r = rhs(ch, Qx, Qy, Fz, okc, ext(ch), open_mask, z_variant, bool(np.any(Fz != 0.0)))
+        fx = lambda: faces_h(ch, Qx, Qy, okc, ext(ch), open_mask, z_variant)[0]            # noqa: E731
+    out = _safe(q0 + dt * r, V1, C)
+    return out if flux_face_i is None else (out, fx()[..., flux_face_i])
 
 
 # ------------------------------------------------------------------------------------------------ periodic

@@ -18,6 +18,7 @@ WENO_JS = 16
 UNLIMITED = 32
 UP3 = 64
 UP1 = 128
+CENTRED = 256
 
 OCX_E_ARG, OCX_E_HIP, OCX_E_WORKSPACE, OCX_E_RANGE = -1, -2, -3, -4
 OUTFLOW_NOOP, OUTFLOW_OK, OUTFLOW_ACCEPTED, OUTFLOW_EXCEEDS, OUTFLOW_EMPTY_DONOR, OUTFLOW_NEGATIVE = range(6)
@@ -41,6 +42,7 @@ SIGNATURES = {
     "ocx_advect_fct_f64": (I, [P, P, P, P, P, P, P, I, I, I, I, D, U, I, P, P, P, P, Z, P]),
     "ocx_advect_fct_split": (I, [P, P, P, P, P, P, P, I, I, I, I, D, U, I, P, P, P, P, Z, P]),
     "ocx_advect_rk3_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, U, I, I, P, P, P, Z, P]),
+    "ocx_advect_pc_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, U, I, I, P, P, P, Z, P]),
     "ocx_faces_h_f64": (I, [P, P, P, P, P, I, I, I, I, U, P, P, P]),
     "ocx_upwind_faces_h_f64": (I, [P, P, P, P, I, I, I, I, U, P, P, P]),
     "ocx_rhs_f64": (I, [P, P, P, P, P, P, I, I, I, I, U, I, P, P]),

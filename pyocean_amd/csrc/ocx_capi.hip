@@ -138,10 +138,14 @@ dim3 tile_grid(int nt, int nz, int ny, int nx) {
     return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + TY - 1) / TY), (unsigned)std::min(planes, 65535LL));
 }
 
-// Face scheme of the flags.  The caller has rejected flags with more than one of UP3, UP1, WENO_JS.
+// Face scheme of the flags.  The caller has rejected flags with more than one of CENTRED, UP3, UP1, WENO_JS.
 int scheme_of(unsigned flags) {
     using namespace ocx;
-    return (flags & UP1) ? SCH_UP1 : (flags & UP3) ? SCH_UP3 : (flags & WENO_JS) ? SCH_JS : SCH_Z;
+    return (flags & CENTRED) ? SCH_C6
+           : (flags & UP1)   ? SCH_UP1
+           : (flags & UP3)   ? SCH_UP3
+           : (flags & WENO_JS) ? SCH_JS
+                               : SCH_Z;
 }
 
 // Arguments of one k_rk_stage launch.
@@ -161,11 +165,13 @@ void rk_launch(const RkArgs& a, dim3 grid, hipStream_t s) {
         a.flags);
 }
 
-// One RK3 stage (STAGE 0: the RHS only) with the scheme's faces; UP3 / UP1 always form the z faces.
+// One RK3 or 'pc' stage (STAGE 0: the RHS only) with the scheme's faces; UP3 / UP1 / centred always form the z
+// faces.
 template <int STAGE>
 void rk_stage(const RkArgs& a, int sch, bool vert, dim3 grid, hipStream_t s) {
     using namespace ocx;
     switch (sch) {
+        case SCH_C6: rk_launch<STAGE, SCH_C6, true>(a, grid, s); break;
         case SCH_UP3: rk_launch<STAGE, SCH_UP3, true>(a, grid, s); break;
         case SCH_UP1: rk_launch<STAGE, SCH_UP1, true>(a, grid, s); break;
         case SCH_JS: vert ? rk_launch<STAGE, SCH_JS, true>(a, grid, s) : rk_launch<STAGE, SCH_JS, false>(a, grid, s); break;
@@ -173,11 +179,13 @@ void rk_stage(const RkArgs& a, int sch, bool vert, dim3 grid, hipStream_t s) {
     }
 }
 
-// Flags of the RK3 step, the RHS and faces_h: open edges plus at most one of WENO_JS, UP3, UP1.
-int check_rk_flags(unsigned flags, const char* what) {
+// Flags of the RK3 step, the RHS and faces_h: open edges plus at most one of WENO_JS, UP3, UP1 and (when
+// `centred` allows it) CENTRED.
+int check_rk_flags(unsigned flags, const char* what, bool centred = false) {
     using namespace ocx;
-    if (flags & ~(31u | UP3 | UP1)) return fail(OCX_E_ARG, "flags 0x%x not valid for %s", flags, what);
-    const int schemes = !!(flags & WENO_JS) + !!(flags & UP3) + !!(flags & UP1);
+    if (flags & ~(31u | UP3 | UP1 | (centred ? CENTRED : 0u)))
+        return fail(OCX_E_ARG, "flags 0x%x not valid for %s", flags, what);
+    const int schemes = !!(flags & WENO_JS) + !!(flags & UP3) + !!(flags & UP1) + !!(flags & CENTRED);
     if (schemes > 1) return fail(OCX_E_ARG, "flags 0x%x name more than one face scheme", flags);
     return OCX_OK;
 }
@@ -425,6 +433,50 @@ int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, cons
     return OCX_OK;
 }
 
+int ocx_advect_pc_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
+                      const double* Fz, const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx,
+                      double dt, unsigned flags, int vert, int flux_face_i, double* C_out, double* fx_i,
+                      void* workspace, size_t workspace_bytes, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(C, "C"));
+    OCX_TRY(need(V0, "V0"));
+    OCX_TRY(need(V1, "V1"));
+    OCX_TRY(need(Qx, "Qx"));
+    OCX_TRY(need(Qy, "Qy"));
+    OCX_TRY(need(ok, "ok"));
+    OCX_TRY(need(C_out, "C_out"));
+    OCX_TRY(need(Fz, "Fz"));
+    if (C_out == C) return fail(OCX_E_ARG, "C_out must not alias C");
+    OCX_TRY(check_rk_flags(flags, "the pc step"));
+    if (flux_face_i != -1 && (flux_face_i < 0 || flux_face_i > nx))
+        return fail(OCX_E_ARG, "flux_face_i %d outside 0..%d", flux_face_i, nx);
+    if (flux_face_i >= 0 && !fx_i) return fail(OCX_E_ARG, "fx_i is NULL with flux_face_i = %d", flux_face_i);
+    const size_t wsb = rk3_layout(nullptr, nt, nz, ny, nx, nullptr);
+    if (!workspace || workspace_bytes < wsb)
+        return fail(OCX_E_WORKSPACE, "workspace of %zu bytes needed, %zu given", wsb, workspace ? workspace_bytes : 0);
+    Rk3Ws w;
+    rk3_layout(workspace, nt, nz, ny, nx, &w);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid = tile_grid(nt, nz, ny, nx);
+    const unsigned edges = flags & 15u;
+    RkArgs a{C, C, inflow, V0, V1, Qx, Qy, Fz, ok, nullptr, w.c1, nullptr, -1, nt, nz, ny, nx, dt, edges | CENTRED};
+    {
+        ProfScope _ps("pc_predictor", s);
+        rk_stage<4>(a, SCH_C6, true, grid, s);
+        OCX_TRY(launch_check("pc predictor"));
+    }
+    {
+        ProfScope _ps("pc_corrector", s);
+        a.cin = w.c1, a.cout = C_out, a.flags = flags;
+        a.fxs = fx_i ? fx_i : nullptr;
+        a.fi = fx_i ? flux_face_i : -1;
+        rk_stage<5>(a, scheme_of(flags), vert, grid, s);
+        OCX_TRY(launch_check("pc corrector"));
+    }
+    return OCX_OK;
+}
+
 int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const uint8_t* ok, const double* ext,
                     int nt, int nz, int ny, int nx, unsigned flags, double* fx, double* fy, void* stream) {
     using namespace ocx;
@@ -435,10 +487,11 @@ int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const u
     OCX_TRY(need(ok, "ok"));
     OCX_TRY(need(fx, "fx"));
     OCX_TRY(need(fy, "fy"));
-    OCX_TRY(check_rk_flags(flags, "faces_h"));
+    OCX_TRY(check_rk_flags(flags, "faces_h", true));
     hipStream_t s = static_cast<hipStream_t>(stream);
     const dim3 grid = tile_grid(nt, nz, ny, nx), block(TX, TY);
     switch (scheme_of(flags)) {
+        case SCH_C6: k_faces<SCH_C6, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags); break;
         case SCH_UP3: k_faces<SCH_UP3, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags); break;
         case SCH_UP1: k_faces<SCH_UP1, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags); break;
         case SCH_JS: k_faces<SCH_JS, TX, TY><<<grid, block, 0, s>>>(c, ext, Qx, Qy, ok, fx, fy, nt, nz, ny, nx, flags); break;
@@ -473,9 +526,9 @@ int ocx_rhs_f64(const double* c, const double* Qx, const double* Qy, const doubl
     OCX_TRY(need(Qy, "Qy"));
     OCX_TRY(need(ok, "ok"));
     OCX_TRY(need(out, "out"));
-    const bool up = flags & (UP3 | UP1);
+    const bool up = flags & (UP3 | UP1 | CENTRED);
     if (vert || up) OCX_TRY(need(Fz, "Fz"));
-    OCX_TRY(check_rk_flags(flags, "the RHS"));
+    OCX_TRY(check_rk_flags(flags, "the RHS", true));
     hipStream_t s = static_cast<hipStream_t>(stream);
     const RkArgs a{c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr, out, nullptr, -1, nt, nz, ny, nx, 0.0, flags};
     rk_stage<0>(a, scheme_of(flags), vert, tile_grid(nt, nz, ny, nx), s);

@@ -34,8 +34,9 @@ __device__ __forceinline__ void load_tile(double* sc, uint8_t* so, const double*
 // Face fluxes q * c_face of the tile's faces into LDS (sfx: TY rows x TX+1 faces; sfy: TY+1 faces x TX columns),
 // from the input tile sc/so (halo 3).  WENO (SCH_JS, SCH_Z), rules of _advect_faces_h (sigma_baroclinic.py:
 // 265-321): interior faces with q == 0 stay 0; open edges take ext on inflow, the edge cell on outflow; closed
-// edges 0.  UP3 / UP1 (SCH_UP3, SCH_UP1), rules of rhs_numpy (tracers.py:139-151): every interior face is q times
-// the _up3_faces value, a closed edge is q * 0, open edges as WENO.  ext == nullptr: the input is the exterior.
+// edges 0.  UP3 / UP1 / centred (SCH_UP3, SCH_UP1, SCH_C6), rules of rhs_numpy (tracers.py:139-151): every
+// interior face is q times the _up3_faces or _centred_faces value, a closed edge is q * 0, open edges as WENO.
+// ext == nullptr: the input is the exterior.
 template <int SCH, int TX, int TY>
 __device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const double* sc, const uint8_t* so,
                                                 const double* __restrict__ Qxk, const double* __restrict__ Qyk,
@@ -52,7 +53,9 @@ __device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const doubl
             const double* row = sc + (ly + H) * LX;
             const uint8_t* orow = so + (ly + H) * LX;
             if (f > 0 && f < nx) {
-                if constexpr (SCH >= SCH_UP3)
+                if constexpr (SCH == SCH_C6)
+                    val = q * centred_face(row, orow, 1, lf + H);
+                else if constexpr (SCH >= SCH_UP3)
                     val = q * up_face<SCH == SCH_UP3>(row, orow, 1, lf + H, q);
                 else if (q != 0.0)
                     val = q * lds_face<SCH == SCH_Z>(row, orow, 1, lf + H, q);
@@ -83,7 +86,9 @@ __device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const doubl
             const double* col = sc + lx + H;
             const uint8_t* ocol = so + lx + H;
             if (f > 0 && f < ny) {
-                if constexpr (SCH >= SCH_UP3)
+                if constexpr (SCH == SCH_C6)
+                    val = q * centred_face(col, ocol, LX, lf + H);
+                else if constexpr (SCH >= SCH_UP3)
                     val = q * up_face<SCH == SCH_UP3>(col, ocol, LX, lf + H, q);
                 else if (q != 0.0)
                     val = q * lds_face<SCH == SCH_Z>(col, ocol, LX, lf + H, q);
@@ -262,11 +267,29 @@ __device__ __forceinline__ double zface_up(const double* __restrict__ ccol, cons
     return hi;
 }
 
+// Centred face value (not yet times Fz) of interface kf (1 <= kf <= n-1) of one column: _centred_faces along the
+// layer index (rhs_numpy with centred=True, tracers.py:148-150), layers outside 0..n-1 invalid.
+__device__ __forceinline__ double zface_c(const double* __restrict__ ccol, const uint8_t* __restrict__ ocol,
+                                          size_t plane, int n, int kf) {
+    auto okk = [&](int k) { return k >= 0 && k < n && ocol[(size_t)k * plane] != 0; };
+    auto cc = [&](int k) { return ccol[(size_t)k * plane]; };
+    const double a = cc(kf - 1) + cc(kf);
+    const bool m4 = okk(kf - 2) && okk(kf - 1) && okk(kf) && okk(kf + 1);
+    if (m4 && okk(kf - 3) && okk(kf + 2))
+        return (37.0 * a - 8.0 * (cc(kf - 2) + cc(kf + 1)) + (cc(kf - 3) + cc(kf + 2))) / 60.0;
+    if (m4) return (7.0 * a - (cc(kf - 2) + cc(kf + 1))) / 12.0;
+    return 0.5 * a;
+}
+
 // SSP-RK3 stage of TracerAdvection.advect (tracers.py:342-375, _rk_stage sigma_baroclinic.py:856-875) with the
-// fused RHS r = -div(Q c): WENO faces of _advect_div (x, y and, with VERT, z), or UP3 / UP1 faces of rhs_numpy
-// (tracers.py:139-151), whose z faces are always formed (fz = Fz * face, the bed face Fz * 0, the surface 0).
+// fused RHS r = -div(Q c): WENO faces of _advect_div (x, y and, with VERT, z), or UP3 / UP1 / centred faces of
+// rhs_numpy (tracers.py:139-151), whose z faces are always formed (fz = Fz * face, the bed face Fz * 0, the
+// surface 0).
 // STAGE 0: r only.  1: q1 = q0 + dt r, c1 = q1/Va.  2: q2 = 0.75 q0 + 0.25 (q1 + dt r), c2 = q2/Vb (q in place).
-// 3: q3 = q0/3 + 2/3 (q2 + dt r), out = q3/V1.  fxs (optional): the x-face column flux_face_i of this stage.
+// 3: q3 = q0/3 + 2/3 (q2 + dt r), out = q3/V1.  The 'pc' step (advect_numpy, tracers.py:324-330):
+// 4: ch = (q0 + 0.5 dt r)/Vh, Vh = V0 + 0.5 dV (the predictor); 5: out = (q0 + dt r)/V1 (the corrector).
+// Every division falls back to C where the volume is not > 0.  fxs (optional): the x-face column flux_face_i of
+// this stage.
 template <int STAGE, int SCH, bool VERT, int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ cin, const double* __restrict__ C,
                                                      const double* __restrict__ ext, const double* __restrict__ V0,
@@ -302,13 +325,19 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                 const double* ccol = cin + (size_t)t * n * plane + xc;
                 const uint8_t* ocol = ok + xc;
                 double ft = 0.0, fb;
+                auto zf = [&](int kf, double qz) {
+                    if constexpr (SCH == SCH_C6)
+                        return zface_c(ccol, ocol, plane, n, kf);
+                    else
+                        return zface_up<SCH == SCH_UP3>(ccol, ocol, plane, n, kf, qz);
+                };
                 if (k + 1 < n) {
                     const double qz = Fz[(size_t)(k + 1) * plane + xc];
-                    ft = qz * zface_up<SCH == SCH_UP3>(ccol, ocol, plane, n, k + 1, qz);
+                    ft = qz * zf(k + 1, qz);
                 }
                 if (k > 0) {
                     const double qz = Fz[(size_t)k * plane + xc];
-                    fb = qz * zface_up<SCH == SCH_UP3>(ccol, ocol, plane, n, k, qz);
+                    fb = qz * zf(k, qz);
                 } else {
                     fb = Fz[xc] * 0.0;
                 }
@@ -346,8 +375,14 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                     qn = 0.75 * q0 + 0.25 * (q[x] + dt * r);
                     V = v0 + 0.5 * dV;
                     q[x] = qn;
-                } else {
+                } else if (STAGE == 3) {
                     qn = q0 / 3.0 + 2.0 / 3.0 * (q[x] + dt * r);
+                    V = v1;
+                } else if (STAGE == 4) {
+                    qn = q0 + 0.5 * dt * r;
+                    V = v0 + 0.5 * dV;
+                } else {
+                    qn = q0 + dt * r;
                     V = v1;
                 }
                 cout[x] = V > 0 ? qn / V : c0;

@@ -16,9 +16,11 @@ constexpr unsigned WENO_JS = 16u;      // WENO5-JS instead of WENO5-Z
 constexpr unsigned UNLIMITED = 32u;    // FCT coefficients = 1 (the RK3 result)
 constexpr unsigned UP3 = 64u;          // third-order upwind faces (tracer_scheme == "upwind3"), RK3 step only
 constexpr unsigned UP1 = 128u;         // first-order upwind faces (tracer_scheme == "upwind1"), RK3 step only
+constexpr unsigned CENTRED = 256u;     // centred 6th/4th/2nd-order faces (rhs(centred=True), the 'pc' predictor)
 
-// Face schemes of the tile kernels: WENO5-JS, WENO5-Z, UP3, UP1.
-constexpr int SCH_JS = 0, SCH_Z = 1, SCH_UP3 = 2, SCH_UP1 = 3;
+// Face schemes of the tile kernels: WENO5-JS, WENO5-Z, UP3, UP1, centred.  From SCH_UP3 on, the faces are those of
+// the reference's vectorised rhs_numpy: every face is Q times its value (no Q == 0 test), a closed edge Q * 0.
+constexpr int SCH_JS = 0, SCH_Z = 1, SCH_UP3 = 2, SCH_UP1 = 3, SCH_C6 = 4;
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
@@ -114,6 +116,19 @@ __device__ __forceinline__ double up_face(const double* s, const uint8_t* o, int
     if (q >= 0.0)
         return (o[(c - 2) * st] && o[(c - 1) * st] && o[c * st]) ? (-s[(c - 2) * st] + 5.0 * lo + 2.0 * hi) / 6.0 : lo;
     return (o[(c - 1) * st] && o[c * st] && o[(c + 1) * st]) ? (2.0 * lo + 5.0 * hi - s[(c + 1) * st]) / 6.0 : hi;
+}
+
+// Centred face value between cells c-1 and c of a line (values s, validity o, stride st), _centred_faces
+// (sigma_baroclinic.py:632-652): (37 (c[i-1] + c[i]) - 8 (c[i-2] + c[i+1]) + (c[i-3] + c[i+2])) / 60 where cells
+// i-3 .. i+2 are ok, else (7 (c[i-1] + c[i]) - (c[i-2] + c[i+1])) / 12 where cells i-2 .. i+1 are ok, else
+// (c[i-1] + c[i]) / 2, whatever the flux.  The line must hold valid indices c-3 .. c+2 (validity 0 outside).
+__device__ __forceinline__ double centred_face(const double* s, const uint8_t* o, int st, int c) {
+    const double a = s[(c - 1) * st] + s[c * st];
+    const bool m4 = o[(c - 2) * st] && o[(c - 1) * st] && o[c * st] && o[(c + 1) * st];
+    if (m4 && o[(c - 3) * st] && o[(c + 2) * st])
+        return (37.0 * a - 8.0 * (s[(c - 2) * st] + s[(c + 1) * st]) + (s[(c - 3) * st] + s[(c + 2) * st])) / 60.0;
+    if (m4) return (7.0 * a - (s[(c - 2) * st] + s[(c + 1) * st])) / 12.0;
+    return 0.5 * a;
 }
 
 }  // namespace ocx

@@ -72,13 +72,14 @@ _WS = Workspace()
 _WS_CONTENT = Workspace()
 
 
-def flags_of(open_mask: int = 0, js: bool = False, unlimited: bool = False, up: int = 0) -> int:
-    """Flag word of the C-ABI: open edges, WENO5-JS faces, the unlimited FCT result, or ``up`` = 3 / 1 for the
-    UP3 / donor-cell faces of the RK3 step."""
+def flags_of(open_mask: int = 0, js: bool = False, unlimited: bool = False, up: int = 0,
+             centred: bool = False) -> int:
+    """Flag word of the C-ABI: open edges, WENO5-JS faces, the unlimited FCT result, ``up`` = 3 / 1 for the
+    UP3 / donor-cell faces of the RK3 step, or the centred faces of the RHS and faces_h."""
     if up not in (0, 1, 3):
         raise ValueError(f"up = {up}: 0, 1 or 3")
     return (int(open_mask) | (_native.WENO_JS if js else 0) | (_native.UNLIMITED if unlimited else 0)
-            | {0: 0, 1: _native.UP1, 3: _native.UP3}[up])
+            | {0: 0, 1: _native.UP1, 3: _native.UP3}[up] | (_native.CENTRED if centred else 0))
 
 
 PRECISIONS = ("f64", "split")
@@ -119,10 +120,10 @@ def advect_fct(C, V0, V1, Qx, Qy, ok, dt, inflow=None, flags=0, flux_face_i=None
 
 
 def _up_faces(flags, Fz) -> bool:
-    """True for the UP3 / UP1 faces, whose RHS always forms the z faces and so needs Fz."""
-    up = bool(int(flags) & (_native.UP3 | _native.UP1))
+    """True for the UP3 / UP1 / centred faces, whose RHS always forms the z faces and so needs Fz."""
+    up = bool(int(flags) & (_native.UP3 | _native.UP1 | _native.CENTRED))
     if up and Fz is None:
-        raise ValueError("the upwind3 / upwind1 faces need Fz (their RHS always forms the z faces)")
+        raise ValueError("the upwind3 / upwind1 / centred faces need Fz (their RHS always forms the z faces)")
     return up
 
 
@@ -156,8 +157,39 @@ def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, flags=0, vert=None, f
     return out if flux_face_i is None else (out, fxi)
 
 
+def advect_pc(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, flags=0, vert=None, flux_face_i=None, out=None,
+              workspace: Workspace | None = None):
+    """One 'pc' step on device tensors (``ocx_advect_pc_f64``): the centred half step, then the full step with the
+    faces of ``flags`` (WENO5-Z, WENO5-JS, UP3 or UP1) of the mid-step tracer.  Returns out or (out, fx_i)."""
+    nt, n, ny, nx = C.shape
+    _need(C, (nt, n, ny, nx), "C")
+    for t, s, nm in ((V0, (n, ny, nx), "V0"), (V1, (n, ny, nx), "V1"), (Qx, (n, ny, nx + 1), "Qx"),
+                     (Qy, (n, ny + 1, nx), "Qy")):
+        _need(t, s, nm)
+    if Fz is None:
+        raise ValueError("the pc step needs Fz (its centred predictor always forms the z faces)")
+    _need(Fz, (n + 1, ny, nx), "Fz")
+    up = _up_faces(flags, Fz)
+    if vert is None:
+        vert = not up and bool(torch.any(Fz != 0.0).item())
+    ok = _ok(ok, (n, ny, nx))
+    if inflow is not None:
+        _need(inflow, C.shape, "inflow")
+    out = torch.empty_like(C) if out is None else _need(out, C.shape, "out")
+    fxi = torch.empty((nt, n, ny), dtype=_F64, device=C.device) if flux_face_i is not None else None
+    L = lib()
+    ws = (workspace or _WS).get(L.ocx_rk3_workspace_bytes(nt, n, ny, nx))
+    rc = L.ocx_advect_pc_f64(_ptr(C), _ptr(V0), _ptr(V1), _ptr(Qx), _ptr(Qy), _ptr(Fz), _ptr(ok), _ptr(inflow), nt, n,
+                             ny, nx, float(dt), int(flags), int(bool(vert)),
+                             -1 if flux_face_i is None else int(flux_face_i), _ptr(out), _ptr(fxi), _ptr(ws),
+                             ws.numel(), _stream())
+    check(rc, "ocx_advect_pc_f64")
+    return out if flux_face_i is None else (out, fxi)
+
+
 def faces_h(c, Qx, Qy, ok, ext=None, flags=0):
-    """Horizontal face fluxes (fx, fy) of the stack c (``ocx_faces_h_f64``): WENO, or UP3 / UP1 by the flags."""
+    """Horizontal face fluxes (fx, fy) of the stack c (``ocx_faces_h_f64``): WENO, or UP3 / UP1 / centred by the
+    flags."""
     nt, n, ny, nx = c.shape
     _need(c, (nt, n, ny, nx), "c")
     _need(Qx, (n, ny, nx + 1), "Qx")

@@ -14,8 +14,8 @@ Scope: SSP-RK3 with the WENO5 schemes ('weno5' = WENO5-Z, 'weno5js'), with or wi
 'upwind3' / 'upwind1' faces of the unlimited step, open or periodic edges.  The face choice follows the reference's
 dispatch: the RK3 step, its RHS and flux_x take UP3 / donor faces for a scheme that does not start with 'weno5'
 (tracers.py:110-115, 139-164, 194-206); the FCT step and faces_h run WENO5-Z for 'weno5' and WENO5-JS for every other
-scheme (tracers.py:166-176).  The reference's centred faces and 'pc' predictor-corrector are outside this library
-and raise NotImplementedError rather than run anywhere else.
+scheme (tracers.py:166-176).  tracer_time 'pc' (advect_numpy, tracers.py:303-330) runs a centred half step and the
+full step with those faces; rhs and flux_x take ``centred=True`` (_centred_faces, sigma_baroclinic.py:632-652).
 """
 from __future__ import annotations
 
@@ -69,13 +69,16 @@ class TracerAdvection:
         return {}
 
     # ------------------------------------------------------------------ helpers
-    def _flags(self, unlimited=False, rk=False) -> int:
-        """Flag word of the open edges and the faces: ``rk`` (the RK3 step, its RHS, flux_x) gives a scheme that
-        does not start with 'weno5' the UP1 ('upwind1') or UP3 faces (tracers.py:110-115); otherwise the WENO5
-        faces of faces_h, Z for 'weno5' and JS for every other scheme (tracers.py:166-176)."""
+    def _flags(self, unlimited=False, rk=False, centred=False) -> int:
+        """Flag word of the open edges and the faces: ``centred`` the centred faces; ``rk`` (the RK3 step, its
+        RHS, flux_x) gives a scheme that does not start with 'weno5' the UP1 ('upwind1') or UP3 faces
+        (tracers.py:110-115); otherwise the WENO5 faces of faces_h, Z for 'weno5' and JS for every other scheme
+        (tracers.py:166-176)."""
         m = 0
         for e in getattr(self.ob, "open", ()) or ():
             m |= _EDGE_BITS[e]
+        if centred:
+            return ops.flags_of(m, centred=True)
         sch = self.tracer_scheme
         if rk and not sch.startswith("weno5"):
             return ops.flags_of(m, up=1 if sch == "upwind1" else 3)
@@ -141,9 +144,9 @@ class TracerAdvection:
         """
         SSP-RK3 flux-form advection of the stack C with frozen transports Qx, Qy and Fz; layer volumes go linearly
         from V0 to V1.  ``flux_face_i``: also return (F(c0) + F(c1))/6 + 2F(c2)/3 of u-face column i.
+        tracer_time 'pc': the centred half step to c^{n+1/2}, then the full step with its faces; ``flux_face_i``
+        then gives F(c^{n+1/2}) (tracers.py:324-330).
         """
-        if self.tracer_time == "pc":
-            raise NotImplementedError("tracer_time 'pc' (centred predictor) is not part of this GPU library")
         C = np.ascontiguousarray(C, dtype=float)
         if np.shape(V0) != C.shape[1:] or np.shape(V1) != C.shape[1:]:
             raise ValueError(f"_advect: volume shape {np.shape(V0)}/{np.shape(V1)} vs tracer layers {C.shape[1:]}")
@@ -159,8 +162,9 @@ class TracerAdvection:
             H = TRACER_HALO
             d = {k: (None if v is None else self._pad(v, {"Qx": "u", "Qy": "v"}.get(k, "c"))) for k, v in d.items()}
             fi = None if fi is None else fi + (H if self.per.x else 0)
-        res = ops.advect_rk3(d["C"], d["V0"], d["V1"], d["Qx"], d["Qy"], d["Fz"], d["ok"], float(self.dt),
-                             d["inflow"], flags, vert=vert, flux_face_i=fi, workspace=self._ws)
+        step = ops.advect_pc if self.tracer_time == "pc" else ops.advect_rk3
+        res = step(d["C"], d["V0"], d["V1"], d["Qx"], d["Qy"], d["Fz"], d["ok"], float(self.dt), d["inflow"], flags,
+                   vert=vert, flux_face_i=fi, workspace=self._ws)
         out, fxi = (res, None) if fi is None else res
         if self.per:
             out = self._crop(out)
@@ -190,26 +194,24 @@ class TracerAdvection:
 
     def rhs(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, Fz: np.ndarray, ok: np.ndarray,
             centred: bool = False, inflow: np.ndarray | None = None) -> np.ndarray:
-        """Advection RHS -div(Q c): WENO faces on x, y and (any Fz != 0) z, or the UP3 / UP1 faces of rhs_numpy
-        on x, y and z (tracers.py:139-164)."""
-        if centred:
-            raise NotImplementedError("centred faces ('pc' predictor) are not part of this GPU library")
+        """Advection RHS -div(Q c): WENO faces on x, y and (any Fz != 0) z, or the UP3 / UP1 / centred
+        (``centred``) faces of rhs_numpy on x, y and z (tracers.py:139-164)."""
         c = np.ascontiguousarray(c, dtype=float)
         vert = bool(np.any(np.asarray(Fz) != 0.0))
+        flags = self._flags(centred=True) if centred else self._flags(rk=True)
         r = ops.rhs(_dev(c), _dev(Qx), _dev(Qy), _dev(Fz), _dev(np.broadcast_to(ok, c.shape[1:]), np.uint8),
-                    None if inflow is None else _dev(inflow), self._flags(rk=True), vert=vert)
+                    None if inflow is None else _dev(inflow), flags, vert=vert)
         return _host(r)
 
     def flux_x(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, ok: np.ndarray, centred: bool = False,
                inflow: np.ndarray | None = None) -> np.ndarray:
         """x-face fluxes Q c_face (ntr, N, ny, nx+1) that ``rhs`` differences (the TEF transect flux)."""
-        if centred:
-            raise NotImplementedError("centred faces ('pc' predictor) are not part of this GPU library")
-        if self.tracer_scheme.startswith("weno5"):
+        if not centred and self.tracer_scheme.startswith("weno5"):
             return self.faces_h(c, Qx, Qy, np.broadcast_to(ok, np.shape(c)[1:]), inflow=inflow)[0]
         c = np.ascontiguousarray(c, dtype=float)
+        flags = self._flags(centred=True) if centred else self._flags(rk=True)
         fx, _ = ops.faces_h(_dev(c), _dev(Qx), _dev(Qy), _dev(np.broadcast_to(ok, c.shape[1:]), np.uint8),
-                            None if inflow is None else _dev(inflow), self._flags(rk=True))
+                            None if inflow is None else _dev(inflow), flags)
         return _host(fx)
 
     def content(self, C: np.ndarray, V: np.ndarray) -> np.ndarray:

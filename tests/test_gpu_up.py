@@ -1,8 +1,9 @@
 """
 GPU parity of the 'upwind3' / 'upwind1' tracer schemes of TracerAdvection.advect, rhs and flux_x (UP3 / donor faces
-of _up3_faces and rhs_numpy, sigma_baroclinic.py:164-187, tracers.py:139-151, 342-375) against the oracle, bit for
-bit (the bit patterns, so signed zeros count), through the C-ABI; and the reference's dispatch of the FCT step and
-faces_h for those schemes (WENO5-JS).
+of _up3_faces and rhs_numpy, sigma_baroclinic.py:164-187, tracers.py:139-151, 342-375), of the centred faces
+(rhs / flux_x with centred=True, _centred_faces sigma_baroclinic.py:632-652) and of the 'pc' step (advect_numpy,
+tracers.py:303-330) against the oracle, bit for bit (the bit patterns, so signed zeros count), through the C-ABI;
+and the reference's dispatch of the FCT step and faces_h for the upwind schemes (WENO5-JS).
 """
 from __future__ import annotations
 
@@ -25,8 +26,8 @@ def _say(*a):
     print("[gpu case]", *a, flush=True)
 
 
-def _adv(scheme, open_mask=0, dt=30.0, per=None):
-    adv = TracerAdvection(scheme=scheme, time="ssprk3", dt=dt,
+def _adv(scheme, open_mask=0, dt=30.0, per=None, time="ssprk3"):
+    adv = TracerAdvection(scheme=scheme, time=time, dt=dt,
                           ob=OpenEdges([e for b, e in zip((1, 2, 4, 8), EDGES) if open_mask & b]))
     if per:
         adv.per = Periodic(per)
@@ -159,3 +160,81 @@ def test_c_abi_rejects_two_face_schemes_and_a_missing_fz(gpu):
     with pytest.raises(ValueError, match="flag"):
         ops.advect_fct(C, V, V, Qx, Qy, ok, 30.0, flags=ops.flags_of(up=3))
     torch.cuda.synchronize()
+
+
+# ------------------------------------------------------------------------------------------------ centred, 'pc'
+UPO = {"weno5": 0, "weno5js": 0, "upwind3": 3, "upwind1": 1}
+
+
+@pytest.mark.parametrize("scheme", ["weno5", "weno5js", "upwind3", "upwind1"])
+@pytest.mark.parametrize("variant", ["plain", "inflow", "flat", "closed"])
+def test_pc_step_matches_the_oracle_bit_for_bit(gpu, scheme, variant):
+    case = cases.fused_fct_case(open_=True, seed=21)
+    mask = 0 if variant == "closed" else case["open_mask"]
+    _say("pc", scheme, variant)
+    nz, ny, nx = case["V0"].shape
+    Fz = _fz((nz, ny, nx), 5, flat=variant == "flat")
+    inflow = case["C"] + 1.0 if variant == "inflow" else None
+    adv = _adv(scheme, mask, dt=case["dt"], time="pc")
+    args = (case["C"], case["V0"], case["V1"], case["Qx"], case["Qy"], Fz, case["ok"])
+    got, fxi = adv.advect(*args, inflow=inflow, flux_face_i=nx)
+    ref, fref = R.advect_pc(*args, case["dt"], inflow=inflow, open_mask=mask, z_variant=scheme == "weno5",
+                            flux_face_i=nx, up_order=UPO[scheme])
+    assert _same_bits(got, ref)
+    assert _same_bits(fxi, fref)
+    assert _same_bits(adv.advect(*args, inflow=inflow), ref)
+
+
+def test_centred_rhs_and_flux_x_match_the_oracle(gpu):
+    case = cases.fused_fct_case(open_=True, seed=22)
+    _say("centred rhs")
+    nz, ny, nx = case["V0"].shape
+    ext = case["C"] - 0.5
+    for scheme in ("weno5", "upwind1"):
+        adv = _adv(scheme, case["open_mask"])
+        for flat in (False, True):
+            Fz = _fz((nz, ny, nx), 7, flat=flat)
+            got = adv.rhs(case["C"], case["Qx"], case["Qy"], Fz, case["ok"], centred=True, inflow=ext)
+            ref = R.rhs_up(case["C"], case["Qx"], case["Qy"], Fz, case["ok"], ext, case["open_mask"], "c")
+            assert _same_bits(got, ref)
+        fx = adv.flux_x(case["C"], case["Qx"], case["Qy"], case["ok"], centred=True, inflow=ext)
+        assert _same_bits(fx, R.faces_up_h(case["C"], case["Qx"], case["Qy"], case["ok"], ext, case["open_mask"],
+                                           "c")[0])
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1, 1), (1, 2, 1, 7), (1, 7, 6, 1), (3, 5, 37, 70), (2, 3, 130, 129)])
+def test_pc_ragged_and_tiny_shapes(gpu, shape):
+    _say("pc shape", shape)
+    nt, nz, ny, nx = shape
+    rng = np.random.default_rng(sum(shape) + 1)
+    C = rng.normal(5.0, 1.0, shape)
+    V = rng.uniform(0.8, 1.2, (nz, ny, nx)) * 1e6
+    Qx = rng.uniform(-0.3, 0.3, (nz, ny, nx + 1)) * 1e4
+    Qy = rng.uniform(-0.3, 0.3, (nz, ny + 1, nx)) * 1e4
+    Fz = rng.uniform(-0.3, 0.3, (nz + 1, ny, nx)) * 1e4
+    V1 = V - 30.0 * (Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:] - Qy[:, :-1] + Fz[1:] - Fz[:-1])
+    ok = rng.random((nz, ny, nx)) > 0.15
+    for scheme in ("weno5", "upwind3"):
+        got = _adv(scheme, 15, time="pc").advect(C, V, V1, Qx, Qy, Fz, ok)
+        ref = R.advect_pc(C, V, V1, Qx, Qy, Fz, ok, 30.0, open_mask=15, up_order=UPO[scheme])
+        assert _same_bits(got, ref)
+
+
+def test_pc_periodic_pad_crop_matches_the_oracle(gpu):
+    _say("pc periodic")
+    rng = np.random.default_rng(23)
+    nt, nz, ny, nx = 2, 3, 20, 24
+    V = rng.uniform(0.8, 1.2, (nz, ny, nx)) * 1e6
+    Qx = rng.uniform(-0.3, 0.3, (nz, ny, nx + 1)) * 1e4
+    Qy = rng.uniform(-0.3, 0.3, (nz, ny + 1, nx)) * 1e4
+    Qx[..., -1] = Qx[..., 0]
+    Qy[:, -1] = Qy[:, 0]
+    V1 = V - 30.0 * (Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:] - Qy[:, :-1])
+    ok = V1 > 0
+    C = rng.normal(5.0, 1.0, (nt, nz, ny, nx))
+    Fz = _fz((nz, ny, nx), 8)
+    H = R.TRACER_HALO
+    P = lambda a, k: R.pad(a, k, H, True, True)                          # noqa: E731
+    got = _adv("weno5", per=("x", "y"), time="pc").advect(C, V, V1, Qx, Qy, Fz, ok)
+    ref = R.advect_pc(P(C, "c"), P(V, "c"), P(V1, "c"), P(Qx, "u"), P(Qy, "v"), P(Fz, "c"), P(ok, "c"), 30.0)
+    assert _same_bits(got, R.crop(ref, H, True, True))

@@ -55,12 +55,14 @@ def test_flag_word_rejects_unknown_up_orders_and_a_missing_fz():
     assert ops._up_faces(16, None) is False
 
 
-def test_predictor_corrector_fails_loudly():
-    adv = _adv(time="pc")
-    C = np.zeros((1, 2, 3, 4))
-    with pytest.raises(NotImplementedError, match="pc"):
-        adv.advect(C, np.ones((2, 3, 4)), np.ones((2, 3, 4)), np.zeros((2, 3, 5)), np.zeros((2, 4, 4)),
-                   np.zeros((3, 3, 4)), np.ones((2, 3, 4), bool))
+def test_centred_faces_flag_word():
+    """rhs / flux_x with centred=True take the centred faces whatever the scheme (tracers.py:110-113)."""
+    for scheme in ("weno5", "upwind3"):
+        assert _adv(scheme=scheme, open_=("east",))._flags(centred=True) == 256 | 2
+    from pyocean_amd import ops
+    assert ops.flags_of(5, centred=True) == 261
+    with pytest.raises(ValueError, match="need Fz"):
+        ops._up_faces(256, None)
 
 
 def test_periodic_axes():

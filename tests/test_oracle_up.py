@@ -1,10 +1,13 @@
 """
-Pins of the oracle's UP3 / UP1 faces and RHS (oracle.numpy_ref.up_faces, faces_up_h, rhs_up; the reference's
-_up3_faces, sigma_baroclinic.py:164-187, and rhs_numpy, tracers.py:139-151).  The reference's tests hold no vectors
-for the tracer upwind faces, so these checks come from what the mathematics fixes: UP3 reproduces the face value of
-any quadratic from its cell averages, each branch reads only its upwind-biased stencil, a cell that is not ok drops
-the face to the donor value, UP1 is the donor flux of upwind_faces_h, the flux form telescopes, and a linear RHS
-advanced by SSP-RK3 is the third-order Taylor polynomial of the step.
+Pins of the oracle's UP3 / UP1 and centred faces, their RHS and the 'pc' step (oracle.numpy_ref.up_faces,
+centred_faces, faces_up_h, rhs_up, advect_pc; the reference's _up3_faces and _centred_faces,
+sigma_baroclinic.py:164-187, 632-652, rhs_numpy and advect_numpy, tracers.py:139-151, 303-330).  The reference's
+tests hold no vectors for these faces; the checks are its sixth-order test of the centred faces
+(test_sigma_baroclinic.py:133-143) and what the mathematics fixes: UP3 reproduces the face value of any quadratic
+from its cell averages and the centred faces that of a quintic (cubic, linear next to walls), each UP3 branch reads
+only its upwind-biased stencil, a cell that is not ok drops the face to the donor value, UP1 is the donor flux of
+upwind_faces_h, the flux form telescopes, and a linear RHS advanced by SSP-RK3 is the third-order Taylor polynomial
+of the step (by 'pc', the predictor-corrector product).
 """
 from __future__ import annotations
 
@@ -145,3 +148,103 @@ def test_rk3_step_is_the_cubic_taylor_polynomial_of_the_linear_rhs(order):
     B = dt * A
     want = (np.eye(m) + B + B @ B / 2.0 + B @ B @ B / 6.0) @ c.ravel()
     np.testing.assert_allclose(out.ravel(), want, rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------------ centred faces
+def _poly_averages(n, coef):
+    """Cell averages over [m, m+1] of the polynomial sum coef[p] x^p, and the polynomial."""
+    m = np.arange(n, dtype=float)
+    avg = sum(c * ((m + 1.0) ** (p + 1) - m ** (p + 1)) / (p + 1) for p, c in enumerate(coef))
+    return avg, lambda x: sum(c * x ** p for p, c in enumerate(coef))
+
+
+def test_centred_faces_are_sixth_order():
+    """The reference's own test (test_sigma_baroclinic.py:133-143) on the oracle."""
+    errs = []
+    for n in (40, 80):
+        x = (np.arange(n) + 0.5) / n
+        h = 1.0 / n
+        c = (np.cos(2 * np.pi * (x - h / 2)) - np.cos(2 * np.pi * (x + h / 2))) / (2 * np.pi * h)
+        faces = R.centred_faces(c, np.ones(n, bool), axis=0)
+        exact = np.sin(2 * np.pi * np.arange(n + 1) / n)
+        errs.append(np.abs(faces[4:-3] - exact[4:-3]).max())
+    assert np.log2(errs[0] / errs[1]) > 5.0
+
+
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_centred_faces_reproduce_polynomials_by_stencil_width(axis):
+    n = 12
+    shape = [1, 1, 1, 1]
+    shape[axis] = n
+    ok = np.ones(shape, bool)
+    f = np.arange(n + 1, dtype=float)
+    for coef, faces in (([0.3, -1.1, 0.7, 0.25, -0.05, 0.004], range(3, n - 2)),   # 6-cell stencil: quintic
+                        ([0.3, -1.1, 0.7, 0.25], (2, n - 2)),                      # 4-cell stencil: cubic
+                        ([0.3, -1.1], (1, n - 1))):                                # 2 cells: linear
+        avg, p = _poly_averages(n, coef)
+        got = np.moveaxis(R.centred_faces(avg.reshape(shape), ok, axis), axis, -1).ravel()
+        idx = np.array(list(faces))
+        np.testing.assert_allclose(got[idx], p(f[idx]), rtol=0, atol=1e-11)
+        assert got[0] == 0.0 and got[n] == 0.0
+
+
+def test_centred_faces_ignore_the_flux_and_narrow_at_cells_that_are_not_ok():
+    rng = np.random.default_rng(10)
+    n = 12
+    c = rng.normal(size=(1, 1, 1, n))
+    ok = np.ones(c.shape, bool)
+    q = rng.normal(size=(1, 1, 1, n + 1))
+    a = R.faces_up_h(c, q[0], np.zeros((1, 2, n)), ok, c, 0, "c")[0] / q
+    assert np.allclose(a[..., 1:-1], R.centred_faces(c, ok, 3)[..., 1:-1], rtol=1e-15, atol=0)
+    ok[..., 6] = False
+    wide = R.centred_faces(c, np.ones(c.shape, bool), 3).ravel()
+    got = R.centred_faces(c, ok, 3).ravel()
+    cv = c.ravel()
+    assert got[4] == (7.0 * (cv[3] + cv[4]) - (cv[2] + cv[5])) / 12.0      # 6-stencil holds cell 6: 4-point
+    assert got[6] == 0.5 * (cv[5] + cv[6]) and got[7] == 0.5 * (cv[6] + cv[7])
+    assert got[10] == wide[10] and got[3] == wide[3]
+
+
+def test_centred_rhs_telescopes_to_zero():
+    rng = np.random.default_rng(11)
+    nt, n, ny, nx = 2, 6, 8, 11
+    c = rng.normal(size=(nt, n, ny, nx))
+    Qx, Qy, Fz = rng.normal(size=(n, ny, nx + 1)), rng.normal(size=(n, ny + 1, nx)), rng.normal(size=(n + 1, ny, nx))
+    r = R.rhs_up(c, Qx, Qy, Fz, rng.random((n, ny, nx)) > 0.1, c, 0, "c")
+    scale = np.abs(c).max() * (np.abs(Qx).sum() + np.abs(Qy).sum() + np.abs(Fz).sum()) / nt
+    assert np.all(np.abs(r.sum(axis=(1, 2, 3))) < 1e-13 * scale)
+
+
+@pytest.mark.parametrize("up_order", [0, 3, 1])
+def test_pc_step_of_a_linear_rhs_is_the_predictor_corrector_product(up_order):
+    """With V0 = V1 = 1 and linear RHS A_c (centred) and A (the corrector's faces): out = (I + dt A (I + dt/2 A_c)) c.
+    WENO is linear in c only for a fixed stencil choice, so its case uses a constant tracer plus a small ripple
+    and compares against the matrix of the WENO RHS linearised by finite differences (loose)."""
+    rng = np.random.default_rng(12)
+    n, ny, nx = 3, 2, 9
+    Qx, Qy, Fz = rng.normal(size=(n, ny, nx + 1)), rng.normal(size=(n, ny + 1, nx)), rng.normal(size=(n + 1, ny, nx))
+    ok = np.ones((n, ny, nx), bool)
+    V = np.ones((n, ny, nx))
+    dt = 0.05
+    m = n * ny * nx
+
+    def matrix(op):
+        A = np.empty((m, m))
+        for j in range(m):
+            e = np.zeros(m)
+            e[j] = 1.0
+            A[:, j] = op(e.reshape(1, n, ny, nx)).ravel()
+        return A
+    Ac = matrix(lambda e: R.rhs_up(e, Qx, Qy, Fz, ok, None, 0, "c"))
+    c = rng.normal(size=(1, n, ny, nx))
+    if up_order:
+        A = matrix(lambda e: R.rhs_up(e, Qx, Qy, Fz, ok, None, 0, up_order))
+        want = (np.eye(m) + dt * A @ (np.eye(m) + 0.5 * dt * Ac)) @ c.ravel()
+        out = R.advect_pc(c, V, V, Qx, Qy, Fz, ok, dt, up_order=up_order)
+        np.testing.assert_allclose(out.ravel(), want, rtol=0, atol=1e-12)
+    else:
+        # the WENO corrector applied to the exact predictor state
+        ch = (c.ravel() + 0.5 * dt * Ac @ c.ravel()).reshape(c.shape)
+        want = c + dt * R.rhs(ch, Qx, Qy, Fz, ok, ch, 0, True, True)
+        out = R.advect_pc(c, V, V, Qx, Qy, Fz, ok, dt)
+        np.testing.assert_allclose(out, want, rtol=0, atol=1e-12)
