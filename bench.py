@@ -406,6 +406,92 @@ def bench_diffusion(args) -> int:
     return 0
 
 
+# compulsory bytes per tracer-cell of the unlimited step's kernels (the shared Qx, Qy, V0, V1, Fz and ok, 41 B per
+# cell, amortised over the tracers): RK3 stage 1 reads C, writes q1 and c1; stage 2 reads c1, C, q1, writes q2 and
+# c2; stage 3 reads c2, C, q2, writes out; the 'pc' predictor reads C and writes c^{n+1/2}, the corrector reads it
+# and C and writes out
+RK_BYTES = {"rk3_stage1": 24.0, "rk3_stage2": 40.0, "rk3_stage3": 32.0, "pc_predictor": 16.0, "pc_corrector": 24.0}
+
+
+def bench_rk3(args) -> int:
+    """The unlimited step of TracerAdvection.advect on the regional grid in sigma form (V1 = V0, the continuity
+    residual in Fz, so every stage runs the z sweep): SSP-RK3 or the 'pc' predictor-corrector, with the faces of
+    --scheme."""
+    import torch
+
+    from pyocean_amd import _native, ops, synth
+
+    _log(f"generating regional sigma {args.nx}x{args.ny}x{args.nz} nt={args.nt}")
+    case = synth.regional(nx=args.nx, ny=args.ny, nz=args.nz, nt=args.nt, seed=0, sigma=True)
+    nt, nz, ny, nx = case.C.shape
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()       # noqa: E731
+    C, V0, V1, Qx, Qy, Fz = d(case.C), d(case.V0), d(case.V1), d(case.Qx), d(case.Qy), d(case.Fz)
+    ok = d(case.ok.astype(np.uint8))
+    out = torch.empty_like(C)
+    ws = ops.Workspace()
+    up = {"upwind3": 3, "upwind1": 1}.get(args.scheme, 0)
+    flags = ops.flags_of(js=args.scheme == "weno5js", up=up)
+    step_fn = ops.advect_pc if args.time == "pc" else ops.advect_rk3
+    cur = [C, out]
+
+    def step():
+        step_fn(cur[0], V0, V1, Qx, Qy, Fz, ok, case.dt, flags=flags, vert=None if up else True, out=cur[1],
+                workspace=ws)
+        cur.reverse()
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    _native.profile_enable(True)
+    _native.profile_collect()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _native.profile_enable(False)
+    prof = _native.profile_collect()
+    updates = nt * nz * ny * nx
+    value = updates * args.steps / wall
+    kms = {k: v[0] / v[1] for k, v in prof.items()}
+    dom = max((k for k in kms if k in RK_BYTES), key=lambda k: kms[k])
+    dom_bytes = (RK_BYTES[dom] + 41.0 / nt) * updates
+    names = ("pc_predictor", "pc_corrector") if args.time == "pc" else ("rk3_stage1", "rk3_stage2", "rk3_stage3")
+    b_step = sum(RK_BYTES[k] + 41.0 / nt for k in names)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import numpy_ref as R
+        sy, st = min(ny, 64), 1
+        Cs = np.ascontiguousarray(case.C[:st, :, :sy])
+        sl = lambda a: np.ascontiguousarray(a[:, :sy])                    # noqa: E731
+        Qys = np.ascontiguousarray(case.Qy[:, :sy + 1])
+        Qys[:, sy] = 0.0
+        run = R.advect_pc if args.time == "pc" else R.advect_rk3
+        t1 = time.perf_counter()
+        run(Cs, sl(case.V0), sl(case.V1), sl(case.Qx), Qys, sl(case.Fz), sl(case.ok), case.dt,
+            z_variant=args.scheme != "weno5js", up_order=up)
+        tc = time.perf_counter() - t1
+        cpu = {"value": Cs.size / tc, "unit": "tracer-cell updates/s", "cores": 1, "kind": "port",
+               "sample": f"{st} tracer of the first {sy} rows ({nx}x{sy}x{nz}) of the same grid through the "
+                         f"oracle's NumPy {args.time} step ({tc:.1f} s)"}
+    res = {
+        "metric": f"tracer-cell updates/s of the unlimited {args.time} step ({args.scheme} faces, x, y and z sweeps)",
+        "value": value, "unit": "tracer-cell updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded regional grid, sigma form)",
+        "config": {"workload": f"regional sigma {nx}x{ny}x{nz}, {nt} tracers, {args.scheme}, {args.time}"},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": dom_bytes / kms[dom] / 1e6, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": dom_bytes / (kms[dom] * 1e-3) / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes": dom_bytes},
+        "step_roofline": {"bytes_per_update": b_step, "achieved": value * b_step / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "frac": value * b_step / HBM_PEAK},
+        "kernels_ms": kms,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 # algorithmic HBM bytes of the ALE layer-transport kernels per cell-layer (faces of one direction ~ cells): each
 # donor cell is read by two faces, so a face reads one cell's worth of the cell fields
 ALE_BYTES = {
@@ -776,14 +862,19 @@ def main() -> int:
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--rule", choices=("zstar", "hybrid"), default="zstar",
                     help="--form remap: the regrid rule (hybrid: Wright densities of a stratified column)")
-    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion", "ale", "remap", "cycle"),
+    ap.add_argument("--scheme", choices=("weno5", "weno5js", "upwind3", "upwind1"), default="weno5",
+                    help="--form rk3: the face scheme")
+    ap.add_argument("--time", choices=("ssprk3", "pc"), default="ssprk3", help="--form rk3: the time stepping")
+    ap.add_argument("--form", choices=("step", "batch", "serve", "precision", "diffusion", "ale", "remap", "cycle",
+                                       "rk3"),
                     default="step",
                     help="step: one domain (the default workload); batch: 64 regional domains of uneven size in one "
                          "persistent launch; serve: the Poisson job stream through the continuous-batching loop; "
                          "precision: split against FP64 after --checkpoints steps; diffusion: the along-layer and "
                          "vertical tracer diffusion of the stack; ale: the ALE layer-transport step that "
                          "produces the advection's inputs; remap: the vertical regrid + remap of the stack; "
-                         "cycle: the whole ALE tracer cycle (transport, advection, diffusion, regrid + remap)")
+                         "cycle: the whole ALE tracer cycle (transport, advection, diffusion, regrid + remap); "
+                         "rk3: the unlimited step with the z sweep (--scheme, --time)")
     ap.add_argument("--checkpoints", default="1,10,200,2000", help="precision form: step counts to compare at")
     ap.add_argument("--mix", choices=("mixed", "split", "f64"), default="mixed",
                     help="batch form: member precisions (mixed: even members FP64, odd split)")
@@ -822,6 +913,8 @@ def main() -> int:
         return bench_remap(args)
     if args.form == "cycle":
         return bench_cycle(args)
+    if args.form == "rk3":
+        return bench_rk3(args)
 
     if args.rch or args.wpb:
         from pyocean_amd import batch as _batch

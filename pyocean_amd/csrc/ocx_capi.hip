@@ -158,9 +158,11 @@ struct RkArgs {
     unsigned flags;
 };
 
+// k_rk_stage runs one block per (tile, tracer) on a 1-D grid; the host checks the block count fits it.
 template <int STAGE, int SCH, bool VERT>
-void rk_launch(const RkArgs& a, dim3 grid, hipStream_t s) {
-    ocx::k_rk_stage<STAGE, SCH, VERT, TX, TY><<<grid, dim3(TX, TY), 0, s>>>(
+void rk_launch(const RkArgs& a, dim3, hipStream_t s) {
+    const long long nb = (long long)((a.nx + TX - 1) / TX) * ((a.ny + TY - 1) / TY) * a.nt;
+    ocx::k_rk_stage<STAGE, SCH, VERT, TX, TY><<<dim3((unsigned)nb), dim3(TX, TY), 0, s>>>(
         a.cin, a.C, a.ext, a.V0, a.V1, a.Qx, a.Qy, a.Fz, a.ok, a.q, a.cout, a.fxs, a.fi, a.nt, a.nz, a.ny, a.nx, a.dt,
         a.flags);
 }
@@ -177,6 +179,13 @@ void rk_stage(const RkArgs& a, int sch, bool vert, dim3 grid, hipStream_t s) {
         case SCH_JS: vert ? rk_launch<STAGE, SCH_JS, true>(a, grid, s) : rk_launch<STAGE, SCH_JS, false>(a, grid, s); break;
         default: vert ? rk_launch<STAGE, SCH_Z, true>(a, grid, s) : rk_launch<STAGE, SCH_Z, false>(a, grid, s);
     }
+}
+
+// The RK3 / pc / RHS kernel's 1-D grid of (tile, tracer) blocks must fit a launch.
+int check_rk_grid(int nt, int ny, int nx) {
+    const long long nb = (long long)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * nt;
+    if (nb > 0x7fffffffLL) return fail(OCX_E_ARG, "%lld tiles x tracers exceed one launch", nb);
+    return OCX_OK;
 }
 
 // Flags of the RK3 step, the RHS and faces_h: open edges plus at most one of WENO_JS, UP3, UP1 and (when
@@ -391,6 +400,7 @@ int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, cons
     if (vert || up) OCX_TRY(need(Fz, "Fz"));
     if (C_out == C) return fail(OCX_E_ARG, "C_out must not alias C");
     OCX_TRY(check_rk_flags(flags, "the RK3 step"));
+    OCX_TRY(check_rk_grid(nt, ny, nx));
     if (flux_face_i != -1 && (flux_face_i < 0 || flux_face_i > nx))
         return fail(OCX_E_ARG, "flux_face_i %d outside 0..%d", flux_face_i, nx);
     if (flux_face_i >= 0 && !fx_i) return fail(OCX_E_ARG, "fx_i is NULL with flux_face_i = %d", flux_face_i);
@@ -449,6 +459,7 @@ int ocx_advect_pc_f64(const double* C, const double* V0, const double* V1, const
     OCX_TRY(need(Fz, "Fz"));
     if (C_out == C) return fail(OCX_E_ARG, "C_out must not alias C");
     OCX_TRY(check_rk_flags(flags, "the pc step"));
+    OCX_TRY(check_rk_grid(nt, ny, nx));
     if (flux_face_i != -1 && (flux_face_i < 0 || flux_face_i > nx))
         return fail(OCX_E_ARG, "flux_face_i %d outside 0..%d", flux_face_i, nx);
     if (flux_face_i >= 0 && !fx_i) return fail(OCX_E_ARG, "fx_i is NULL with flux_face_i = %d", flux_face_i);
@@ -529,6 +540,7 @@ int ocx_rhs_f64(const double* c, const double* Qx, const double* Qy, const doubl
     const bool up = flags & (UP3 | UP1 | CENTRED);
     if (vert || up) OCX_TRY(need(Fz, "Fz"));
     OCX_TRY(check_rk_flags(flags, "the RHS", true));
+    OCX_TRY(check_rk_grid(nt, ny, nx));
     hipStream_t s = static_cast<hipStream_t>(stream);
     const RkArgs a{c, c, ext, nullptr, nullptr, Qx, Qy, Fz, ok, nullptr, out, nullptr, -1, nt, nz, ny, nx, 0.0, flags};
     rk_stage<0>(a, scheme_of(flags), vert, tile_grid(nt, nz, ny, nx), s);

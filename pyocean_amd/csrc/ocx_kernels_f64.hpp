@@ -226,70 +226,29 @@ __device__ __forceinline__ double coeff(double a, double rin_hi, double rout_lo,
     return a >= 0.0 ? dmin(rin_hi, rout_lo) : dmin(rin_lo, rout_hi);
 }
 
-// Vertical WENO face flux of interface kf (between layers kf-1 and kf, 1 <= kf <= n-1) of one column; same
-// validity rules along the layer index with each layer's own ok (_advect_div, sigma_baroclinic.py:335-353).
-template <bool Z>
-__device__ __forceinline__ double zface(const double* __restrict__ ccol, const uint8_t* __restrict__ ocol,
-                                        size_t plane, int n, int kf, double q) {
-    const bool pos = q >= 0.0;
-    int id[5];
-    id[0] = pos ? kf - 3 : kf + 2;
-    id[1] = pos ? kf - 2 : kf + 1;
-    id[2] = pos ? kf - 1 : kf;
-    id[3] = pos ? kf : kf - 1;
-    id[4] = pos ? kf + 1 : kf - 2;
-    double v[5];
-    bool b[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        const bool in = id[m] >= 0 && id[m] < n;
-        v[m] = in ? ccol[(size_t)id[m] * plane] : 0.0;
-        b[m] = in && ocol[(size_t)id[m] * plane] != 0;
+// Interface kf's z face (1 <= kf <= n-1, between layers kf-1 and kf) from a column window w[0..5] = c[kf-3 .. kf+2]
+// (each layer's own ok in b; false and value 0 outside the column): the WENO face times q, upwind-ordered from the
+// sign of q as along x and y (_advect_div's vertical pass, sigma_baroclinic.py:335-353), or the UP3 / UP1 / centred
+// face value, not yet times q (_up3_faces, _centred_faces along the layer index, rhs_numpy tracers.py:148-150).
+template <int SCH>
+__device__ __forceinline__ double zface_w(const double* w, const bool* b, double q) {
+    if constexpr (SCH == SCH_C6) {
+        const double a = w[2] + w[3];
+        const bool m4 = b[1] && b[2] && b[3] && b[4];
+        if (m4 && b[0] && b[5]) return (37.0 * a - 8.0 * (w[1] + w[4]) + (w[0] + w[5])) / 60.0;
+        if (m4) return (7.0 * a - (w[1] + w[4])) / 12.0;
+        return 0.5 * a;
+    } else if constexpr (SCH == SCH_UP1) {
+        return q >= 0.0 ? w[2] : w[3];
+    } else if constexpr (SCH == SCH_UP3) {
+        if (q >= 0.0) return (b[1] && b[2] && b[3]) ? (-w[1] + 5.0 * w[2] + 2.0 * w[3]) / 6.0 : w[2];
+        return (b[2] && b[3] && b[4]) ? (2.0 * w[2] + 5.0 * w[3] - w[4]) / 6.0 : w[3];
+    } else {
+        if (q >= 0.0) return q * face_value<SCH == SCH_Z>(w[0], w[1], w[2], w[3], w[4], b[0], b[1], b[2], b[3], b[4]);
+        return q * face_value<SCH == SCH_Z>(w[5], w[4], w[3], w[2], w[1], b[5], b[4], b[3], b[2], b[1]);
     }
-    return q * face_value<Z>(v[0], v[1], v[2], v[3], v[4], b[0], b[1], b[2], b[3], b[4]);
 }
 
-// UP3 / UP1 face value (not yet times Fz) of interface kf (1 <= kf <= n-1) of one column: _up3_faces along the
-// layer index (rhs_numpy, tracers.py:148-150), layers outside 0..n-1 invalid.
-template <bool O3>
-__device__ __forceinline__ double zface_up(const double* __restrict__ ccol, const uint8_t* __restrict__ ocol,
-                                           size_t plane, int n, int kf, double q) {
-    const double lo = ccol[(size_t)(kf - 1) * plane], hi = ccol[(size_t)kf * plane];
-    if (!O3) return q >= 0.0 ? lo : hi;
-    const bool olo = ocol[(size_t)(kf - 1) * plane] != 0, ohi = ocol[(size_t)kf * plane] != 0;
-    if (q >= 0.0) {
-        if (kf >= 2 && ocol[(size_t)(kf - 2) * plane] != 0 && olo && ohi)
-            return (-ccol[(size_t)(kf - 2) * plane] + 5.0 * lo + 2.0 * hi) / 6.0;
-        return lo;
-    }
-    if (kf + 1 < n && olo && ohi && ocol[(size_t)(kf + 1) * plane] != 0)
-        return (2.0 * lo + 5.0 * hi - ccol[(size_t)(kf + 1) * plane]) / 6.0;
-    return hi;
-}
-
-// Centred face value (not yet times Fz) of interface kf (1 <= kf <= n-1) of one column: _centred_faces along the
-// layer index (rhs_numpy with centred=True, tracers.py:148-150), layers outside 0..n-1 invalid.
-__device__ __forceinline__ double zface_c(const double* __restrict__ ccol, const uint8_t* __restrict__ ocol,
-                                          size_t plane, int n, int kf) {
-    auto okk = [&](int k) { return k >= 0 && k < n && ocol[(size_t)k * plane] != 0; };
-    auto cc = [&](int k) { return ccol[(size_t)k * plane]; };
-    const double a = cc(kf - 1) + cc(kf);
-    const bool m4 = okk(kf - 2) && okk(kf - 1) && okk(kf) && okk(kf + 1);
-    if (m4 && okk(kf - 3) && okk(kf + 2))
-        return (37.0 * a - 8.0 * (cc(kf - 2) + cc(kf + 1)) + (cc(kf - 3) + cc(kf + 2))) / 60.0;
-    if (m4) return (7.0 * a - (cc(kf - 2) + cc(kf + 1))) / 12.0;
-    return 0.5 * a;
-}
-
-// SSP-RK3 stage of TracerAdvection.advect (tracers.py:342-375, _rk_stage sigma_baroclinic.py:856-875) with the
-// fused RHS r = -div(Q c): WENO faces of _advect_div (x, y and, with VERT, z), or UP3 / UP1 / centred faces of
-// rhs_numpy (tracers.py:139-151), whose z faces are always formed (fz = Fz * face, the bed face Fz * 0, the
-// surface 0).
-// STAGE 0: r only.  1: q1 = q0 + dt r, c1 = q1/Va.  2: q2 = 0.75 q0 + 0.25 (q1 + dt r), c2 = q2/Vb (q in place).
-// 3: q3 = q0/3 + 2/3 (q2 + dt r), out = q3/V1.  The 'pc' step (advect_numpy, tracers.py:324-330):
-// 4: ch = (q0 + 0.5 dt r)/Vh, Vh = V0 + 0.5 dV (the predictor); 5: out = (q0 + dt r)/V1 (the corrector).
-// Every division falls back to C where the volume is not > 0.  fxs (optional): the x-face column flux_face_i of
-// this stage.
 template <int STAGE, int SCH, bool VERT, int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ cin, const double* __restrict__ C,
                                                      const double* __restrict__ ext, const double* __restrict__ V0,
@@ -299,16 +258,45 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                                                      double* __restrict__ cout, double* __restrict__ fxs,
                                                      int flux_face_i, int nt, int n, int ny, int nx, double dt,
                                                      unsigned flags) {
+    // One block marches one tile of one tracer up its layers (1-D grid of gx * gy * nt blocks, XCD-contiguous):
+    // each layer's x and y faces go through the LDS tile; each thread keeps its column's window c[k-2 .. k+3] in
+    // registers and forms the z face between layers k and k+1 once, carrying it as the next layer's bottom face.
+    constexpr bool ZF = VERT || SCH >= SCH_UP3;
     constexpr int H = 3, LX = TX + 2 * H, LY = TY + 2 * H;
     __shared__ double sc[LY * LX];
     __shared__ uint8_t so[LY * LX];
     __shared__ double sfx[TY * (TX + 1)];
     __shared__ double sfy[(TY + 1) * TX];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
+    const long long nb = (long long)gx * gy * nt;
+    long long L = xcd_logical((long long)blockIdx.x, nb);
+    const int bx = (int)(L % gx);
+    L /= gx;
+    const int by = (int)(L % gy);
+    const int t = (int)(L / gy);
+    const int i0 = bx * TX, j0 = by * TY;
     const size_t plane = (size_t)ny * nx;
-    for (int p = blockIdx.z; p < nt * n; p += gridDim.z) {
-        const int t = p / n, k = p - t * n;
+    const int j = j0 + ty, i = i0 + tx;
+    const bool cell = j < ny && i < nx;
+    const size_t xc = cell ? (size_t)j * nx + i : 0;
+    const double* ccol = cin + (size_t)t * n * plane + xc;
+    double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    bool b[6] = {false, false, false, false, false, false};
+    auto wload = [&](int m, int kk) {
+        const bool in = cell && kk >= 0 && kk < n;
+        w[m] = in ? ccol[(size_t)kk * plane] : 0.0;
+        b[m] = in && ok[(size_t)kk * plane + xc] != 0;
+    };
+    if constexpr (ZF) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) wload(m, m - 2);
+    }
+    // the bottom face of layer 0: 0 for WENO, Fz[0] * 0 for the rhs_numpy faces
+    double fb = 0.0;
+    if constexpr (SCH >= SCH_UP3) fb = cell ? Fz[xc] * 0.0 : 0.0;
+    for (int k = 0; k < n; ++k) {
+        const int p = t * n + k;
         const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
         __syncthreads();
         load_tile<TX, TY, H>(sc, so, cin + pc, ok + kc, i0, j0, ny, nx);
@@ -316,46 +304,22 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
         tile_faces<SCH, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
                                 ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
         __syncthreads();
-        const int j = j0 + ty, i = i0 + tx;
-        if (j < ny && i < nx) {
-            const size_t xc = (size_t)j * nx + i, x = pc + xc, xv = kc + xc;
+        double ft = 0.0;
+        if constexpr (ZF) {
+            // window w = c[k-2 .. k+3] = c[kf-3 .. kf+2] of the interface kf = k+1
+            if (cell && k + 1 < n) {
+                const double qz = Fz[(size_t)(k + 1) * plane + xc];
+                if constexpr (SCH >= SCH_UP3)
+                    ft = qz * zface_w<SCH>(w, b, qz);
+                else if (qz != 0.0)
+                    ft = zface_w<SCH>(w, b, qz);
+            }
+        }
+        if (cell) {
+            const size_t x = pc + xc, xv = kc + xc;
             double d = sfx[ty * (TX + 1) + tx + 1] - sfx[ty * (TX + 1) + tx] + sfy[(ty + 1) * TX + tx] -
                        sfy[ty * TX + tx];
-            if constexpr (SCH >= SCH_UP3) {
-                const double* ccol = cin + (size_t)t * n * plane + xc;
-                const uint8_t* ocol = ok + xc;
-                double ft = 0.0, fb;
-                auto zf = [&](int kf, double qz) {
-                    if constexpr (SCH == SCH_C6)
-                        return zface_c(ccol, ocol, plane, n, kf);
-                    else
-                        return zface_up<SCH == SCH_UP3>(ccol, ocol, plane, n, kf, qz);
-                };
-                if (k + 1 < n) {
-                    const double qz = Fz[(size_t)(k + 1) * plane + xc];
-                    ft = qz * zf(k + 1, qz);
-                }
-                if (k > 0) {
-                    const double qz = Fz[(size_t)k * plane + xc];
-                    fb = qz * zf(k, qz);
-                } else {
-                    fb = Fz[xc] * 0.0;
-                }
-                d = d + ft - fb;
-            } else if (VERT) {
-                const double* ccol = cin + (size_t)t * n * plane + xc;
-                const uint8_t* ocol = ok + xc;
-                double ft = 0.0, fb = 0.0;
-                if (k + 1 < n) {
-                    const double qz = Fz[(size_t)(k + 1) * plane + xc];
-                    if (qz != 0.0) ft = zface<SCH == SCH_Z>(ccol, ocol, plane, n, k + 1, qz);
-                }
-                if (k > 0) {
-                    const double qz = Fz[(size_t)k * plane + xc];
-                    if (qz != 0.0) fb = zface<SCH == SCH_Z>(ccol, ocol, plane, n, k, qz);
-                }
-                d = d + ft - fb;
-            }
+            if constexpr (ZF) d = d + ft - fb;
             const double r = -d;
             if (fxs) {
                 if (flux_face_i == i) fxs[(size_t)p * ny + j] = sfx[ty * (TX + 1) + tx];
@@ -387,6 +351,15 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                 }
                 cout[x] = V > 0 ? qn / V : c0;
             }
+        }
+        if constexpr (ZF) {
+            fb = ft;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                w[m] = w[m + 1];
+                b[m] = b[m + 1];
+            }
+            wload(5, k + 4);
         }
     }
 }
