@@ -30,6 +30,7 @@ namespace {
 using namespace ocx_host;
 
 constexpr int TX = 64, TY = 4;
+constexpr int RK_TY = 4;     // tile rows of the RK3 / pc / RHS layer-marching kernel
 
 int check_shape(int nt, int nz, int ny, int nx) {
     if (nt < 1 || nz < 1 || ny < 1 || nx < 1)
@@ -161,8 +162,8 @@ struct RkArgs {
 // k_rk_stage runs one block per (tile, tracer) on a 1-D grid; the host checks the block count fits it.
 template <int STAGE, int SCH, bool VERT>
 void rk_launch(const RkArgs& a, dim3, hipStream_t s) {
-    const long long nb = (long long)((a.nx + TX - 1) / TX) * ((a.ny + TY - 1) / TY) * a.nt;
-    ocx::k_rk_stage<STAGE, SCH, VERT, TX, TY><<<dim3((unsigned)nb), dim3(TX, TY), 0, s>>>(
+    const long long nb = (long long)((a.nx + TX - 1) / TX) * ((a.ny + RK_TY - 1) / RK_TY) * a.nt;
+    ocx::k_rk_stage<STAGE, SCH, VERT, TX, RK_TY><<<dim3((unsigned)nb), dim3(TX, RK_TY), 0, s>>>(
         a.cin, a.C, a.ext, a.V0, a.V1, a.Qx, a.Qy, a.Fz, a.ok, a.q, a.cout, a.fxs, a.fi, a.nt, a.nz, a.ny, a.nx, a.dt,
         a.flags);
 }
@@ -183,7 +184,7 @@ void rk_stage(const RkArgs& a, int sch, bool vert, dim3 grid, hipStream_t s) {
 
 // The RK3 / pc / RHS kernel's 1-D grid of (tile, tracer) blocks must fit a launch.
 int check_rk_grid(int nt, int ny, int nx) {
-    const long long nb = (long long)((nx + TX - 1) / TX) * ((ny + TY - 1) / TY) * nt;
+    const long long nb = (long long)((nx + TX - 1) / TX) * ((ny + RK_TY - 1) / RK_TY) * nt;
     if (nb > 0x7fffffffLL) return fail(OCX_E_ARG, "%lld tiles x tracers exceed one launch", nb);
     return OCX_OK;
 }

@@ -295,11 +295,39 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
     // the bottom face of layer 0: 0 for WENO, Fz[0] * 0 for the rhs_numpy faces
     double fb = 0.0;
     if constexpr (SCH >= SCH_UP3) fb = cell ? Fz[xc] * 0.0 : 0.0;
+    // the tile of layer k (with its halo) is fetched into registers one layer ahead and stored to LDS at the top of
+    // the layer's iteration, so its loads are in flight while the previous layer's faces are formed
+    constexpr int NT = TX * TY, NL = (LY * LX + NT - 1) / NT;
+    const int tid = ty * TX + tx;
+    double pv[NL];
+    uint8_t po[NL];
+    auto fetch = [&](int kk) {
+        const double* src = cin + ((size_t)t * n + kk) * plane;
+        const uint8_t* okp = ok + (size_t)kk * plane;
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const int idx = tid + m * NT;
+            const int ly = idx / LX, lx = idx - ly * LX;
+            const int jj = j0 - H + ly, ii = i0 - H + lx;
+            const bool in = idx < LY * LX && jj >= 0 && jj < ny && ii >= 0 && ii < nx;
+            pv[m] = in ? src[(size_t)jj * nx + ii] : 0.0;
+            po[m] = in ? okp[(size_t)jj * nx + ii] : (uint8_t)0;
+        }
+    };
+    fetch(0);
     for (int k = 0; k < n; ++k) {
         const int p = t * n + k;
         const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
         __syncthreads();
-        load_tile<TX, TY, H>(sc, so, cin + pc, ok + kc, i0, j0, ny, nx);
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const int idx = tid + m * NT;
+            if (idx < LY * LX) {
+                sc[idx] = pv[m];
+                so[idx] = po[m];
+            }
+        }
+        if (k + 1 < n) fetch(k + 1);
         __syncthreads();
         tile_faces<SCH, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
                                 ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
