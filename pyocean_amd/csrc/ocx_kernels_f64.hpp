@@ -328,6 +328,20 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
             }
         }
         if (k + 1 < n) fetch(k + 1);
+        // the cell's update inputs, issued before the face arithmetic so that their loads overlap it
+        const size_t x = pc + xc, xv = kc + xc;
+        double c0 = 0.0, v0 = 0.0, v1 = 0.0, qo = 0.0, qz = 0.0;
+        if (cell) {
+            if constexpr (STAGE != 0) {
+                c0 = C[x];
+                v0 = V0[xv];
+                v1 = V1[xv];
+            }
+            if constexpr (STAGE == 2 || STAGE == 3) qo = q[x];
+            if constexpr (ZF) {
+                if (k + 1 < n) qz = Fz[(size_t)(k + 1) * plane + xc];
+            }
+        }
         __syncthreads();
         tile_faces<SCH, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
                                 ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
@@ -336,7 +350,6 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
         if constexpr (ZF) {
             // window w = c[k-2 .. k+3] = c[kf-3 .. kf+2] of the interface kf = k+1
             if (cell && k + 1 < n) {
-                const double qz = Fz[(size_t)(k + 1) * plane + xc];
                 if constexpr (SCH >= SCH_UP3)
                     ft = qz * zface_w<SCH>(w, b, qz);
                 else if (qz != 0.0)
@@ -344,7 +357,6 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
             }
         }
         if (cell) {
-            const size_t x = pc + xc, xv = kc + xc;
             double d = sfx[ty * (TX + 1) + tx + 1] - sfx[ty * (TX + 1) + tx] + sfy[(ty + 1) * TX + tx] -
                        sfy[ty * TX + tx];
             if constexpr (ZF) d = d + ft - fb;
@@ -356,7 +368,6 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
             if (STAGE == 0) {
                 cout[x] = r;
             } else {
-                const double c0 = C[x], v0 = V0[xv], v1 = V1[xv];
                 const double q0 = v0 * c0, dV = v1 - v0;
                 double qn, V;
                 if (STAGE == 1) {
@@ -364,11 +375,11 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                     V = v0 + dV;
                     q[x] = qn;
                 } else if (STAGE == 2) {
-                    qn = 0.75 * q0 + 0.25 * (q[x] + dt * r);
+                    qn = 0.75 * q0 + 0.25 * (qo + dt * r);
                     V = v0 + 0.5 * dV;
                     q[x] = qn;
                 } else if (STAGE == 3) {
-                    qn = q0 / 3.0 + 2.0 / 3.0 * (q[x] + dt * r);
+                    qn = q0 / 3.0 + 2.0 / 3.0 * (qo + dt * r);
                     V = v1;
                 } else if (STAGE == 4) {
                     qn = q0 + 0.5 * dt * r;
