@@ -37,19 +37,52 @@ __device__ __forceinline__ void load_tile(double* sc, uint8_t* so, const double*
 // edges 0.  UP3 / UP1 / centred (SCH_UP3, SCH_UP1, SCH_C6), rules of rhs_numpy (tracers.py:139-151): every
 // interior face is q times the _up3_faces or _centred_faces value, a closed edge is q * 0, open edges as WENO.
 // ext == nullptr: the input is the exterior.
+// Face items per thread of a TX x TY tile: x faces TY rows x TX+1, y faces TY+1 rows x TX, item m of thread tid
+// at index tid + m * TX * TY.
+template <int TX, int TY>
+struct TileItems {
+    static constexpr int NT = TX * TY, NX = (TY * (TX + 1) + NT - 1) / NT, NY = ((TY + 1) * TX + NT - 1) / NT;
+};
+
+// The face transports of the thread's items (0 outside the domain), for a caller that loads them ahead.
+template <int TX, int TY>
+__device__ __forceinline__ void tile_face_q(double* qx, double* qy, const double* __restrict__ Qxk,
+                                            const double* __restrict__ Qyk, int i0, int j0, int ny, int nx) {
+    using IT = TileItems<TX, TY>;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < IT::NX; ++m) {
+        const int idx = tid + m * IT::NT, ly = idx / (TX + 1), lf = idx - ly * (TX + 1);
+        const int j = j0 + ly, f = i0 + lf;
+        qx[m] = (idx < TY * (TX + 1) && j < ny && f <= nx) ? Qxk[(size_t)j * (nx + 1) + f] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < IT::NY; ++m) {
+        const int idx = tid + m * IT::NT, lf = idx / TX, lx = idx - lf * TX;
+        const int f = j0 + lf, i = i0 + lx;
+        qy[m] = (idx < (TY + 1) * TX && i < nx && f <= ny) ? Qyk[(size_t)f * nx + i] : 0.0;
+    }
+}
+
+// qx / qy (optional): the transports of the thread's items from tile_face_q instead of loads from Qxk / Qyk.
 template <int SCH, int TX, int TY>
 __device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const double* sc, const uint8_t* so,
                                                 const double* __restrict__ Qxk, const double* __restrict__ Qyk,
                                                 const double* __restrict__ extp, int i0, int j0, int ny, int nx,
-                                                unsigned flags) {
+                                                unsigned flags, const double* qx = nullptr,
+                                                const double* qy = nullptr) {
     constexpr int H = 3, LX = TX + 2 * H;
+    using IT = TileItems<TX, TY>;
     const int tid = threadIdx.y * TX + threadIdx.x;
-    for (int idx = tid; idx < TY * (TX + 1); idx += TX * TY) {
+#pragma unroll
+    for (int m = 0; m < IT::NX; ++m) {
+        const int idx = tid + m * IT::NT;
+        if (idx >= TY * (TX + 1)) break;
         const int ly = idx / (TX + 1), lf = idx - ly * (TX + 1);
         const int j = j0 + ly, f = i0 + lf;
         double val = 0.0;
         if (j < ny && f <= nx) {
-            const double q = Qxk[(size_t)j * (nx + 1) + f];
+            const double q = qx ? qx[m] : Qxk[(size_t)j * (nx + 1) + f];
             const double* row = sc + (ly + H) * LX;
             const uint8_t* orow = so + (ly + H) * LX;
             if (f > 0 && f < nx) {
@@ -77,12 +110,15 @@ __device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const doubl
         }
         sfx[idx] = val;
     }
-    for (int idx = tid; idx < (TY + 1) * TX; idx += TX * TY) {
+#pragma unroll
+    for (int m = 0; m < IT::NY; ++m) {
+        const int idx = tid + m * IT::NT;
+        if (idx >= (TY + 1) * TX) break;
         const int lf = idx / TX, lx = idx - lf * TX;
         const int f = j0 + lf, i = i0 + lx;
         double val = 0.0;
         if (i < nx && f <= ny) {
-            const double q = Qyk[(size_t)f * nx + i];
+            const double q = qy ? qy[m] : Qyk[(size_t)f * nx + i];
             const double* col = sc + lx + H;
             const uint8_t* ocol = so + lx + H;
             if (f > 0 && f < ny) {
@@ -314,7 +350,15 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
             po[m] = in ? okp[(size_t)jj * nx + ii] : (uint8_t)0;
         }
     };
+    // the face transports of the thread's face items, likewise one layer ahead
+    using IT = TileItems<TX, TY>;
+    double qxn[IT::NX], qyn[IT::NY], qxc[IT::NX], qyc[IT::NY];
+    auto fetch_q = [&](int kk) {
+        tile_face_q<TX, TY>(qxn, qyn, Qx + (size_t)kk * ny * (nx + 1), Qy + (size_t)kk * (ny + 1) * nx, i0, j0, ny,
+                            nx);
+    };
     fetch(0);
+    fetch_q(0);
     for (int k = 0; k < n; ++k) {
         const int p = t * n + k;
         const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
@@ -327,7 +371,14 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                 so[idx] = po[m];
             }
         }
-        if (k + 1 < n) fetch(k + 1);
+#pragma unroll
+        for (int m = 0; m < IT::NX; ++m) qxc[m] = qxn[m];
+#pragma unroll
+        for (int m = 0; m < IT::NY; ++m) qyc[m] = qyn[m];
+        if (k + 1 < n) {
+            fetch(k + 1);
+            fetch_q(k + 1);
+        }
         // the cell's update inputs, issued before the face arithmetic so that their loads overlap it
         const size_t x = pc + xc, xv = kc + xc;
         double c0 = 0.0, v0 = 0.0, v1 = 0.0, qo = 0.0, qz = 0.0;
@@ -344,7 +395,7 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
         }
         __syncthreads();
         tile_faces<SCH, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
-                                ext ? ext + pc : nullptr, i0, j0, ny, nx, flags);
+                                ext ? ext + pc : nullptr, i0, j0, ny, nx, flags, qxc, qyc);
         __syncthreads();
         double ft = 0.0;
         if constexpr (ZF) {
