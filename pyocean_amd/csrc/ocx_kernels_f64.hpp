@@ -362,7 +362,8 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
     for (int k = 0; k < n; ++k) {
         const int p = t * n + k;
         const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
-        __syncthreads();
+        // no barrier needed before the store: the tile (sc, so) was last read by the previous layer's face phase,
+        // which a barrier closes; the faces (sfx, sfy) are rewritten only after the barrier below
 #pragma unroll
         for (int m = 0; m < NL; ++m) {
             const int idx = tid + m * NT;
