@@ -218,3 +218,19 @@ class TracerAdvection:
         """Per-tracer content sum V c in FP64 (deterministic order)."""
         C = np.ascontiguousarray(C, dtype=float)
         return _host(ops.content(_dev(C), _dev(V)))
+
+    # ------------------------------------------------------------------ the reference's NumPy twins (tracers.py:139,
+    # 256, 303): same signatures; here they run the same GPU kernels as advect_fct, advect and rhs.  The reference's
+    # tests hold its twins to its fused kernels within 1e-13 (FCT) and 1e-12 (RHS) and bit for bit (RK3); the GPU
+    # kernels equal the fused ones bit for bit.
+    def advect_fct_numpy(self, C, V0, V1, Qx, Qy, ok, unlimited=False, inflow=None, flux_face_i=None):
+        """``advect_fct`` (tracers.py:256-301 is the reference's NumPy twin)."""
+        return self.advect_fct(C, V0, V1, Qx, Qy, ok, unlimited=unlimited, inflow=inflow, flux_face_i=flux_face_i)
+
+    def advect_numpy(self, C, V0, V1, Qx, Qy, Fz, ok, inflow=None, flux_face_i=None):
+        """``advect`` (tracers.py:303-340 is the reference's NumPy twin, and its 'pc' path)."""
+        return self.advect(C, V0, V1, Qx, Qy, Fz, ok, inflow=inflow, flux_face_i=flux_face_i)
+
+    def rhs_numpy(self, c, Qx, Qy, Fz, ok, centred=False, inflow=None):
+        """``rhs`` (tracers.py:139-151 is the reference's vectorised twin)."""
+        return self.rhs(c, Qx, Qy, Fz, ok, centred=centred, inflow=inflow)

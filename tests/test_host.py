@@ -55,6 +55,20 @@ def test_flag_word_rejects_unknown_up_orders_and_a_missing_fz():
     assert ops._up_faces(16, None) is False
 
 
+def test_numpy_twin_names_are_the_gpu_methods():
+    """advect_fct_numpy / advect_numpy / rhs_numpy keep the reference's names and signatures (tracers.py:139, 256,
+    303) and call the GPU methods: no CPU path behind them."""
+    import inspect
+    adv = _adv()
+    for twin, gpu in (("advect_fct_numpy", "advect_fct"), ("advect_numpy", "advect"), ("rhs_numpy", "rhs")):
+        assert list(inspect.signature(getattr(adv, twin)).parameters) == \
+            list(inspect.signature(getattr(adv, gpu)).parameters)
+    C = np.zeros((1, 2, 3, 4))
+    with pytest.raises(ValueError, match="volume shape"):
+        adv.advect_numpy(C, np.zeros((2, 3, 5)), np.zeros((2, 3, 4)), np.zeros((2, 3, 5)), np.zeros((2, 4, 4)),
+                         np.zeros((3, 3, 4)), np.ones((2, 3, 4), bool))
+
+
 def test_centred_faces_flag_word():
     """rhs / flux_x with centred=True take the centred faces whatever the scheme (tracers.py:110-113)."""
     for scheme in ("weno5", "upwind3"):
