@@ -238,3 +238,12 @@ def test_pc_periodic_pad_crop_matches_the_oracle(gpu):
     got = _adv("weno5", per=("x", "y"), time="pc").advect(C, V, V1, Qx, Qy, Fz, ok)
     ref = R.advect_pc(P(C, "c"), P(V, "c"), P(V1, "c"), P(Qx, "u"), P(Qy, "v"), P(Fz, "c"), P(ok, "c"), 30.0)
     assert _same_bits(got, R.crop(ref, H, True, True))
+    # the face column of a periodic step: index shifted by the halo, the y halo cropped (tracers.py:80-102)
+    for scheme, time_, run in (("upwind1", "pc", R.advect_pc), ("upwind3", "ssprk3", R.advect_rk3)):
+        got, fxi = _adv(scheme, per=("x", "y"), time=time_).advect(C, V, V1, Qx, Qy, Fz, ok, flux_face_i=3)
+        ref, fref = run(P(C, "c"), P(V, "c"), P(V1, "c"), P(Qx, "u"), P(Qy, "v"), P(Fz, "c"), P(ok, "c"), 30.0,
+                        flux_face_i=3 + H, up_order=ORDER[scheme])
+        assert _same_bits(got, R.crop(ref, H, True, True))
+        assert _same_bits(fxi, fref[..., H:-H])
+    adv = _adv("upwind3", time="pc")
+    assert _same_bits(adv.advect_numpy(C, V, V1, Qx, Qy, Fz, ok), adv.advect(C, V, V1, Qx, Qy, Fz, ok))
