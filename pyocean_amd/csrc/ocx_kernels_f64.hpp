@@ -294,7 +294,7 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                                                      double* __restrict__ cout, double* __restrict__ fxs,
                                                      int flux_face_i, int nt, int n, int ny, int nx, double dt,
                                                      unsigned flags) {
-    // One block marches one tile of one tracer up its layers (1-D grid of gx * gy * nt blocks, XCD-contiguous):
+    // One block marches one tile of one tracer up its layers (1-D grid of nt * gx * gy blocks, XCD-contiguous):
     // each layer's x and y faces go through the LDS tile; each thread keeps its column's window c[k-2 .. k+3] in
     // registers and forms the z face between layers k and k+1 once, carrying it as the next layer's bottom face.
     constexpr bool ZF = VERT || SCH >= SCH_UP3;
@@ -306,11 +306,12 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
     const long long nb = (long long)gx * gy * nt;
+    // tracer fastest: the nt blocks of a tile run together on one XCD and share its per-layer fields through L2
     long long L = xcd_logical((long long)blockIdx.x, nb);
+    const int t = (int)(L % nt);
+    L /= nt;
     const int bx = (int)(L % gx);
-    L /= gx;
-    const int by = (int)(L % gy);
-    const int t = (int)(L / gy);
+    const int by = (int)(L / gx);
     const int i0 = bx * TX, j0 = by * TY;
     const size_t plane = (size_t)ny * nx;
     const int j = j0 + ty, i = i0 + tx;
