@@ -885,7 +885,7 @@ def main() -> int:
     ap.add_argument("--jobs", type=int, default=500, help="serve form: number of jobs")
     ap.add_argument("--rate", type=float, default=20.0, help="serve form: mean arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve form: HBM budget")
-    ap.add_argument("--workers", type=int, default=16, help="batch/serve forms: host processes generating grids")
+    ap.add_argument("--workers", type=int, default=16, help="batch/serve forms: host threads generating grids")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy the tracer stack once (device to device) before the steps: a kernel of known bytes "
                          "for scaling the HBM counters")

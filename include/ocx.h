@@ -118,6 +118,24 @@ int ocx_advect_pc_f64(const double* C, const double* V0, const double* V1, const
 int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const uint8_t* ok, const double* ext,
                     int nt, int nz, int ny, int nx, unsigned flags, double* fx, double* fy, void* stream);
 
+/* Face VALUES (not fluxes) along one axis of any array, as TracerAdvection.faces (core/components/tracers.py:110-115)
+ * returns them: _weno5_faces (WENO5-Z, or WENO5-JS with OCX_WENO_JS; sigma_baroclinic.py:564-578 with _weno5_rows
+ * :236-262), _up3_faces (OCX_UP3 / OCX_UP1; :164-187) or _centred_faces (OCX_CENTRED, flux unused and may be NULL;
+ * :632-652).  The arrays are viewed as c, ok (outer, n, inner) and flux, out (outer, nf, inner), the face axis in
+ * the middle (any axis of a C-contiguous array: outer = the product of the dimensions before it, inner = of those
+ * after it).  Face f lies between cells f-1 and f; faces 1 .. n-1 take the scheme's value, upwind-ordered by the
+ * sign of flux (flux >= 0: the low side; NaN: the high side), cells outside 0 .. n-1 invalid; all other faces 0. */
+int ocx_faces_f64(const double* c, const double* flux, const uint8_t* ok, long long outer, int n, int nf,
+                  long long inner, unsigned flags, double* out, void* stream);
+
+/* Open-edge face values of one horizontal axis, in place: face (nt, nz, ny, nx+1) with Q (nz, ny, nx+1) for axis 0
+ * (west and east edges), face (nt, nz, ny+1, nx) with Q (nz, ny+1, nx) for axis 1 (south and north).  On each
+ * edge in open_mask (OCX_OPEN_*) the face takes ext (nt, nz, ny, nx; NULL = c) of its boundary cell where the flux
+ * enters (Q > 0 west / south, Q < 0 east / north), else the boundary cell of c.  Replaces
+ * TracerAdvection.open_faces (core/components/tracers.py:117-137). */
+int ocx_open_faces_f64(double* face, const double* c, const double* Q, const double* ext, int nt, int nz, int ny,
+                       int nx, int axis, unsigned open_mask, void* stream);
+
 /* Donor-cell face fluxes.  Replaces TracerAdvection.upwind_faces_h (tracers.py:178-193). */
 int ocx_upwind_faces_h_f64(const double* c, const double* Qx, const double* Qy, const double* ext, int nt, int nz,
                            int ny, int nx, unsigned flags, double* fx, double* fy, void* stream);
@@ -175,6 +193,15 @@ int ocx_mix_num_f64(const double* c, const double* c2, const double* V1, int nz,
 size_t ocx_content_workspace_bytes(int nt);
 int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, int nx, double* out, void* workspace,
                     size_t workspace_bytes, void* stream);
+
+/* Self-test of the correctly rounded fast divisions the FP64 kernels use in place of the IEEE division (no
+ * counterpart in the reference, whose Numba kernels divide with IEEE semantics, sigma_baroclinic.py:212-231): for n
+ * device operand pairs (x[e], y[e]), out (5, n) receives div_const(x, 6), div_const(x, 3) (the divisions by the
+ * WENO and F_H constants), div_nr(x, y) (reciprocal, two Newton steps and one residual correction; defined for
+ * y > 0 with both operands inside 2^-500 .. 2^500 or x = +-0), qdiv(x, y) (div_nr inside that range, IEEE outside)
+ * and the device's IEEE quotient x / y.  Each of the first four equals the IEEE quotient bit for bit in its
+ * stated range. */
+int ocx_selftest_div_f64(const double* x, const double* y, long long n, double* out, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Batched form (no counterpart in the reference, which advances one domain per call): many independent domains

@@ -110,6 +110,19 @@ def advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, dt, inflow=None, open_mask=0, z_varian
     return out
 
 
+def rhs(c, Qx, Qy, Fz, ok, ext, open_mask=0, z_variant=True, vert=None):
+    """The advection RHS by the C loops (same meaning as ``numpy_ref.rhs``; ``vert`` None: any Fz != 0)."""
+    c = _f64(c)
+    nt, n, ny, nx = c.shape
+    Fz = _f64(Fz)
+    if vert is None:
+        vert = bool(np.any(Fz != 0.0))
+    out = np.empty_like(c)
+    lib().oc_advect_div(_p(c), _p(_f64(Qx)), _p(_f64(Qy)), _p(Fz), _p(_u8(ok, (n, ny, nx))), _p(_f64(ext)), nt, n,
+                        ny, nx, int(open_mask), int(bool(z_variant)), int(bool(vert)), _p(out))
+    return out
+
+
 def faces_h(c, Qx, Qy, ok, ext, open_mask=0, z_variant=True):
     c = _f64(c)
     nt, n, ny, nx = c.shape

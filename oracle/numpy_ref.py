@@ -136,6 +136,30 @@ def _open_edges(fx, fy, c, Qx, Qy, ext, open_mask, donor):
         fy[..., -1, :] = q * np.where(q < 0.0, ext[..., -1, :], c[..., -1, :])
 
 
+def open_faces(face, c, Q, axis, open_mask, ext=None):
+    """
+    Open-edge face VALUES (not fluxes) of one axis of a face stack: ``axis`` 3 the x faces (west, east), 2 the y
+    faces (south, north); on each open edge the exterior value (``ext``, default c) of the boundary cell where the
+    flux enters (Q > 0 on the low edge, Q < 0 on the high edge), the boundary cell otherwise; a copy, other faces
+    unchanged.  Follows ``TracerAdvection.open_faces`` (tracers.py:117-137).
+    """
+    ext = c if ext is None else ext
+    face = np.array(face, dtype=float, copy=True)
+    lo, hi = (OPEN_W, OPEN_E) if axis == 3 else (OPEN_S, OPEN_N)
+    for bit, f_i, c_i, sign in ((lo, 0, 0, 1.0), (hi, -1, -1, -1.0)):
+        if not open_mask & bit:
+            continue
+        q = np.take(Q, f_i, axis=axis - 1)
+        cell = np.take(c, c_i, axis=axis)
+        out = np.take(ext, c_i, axis=axis)
+        val = np.where(sign * q > 0, out, cell)
+        if axis == 3:
+            face[..., f_i] = val
+        else:
+            face[..., f_i, :] = val
+    return face
+
+
 def upwind_faces_h(c, Qx, Qy, ext, open_mask=0):
     """
     Donor-cell face fluxes F_L = Q * c_up on all interior faces (Q >= 0: the low-side cell), open edges as

@@ -44,6 +44,8 @@ SIGNATURES = {
     "ocx_advect_rk3_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, U, I, I, P, P, P, Z, P]),
     "ocx_advect_pc_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, U, I, I, P, P, P, Z, P]),
     "ocx_faces_h_f64": (I, [P, P, P, P, P, I, I, I, I, U, P, P, P]),
+    "ocx_faces_f64": (I, [P, P, P, L, I, I, L, U, P, P]),
+    "ocx_open_faces_f64": (I, [P, P, P, P, I, I, I, I, I, U, P]),
     "ocx_upwind_faces_h_f64": (I, [P, P, P, P, I, I, I, I, U, P, P, P]),
     "ocx_rhs_f64": (I, [P, P, P, P, P, P, I, I, I, I, U, I, P, P]),
     "ocx_fct_coefficients_f64": (I, [P, P, P, P, P, P, D, I, I, I, I, P, P, P, Z, P]),
@@ -52,6 +54,7 @@ SIGNATURES = {
     "ocx_diffuse_h_f64": (I, [P, P, P, P, P, P, P, P, P, I, I, I, I, D, D, P, P]),
     "ocx_vertical_implicit_workspace_bytes": (Z, [I, L]),
     "ocx_vertical_implicit_f64": (I, [P, P, P, P, I, P, I, I, L, D, P, P, Z, P]),
+    "ocx_selftest_div_f64": (I, [P, P, L, P, P]),
     "ocx_content_workspace_bytes": (Z, [I]),
     "ocx_content_f64": (I, [P, P, I, I, I, I, P, P, Z, P]),
     "ocx_transport_velocity_f64": (I, [P, P, P, P, P, P, I, L, D, D, P, P, P]),

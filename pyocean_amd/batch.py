@@ -71,6 +71,9 @@ class Batch:
                 ops._need(dom.inflow, dom.C.shape, "inflow")
             if dom.precision not in ops.PRECISIONS:
                 raise ValueError(f"domain {d}: precision {dom.precision!r}")
+            if int(dom.flags) & ~31:
+                raise ValueError(f"domain {d}: flags 0x{int(dom.flags):x}; a batch member takes only the open-edge "
+                                 f"and WENO5-JS flags (0x1f)")
             other = dom.C_alt if dom.C_alt is not None else torch.empty_like(dom.C)
             ops._need(other, dom.C.shape, "C_alt")
             ws = dom.workspace
@@ -79,7 +82,7 @@ class Batch:
             self._bufs.append((other, ws, ok))
             p = lambda t: None if t is None else t.data_ptr()                # noqa: E731
             descs[d] = OcxDomain(p(dom.C), p(other), p(dom.V0), p(dom.V1), p(dom.Qx), p(dom.Qy), p(dom.inflow), p(ok),
-                                 nt, nz, ny, nx, float(dom.dt), int(dom.flags) & 31,
+                                 nt, nz, ny, nx, float(dom.dt), int(dom.flags),
                                  1 if dom.precision == "split" else 0, p(ws), ws.numel())
         self._descs = descs
         handle = ctypes.c_void_p()

@@ -512,6 +512,52 @@ int ocx_faces_h_f64(const double* c, const double* Qx, const double* Qy, const u
     return launch_check("faces_h");
 }
 
+int ocx_faces_f64(const double* c, const double* flux, const uint8_t* ok, long long outer, int n, int nf,
+                  long long inner, unsigned flags, double* out, void* stream) {
+    using namespace ocx;
+    OCX_TRY(need(c, "c"));
+    OCX_TRY(need(ok, "ok"));
+    OCX_TRY(need(out, "out"));
+    if (flags & ~(WENO_JS | UP3 | UP1 | CENTRED))
+        return fail(OCX_E_ARG, "flags 0x%x: faces take only OCX_WENO_JS, OCX_UP3, OCX_UP1 or OCX_CENTRED", flags);
+    OCX_TRY(check_rk_flags(flags, "faces", true));
+    if (!(flags & CENTRED)) OCX_TRY(need(flux, "flux"));
+    if (outer < 1 || inner < 1 || n < 1 || nf < 1)
+        return fail(OCX_E_ARG, "shape (outer=%lld, n=%d, nf=%d, inner=%lld) must be positive", outer, n, nf, inner);
+    if (outer * nf > (1LL << 40) / inner || outer * n > (1LL << 40) / inner)
+        return fail(OCX_E_ARG, "shape (outer=%lld, n=%d, nf=%d, inner=%lld) too large", outer, n, nf, inner);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long total = outer * nf * inner;
+    const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 1LL << 20);
+    switch (scheme_of(flags)) {
+        case SCH_C6: k_line_faces<SCH_C6><<<blocks, 256, 0, s>>>(c, flux, ok, outer, n, nf, inner, out); break;
+        case SCH_UP3: k_line_faces<SCH_UP3><<<blocks, 256, 0, s>>>(c, flux, ok, outer, n, nf, inner, out); break;
+        case SCH_UP1: k_line_faces<SCH_UP1><<<blocks, 256, 0, s>>>(c, flux, ok, outer, n, nf, inner, out); break;
+        case SCH_JS: k_line_faces<SCH_JS><<<blocks, 256, 0, s>>>(c, flux, ok, outer, n, nf, inner, out); break;
+        default: k_line_faces<SCH_Z><<<blocks, 256, 0, s>>>(c, flux, ok, outer, n, nf, inner, out);
+    }
+    return launch_check("faces");
+}
+
+int ocx_open_faces_f64(double* face, const double* c, const double* Q, const double* ext, int nt, int nz, int ny,
+                       int nx, int axis, unsigned open_mask, void* stream) {
+    using namespace ocx;
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    OCX_TRY(need(face, "face"));
+    OCX_TRY(need(c, "c"));
+    OCX_TRY(need(Q, "Q"));
+    if (axis != 0 && axis != 1) return fail(OCX_E_ARG, "axis %d: 0 (x faces) or 1 (y faces)", axis);
+    if (open_mask & ~15u) return fail(OCX_E_ARG, "open mask 0x%x", open_mask);
+    const int lo = axis == 0 ? !!(open_mask & OPEN_W) : !!(open_mask & OPEN_S);
+    const int hi = axis == 0 ? !!(open_mask & OPEN_E) : !!(open_mask & OPEN_N);
+    if (!lo && !hi) return OCX_OK;
+    const long long total = (long long)nt * nz * (axis == 0 ? ny : nx) * 2;
+    const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 1LL << 20);
+    k_open_faces<<<blocks, 256, 0, static_cast<hipStream_t>(stream)>>>(face, c, Q, ext ? ext : c, nt, nz, ny, nx,
+                                                                        axis, lo, hi);
+    return launch_check("open_faces");
+}
+
 int ocx_upwind_faces_h_f64(const double* c, const double* Qx, const double* Qy, const double* ext, int nt, int nz,
                            int ny, int nx, unsigned flags, double* fx, double* fy, void* stream) {
     using namespace ocx;
@@ -970,6 +1016,16 @@ int ocx_vertical_implicit_f64(const double* u, const double* dz, const double* n
         k_thomas_solve<<<gs, 256, 0, s>>>(u, dz, nu, r_bed, r_layers, tau_top, nt, n, columns, dt, cp, out);
     }
     return launch_check("vertical_implicit");
+}
+
+int ocx_selftest_div_f64(const double* x, const double* y, long long n, double* out, void* stream) {
+    OCX_TRY(need(x, "x"));
+    OCX_TRY(need(y, "y"));
+    OCX_TRY(need(out, "out"));
+    if (n < 1 || n > (1LL << 30)) return fail(OCX_E_ARG, "n = %lld outside 1 .. 2^30", n);
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    ocx::k_selftest_div<<<blocks, 256, 0, static_cast<hipStream_t>(stream)>>>(x, y, (size_t)n, out);
+    return launch_check("selftest div");
 }
 
 int ocx_mix_num_f64(const double* c, const double* c2, const double* V1, int nz, int ny, int nx, double dt,

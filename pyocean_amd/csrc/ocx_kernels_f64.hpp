@@ -463,6 +463,83 @@ __global__ void k_rk_face_column(const double* __restrict__ f0, const double* __
     if (x < count) out[x] = (f0[x] + f1[x]) / 6.0 + 2.0 * f2[x] / 3.0;
 }
 
+// Face values along one axis of an array viewed as (outer, n, inner) cells with (outer, nf, inner) faces: face f
+// lies between cells f-1 and f of its line, faces 1 .. n-1 take the scheme's value (WENO5-Z/JS upwind-ordered by
+// the sign of flux, flux >= 0 taking the low side; UP3 / UP1 likewise; centred whatever the flux), every other
+// face 0.  Cells outside 0 .. n-1 read value 0 and are not valid.  One thread per face; consecutive threads take
+// consecutive inner indices (or, with inner = 1, consecutive faces), so the loads are coalesced either way.
+// _weno5_faces / _weno5_rows, _up3_faces, _centred_faces (core/dynamics/sigma_baroclinic.py:236-262, 564-578,
+// 164-187, 632-652) as TracerAdvection.faces calls them (core/components/tracers.py:110-115).
+template <int SCH>
+__global__ __launch_bounds__(256) void k_line_faces(const double* __restrict__ c, const double* __restrict__ flux,
+                                                    const uint8_t* __restrict__ ok, long long outer, int n, int nf,
+                                                    long long inner, double* __restrict__ out) {
+    const long long total = outer * nf * inner;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long r = e % inner;
+        const long long q = e / inner;
+        const int f = (int)(q % nf);
+        const long long o = q / nf;
+        if (f < 1 || f > n - 1) {
+            out[e] = 0.0;
+            continue;
+        }
+        const long long base = o * n * inner + r;
+        double u[6];
+        uint8_t v[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const int cell = f - 3 + m;
+            const bool in = cell >= 0 && cell < n;
+            const long long x = base + (long long)(in ? cell : 0) * inner;
+            u[m] = in ? c[x] : 0.0;
+            v[m] = in && ok[x] != 0;
+        }
+        const double qf = SCH == SCH_C6 ? 0.0 : flux[e];
+        double val;
+        if (SCH == SCH_C6) val = centred_face(u, v, 1, 3);
+        else if (SCH == SCH_UP3) val = up_face<true>(u, v, 1, 3, qf);
+        else if (SCH == SCH_UP1) val = up_face<false>(u, v, 1, 3, qf);
+        else val = lds_face<SCH == SCH_Z>(u, v, 1, 3, qf);
+        out[e] = val;
+    }
+}
+
+// Open-edge face values of one horizontal axis (TracerAdvection.open_faces, core/components/tracers.py:117-137), in
+// place on face (nt, nz, ny, nx+1) (axis 0) or (nt, nz, ny+1, nx) (axis 1): on each open edge of the axis the
+// exterior value ext where the flux Q (nz, ny, nx+1) / (nz, ny+1, nx) enters the domain (Q > 0 on the west / south
+// edge, Q < 0 on the east / north edge), the boundary cell of c otherwise (also where Q is 0 or NaN).
+__global__ void k_open_faces(double* __restrict__ face, const double* __restrict__ c, const double* __restrict__ Q,
+                             const double* __restrict__ ext, int nt, int nz, int ny, int nx, int axis, int lo_open,
+                             int hi_open) {
+    const int m = axis == 0 ? ny : nx;                   // faces along one edge of one (t, k) plane
+    const long long total = (long long)nt * nz * m * 2;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int side = (int)(e % 2);
+        const long long q = e / 2;
+        const int a = (int)(q % m);
+        const long long p = q / m;                       // plane t * nz + k
+        const int k = (int)(p % nz);
+        if (side == 0 ? !lo_open : !hi_open) continue;
+        long long fidx, cidx, qidx;
+        if (axis == 0) {
+            const int fi = side == 0 ? 0 : nx, ci = side == 0 ? 0 : nx - 1;
+            fidx = (p * ny + a) * (nx + 1) + fi;
+            cidx = (p * ny + a) * nx + ci;
+            qidx = ((long long)k * ny + a) * (nx + 1) + fi;
+        } else {
+            const int fj = side == 0 ? 0 : ny, cj = side == 0 ? 0 : ny - 1;
+            fidx = (p * (ny + 1) + fj) * nx + a;
+            cidx = (p * ny + cj) * nx + a;
+            qidx = ((long long)k * (ny + 1) + fj) * nx + a;
+        }
+        const double sgn = side == 0 ? 1.0 : -1.0;
+        face[fidx] = (sgn * Q[qidx] > 0.0) ? ext[cidx] : c[cidx];
+    }
+}
+
 // Donor-cell face fluxes of every face (TracerAdvection.upwind_faces_h, tracers.py:178-193).
 __global__ void k_upwind_faces(const double* __restrict__ c, const double* __restrict__ ext,
                                const double* __restrict__ Qx, const double* __restrict__ Qy, double* __restrict__ fx,
@@ -551,6 +628,20 @@ __global__ void k_open_flux(const double* __restrict__ gedge, double* __restrict
 __global__ void k_fill(double* __restrict__ dst, double value, size_t count) {
     for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < count; x += (size_t)gridDim.x * blockDim.x)
         dst[x] = value;
+}
+
+// Self-test of the fast divisions: for n operand pairs (x, y), out[0..n) = div_const(x, 6), [n..2n) = div_const(x, 3),
+// [2n..3n) = div_nr(x, y), [3n..4n) = qdiv(x, y), [4n..5n) = the IEEE quotient x / y.
+__global__ void k_selftest_div(const double* __restrict__ x, const double* __restrict__ y, size_t n,
+                               double* __restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double a = x[e], b = y[e];
+    out[e] = div_const(a, 6.0, 1.0 / 6.0);
+    out[n + e] = div_const(a, 3.0, 1.0 / 3.0);
+    out[2 * n + e] = div_nr(a, b);
+    out[3 * n + e] = qdiv(a, b);
+    out[4 * n + e] = a / b;
 }
 
 // Numerical mixing of the advection of c (Burchard & Rennau 2008, eq. 24), with c^2 advected in the same stack:

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _native
@@ -56,16 +57,23 @@ def _ok(ok, shape):
 
 
 class Workspace:
-    """A growing device scratch buffer reused across calls (one per stream user)."""
+    """A growing device scratch buffer, one per stream, reused across the calls made on that stream.  The caching
+    allocator hands a freed block only to work ordered after the free on the same stream, so growing the buffer of
+    one stream never releases memory that another stream's kernels may still be using, and calls on two streams
+    never share a buffer."""
 
     def __init__(self):
-        self.buf = None
+        self.bufs = {}
 
     def get(self, nbytes: int) -> torch.Tensor:
-        if self.buf is None or self.buf.numel() < nbytes:
-            self.buf = None
-            self.buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=require_gpu())
-        return self.buf
+        s = torch.cuda.current_stream(require_gpu())
+        key = (s.device_index, s.cuda_stream)
+        buf = self.bufs.get(key)
+        if buf is None or buf.numel() < nbytes:
+            self.bufs.pop(key, None)
+            buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=s.device)
+            self.bufs[key] = buf
+        return buf
 
 
 _WS = Workspace()
@@ -202,6 +210,53 @@ def faces_h(c, Qx, Qy, ok, ext=None, flags=0):
     check(lib().ocx_faces_h_f64(_ptr(c), _ptr(Qx), _ptr(Qy), _ptr(ok), _ptr(ext), nt, n, ny, nx, int(flags), _ptr(fx),
                                 _ptr(fy), _stream()), "ocx_faces_h_f64")
     return fx, fy
+
+
+def faces(c, flux, ok, axis: int, flags: int = 0):
+    """
+    Face values (not fluxes) of c along ``axis`` (``ocx_faces_f64``): WENO5-Z, or by the flags WENO5-JS, UP3, UP1
+    or centred, upwind-ordered by the sign of ``flux`` (which also sets the number of faces; unused by the centred
+    faces).  c, flux and ok broadcast against each other on every other axis, as the reference's face functions
+    do (core/dynamics/sigma_baroclinic.py:564-578).  Returns the (lead..., nf) array with the face axis at ``axis``.
+    """
+    cm = torch.movedim(c, axis, -1)
+    fm = torch.movedim(flux, axis, -1)
+    om = torch.movedim(ok.view(torch.uint8) if ok.dtype == torch.bool else ok, axis, -1)
+    for t, nm in ((cm, "c"), (fm, "flux")):
+        if t.dtype != _F64 or not t.is_cuda:
+            raise ValueError(f"{nm}: a float64 GPU tensor")
+    if om.dtype != torch.uint8 or not om.is_cuda:
+        raise ValueError("ok: a bool or uint8 GPU tensor")
+    lead = torch.broadcast_shapes(cm.shape[:-1], fm.shape[:-1], om.shape[:-1])
+    n, nf = cm.shape[-1], fm.shape[-1]
+    ax = axis % (len(lead) + 1)
+    place = lambda t, m: torch.movedim(t.expand(tuple(lead) + (m,)), -1, ax).contiguous()     # noqa: E731
+    c3, f3, o3 = place(cm, n), place(fm, nf), place(om, n)
+    shape = tuple(f3.shape)
+    outer = int(np.prod(shape[:ax], dtype=np.int64))
+    inner = int(np.prod(shape[ax + 1:], dtype=np.int64))
+    out = torch.empty(shape, dtype=_F64, device=c.device)
+    if out.numel() == 0:
+        return out
+    check(lib().ocx_faces_f64(_ptr(c3), _ptr(f3), _ptr(o3), outer, n, nf, inner, int(flags), _ptr(out), _stream()),
+          "ocx_faces_f64")
+    return out
+
+
+def open_faces(face, c, Q, axis: int, open_mask: int, ext=None):
+    """Open-edge face values in place on ``face`` (``ocx_open_faces_f64``): axis 0 the x faces (nt, nz, ny, nx+1)
+    with Q (nz, ny, nx+1), axis 1 the y faces (nt, nz, ny+1, nx) with Q (nz, ny+1, nx); on each open edge of the
+    axis the exterior value ``ext`` (default c) where the flux enters, else the boundary cell."""
+    nt, n, ny, nx = c.shape
+    _need(c, (nt, n, ny, nx), "c")
+    fshape = (nt, n, ny, nx + 1) if axis == 0 else (nt, n, ny + 1, nx)
+    _need(face, fshape, "face")
+    _need(Q, fshape[1:], "Q")
+    if ext is not None:
+        _need(ext, c.shape, "ext")
+    check(lib().ocx_open_faces_f64(_ptr(face), _ptr(c), _ptr(Q), _ptr(ext), nt, n, ny, nx, int(axis),
+                                   int(open_mask) & 15, _stream()), "ocx_open_faces_f64")
+    return face
 
 
 def upwind_faces_h(c, Qx, Qy, ext=None, flags=0):

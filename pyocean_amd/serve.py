@@ -89,18 +89,19 @@ def _regional_case(args):
 
 def make_grids(shapes, nt: int = 10, seed: int = 100, workers: int = 8, log=None) -> list[Grid]:
     """Resident grids for ``shapes`` [(nx, ny, nz)]: seeded regional domains (own mask, streamfunction and tracers
-    each), generated on the host by ``workers`` processes and uploaded one at a time as they arrive."""
-    import multiprocessing as mp
+    each), generated on the host by ``workers`` threads of this process (NumPy releases the interpreter lock in its
+    array work; no child process is started, so nothing is forked from a process that holds the GPU) and uploaded
+    in order as they are done."""
+    from concurrent.futures import ThreadPoolExecutor
     dev = ops.require_gpu()
     jobs = [(nx, ny, nz, nt, seed + m) for m, (nx, ny, nz) in enumerate(shapes)]
     out = []
     if workers <= 1:
-        cases = map(_regional_case, jobs)
-        for c in cases:
+        for c in map(_regional_case, jobs):
             out.append(Grid.from_case(c, dev))
         return out
-    with mp.get_context("spawn").Pool(workers) as pool:
-        for m, c in enumerate(pool.imap(_regional_case, jobs)):
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for m, c in enumerate(pool.map(_regional_case, jobs)):
             out.append(Grid.from_case(c, dev))
             if log and (m + 1) % 16 == 0:
                 log(f"grids {m + 1}/{len(jobs)}")

@@ -185,6 +185,35 @@ class TracerAdvection:
                              None if e is None else _dev(e), self._flags())
         return _host(fx), _host(fy)
 
+    def faces(self, c: np.ndarray, flux: np.ndarray, ok: np.ndarray, axis: int, centred: bool = False) -> np.ndarray:
+        """Face values of c along ``axis`` (tracers.py:110-115): the centred faces (``centred``), the WENO5 faces
+        ('weno5' Z, 'weno5js' JS) or the UP3 / donor faces ('upwind1') of every other scheme, upwind-ordered by the
+        sign of ``flux``."""
+        if centred:
+            return centred_faces(c, ok, axis, n_out=np.shape(flux)[axis])
+        if self.tracer_scheme.startswith("weno5"):
+            return weno5_faces(c, flux, ok, axis, variant="js" if self.tracer_scheme == "weno5js" else "z")
+        return up3_faces(c, flux, ok, axis, order=1 if self.tracer_scheme == "upwind1" else 3)
+
+    def open_faces(self, face: np.ndarray, c: np.ndarray, Q: np.ndarray, axis: int,
+                   inflow: np.ndarray | None = None) -> np.ndarray:
+        """
+        Tracer values on the open-boundary faces (tracers.py:117-137): upstream - the boundary cell where the flow
+        leaves, the exterior value (``inflow``, default c) where it enters - on the open edges of the axis (3: west
+        and east, 2: south and north) of a face stack ``face`` (ntr, N, ny, nx+1) or (ntr, N, ny+1, nx); a new
+        array, the other faces copied.  Without open edges ``face`` itself is returned.
+        """
+        if not self.ob.open:
+            return face
+        m = 0
+        for e in self.ob.open:
+            m |= _EDGE_BITS[e]
+        c = np.ascontiguousarray(c, dtype=float)
+        ax = 0 if axis == 3 else 1
+        f = _dev(face)
+        ops.open_faces(f, _dev(c), _dev(Q), ax, m, None if inflow is None else _dev(inflow))
+        return _host(f)
+
     def upwind_faces_h(self, c: np.ndarray, Qx: np.ndarray, Qy: np.ndarray, inflow: np.ndarray | None = None):
         """Donor-cell face fluxes Q c_up (the FCT low-order flux); open edges take the inflow values on inflow."""
         c = np.ascontiguousarray(c, dtype=float)
@@ -234,3 +263,36 @@ class TracerAdvection:
     def rhs_numpy(self, c, Qx, Qy, Fz, ok, centred=False, inflow=None):
         """``rhs`` (tracers.py:139-151 is the reference's vectorised twin)."""
         return self.rhs(c, Qx, Qy, Fz, ok, centred=centred, inflow=inflow)
+
+
+# ---------------------------------------------------------------------------- face values along any axis (module level)
+def _faces_call(c, flux, ok, axis, flags):
+    c = np.asarray(c, dtype=float)
+    flux = np.asarray(flux, dtype=float)
+    okb = np.asarray(ok, dtype=bool)
+    return _host(ops.faces(_dev(c), _dev(flux), _dev(okb, np.uint8), axis, flags))
+
+
+def weno5_faces(c: np.ndarray, flux: np.ndarray, ok: np.ndarray, axis: int, variant: str = "z") -> np.ndarray:
+    """WENO5 face values of c along ``axis`` on the GPU, as ``_weno5_faces`` (sigma_baroclinic.py:564-578): faces
+    1 .. n-1 upwind-ordered by the sign of ``flux`` (which sets the face count), the others 0; variant 'z' (WENO5-Z)
+    or 'js' (WENO5-JS), anything else a ValueError."""
+    if variant not in ("z", "js"):
+        raise ValueError(f"unknown WENO variant {variant!r}")
+    return _faces_call(c, flux, ok, axis, ops.flags_of(js=variant == "js"))
+
+
+def up3_faces(c: np.ndarray, flux: np.ndarray, ok: np.ndarray, axis: int, order: int = 3) -> np.ndarray:
+    """UP3 (or, ``order`` 1, donor-cell) face values along ``axis`` on the GPU, as ``_up3_faces``
+    (sigma_baroclinic.py:164-187)."""
+    return _faces_call(c, flux, ok, axis, ops.flags_of(up=1 if order == 1 else 3))
+
+
+def centred_faces(c: np.ndarray, ok: np.ndarray, axis: int, n_out: int | None = None) -> np.ndarray:
+    """Centred 6th/4th/2nd-order face values along ``axis`` on the GPU, as ``_centred_faces``
+    (sigma_baroclinic.py:632-652); ``n_out`` faces (default n+1)."""
+    c = np.asarray(c, dtype=float)
+    n = c.shape[axis]
+    shape = list(c.shape)
+    shape[axis] = n + 1 if n_out is None else n_out
+    return _faces_call(c, np.zeros(shape), ok, axis, ops.flags_of(centred=True))

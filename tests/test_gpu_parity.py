@@ -244,10 +244,10 @@ def test_regional_case_matches_the_oracle(gpu, oracle_c):
 
 
 def test_full_size_properties(gpu):
-    # the 1024 x 1024 x 50 regional grid (two tracers to keep the check quick): closed edges conserve content to
-    # 1e-13 per step, the limiter makes no new extrema, and two runs are bit-identical
-    _say("full size 1024x1024x50")
-    case = synth.regional(nx=1024, ny=1024, nz=50, nt=2, seed=0)
+    # the 1024 x 1024 x 50 regional grid with its ten tracers: closed edges conserve content to 1e-13 per step, the
+    # limiter makes no new extrema, and two runs are bit-identical
+    _say("full size 1024x1024x50 nt=10")
+    case = synth.regional(nx=1024, ny=1024, nz=50, nt=10, seed=0)
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()        # noqa: E731
     V0, V1, Qx, Qy = d(case.V0), d(case.V1), d(case.Qx), d(case.Qy)
     ok = d(case.ok.astype(np.uint8))
@@ -307,3 +307,25 @@ def test_split_precision_open_edges_and_inflow(gpu, oracle_c):
     with pytest.raises(ValueError, match="split"):
         ops.advect_fct(d(case["C"]), d(case["V0"]), d(case["V1"]), d(case["Qx"]), d(case["Qy"]),
                        d(case["ok"].astype(np.uint8)), case["dt"], flags=32, precision="split")
+
+
+def test_full_size_sigma_uniform_tracer_stays_uniform_and_is_conserved(gpu):
+    # the sigma form of the 1024 x 1024 x 50 regional grid: every layer's divergence beyond the column's share leaves
+    # through the interfaces (Fz cumulated from the bed), so every RK3 stage runs the z sweep.  A uniform tracer
+    # stays uniform to 1e-11 and keeps its content to 1e-13 (test_sigma_baroclinic.py:28-40) over 5 steps
+    _say("full size sigma 1024x1024x50 uniform")
+    s = synth.regional(nx=1024, ny=1024, nz=50, nt=1, seed=0, sigma=True)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()        # noqa: E731
+    V0, V1, Qx, Qy, Fz = d(s.V0), d(s.V1), d(s.Qx), d(s.Qy), d(s.Fz)
+    ok = d(s.ok.astype(np.uint8))
+    assert (Fz[1:-1] != 0).float().mean().item() > 0.5
+    C = torch.empty((2,) + tuple(V0.shape), dtype=torch.float64, device=V0.device)
+    C[0].fill_(7.0)
+    C[1].fill_(-3.25)
+    m0 = ops.content(C, V0).cpu().numpy()
+    for _ in range(5):
+        C = ops.advect_rk3(C, V0, V1, Qx, Qy, Fz, ok, s.dt)
+    m1 = ops.content(C, V1).cpu().numpy()
+    assert (C[0] - 7.0).abs().max().item() <= 1e-11
+    assert (C[1] + 3.25).abs().max().item() <= 1e-11
+    np.testing.assert_allclose(m1, m0, rtol=1e-13)

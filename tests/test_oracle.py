@@ -567,3 +567,35 @@ def test_vertical_implicit_matches_a_banded_solver_and_conserves():
     np.testing.assert_allclose((dz * got).sum(0), (dz * u).sum(0), rtol=1e-13)
     U = np.full((n, M), 2.0)
     np.testing.assert_allclose(R.vertical_implicit(U, dz, nu, np.zeros(M), dt), U, rtol=1e-13)
+
+
+# ------------------------------------------------------------------------------------------------ open-edge face values
+def test_open_face_values_are_upstream_on_open_edges_only():
+    # TracerAdvection.open_faces (tracers.py:117-137): on an open edge the face takes the exterior value where the
+    # flux enters and the boundary cell where it leaves (test_sigma_ale.py:1387-1394); interior faces and closed
+    # edges keep what they held; and Q times these values is the open-edge flux that faces_h forms
+    rng = np.random.default_rng(5)
+    nt, n, ny, nx = 2, 3, 5, 7
+    c = rng.normal(10.0, 1.0, (nt, n, ny, nx))
+    ext = rng.normal(20.0, 1.0, (nt, n, ny, nx))
+    Qx = rng.normal(size=(n, ny, nx + 1))
+    Qy = rng.normal(size=(n, ny + 1, nx))
+    fx0 = rng.normal(size=(nt, n, ny, nx + 1))
+    fy0 = rng.normal(size=(nt, n, ny + 1, nx))
+    for mask in (1, 2, 4, 8, 15, 5):
+        fx = R.open_faces(fx0, c, Qx, 3, mask, ext)
+        fy = R.open_faces(fy0, c, Qy, 2, mask, ext)
+        assert np.array_equal(fx[..., 1:-1], fx0[..., 1:-1]) and np.array_equal(fy[..., 1:-1, :], fy0[..., 1:-1, :])
+        for bit, got, was, want in ((1, fx[..., 0], fx0[..., 0], np.where(Qx[..., 0] > 0, ext[..., 0], c[..., 0])),
+                                    (2, fx[..., -1], fx0[..., -1], np.where(Qx[..., -1] < 0, ext[..., -1], c[..., -1])),
+                                    (4, fy[..., 0, :], fy0[..., 0, :],
+                                     np.where(Qy[:, 0] > 0, ext[..., 0, :], c[..., 0, :])),
+                                    (8, fy[..., -1, :], fy0[..., -1, :],
+                                     np.where(Qy[:, -1] < 0, ext[..., -1, :], c[..., -1, :]))):
+            assert np.array_equal(got, want if mask & bit else was), (mask, bit)
+        # the fluxes of faces_h on the open edges are Q times these values
+        ffx, ffy = R.faces_h(c, Qx, Qy, np.ones((n, ny, nx), bool), ext, mask)
+        if mask & 1:
+            assert np.array_equal(ffx[..., 0], Qx[None, ..., 0] * fx[..., 0])
+        if mask & 8:
+            assert np.array_equal(ffy[..., -1, :], Qy[None, :, -1, :] * fy[..., -1, :])
