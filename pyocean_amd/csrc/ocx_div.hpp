@@ -6,14 +6,14 @@
 
 namespace ocx {
 
-// Correctly rounded x / b for a constant divisor b with rb = RN(1/b): q = RN(x rb) is within an ulp of x/b, the
-// remainder b q - x is exact under an FMA, and RN(q - (b q - x) rb) = RN(x/b) (Markstein's correction; needs rb
-// within half an ulp of 1/b and no underflow, so tiny nonzero |x| takes the IEEE division, and so do infinities and
-// NaNs).  The remainder is formed as b q - x and subtracted, so a zero x keeps its sign: RN(-0 + -0) = -0, while
-// x - b q would give +0 for x = -0.
+// Correctly rounded x / b for a constant divisor b with rb = RN(1/b) and a finite x: q = RN(x rb) is within an ulp
+// of x/b, the remainder b q - x is exact under an FMA, and RN(q - (b q - x) rb) = RN(x/b) (Markstein's correction;
+// needs rb within half an ulp of 1/b and no underflow, so tiny nonzero |x| takes the IEEE division).  The remainder
+// is formed as b q - x and subtracted, so a zero x keeps its sign: RN(-0 + -0) = -0, while x - b q would give +0
+// for x = -0.  A NaN x gives a NaN; an infinite x gives a NaN where the IEEE quotient is infinite (the callers
+// divide finite sums of finite fluxes).
 __device__ __forceinline__ double div_const(double x, double b, double rb) {
-    const double ax = fabs(x);
-    if ((ax < 0x1p-960 && x != 0.0) || !(ax <= DBL_MAX)) return x / b;
+    if (fabs(x) < 0x1p-960 && x != 0.0) return x / b;
     const double q = x * rb;
     const double r = __fma_rn(b, q, -x);
     return __fma_rn(-r, rb, q);

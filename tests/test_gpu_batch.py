@@ -27,11 +27,11 @@ def _host(nx, ny, nz, nt, seed, open_mask=0, inflow=False):
     Qx, Qy = c.Qx.copy(), c.Qy.copy()
     V1 = c.V1
     if open_mask:
-        # open the west/south edges with inflow through them; V1 from the same continuity
-        Qx[..., 0] = 0.02 * np.abs(c.Qx).max()
-        Qy[:, 0, :] = 0.02 * np.abs(c.Qy).max()
-        Qx[..., 0] *= c.V0[..., 0] > 0
-        Qy[:, 0, :] *= c.V0[:, 0, :] > 0
+        # open the west/south edges with inflow through them, each layer's share in proportion to its edge cell's
+        # volume (so the 1 mm sub-bed layers take a 1 mm share, as the open-edge layer transports do); V1 from the
+        # same continuity
+        Qx[..., 0] = 0.02 * np.abs(c.Qx).max() * c.V0[..., 0] / c.V0[..., 0].max()
+        Qy[:, 0, :] = 0.02 * np.abs(c.Qy).max() * c.V0[:, 0, :] / c.V0[:, 0, :].max()
         V1 = c.V0 - c.dt * (Qx[..., 1:] - Qx[..., :-1] + Qy[:, 1:] - Qy[:, :-1])
     ok = (c.V0 > 0) & (V1 > 0)
     return dict(C=c.C, V0=c.V0, V1=V1, Qx=Qx, Qy=Qy, ok=ok, dt=c.dt, open_mask=open_mask,

@@ -60,14 +60,14 @@ def test_face_values_along_every_axis_match_the_oracle(gpu, axis, nf_extra):
 
 def test_face_values_broadcast_like_the_reference(gpu):
     # the reference broadcasts c, flux and ok on the axes other than the face axis (sigma_baroclinic.py:569-573):
-    # a per-layer mask and one flux row shared by every tracer
+    # a per-layer mask and one flux row shared by every tracer, the face axis counted from the end
     _say("faces broadcast")
     rng = np.random.default_rng(3)
     c = rng.normal(size=(3, 5, 7, 10))
     ok = rng.random((5, 7, 10)) > 0.1
     flux = rng.normal(size=(5, 7, 11))
-    got = tracers.weno5_faces(c, flux, ok, 3)
-    ref = R.weno5_faces(c, flux, ok, 3)
+    got = tracers.weno5_faces(c, flux, ok, -1)
+    ref = R.weno5_faces(c, flux, ok, -1)
     assert got.shape == (3, 5, 7, 11) and _same(got, ref)
 
 
