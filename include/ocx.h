@@ -89,6 +89,20 @@ int ocx_advect_fct_split(const double* C, const double* V0, const double* V1, co
                          unsigned flags, int flux_face_i, double* C_out, double* fx_i, double* open_flux,
                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* The FP64 step of ocx_advect_fct_f64 for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc); no counterpart in the reference.  Same arguments, meaning, results and
+ * errors as ocx_advect_fct_f64, without the workspace: the step's workspace is a device array inside the library of
+ * OCX_NOALLOC_SCRATCH_BYTES (64 MiB, held on each GPU the library runs on until the process ends), so a shape whose
+ * ocx_fct_workspace_bytes exceeds it is rejected with OCX_E_WORKSPACE (e.g. 2 x 20 x 128 x 128 needs 32 MB and
+ * fits).  The call allocates nothing (no hipMalloc, hipFree or allocating library routine).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other host threads, other streams not ordered after this
+ * one) race and must be serialised by the caller. */
+#define OCX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int ocx_advect_fct_f64_noalloc(const double* C, const double* V0, const double* V1, const double* Qx,
+                               const double* Qy, const uint8_t* ok, const double* inflow, int nt, int nz, int ny,
+                               int nx, double dt, unsigned flags, int flux_face_i, double* C_out, double* fx_i,
+                               double* open_flux, void* stream);
+
 /* One SSP-RK3 step without limiter, x, y and (if vert) z sweeps.  Replaces TracerAdvection.advect
  * (core/components/tracers.py:342-375) with _advect_div and _rk_stage (sigma_baroclinic.py:324-364, 856-875).
  * Fz (nz+1, ny, nx) may be NULL when vert == 0; vert as the reference's any(Fz != 0).  fx_i as above

@@ -41,6 +41,7 @@ SIGNATURES = {
     "ocx_rk3_workspace_bytes": (Z, [I, I, I, I]),
     "ocx_advect_fct_f64": (I, [P, P, P, P, P, P, P, I, I, I, I, D, U, I, P, P, P, P, Z, P]),
     "ocx_advect_fct_split": (I, [P, P, P, P, P, P, P, I, I, I, I, D, U, I, P, P, P, P, Z, P]),
+    "ocx_advect_fct_f64_noalloc": (I, [P, P, P, P, P, P, P, I, I, I, I, D, U, I, P, P, P, P]),
     "ocx_advect_rk3_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, U, I, I, P, P, P, Z, P]),
     "ocx_advect_pc_f64": (I, [P, P, P, P, P, P, P, P, I, I, I, I, D, U, I, I, P, P, P, Z, P]),
     "ocx_faces_h_f64": (I, [P, P, P, P, P, I, I, I, I, U, P, P, P]),

@@ -384,6 +384,57 @@ int ocx_advect_fct_split(const double* C, const double* V0, const double* V1, co
                                            C_out, fx_i, open_flux, workspace, workspace_bytes, stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of the allocation-free FCT entry: the step's workspace (stage values, face stacks, open-edge fluxes) in a
+// file-scope device array.  Each GPU the library runs on holds its own copy, at its own address, until the process
+// ends.  In host code `fct_scratch` names only a host-side stand-in: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is ever passed to a kernel.
+constexpr size_t NOALLOC_SCRATCH_BYTES = OCX_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char fct_scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's address of fct_scratch (looked up on the first call made on that device).
+int scratch_address(void** out) {
+    int dev = 0;
+    OCX_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(OCX_E_ARG, "device %d beyond %d", dev, NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        OCX_TRY(hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(fct_scratch)), "hipGetSymbolAddress(fct_scratch)"));
+        if (!p) return fail(OCX_E_HIP, "hipGetSymbolAddress(fct_scratch) gave a null address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return OCX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocx_advect_fct_f64_noalloc(const double* C, const double* V0, const double* V1, const double* Qx,
+                               const double* Qy, const uint8_t* ok, const double* inflow, int nt, int nz, int ny,
+                               int nx, double dt, unsigned flags, int flux_face_i, double* C_out, double* fx_i,
+                               double* open_flux, void* stream) {
+    OCX_TRY(check_shape(nt, nz, ny, nx));
+    const size_t need_b = fct_layout<ocx::PrecF64>(nullptr, nt, nz, ny, nx, nullptr);
+    if (need_b > NOALLOC_SCRATCH_BYTES)
+        return fail(OCX_E_WORKSPACE,
+                    "shape (nt=%d, nz=%d, ny=%d, nx=%d) needs %zu bytes of workspace; the allocation-free entry holds "
+                    "%zu (use ocx_advect_fct_f64 with a workspace)", nt, nz, ny, nx, need_b, NOALLOC_SCRATCH_BYTES);
+    void* ws = nullptr;
+    OCX_TRY(scratch_address(&ws));
+    return advect_fct_impl<ocx::PrecF64>(C, V0, V1, Qx, Qy, ok, inflow, nt, nz, ny, nx, dt, flags, flux_face_i,
+                                         C_out, fx_i, open_flux, ws, NOALLOC_SCRATCH_BYTES, stream);
+}
+
 int ocx_advect_rk3_f64(const double* C, const double* V0, const double* V1, const double* Qx, const double* Qy,
                        const double* Fz, const uint8_t* ok, const double* inflow, int nt, int nz, int ny, int nx,
                        double dt, unsigned flags, int vert, int flux_face_i, double* C_out, double* fx_i,
