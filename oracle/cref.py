@@ -42,6 +42,7 @@ def lib():
             build()
         L = ctypes.CDLL(str(LIB))
         P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
+        L.oc_set_threads.argtypes = [I]
         L.oc_advect_fct.argtypes = [P, P, P, P, P, P, P, I, I, I, I, D, I, I, I, I, P, P, P, P]
         L.oc_advect_fct.restype = I
         L.oc_advect_rk3.argtypes = [P, P, P, P, P, P, P, P, I, I, I, I, D, I, I, P]
@@ -60,6 +61,11 @@ def lib():
         L.oc_regrid_hybrid.argtypes = [P, P, P, P, P, I, LL, P, P, P, P, D, D, D, I, P, P]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP team size of the C loops."""
+    lib().oc_set_threads(int(n))
 
 
 def _p(a):

@@ -14,6 +14,10 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
+
+/* OpenMP team size of the loops below (the CPU baseline times them on all threads and on one) */
+void oc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 #define OPEN_W 1
 #define OPEN_E 2

@@ -15,7 +15,8 @@ loop runs on one host thread:
   budget.
 
 Each job's result is bit-identical to the same job run alone (``advance_alone``): a batch member's arithmetic does
-not depend on the other members.
+not depend on the other members.  In split precision each step is within 1e-5 of each tracer's range of the FP64
+step; that bound holds per step, not over a job's whole run, in which any perturbation grows with the flow.
 """
 from __future__ import annotations
 
