@@ -148,6 +148,110 @@ __device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const doubl
     }
 }
 
+// One x-face (tile row ly, face lf = 0 .. TX of the tile's row, the face between columns i0+lf-1 and i0+lf) or one
+// y-face (face row lf = 0 .. TY, column lx) of the LDS tile, with the same rules as tile_faces.
+template <int SCH, int TX, int TY>
+__device__ __forceinline__ double tile_xface(const double* sc, const uint8_t* so, const double* __restrict__ extp,
+                                             int ly, int lf, int i0, int j0, int ny, int nx, unsigned flags, double q) {
+    constexpr int H = 3, LX = TX + 2 * H;
+    const int j = j0 + ly, f = i0 + lf;
+    double val = 0.0;
+    if (j < ny && f <= nx) {
+        const double* row = sc + (ly + H) * LX;
+        const uint8_t* orow = so + (ly + H) * LX;
+        if (f > 0 && f < nx) {
+            if constexpr (SCH == SCH_C6)
+                val = q * centred_face(row, orow, 1, lf + H);
+            else if constexpr (SCH >= SCH_UP3)
+                val = q * up_face<SCH == SCH_UP3>(row, orow, 1, lf + H, q);
+            else if (q != 0.0)
+                val = q * lds_face<SCH == SCH_Z>(row, orow, 1, lf + H, q);
+        } else if (f == 0) {
+            if (flags & OPEN_W) {
+                const double cc = row[lf + H];
+                const double e = extp ? extp[(size_t)j * nx] : cc;
+                val = q * (q > 0.0 ? e : cc);
+            } else if constexpr (SCH >= SCH_UP3) {
+                val = q * 0.0;
+            }
+        } else if (flags & OPEN_E) {
+            const double cc = row[lf + H - 1];
+            const double e = extp ? extp[(size_t)j * nx + nx - 1] : cc;
+            val = q * (q < 0.0 ? e : cc);
+        } else if constexpr (SCH >= SCH_UP3) {
+            val = q * 0.0;
+        }
+    }
+    return val;
+}
+
+template <int SCH, int TX, int TY>
+__device__ __forceinline__ double tile_yface(const double* sc, const uint8_t* so, const double* __restrict__ extp,
+                                             int lf, int lx, int i0, int j0, int ny, int nx, unsigned flags, double q) {
+    constexpr int H = 3, LX = TX + 2 * H;
+    const int f = j0 + lf, i = i0 + lx;
+    double val = 0.0;
+    if (i < nx && f <= ny) {
+        const double* col = sc + lx + H;
+        const uint8_t* ocol = so + lx + H;
+        if (f > 0 && f < ny) {
+            if constexpr (SCH == SCH_C6)
+                val = q * centred_face(col, ocol, LX, lf + H);
+            else if constexpr (SCH >= SCH_UP3)
+                val = q * up_face<SCH == SCH_UP3>(col, ocol, LX, lf + H, q);
+            else if (q != 0.0)
+                val = q * lds_face<SCH == SCH_Z>(col, ocol, LX, lf + H, q);
+        } else if (f == 0) {
+            if (flags & OPEN_S) {
+                const double cc = col[(lf + H) * LX];
+                const double e = extp ? extp[i] : cc;
+                val = q * (q > 0.0 ? e : cc);
+            } else if constexpr (SCH >= SCH_UP3) {
+                val = q * 0.0;
+            }
+        } else if (flags & OPEN_N) {
+            const double cc = col[(lf + H - 1) * LX];
+            const double e = extp ? extp[(size_t)(ny - 1) * nx + i] : cc;
+            val = q * (q < 0.0 ? e : cc);
+        } else if constexpr (SCH >= SCH_UP3) {
+            val = q * 0.0;
+        }
+    }
+    return val;
+}
+
+// The faces of the tile with one wavefront per tile row (TX = 64, blockDim (64, TY)): wave w forms the west faces
+// of row w and the north faces of row w; the tile's last x-faces (east of each row) and its first y-faces (south of
+// row 0) go to two waves that alternate with the layer parity, so every wave forms the same number of faces over
+// two layers.  q*: the transports of those items, loaded ahead (tile_face_q_waves).
+template <int SCH, int TX, int TY>
+__device__ __forceinline__ void tile_faces_waves(double* sfx, double* sfy, const double* sc, const uint8_t* so,
+                                                 const double* __restrict__ extp, int i0, int j0, int ny, int nx,
+                                                 unsigned flags, int k, const double* q4) {
+    static_assert(TX == 64 && TY >= 2, "one wavefront per tile row");
+    const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
+    sfx[w * (TX + 1) + lane] = tile_xface<SCH, TX, TY>(sc, so, extp, w, lane, i0, j0, ny, nx, flags, q4[0]);
+    sfy[(w + 1) * TX + lane] = tile_yface<SCH, TX, TY>(sc, so, extp, w + 1, lane, i0, j0, ny, nx, flags, q4[1]);
+    const int ws = (k & 1) ? TY - 2 : 0, we = (k & 1) ? TY - 1 : 1;
+    if (w == ws) sfy[lane] = tile_yface<SCH, TX, TY>(sc, so, extp, 0, lane, i0, j0, ny, nx, flags, q4[2]);
+    if (w == we && lane < TY)
+        sfx[lane * (TX + 1) + TX] = tile_xface<SCH, TX, TY>(sc, so, extp, lane, TX, i0, j0, ny, nx, flags, q4[3]);
+}
+
+// The transports of tile_faces_waves's items for one layer: Qx on the west face of the thread's cell, Qy on its north
+// face, Qy on the south face of its column in row 0, Qx on the east face of tile row `lane` (lanes < TY); 0 outside
+// the domain.
+template <int TX, int TY>
+__device__ __forceinline__ void tile_face_q_waves(double* q4, const double* __restrict__ Qxk,
+                                                  const double* __restrict__ Qyk, int i0, int j0, int ny, int nx) {
+    const int w = threadIdx.y, lane = threadIdx.x;
+    const int j = j0 + w, i = i0 + lane;
+    q4[0] = (j < ny && i <= nx) ? Qxk[(size_t)j * (nx + 1) + i] : 0.0;
+    q4[1] = (i < nx && j + 1 <= ny) ? Qyk[(size_t)(j + 1) * nx + i] : 0.0;
+    q4[2] = (i < nx && j0 <= ny) ? Qyk[(size_t)j0 * nx + i] : 0.0;
+    q4[3] = (lane < TY && j0 + lane < ny && i0 + TX <= nx) ? Qxk[(size_t)(j0 + lane) * (nx + 1) + i0 + TX] : 0.0;
+}
+
 // Donor-cell face flux F_L = q * c_up of x-face f (row pointers of C and of the exterior values), rules of
 // _fct_fluxes (sigma_baroclinic.py:482-491): interior q >= 0 takes the low side; open edges as the WENO faces;
 // closed edges 0.
@@ -352,11 +456,10 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
         }
     };
     // the face transports of the thread's face items, likewise one layer ahead
-    using IT = TileItems<TX, TY>;
-    double qxn[IT::NX], qyn[IT::NY], qxc[IT::NX], qyc[IT::NY];
+    double qn4[4], qc4[4];
     auto fetch_q = [&](int kk) {
-        tile_face_q<TX, TY>(qxn, qyn, Qx + (size_t)kk * ny * (nx + 1), Qy + (size_t)kk * (ny + 1) * nx, i0, j0, ny,
-                            nx);
+        tile_face_q_waves<TX, TY>(qn4, Qx + (size_t)kk * ny * (nx + 1), Qy + (size_t)kk * (ny + 1) * nx, i0, j0, ny,
+                                  nx);
     };
     fetch(0);
     fetch_q(0);
@@ -374,9 +477,7 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
             }
         }
 #pragma unroll
-        for (int m = 0; m < IT::NX; ++m) qxc[m] = qxn[m];
-#pragma unroll
-        for (int m = 0; m < IT::NY; ++m) qyc[m] = qyn[m];
+        for (int m = 0; m < 4; ++m) qc4[m] = qn4[m];
         if (k + 1 < n) {
             fetch(k + 1);
             fetch_q(k + 1);
@@ -396,8 +497,7 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
             }
         }
         __syncthreads();
-        tile_faces<SCH, TX, TY>(sfx, sfy, sc, so, Qx + (size_t)k * ny * (nx + 1), Qy + (size_t)k * (ny + 1) * nx,
-                                ext ? ext + pc : nullptr, i0, j0, ny, nx, flags, qxc, qyc);
+        tile_faces_waves<SCH, TX, TY>(sfx, sfy, sc, so, ext ? ext + pc : nullptr, i0, j0, ny, nx, flags, k, qc4);
         __syncthreads();
         double ft = 0.0;
         if constexpr (ZF) {

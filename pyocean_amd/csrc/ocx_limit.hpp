@@ -48,7 +48,10 @@ struct LimitArgs {
 // newest row in LDS, so a wave reads those of layers k +- 1 from the waves above and below it and only the group's
 // bottom and top layers read their outer neighbour layer from memory; the waves of one layer (its tracers) read the
 // shared per-layer fields together, through the compute unit's cache.
-constexpr int LG = 4;      // layers per limiter workgroup
+#ifndef OCX_LG
+#define OCX_LG 4
+#endif
+constexpr int LG = OCX_LG;  // layers per limiter workgroup
 constexpr int LMAXW = 4;   // tracers per limiter workgroup at most (LMAXW x LG waves <= 1024 threads)
 
 // Logical blocks of one limiter launch over a domain: tracer groups x layer groups x strips x chunks.

@@ -161,7 +161,7 @@ __device__ __forceinline__ float weno3f(float v3, const Diff5& d) {
 
 template <bool Z>
 __device__ __forceinline__ float face_value(float v1, float v2, float v3, float v4, float v5, bool k1, bool k2,
-                                            bool k3, bool k4, bool k5) {
+                                            bool k3, bool k4, bool k5, bool = false) {
     const Diff5 d = diff5(v1, v2, v3, v4, v5);
     if (k1 && k2 && k3 && k4 && k5) return weno5f<Z>(v1, v2, v3, v4, v5, d);
     if (k2 && k3 && k4) return weno3f(v3, d);
@@ -173,7 +173,7 @@ __device__ __forceinline__ float face_value(float v1, float v2, float v3, float 
 // q * value is formed in it too).
 template <bool Z, class W>
 __device__ __forceinline__ W line_face(double qd, int f, int nf, const W* u, const bool* o, bool open_lo,
-                                       bool open_hi, W ext_lo, W ext_hi) {
+                                       bool open_hi, W ext_lo, W ext_hi, bool known = false) {
     const W q = (W)qd;
     if (f > 0 && f < nf) {
         if (qd == 0.0) return (W)0;
@@ -182,7 +182,7 @@ __device__ __forceinline__ W line_face(double qd, int f, int nf, const W* u, con
         const W v4 = pos ? u[3] : u[2], v5 = pos ? u[4] : u[1];
         const bool k1 = pos ? o[0] : o[5], k2 = pos ? o[1] : o[4], k3 = pos ? o[2] : o[3];
         const bool k4 = pos ? o[3] : o[2], k5 = pos ? o[4] : o[1];
-        return q * face_value<Z>(v1, v2, v3, v4, v5, k1, k2, k3, k4, k5);
+        return q * face_value<Z>(v1, v2, v3, v4, v5, k1, k2, k3, k4, k5, known);
     }
     if (f == 0) return open_lo ? q * (qd > 0.0 ? ext_lo : u[3]) : (W)0;
     return open_hi ? q * (qd < 0.0 ? ext_hi : u[2]) : (W)0;
@@ -346,6 +346,17 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
     bool hk;
     use(ldh(jb), hv, hk);
     RowIn cur = mask_row(ld_row(jb));
+    // FP64: the last row of the window (or its x-stencil halo) holding a value outside fast_value's range in any
+    // lane; the faces of row j take the untested fast divisions when that row lies below j - 3 (wave-uniform)
+    auto row_bad = [&](W v) -> bool {
+        if constexpr (PR::split) return false;
+        else return __builtin_amdgcn_ballot_w64(!fast_value(v)) != 0;
+    };
+    int bad_row = jb - 4;
+#pragma unroll
+    for (int m = 0; m < 7; ++m)
+        if (row_bad(w[m])) bad_row = jb - 3 + m;
+    if (row_bad(hv)) bad_row = max(bad_row, jb);
 
     // y-face jb (the chunk's first south face)
     double fys;
@@ -387,6 +398,7 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bool known = !PR::split && bad_row < j - 3;
         double fxw = 0.0;
         if (xface) {
             W u[6];
@@ -397,13 +409,13 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
                 o[m] = obw[lane + m] != 0;
             }
             fxw = (double)line_face<Z>(cur.qx, i, nx, u, o, ow, oe, ex((size_t)j * nx, u[3]),
-                                       ex((size_t)j * nx + nx - 1, u[2]));
+                                       ex((size_t)j * nx + nx - 1, u[2]), known);
         }
         // y-face j+1 (north of row j): rows j-2 .. j+3 = w[1..6]
         double fyn = 0.0;
         const double qyn = cur.qy;
         if (col_in) fyn = (double)line_face<Z>(qyn, j + 1, ny, w + 1, kw + 1, os, on, ex(i, w[4]),
-                                               ex((size_t)(ny - 1) * nx + i, w[3]));
+                                               ex((size_t)(ny - 1) * nx + i, w[3]), known);
         const double fxe = shfl_down1(fxw);
         const bool yown = (j + 1 < je) || (j + 1 == ny);
 
@@ -490,6 +502,8 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
         }
         use(wn_raw, w[6], kw[6]);
         use(hn_raw, hv, hk);
+        if (row_bad(w[6])) bad_row = j + 4;
+        if (row_bad(hv)) bad_row = max(bad_row, j + 1);
         cur = mask_row(nxt_raw);
     }
 }

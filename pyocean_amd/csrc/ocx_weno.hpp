@@ -36,18 +36,38 @@ __device__ __forceinline__ long long xcd_logical(long long b, long long nb) {
 // (tracer, plane base pointers) stays in scalar registers instead of being recomputed per lane.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
+// x / b for the WENO constant divisors, without div_const's tiny-x test: for x that is 0 or at least 2^-960 in
+// magnitude (a sum of products of stencil values that are each 0 or at least 2^-400 in magnitude is a multiple of
+// 2^-452, so it is either 0 or beyond that bound).
+__device__ __forceinline__ double div_const_n(double x, double b, double rb) {
+    const double q = x * rb;
+    const double r = __fma_rn(b, q, -x);
+    return __fma_rn(-r, rb, q);
+}
+
+// Division modes of the face value: IEEE divisions; the fast divisions (div_const with its tiny-x test, div_nr);
+// the fast divisions when every stencil value is known to be 0 or of magnitude in [2^-400, 2^100] (no test at all).
+constexpr int DIV_IEEE = 0, DIV_FAST = 1, DIV_FAST_KNOWN = 2;
+
+// A stencil value that keeps every division of its faces on the fast path: 0, or of magnitude in [2^-400, 2^100].
+__device__ __forceinline__ bool fast_value(double v) {
+    const double a = fabs(v);
+    return a <= 0x1p100 && (a >= 0x1p-400 || a == 0.0);
+}
+
 // WENO5 value from the upwind-ordered stencil v1..v5 (v3 = upwind cell).  Z: WENO5-Z (Borges et al. 2008,
 // tau5 = |b2 - b0|, eps = 1e-40); otherwise WENO5-JS (Jiang & Shu 1996, eps = 1e-6).  d = (0.1, 0.6, 0.3).
-// FAST: the divisions by div_const / div_nr (identical results for stencil values in the guarded range).
-template <bool Z, bool FAST>
+// MODE: the divisions (DIV_IEEE, DIV_FAST, DIV_FAST_KNOWN); all three give identical results in their ranges.
+template <bool Z, int MODE>
 __device__ __forceinline__ double weno5(double v1, double v2, double v3, double v4, double v5) {
+    constexpr bool FAST = MODE != DIV_IEEE;
     constexpr double r6 = 1.0 / 6.0;
     const double n0 = 2.0 * v1 - 7.0 * v2 + 11.0 * v3;
     const double n1 = -v2 + 5.0 * v3 + 2.0 * v4;
     const double n2 = 2.0 * v3 + 5.0 * v4 - v5;
-    const double q0 = FAST ? div_const(n0, 6.0, r6) : n0 / 6.0;
-    const double q1 = FAST ? div_const(n1, 6.0, r6) : n1 / 6.0;
-    const double q2 = FAST ? div_const(n2, 6.0, r6) : n2 / 6.0;
+    const double q0 = MODE == DIV_FAST_KNOWN ? div_const_n(n0, 6.0, r6) : FAST ? div_const(n0, 6.0, r6) : n0 / 6.0;
+    const double q1 = MODE == DIV_FAST_KNOWN ? div_const_n(n1, 6.0, r6) : FAST ? div_const(n1, 6.0, r6) : n1 / 6.0;
+    const double q2 = MODE == DIV_FAST_KNOWN ? div_const_n(n2, 6.0, r6) : FAST ? div_const(n2, 6.0, r6) : n2 / 6.0;
     constexpr double c1312 = 13.0 / 12.0;
     const double b0 = c1312 * sq(v1 - 2.0 * v2 + v3) + 0.25 * sq(v1 - 4.0 * v2 + 3.0 * v3);
     const double b1 = c1312 * sq(v2 - 2.0 * v3 + v4) + 0.25 * sq(v2 - v4);
@@ -82,14 +102,20 @@ __device__ __forceinline__ double weno3(double v2, double v3, double v4) {
 // are, else the upwind cell.  Cells outside the domain carry value 0 and validity 0.  The fast divisions are used
 // when the stencil's largest magnitude m lies in [2^-400, 2^100] (or m = 0): then no operand of a division is near
 // the overflow or denormal range (the smoothness indicators stay below 2^210, the weight sums above 2^-430, a
-// nonzero numerator above 2^-520), which is where the plain sequence and the IEEE division can differ.
+// nonzero numerator above 2^-520), which is where the plain sequence and the IEEE division can differ.  known: the
+// caller has checked that every value of the stencil is 0 or in that range (fast_value), which implies that test.
 template <bool Z>
 __device__ __forceinline__ double face_value(double v1, double v2, double v3, double v4, double v5,
-                                             bool k1, bool k2, bool k3, bool k4, bool k5) {
+                                             bool k1, bool k2, bool k3, bool k4, bool k5, bool known = false) {
+    if (known) {
+        if (k1 && k2 && k3 && k4 && k5) return weno5<Z, DIV_FAST_KNOWN>(v1, v2, v3, v4, v5);
+        if (k2 && k3 && k4) return weno3<true>(v2, v3, v4);
+        return v3;
+    }
     const double m = fmax(fmax(fmax(fabs(v1), fabs(v2)), fmax(fabs(v3), fabs(v4))), fabs(v5));
     const bool fast = m <= 0x1p100 && (m >= 0x1p-400 || m == 0.0);
     if (k1 && k2 && k3 && k4 && k5)
-        return fast ? weno5<Z, true>(v1, v2, v3, v4, v5) : weno5<Z, false>(v1, v2, v3, v4, v5);
+        return fast ? weno5<Z, DIV_FAST>(v1, v2, v3, v4, v5) : weno5<Z, DIV_IEEE>(v1, v2, v3, v4, v5);
     if (k2 && k3 && k4) return fast ? weno3<true>(v2, v3, v4) : weno3<false>(v2, v3, v4);
     return v3;
 }
