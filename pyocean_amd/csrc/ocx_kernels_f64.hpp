@@ -152,7 +152,8 @@ __device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const doubl
 // y-face (face row lf = 0 .. TY, column lx) of the LDS tile, with the same rules as tile_faces.
 template <int SCH, int TX, int TY>
 __device__ __forceinline__ double tile_xface(const double* sc, const uint8_t* so, const double* __restrict__ extp,
-                                             int ly, int lf, int i0, int j0, int ny, int nx, unsigned flags, double q) {
+                                             int ly, int lf, int i0, int j0, int ny, int nx, unsigned flags, double q,
+                                             bool known) {
     constexpr int H = 3, LX = TX + 2 * H;
     const int j = j0 + ly, f = i0 + lf;
     double val = 0.0;
@@ -165,7 +166,7 @@ __device__ __forceinline__ double tile_xface(const double* sc, const uint8_t* so
             else if constexpr (SCH >= SCH_UP3)
                 val = q * up_face<SCH == SCH_UP3>(row, orow, 1, lf + H, q);
             else if (q != 0.0)
-                val = q * lds_face<SCH == SCH_Z>(row, orow, 1, lf + H, q);
+                val = q * lds_face<SCH == SCH_Z>(row, orow, 1, lf + H, q, known);
         } else if (f == 0) {
             if (flags & OPEN_W) {
                 const double cc = row[lf + H];
@@ -187,7 +188,8 @@ __device__ __forceinline__ double tile_xface(const double* sc, const uint8_t* so
 
 template <int SCH, int TX, int TY>
 __device__ __forceinline__ double tile_yface(const double* sc, const uint8_t* so, const double* __restrict__ extp,
-                                             int lf, int lx, int i0, int j0, int ny, int nx, unsigned flags, double q) {
+                                             int lf, int lx, int i0, int j0, int ny, int nx, unsigned flags, double q,
+                                             bool known) {
     constexpr int H = 3, LX = TX + 2 * H;
     const int f = j0 + lf, i = i0 + lx;
     double val = 0.0;
@@ -200,7 +202,7 @@ __device__ __forceinline__ double tile_yface(const double* sc, const uint8_t* so
             else if constexpr (SCH >= SCH_UP3)
                 val = q * up_face<SCH == SCH_UP3>(col, ocol, LX, lf + H, q);
             else if (q != 0.0)
-                val = q * lds_face<SCH == SCH_Z>(col, ocol, LX, lf + H, q);
+                val = q * lds_face<SCH == SCH_Z>(col, ocol, LX, lf + H, q, known);
         } else if (f == 0) {
             if (flags & OPEN_S) {
                 const double cc = col[(lf + H) * LX];
@@ -223,19 +225,22 @@ __device__ __forceinline__ double tile_yface(const double* sc, const uint8_t* so
 // The faces of the tile with one wavefront per tile row (TX = 64, blockDim (64, TY)): wave w forms the west faces
 // of row w and the north faces of row w; the tile's last x-faces (east of each row) and its first y-faces (south of
 // row 0) go to two waves that alternate with the layer parity, so every wave forms the same number of faces over
-// two layers.  q*: the transports of those items, loaded ahead (tile_face_q_waves).
+// two layers.  q*: the transports of those items, loaded ahead (tile_face_q_waves).  known: every value of the LDS
+// tile is 0 or of magnitude in [2^-400, 2^100], so the WENO faces skip their per-face range test.
 template <int SCH, int TX, int TY>
 __device__ __forceinline__ void tile_faces_waves(double* sfx, double* sfy, const double* sc, const uint8_t* so,
                                                  const double* __restrict__ extp, int i0, int j0, int ny, int nx,
-                                                 unsigned flags, int k, const double* q4) {
+                                                 unsigned flags, int k, const double* q4, bool known) {
     static_assert(TX == 64 && TY >= 2, "one wavefront per tile row");
     const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
-    sfx[w * (TX + 1) + lane] = tile_xface<SCH, TX, TY>(sc, so, extp, w, lane, i0, j0, ny, nx, flags, q4[0]);
-    sfy[(w + 1) * TX + lane] = tile_yface<SCH, TX, TY>(sc, so, extp, w + 1, lane, i0, j0, ny, nx, flags, q4[1]);
+    sfx[w * (TX + 1) + lane] = tile_xface<SCH, TX, TY>(sc, so, extp, w, lane, i0, j0, ny, nx, flags, q4[0], known);
+    sfy[(w + 1) * TX + lane] =
+        tile_yface<SCH, TX, TY>(sc, so, extp, w + 1, lane, i0, j0, ny, nx, flags, q4[1], known);
     const int ws = (k & 1) ? TY - 2 : 0, we = (k & 1) ? TY - 1 : 1;
-    if (w == ws) sfy[lane] = tile_yface<SCH, TX, TY>(sc, so, extp, 0, lane, i0, j0, ny, nx, flags, q4[2]);
+    if (w == ws) sfy[lane] = tile_yface<SCH, TX, TY>(sc, so, extp, 0, lane, i0, j0, ny, nx, flags, q4[2], known);
     if (w == we && lane < TY)
-        sfx[lane * (TX + 1) + TX] = tile_xface<SCH, TX, TY>(sc, so, extp, lane, TX, i0, j0, ny, nx, flags, q4[3]);
+        sfx[lane * (TX + 1) + TX] =
+            tile_xface<SCH, TX, TY>(sc, so, extp, lane, TX, i0, j0, ny, nx, flags, q4[3], known);
 }
 
 // The transports of tile_faces_waves's items for one layer: Qx on the west face of the thread's cell, Qy on its north
@@ -370,8 +375,9 @@ __device__ __forceinline__ double coeff(double a, double rin_hi, double rout_lo,
 // (each layer's own ok in b; false and value 0 outside the column): the WENO face times q, upwind-ordered from the
 // sign of q as along x and y (_advect_div's vertical pass, sigma_baroclinic.py:335-353), or the UP3 / UP1 / centred
 // face value, not yet times q (_up3_faces, _centred_faces along the layer index, rhs_numpy tracers.py:148-150).
+// known: every value of the window is 0 or of magnitude in [2^-400, 2^100] (face_value's fast range).
 template <int SCH>
-__device__ __forceinline__ double zface_w(const double* w, const bool* b, double q) {
+__device__ __forceinline__ double zface_w(const double* w, const bool* b, double q, bool known = false) {
     if constexpr (SCH == SCH_C6) {
         const double a = w[2] + w[3];
         const bool m4 = b[1] && b[2] && b[3] && b[4];
@@ -384,13 +390,19 @@ __device__ __forceinline__ double zface_w(const double* w, const bool* b, double
         if (q >= 0.0) return (b[1] && b[2] && b[3]) ? (-w[1] + 5.0 * w[2] + 2.0 * w[3]) / 6.0 : w[2];
         return (b[2] && b[3] && b[4]) ? (2.0 * w[2] + 5.0 * w[3] - w[4]) / 6.0 : w[3];
     } else {
-        if (q >= 0.0) return q * face_value<SCH == SCH_Z>(w[0], w[1], w[2], w[3], w[4], b[0], b[1], b[2], b[3], b[4]);
-        return q * face_value<SCH == SCH_Z>(w[5], w[4], w[3], w[2], w[1], b[5], b[4], b[3], b[2], b[1]);
+        if (q >= 0.0)
+            return q * face_value<SCH == SCH_Z>(w[0], w[1], w[2], w[3], w[4], b[0], b[1], b[2], b[3], b[4], known);
+        return q * face_value<SCH == SCH_Z>(w[5], w[4], w[3], w[2], w[1], b[5], b[4], b[3], b[2], b[1], known);
     }
 }
 
+// waves per SIMD the RK stage kernel is compiled for (its register budget: 512 / waves VGPRs)
+#ifndef OCX_RK_WAVES
+#define OCX_RK_WAVES 3
+#endif
+
 template <int STAGE, int SCH, bool VERT, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ cin, const double* __restrict__ C,
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(OCX_RK_WAVES))) void k_rk_stage(const double* __restrict__ cin, const double* __restrict__ C,
                                                      const double* __restrict__ ext, const double* __restrict__ V0,
                                                      const double* __restrict__ V1, const double* __restrict__ Qx,
                                                      const double* __restrict__ Qy, const double* __restrict__ Fz,
@@ -407,6 +419,9 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
     __shared__ uint8_t so[LY * LX];
     __shared__ double sfx[TY * (TX + 1)];
     __shared__ double sfy[(TY + 1) * TX];
+    // per layer parity: set when a value of that layer's tile lies outside the WENO divisions' fast range
+    constexpr bool WENO = SCH <= SCH_Z;
+    __shared__ int sbad[2];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
     const long long nb = (long long)gx * gy * nt;
@@ -429,9 +444,14 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
         w[m] = in ? ccol[(size_t)kk * plane] : 0.0;
         b[m] = in && ok[(size_t)kk * plane + xc] != 0;
     };
+    // bit m: w[m] is 0 or of magnitude in [2^-400, 2^100]
+    unsigned wfast = 0;
     if constexpr (ZF) {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) wload(m, m - 2);
+        for (int m = 0; m < 6; ++m) {
+            wload(m, m - 2);
+            if constexpr (WENO) wfast |= (unsigned)fast_value(w[m]) << m;
+        }
     }
     // the bottom face of layer 0: 0 for WENO, Fz[0] * 0 for the rhs_numpy faces
     double fb = 0.0;
@@ -442,6 +462,7 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
     const int tid = ty * TX + tx;
     double pv[NL];
     uint8_t po[NL];
+    bool pfast = true;
     auto fetch = [&](int kk) {
         const double* src = cin + ((size_t)t * n + kk) * plane;
         const uint8_t* okp = ok + (size_t)kk * plane;
@@ -455,6 +476,11 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
             po[m] = in ? okp[(size_t)jj * nx + ii] : (uint8_t)0;
         }
     };
+    auto check = [&]() {
+        pfast = true;
+#pragma unroll
+        for (int m = 0; m < NL; ++m) pfast = pfast && fast_value(pv[m]);
+    };
     // the face transports of the thread's face items, likewise one layer ahead
     double qn4[4], qc4[4];
     auto fetch_q = [&](int kk) {
@@ -463,6 +489,10 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
     };
     fetch(0);
     fetch_q(0);
+    if constexpr (WENO) {
+        if (tid < 2) sbad[tid] = 0;
+        __syncthreads();
+    }
     for (int k = 0; k < n; ++k) {
         const int p = t * n + k;
         const size_t pc = (size_t)p * plane, kc = (size_t)k * plane;
@@ -475,6 +505,10 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                 sc[idx] = pv[m];
                 so[idx] = po[m];
             }
+        }
+        if constexpr (WENO) {
+            check();
+            if (__builtin_amdgcn_ballot_w64(!pfast) != 0 && tx == 0) sbad[k & 1] = 1;
         }
 #pragma unroll
         for (int m = 0; m < 4; ++m) qc4[m] = qn4[m];
@@ -497,7 +531,15 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
             }
         }
         __syncthreads();
-        tile_faces_waves<SCH, TX, TY>(sfx, sfy, sc, so, ext ? ext + pc : nullptr, i0, j0, ny, nx, flags, k, qc4);
+        // the flag of layer k is read below; the other parity's is cleared for layer k + 1 (its readers of layer
+        // k - 1 passed the previous barrier, its writers of layer k + 1 come after the next one)
+        bool known = false;
+        if constexpr (WENO) {
+            known = sbad[k & 1] == 0;
+            if (tid == 0) sbad[(k + 1) & 1] = 0;
+        }
+        tile_faces_waves<SCH, TX, TY>(sfx, sfy, sc, so, ext ? ext + pc : nullptr, i0, j0, ny, nx, flags, k, qc4,
+                                      known);
         __syncthreads();
         double ft = 0.0;
         if constexpr (ZF) {
@@ -506,7 +548,7 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                 if constexpr (SCH >= SCH_UP3)
                     ft = qz * zface_w<SCH>(w, b, qz);
                 else if (qz != 0.0)
-                    ft = zface_w<SCH>(w, b, qz);
+                    ft = zface_w<SCH>(w, b, qz, wfast == 0x3fu);
             }
         }
         if (cell) {
@@ -552,6 +594,7 @@ __global__ __launch_bounds__(TX* TY) void k_rk_stage(const double* __restrict__ 
                 b[m] = b[m + 1];
             }
             wload(5, k + 4);
+            if constexpr (WENO) wfast = (wfast >> 1) | ((unsigned)fast_value(w[5]) << 5);
         }
     }
 }

@@ -121,14 +121,17 @@ __device__ __forceinline__ double face_value(double v1, double v2, double v3, do
 }
 
 // Face value between cells c-1 and c of a line held in LDS (values s, validity o, element stride st), upwind
-// side from the sign of q (q >= 0: low side).  The line must hold valid indices c-3 .. c+2.
+// side from the sign of q (q >= 0: low side).  The line must hold valid indices c-3 .. c+2.  known: as for
+// face_value, every value of the line's stencil is 0 or of magnitude in [2^-400, 2^100].
 template <bool Z>
-__device__ __forceinline__ double lds_face(const double* s, const uint8_t* o, int st, int c, double q) {
+__device__ __forceinline__ double lds_face(const double* s, const uint8_t* o, int st, int c, double q,
+                                           bool known = false) {
     const bool pos = q >= 0.0;
     const int i1 = pos ? c - 3 : c + 2, i2 = pos ? c - 2 : c + 1, i3 = pos ? c - 1 : c;
     const int i4 = pos ? c : c - 1, i5 = pos ? c + 1 : c - 2;
     return face_value<Z>(s[i1 * st], s[i2 * st], s[i3 * st], s[i4 * st], s[i5 * st],
-                         o[i1 * st] != 0, o[i2 * st] != 0, o[i3 * st] != 0, o[i4 * st] != 0, o[i5 * st] != 0);
+                         o[i1 * st] != 0, o[i2 * st] != 0, o[i3 * st] != 0, o[i4 * st] != 0, o[i5 * st] != 0,
+                         known);
 }
 
 // UP3 / UP1 face value between cells c-1 and c of a line (values s, validity o, stride st), _up3_faces
