@@ -30,7 +30,10 @@ namespace {
 using namespace ocx_host;
 
 constexpr int TX = 64, TY = 4;
-constexpr int RK_TY = 4;     // tile rows of the RK3 / pc / RHS layer-marching kernel
+#ifndef OCX_RK_TY
+#define OCX_RK_TY 4
+#endif
+constexpr int RK_TY = OCX_RK_TY;     // tile rows of the RK3 / pc / RHS layer-marching kernel
 
 int check_shape(int nt, int nz, int ny, int nx) {
     if (nt < 1 || nz < 1 || ny < 1 || nx < 1)

@@ -508,8 +508,14 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
     }
 }
 
+// waves per SIMD the FP64 stage kernels are compiled for (their register budget: 512 / waves VGPRs)
+#ifndef OCX_MARCH_WAVES
+#define OCX_MARCH_WAVES 4
+#endif
+
 template <int STAGE, bool Z, bool UNLIM, class PR>
-__global__ __launch_bounds__(64 * MAXW) void k_march(MarchArgs a) {
+__global__ __launch_bounds__(64 * MAXW) __attribute__((amdgpu_waves_per_eu(OCX_MARCH_WAVES))) void k_march(
+    MarchArgs a) {
     __shared__ typename PR::w rbuf[MAXW][MRB];
     __shared__ uint8_t obuf[MAXW][MRB];
     const int wv = wave_id();

@@ -15,6 +15,10 @@ using namespace ocx::rmp;
 #ifndef REMAP_G
 #define REMAP_G 2
 #endif
+#ifndef REMAP_STAGE
+#define REMAP_STAGE 1
+#endif
+constexpr int REMAP_LDS_LAYERS = 128;   // 64 KB of target thicknesses per workgroup
 
 __global__ void k_set_bad(unsigned long long* p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = ~0ull;
@@ -37,16 +41,25 @@ int ocx_remap_columns_f64(const double* h_src, const double* f, const double* h_
     RemapArgs r{h_src, f, h_tgt, wet, out, n0, n1, nf, scheme, columns};
     hipStream_t s = static_cast<hipStream_t>(stream);
     ProfScope _ps("remap", s);
-    // one field per thread for a single field; otherwise groups of REMAP_G fields per thread, four groups (waves)
-    // of the same 64 columns per workgroup
+    // one field per thread for a single field; otherwise groups of REMAP_G fields per thread, up to REMAP_FW_MAX
+    // groups (waves) of the same 64 columns per workgroup, balanced over the workgroups of a column block; with at
+    // most REMAP_LDS_LAYERS target layers the workgroup stages the 64 columns' target thicknesses in LDS once for
+    // its waves.  Workgroups of at most 4 waves (launch bound 256) ran the 10-field remap at 1024^2 x 50 in 8.7 ms;
+    // 5 waves under a launch bound of 512 took 16.4 ms.
     const int g = nf == 1 ? 1 : REMAP_G;
     const int groups = (nf + g - 1) / g;
-    const int fw = std::min(groups, 4);
-    const dim3 grid((unsigned)((columns + 63) / 64), (unsigned)((groups + fw - 1) / fw)), block(64, fw);
-    if (g == 1)
-        k_remap<<<grid, block, 0, s>>>(r);
-    else
-        k_remap_g<REMAP_G><<<grid, block, 0, s>>>(r);
+    const int nby = (groups + REMAP_FW_MAX - 1) / REMAP_FW_MAX;
+    const int fw = (groups + nby - 1) / nby;
+    const dim3 grid((unsigned)((columns + 63) / 64), (unsigned)nby), block(64, fw);
+    const bool staged = REMAP_STAGE && n1 <= REMAP_LDS_LAYERS;
+    const size_t lds = staged ? (size_t)n1 * 64 * sizeof(double) : 0;
+    if (g == 1) {
+        if (staged) k_remap<true><<<grid, block, lds, s>>>(r);
+        else k_remap<false><<<grid, block, 0, s>>>(r);
+    } else {
+        if (staged) k_remap_g<REMAP_G, true><<<grid, block, lds, s>>>(r);
+        else k_remap_g<REMAP_G, false><<<grid, block, 0, s>>>(r);
+    }
     return launch_check("remap_columns");
 }
 

@@ -553,8 +553,8 @@ struct RemapArgs {
     long long M;
 };
 
-// column col of field q
-__device__ void remap_one(const RemapArgs& r, long long col, int q) {
+// column col of field q; ht: the column's target thicknesses (in HBM, or staged in LDS)
+__device__ void remap_one(const RemapArgs& r, long long col, int q, Col ht) {
     const long long M = r.M;
     const Col hs{r.h_src + col, M};
     const Col u{r.f + (size_t)q * r.n0 * M + col, M};
@@ -583,7 +583,7 @@ __device__ void remap_one(const RemapArgs& r, long long col, int q) {
     if ((scheme == PLM && n0 < 3) || (scheme == PPM_H4 && n0 < 4)) scheme = PCM;
 
     Targets T;
-    T.ht = Col{r.h_tgt + col, M};
+    T.ht = ht;
     T.out = out;
     T.M = M;
     T.n1 = n1;
@@ -701,7 +701,7 @@ __device__ void remap_one(const RemapArgs& r, long long col, int q) {
 // fields q0 .. q0 + G - 1 of column col (those beyond nf are computed on field q0 and not stored): the PPM_H4 path
 // shares the thickness work; the other schemes and dry or vanished columns take the single-field path per field
 template <int G>
-__device__ void remap_group(const RemapArgs& r, long long col, int q0) {
+__device__ void remap_group(const RemapArgs& r, long long col, int q0, Col ht) {
     const long long M = r.M;
     const int n0 = r.n0, n1 = r.n1;
     const int ng = r.nf - q0 < G ? r.nf - q0 : G;
@@ -719,7 +719,7 @@ __device__ void remap_group(const RemapArgs& r, long long col, int q0) {
         ppm = n_active > 3;
     }
     if (!ppm) {
-        for (int g = 0; g < ng; ++g) remap_one(r, col, q0 + g);
+        for (int g = 0; g < ng; ++g) remap_one(r, col, q0 + g, ht);
         return;
     }
     Col u[G];
@@ -731,7 +731,7 @@ __device__ void remap_group(const RemapArgs& r, long long col, int q0) {
         T.out[g] = r.out + (size_t)q * n1 * M + col;
         T.content[g] = 0.0;
     }
-    T.ht = Col{r.h_tgt + col, M};
+    T.ht = ht;
     T.M = M;
     T.n1 = n1;
     T.ng = ng;
@@ -855,20 +855,47 @@ __device__ void remap_group(const RemapArgs& r, long long col, int q0) {
     }
 }
 
-template <int G>
-__global__ __launch_bounds__(256) void k_remap_g(RemapArgs r) {
+// The target thicknesses of the workgroup's 64 columns (blockIdx.x), staged in LDS as [layer][lane]: the overlap
+// walks read them at layers that differ from lane to lane, which from HBM costs a cache line per lane and an L2
+// round trip per step of the walk; in LDS a lane's column sits at stride 64, so every lane reads its own bank.
+__device__ __forceinline__ Col stage_targets(double* sht, const RemapArgs& r) {
+    const long long col0 = (long long)blockIdx.x * 64;
+    const int nthr = blockDim.x * blockDim.y, tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int idx = tid; idx < r.n1 * 64; idx += nthr) {
+        const int k = idx >> 6, l = idx & 63;
+        sht[idx] = col0 + l < r.M ? r.h_tgt[(size_t)k * r.M + col0 + l] : 0.0;
+    }
+    __syncthreads();
+    return Col{sht + threadIdx.x, 64};
+}
+
+// block (64, FW): x over 64 columns, y over groups of G fields; grid (ceil(M / 64), ceil(groups / FW)).  STAGED:
+// the target thicknesses come from LDS (n1 * 512 bytes of dynamic shared memory), shared by the FW waves.
+#ifndef REMAP_FW_MAX
+#define REMAP_FW_MAX 4
+#endif
+
+template <int G, bool STAGED>
+__global__ __launch_bounds__(64 * REMAP_FW_MAX) void k_remap_g(RemapArgs r) {
+    extern __shared__ double sht[];
     const long long col = (long long)blockIdx.x * 64 + threadIdx.x;
     const int q0 = (blockIdx.y * blockDim.y + threadIdx.y) * G;
+    Col ht{r.h_tgt + col, r.M};
+    if constexpr (STAGED) ht = stage_targets(sht, r);
     if (col >= r.M || q0 >= r.nf) return;
-    remap_group<G>(r, col, q0);
+    remap_group<G>(r, col, q0, ht);
 }
 
 // block (64, FW): x over columns, y over fields; grid (ceil(M / 64), ceil(nf / FW))
-__global__ __launch_bounds__(256) void k_remap(RemapArgs r) {
+template <bool STAGED>
+__global__ __launch_bounds__(64 * REMAP_FW_MAX) void k_remap(RemapArgs r) {
+    extern __shared__ double sht[];
     const long long col = (long long)blockIdx.x * 64 + threadIdx.x;
     const int q = blockIdx.y * blockDim.y + threadIdx.y;
+    Col ht{r.h_tgt + col, r.M};
+    if constexpr (STAGED) ht = stage_targets(sht, r);
     if (col >= r.M || q >= r.nf) return;
-    remap_one(r, col, q);
+    remap_one(r, col, q, ht);
 }
 
 // ------------------------------------------------------------------------------------------ regrid

@@ -37,7 +37,7 @@ def _eq(gpu_t, ref, what):
 
 
 @pytest.mark.parametrize("scheme", [vertical.PCM, vertical.PLM, vertical.PPM_H4])
-@pytest.mark.parametrize("n0,n1", [(50, 50), (30, 45), (40, 12), (4, 4)])
+@pytest.mark.parametrize("n0,n1", [(50, 50), (30, 45), (40, 12), (4, 4), (60, 140)])
 def test_remap_matches_the_oracle_bit_for_bit(gpu, scheme, n0, n1):
     _say("remap", scheme, n0, n1)
     hs, f, ht, wet = cases.remap_case(10 + n0 + n1, n0=n0, n1=n1)
@@ -45,6 +45,17 @@ def test_remap_matches_the_oracle_bit_for_bit(gpu, scheme, n0, n1):
     _eq(out, cref.remap_columns(hs, f, ht, scheme, wet), "stack")
     one = vertical.remap_columns(_d(hs), _d(f[1]), _d(ht), scheme, _d(wet))
     _eq(one, cref.remap_columns(hs, f[1], ht, scheme, wet), "single field")
+
+
+@pytest.mark.parametrize("nf", [2, 10, 17])
+def test_remap_of_many_fields_matches_the_oracle(gpu, nf):
+    # the fields go two to a thread and up to eight threads to a column block (17 fields: two workgroups of five
+    # field groups, the last group holding one field and one group idle); the workgroup's target thicknesses are
+    # staged in LDS once for all of them (the 140-layer targets above take them from HBM)
+    _say("remap fields", nf)
+    hs, f, ht, wet = cases.remap_case(77 + nf, n0=24, n1=20, nf=nf)
+    out = vertical.remap_columns(_d(hs), _d(f), _d(ht), vertical.PPM_H4, _d(wet))
+    _eq(out, cref.remap_columns(hs, f, ht, vertical.PPM_H4, wet), f"{nf} fields")
 
 
 def test_remap_known_answers(gpu):
