@@ -54,7 +54,7 @@ struct LimitArgs {
 constexpr int LG = OCX_LG;  // layers per limiter workgroup
 constexpr int LMAXW = 4;   // tracers per limiter workgroup at most (LMAXW x LG waves <= 1024 threads)
 
-// Logical blocks of one limiter launch over a domain: tracer groups x layer groups x strips x chunks.
+// Logical blocks of one limiter launch over a domain: tracer groups x strips x layer groups x chunks.
 __host__ __device__ inline long long limit_blocks(const LimitArgs& a, int wpb) {
     return (long long)((a.nt + wpb - 1) / wpb) * ((a.n + LG - 1) / LG) * a.nsx * a.ncy;
 }
@@ -81,12 +81,15 @@ __device__ __forceinline__ void limit_block(const LimitArgs& a, long long L, int
     const int lane = threadIdx.x & 63;
     const int ntb = (a.nt + wpb - 1) / wpb;
     const int nkg = (a.n + LG - 1) / LG;
+    // tracer groups fastest, then strips: the strips of a row band run together, so the cache lines a strip's
+    // halo lanes share with its neighbours are read from L2 (7.9 -> 7.4 ms at 1024 x 1024 x 50 x 10 against
+    // layer groups before strips)
     const int tg = (int)(L % ntb);
     L /= ntb;
-    const int kg = (int)(L % nkg);
-    L /= nkg;
     const int sx = (int)(L % a.nsx);
-    const int cy = (int)(L / a.nsx);
+    L /= a.nsx;
+    const int kg = (int)(L % nkg);
+    const int cy = (int)(L / nkg);
     const int tw = wv % wpb, kw = wv / wpb;
     const int t_raw = tg * wpb + tw, k_raw = kg * LG + kw;
     const bool active = t_raw < a.nt && k_raw < a.n;

@@ -95,3 +95,24 @@ def test_weno_faces_and_fct_step_on_fields_scaled_across_the_fast_range_edges(gp
         ref = oracle_c.advect_fct(C, case["V0"], case["V1"], case["Qx"], case["Qy"], case["ok"], case["dt"],
                                   open_mask=case["open_mask"])
         assert _same(out.cpu().numpy(), ref).all()
+
+
+@pytest.mark.parametrize("scale_exp", [-402, -400, -398, 98, 100, 102])
+@pytest.mark.parametrize("variant", ["z", "js"])
+def test_rk3_step_with_z_sweep_on_fields_scaled_across_the_fast_range_edges(gpu, oracle_c, scale_exp, variant):
+    # the RK3 stage kernel checks the fast range once per layer tile (x and y faces) and per column window (z faces):
+    # with a field spanning a factor ~16 next to an edge some tiles and windows are inside the range and some are not
+    from tests import cases
+    case = cases.fused_fct_case(open_=True, nx=70, ny=40, nz=6, seed=5)
+    print(f"[gpu case] rk3 scaled fields 2^{scale_exp} {variant}", flush=True)
+    nz, ny, nx = case["V0"].shape
+    C = np.ldexp(case["C"], scale_exp)
+    Fz = np.random.default_rng(4).normal(0.0, 2e3, (nz + 1, ny, nx))
+    Fz[0] = Fz[-1] = 0.0
+    flags = case["open_mask"] | (16 if variant == "js" else 0)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)       # noqa: E731
+    out = ops.advect_rk3(d(C), d(case["V0"]), d(case["V1"]), d(case["Qx"]), d(case["Qy"]), d(Fz),
+                         d(case["ok"].astype(np.uint8)), case["dt"], flags=flags)
+    ref = oracle_c.advect_rk3(C, case["V0"], case["V1"], case["Qx"], case["Qy"], Fz, case["ok"], case["dt"],
+                              open_mask=case["open_mask"], z_variant=variant == "z")
+    assert _same(out.cpu().numpy(), ref).all()
