@@ -161,7 +161,7 @@ __device__ __forceinline__ float weno3f(float v3, const Diff5& d) {
 
 template <bool Z>
 __device__ __forceinline__ float face_value(float v1, float v2, float v3, float v4, float v5, bool k1, bool k2,
-                                            bool k3, bool k4, bool k5, bool = false) {
+                                            bool k3, bool k4, bool k5, bool = false, const WenoK& = WenoK{}) {
     const Diff5 d = diff5(v1, v2, v3, v4, v5);
     if (k1 && k2 && k3 && k4 && k5) return weno5f<Z>(v1, v2, v3, v4, v5, d);
     if (k2 && k3 && k4) return weno3f(v3, d);
@@ -173,7 +173,8 @@ __device__ __forceinline__ float face_value(float v1, float v2, float v3, float 
 // q * value is formed in it too).
 template <bool Z, class W>
 __device__ __forceinline__ W line_face(double qd, int f, int nf, const W* u, const bool* o, bool open_lo,
-                                       bool open_hi, W ext_lo, W ext_hi, bool known = false) {
+                                       bool open_hi, W ext_lo, W ext_hi, bool known = false,
+                                       const WenoK& K = WenoK{}) {
     const W q = (W)qd;
     if (f > 0 && f < nf) {
         if (qd == 0.0) return (W)0;
@@ -182,7 +183,7 @@ __device__ __forceinline__ W line_face(double qd, int f, int nf, const W* u, con
         const W v4 = pos ? u[3] : u[2], v5 = pos ? u[4] : u[1];
         const bool k1 = pos ? o[0] : o[5], k2 = pos ? o[1] : o[4], k3 = pos ? o[2] : o[3];
         const bool k4 = pos ? o[3] : o[2], k5 = pos ? o[4] : o[1];
-        return q * face_value<Z>(v1, v2, v3, v4, v5, k1, k2, k3, k4, k5, known);
+        return q * face_value<Z>(v1, v2, v3, v4, v5, k1, k2, k3, k4, k5, known, K);
     }
     if (f == 0) return open_lo ? q * (qd > 0.0 ? ext_lo : u[3]) : (W)0;
     return open_hi ? q * (qd < 0.0 ? ext_hi : u[2]) : (W)0;
@@ -338,6 +339,17 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
         return r;
     };
 
+    // FP64: WENO constants held in vector registers, since the stage kernels are short of scalar registers: all of
+    // them in stage 1 (scalar spills 74 -> 10), 1/6 and 13/12 in stages 2 and 3 (their VGPR budget of 128 has no
+    // room for more); stage 1 7.59 -> 7.50 ms, stage 2 8.70 -> 8.66 at 1024 x 1024 x 50 x 10
+#ifndef OCX_WENOK_S1
+#define OCX_WENOK_S1 15u
+#endif
+#ifndef OCX_WENOK_S23
+#define OCX_WENOK_S23 3u
+#endif
+    constexpr unsigned KSEL = PR::split ? 0u : (STAGE == 1 ? OCX_WENOK_S1 : OCX_WENOK_S23);
+    const WenoK K = weno_k_v<KSEL>();
     W w[7];
     bool kw[7];
 #pragma unroll
@@ -409,13 +421,13 @@ __device__ __forceinline__ void march_block(const MarchArgs& a, long long L, int
                 o[m] = obw[lane + m] != 0;
             }
             fxw = (double)line_face<Z>(cur.qx, i, nx, u, o, ow, oe, ex((size_t)j * nx, u[3]),
-                                       ex((size_t)j * nx + nx - 1, u[2]), known);
+                                       ex((size_t)j * nx + nx - 1, u[2]), known, K);
         }
         // y-face j+1 (north of row j): rows j-2 .. j+3 = w[1..6]
         double fyn = 0.0;
         const double qyn = cur.qy;
         if (col_in) fyn = (double)line_face<Z>(qyn, j + 1, ny, w + 1, kw + 1, os, on, ex(i, w[4]),
-                                               ex((size_t)(ny - 1) * nx + i, w[3]), known);
+                                               ex((size_t)(ny - 1) * nx + i, w[3]), known, K);
         const double fxe = shfl_down1(fxw);
         const bool yown = (j + 1 < je) || (j + 1 == ny);
 

@@ -55,33 +55,60 @@ __device__ __forceinline__ bool fast_value(double v) {
     return a <= 0x1p100 && (a >= 0x1p-400 || a == 0.0);
 }
 
+// The constants of the WENO5 value that are not exact 32-bit literals of an FP64 instruction.  By default they are
+// compile-time constants; a kernel short of scalar registers can hold them in vector registers instead (weno_k_v).
+struct WenoK {
+    double r6 = 1.0 / 6.0, c1312 = 13.0 / 12.0, d0 = 0.1, d1 = 0.6, d2 = 0.3, eps_z = 1e-40, eps_js = 1e-6;
+};
+// the same values; those selected by SEL (bit 0: 1/6, 1: 13/12, 2: the linear weights d, 3: the two epsilons) each
+// pass through an empty asm statement that places them in a vector register pair
+template <unsigned SEL = 15u>
+__device__ __forceinline__ WenoK weno_k_v() {
+    WenoK K;
+    if constexpr ((SEL & 1u) != 0) asm volatile("" : "+v"(K.r6));
+    if constexpr ((SEL & 2u) != 0) asm volatile("" : "+v"(K.c1312));
+    if constexpr ((SEL & 4u) != 0) {
+        asm volatile("" : "+v"(K.d0));
+        asm volatile("" : "+v"(K.d1));
+        asm volatile("" : "+v"(K.d2));
+    }
+    if constexpr ((SEL & 8u) != 0) {
+        asm volatile("" : "+v"(K.eps_z));
+        asm volatile("" : "+v"(K.eps_js));
+    }
+    return K;
+}
+
 // WENO5 value from the upwind-ordered stencil v1..v5 (v3 = upwind cell).  Z: WENO5-Z (Borges et al. 2008,
 // tau5 = |b2 - b0|, eps = 1e-40); otherwise WENO5-JS (Jiang & Shu 1996, eps = 1e-6).  d = (0.1, 0.6, 0.3).
 // MODE: the divisions (DIV_IEEE, DIV_FAST, DIV_FAST_KNOWN); all three give identical results in their ranges.
 template <bool Z, int MODE>
-__device__ __forceinline__ double weno5(double v1, double v2, double v3, double v4, double v5) {
+__device__ __forceinline__ double weno5(double v1, double v2, double v3, double v4, double v5,
+                                       const WenoK& K = WenoK{}) {
     constexpr bool FAST = MODE != DIV_IEEE;
-    constexpr double r6 = 1.0 / 6.0;
+    const double r6 = K.r6;
     const double n0 = 2.0 * v1 - 7.0 * v2 + 11.0 * v3;
     const double n1 = -v2 + 5.0 * v3 + 2.0 * v4;
     const double n2 = 2.0 * v3 + 5.0 * v4 - v5;
     const double q0 = MODE == DIV_FAST_KNOWN ? div_const_n(n0, 6.0, r6) : FAST ? div_const(n0, 6.0, r6) : n0 / 6.0;
     const double q1 = MODE == DIV_FAST_KNOWN ? div_const_n(n1, 6.0, r6) : FAST ? div_const(n1, 6.0, r6) : n1 / 6.0;
     const double q2 = MODE == DIV_FAST_KNOWN ? div_const_n(n2, 6.0, r6) : FAST ? div_const(n2, 6.0, r6) : n2 / 6.0;
-    constexpr double c1312 = 13.0 / 12.0;
+    const double c1312 = K.c1312;
     const double b0 = c1312 * sq(v1 - 2.0 * v2 + v3) + 0.25 * sq(v1 - 4.0 * v2 + 3.0 * v3);
     const double b1 = c1312 * sq(v2 - 2.0 * v3 + v4) + 0.25 * sq(v2 - v4);
     const double b2 = c1312 * sq(v3 - 2.0 * v4 + v5) + 0.25 * sq(3.0 * v3 - 4.0 * v4 + v5);
     double a0, a1, a2;
     if (Z) {
         const double tau = fabs(b2 - b0);
-        a0 = 0.1 * (1.0 + (FAST ? div_nr(tau, 1e-40 + b0) : tau / (1e-40 + b0)));
-        a1 = 0.6 * (1.0 + (FAST ? div_nr(tau, 1e-40 + b1) : tau / (1e-40 + b1)));
-        a2 = 0.3 * (1.0 + (FAST ? div_nr(tau, 1e-40 + b2) : tau / (1e-40 + b2)));
+        const double e = K.eps_z;
+        a0 = K.d0 * (1.0 + (FAST ? div_nr(tau, e + b0) : tau / (e + b0)));
+        a1 = K.d1 * (1.0 + (FAST ? div_nr(tau, e + b1) : tau / (e + b1)));
+        a2 = K.d2 * (1.0 + (FAST ? div_nr(tau, e + b2) : tau / (e + b2)));
     } else {
-        a0 = FAST ? div_nr(0.1, sq(1e-6 + b0)) : 0.1 / sq(1e-6 + b0);
-        a1 = FAST ? div_nr(0.6, sq(1e-6 + b1)) : 0.6 / sq(1e-6 + b1);
-        a2 = FAST ? div_nr(0.3, sq(1e-6 + b2)) : 0.3 / sq(1e-6 + b2);
+        const double e = K.eps_js;
+        a0 = FAST ? div_nr(K.d0, sq(e + b0)) : K.d0 / sq(e + b0);
+        a1 = FAST ? div_nr(K.d1, sq(e + b1)) : K.d1 / sq(e + b1);
+        a2 = FAST ? div_nr(K.d2, sq(e + b2)) : K.d2 / sq(e + b2);
     }
     const double num = a0 * q0 + a1 * q1 + a2 * q2, den = a0 + a1 + a2;
     return FAST ? div_nr(num, den) : num / den;
@@ -106,16 +133,17 @@ __device__ __forceinline__ double weno3(double v2, double v3, double v4) {
 // caller has checked that every value of the stencil is 0 or in that range (fast_value), which implies that test.
 template <bool Z>
 __device__ __forceinline__ double face_value(double v1, double v2, double v3, double v4, double v5,
-                                             bool k1, bool k2, bool k3, bool k4, bool k5, bool known = false) {
+                                             bool k1, bool k2, bool k3, bool k4, bool k5, bool known = false,
+                                             const WenoK& K = WenoK{}) {
     if (known) {
-        if (k1 && k2 && k3 && k4 && k5) return weno5<Z, DIV_FAST_KNOWN>(v1, v2, v3, v4, v5);
+        if (k1 && k2 && k3 && k4 && k5) return weno5<Z, DIV_FAST_KNOWN>(v1, v2, v3, v4, v5, K);
         if (k2 && k3 && k4) return weno3<true>(v2, v3, v4);
         return v3;
     }
     const double m = fmax(fmax(fmax(fabs(v1), fabs(v2)), fmax(fabs(v3), fabs(v4))), fabs(v5));
     const bool fast = m <= 0x1p100 && (m >= 0x1p-400 || m == 0.0);
     if (k1 && k2 && k3 && k4 && k5)
-        return fast ? weno5<Z, DIV_FAST>(v1, v2, v3, v4, v5) : weno5<Z, DIV_IEEE>(v1, v2, v3, v4, v5);
+        return fast ? weno5<Z, DIV_FAST>(v1, v2, v3, v4, v5, K) : weno5<Z, DIV_IEEE>(v1, v2, v3, v4, v5, K);
     if (k2 && k3 && k4) return fast ? weno3<true>(v2, v3, v4) : weno3<false>(v2, v3, v4);
     return v3;
 }
