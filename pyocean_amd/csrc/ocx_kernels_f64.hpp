@@ -149,18 +149,19 @@ __device__ __forceinline__ void tile_faces(double* sfx, double* sfy, const doubl
 }
 
 // One x-face (tile row ly, face lf = 0 .. TX of the tile's row, the face between columns i0+lf-1 and i0+lf) or one
-// y-face (face row lf = 0 .. TY, column lx) of the LDS tile, with the same rules as tile_faces.
-template <int SCH, int TX, int TY>
+// y-face (face row lf = 0 .. TY, column lx) of the LDS tile, with the same rules as tile_faces.  INNER: the tile's
+// faces are all interior faces of the domain (no edge rules, no bounds tests).
+template <int SCH, int TX, int TY, bool INNER = false>
 __device__ __forceinline__ double tile_xface(const double* sc, const uint8_t* so, const double* __restrict__ extp,
                                              int ly, int lf, int i0, int j0, int ny, int nx, unsigned flags, double q,
                                              bool known) {
     constexpr int H = 3, LX = TX + 2 * H;
     const int j = j0 + ly, f = i0 + lf;
     double val = 0.0;
-    if (j < ny && f <= nx) {
+    if (INNER || (j < ny && f <= nx)) {
         const double* row = sc + (ly + H) * LX;
         const uint8_t* orow = so + (ly + H) * LX;
-        if (f > 0 && f < nx) {
+        if (INNER || (f > 0 && f < nx)) {
             if constexpr (SCH == SCH_C6)
                 val = q * centred_face(row, orow, 1, lf + H);
             else if constexpr (SCH >= SCH_UP3)
@@ -186,17 +187,17 @@ __device__ __forceinline__ double tile_xface(const double* sc, const uint8_t* so
     return val;
 }
 
-template <int SCH, int TX, int TY>
+template <int SCH, int TX, int TY, bool INNER = false>
 __device__ __forceinline__ double tile_yface(const double* sc, const uint8_t* so, const double* __restrict__ extp,
                                              int lf, int lx, int i0, int j0, int ny, int nx, unsigned flags, double q,
                                              bool known) {
     constexpr int H = 3, LX = TX + 2 * H;
     const int f = j0 + lf, i = i0 + lx;
     double val = 0.0;
-    if (i < nx && f <= ny) {
+    if (INNER || (i < nx && f <= ny)) {
         const double* col = sc + lx + H;
         const uint8_t* ocol = so + lx + H;
-        if (f > 0 && f < ny) {
+        if (INNER || (f > 0 && f < ny)) {
             if constexpr (SCH == SCH_C6)
                 val = q * centred_face(col, ocol, LX, lf + H);
             else if constexpr (SCH >= SCH_UP3)
@@ -227,20 +228,22 @@ __device__ __forceinline__ double tile_yface(const double* sc, const uint8_t* so
 // row 0) go to two waves that alternate with the layer parity, so every wave forms the same number of faces over
 // two layers.  q*: the transports of those items, loaded ahead (tile_face_q_waves).  known: every value of the LDS
 // tile is 0 or of magnitude in [2^-400, 2^100], so the WENO faces skip their per-face range test.
-template <int SCH, int TX, int TY>
+template <int SCH, int TX, int TY, bool INNER = false>
 __device__ __forceinline__ void tile_faces_waves(double* sfx, double* sfy, const double* sc, const uint8_t* so,
                                                  const double* __restrict__ extp, int i0, int j0, int ny, int nx,
                                                  unsigned flags, int k, const double* q4, bool known) {
     static_assert(TX == 64 && TY >= 2, "one wavefront per tile row");
     const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
-    sfx[w * (TX + 1) + lane] = tile_xface<SCH, TX, TY>(sc, so, extp, w, lane, i0, j0, ny, nx, flags, q4[0], known);
+    sfx[w * (TX + 1) + lane] =
+        tile_xface<SCH, TX, TY, INNER>(sc, so, extp, w, lane, i0, j0, ny, nx, flags, q4[0], known);
     sfy[(w + 1) * TX + lane] =
-        tile_yface<SCH, TX, TY>(sc, so, extp, w + 1, lane, i0, j0, ny, nx, flags, q4[1], known);
+        tile_yface<SCH, TX, TY, INNER>(sc, so, extp, w + 1, lane, i0, j0, ny, nx, flags, q4[1], known);
     const int ws = (k & 1) ? TY - 2 : 0, we = (k & 1) ? TY - 1 : 1;
-    if (w == ws) sfy[lane] = tile_yface<SCH, TX, TY>(sc, so, extp, 0, lane, i0, j0, ny, nx, flags, q4[2], known);
+    if (w == ws)
+        sfy[lane] = tile_yface<SCH, TX, TY, INNER>(sc, so, extp, 0, lane, i0, j0, ny, nx, flags, q4[2], known);
     if (w == we && lane < TY)
         sfx[lane * (TX + 1) + TX] =
-            tile_xface<SCH, TX, TY>(sc, so, extp, lane, TX, i0, j0, ny, nx, flags, q4[3], known);
+            tile_xface<SCH, TX, TY, INNER>(sc, so, extp, lane, TX, i0, j0, ny, nx, flags, q4[3], known);
 }
 
 // The transports of tile_faces_waves's items for one layer: Qx on the west face of the thread's cell, Qy on its north
@@ -436,6 +439,9 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(OCX_RK_W
     const int j = j0 + ty, i = i0 + tx;
     const bool cell = j < ny && i < nx;
     const size_t xc = cell ? (size_t)j * nx + i : 0;
+    // every face of the tile is an interior face of the domain (block-uniform): those tiles take a copy of the face
+    // code with no edge rules and no bounds tests (39.4 -> 38.3 ms for the sigma RK3 step at 1024^2 x 50 x 10)
+    const bool inner = i0 > 0 && i0 + TX < nx && j0 > 0 && j0 + TY < ny;
     const double* ccol = cin + (size_t)t * n * plane + xc;
     double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     bool b[6] = {false, false, false, false, false, false};
@@ -538,8 +544,12 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(OCX_RK_W
             known = sbad[k & 1] == 0;
             if (tid == 0) sbad[(k + 1) & 1] = 0;
         }
-        tile_faces_waves<SCH, TX, TY>(sfx, sfy, sc, so, ext ? ext + pc : nullptr, i0, j0, ny, nx, flags, k, qc4,
-                                      known);
+        if (inner)
+            tile_faces_waves<SCH, TX, TY, true>(sfx, sfy, sc, so, ext ? ext + pc : nullptr, i0, j0, ny, nx, flags, k,
+                                                qc4, known);
+        else
+            tile_faces_waves<SCH, TX, TY>(sfx, sfy, sc, so, ext ? ext + pc : nullptr, i0, j0, ny, nx, flags, k, qc4,
+                                          known);
         __syncthreads();
         double ft = 0.0;
         if constexpr (ZF) {
