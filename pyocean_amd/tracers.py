@@ -47,6 +47,14 @@ def _host(t):
     return t.cpu().numpy()
 
 
+def _empty_result(C, flux_face_i):
+    """The step's result for a stack with no cells: C_new of C's shape, and the face column (ntr, N, ny)."""
+    out = np.empty_like(C)
+    if flux_face_i is None:
+        return out
+    return out, np.empty(C.shape[:3])
+
+
 class TracerAdvection:
     name = "tracer_advection"
     role = Role.TRACER_ADVECTION
@@ -113,6 +121,11 @@ class TracerAdvection:
         """
         C = np.ascontiguousarray(C, dtype=float)
         self._shapes(C, V0, V1, Qx, Qy)
+        if C.size == 0:
+            # an empty stack (no tracers, layers or cells): nothing to advect, as the reference's loops over none
+            if self.record_open_flux and not unlimited:
+                self.last_open_flux = np.zeros(C.shape[0])
+            return _empty_result(C, flux_face_i)
         flags = self._flags(unlimited)
         okc = np.broadcast_to(ok, C.shape[1:])
         d = {k: _dev(v) for k, v in (("C", C), ("V0", V0), ("V1", V1), ("Qx", Qx), ("Qy", Qy))}
@@ -151,6 +164,8 @@ class TracerAdvection:
         if np.shape(V0) != C.shape[1:] or np.shape(V1) != C.shape[1:]:
             raise ValueError(f"_advect: volume shape {np.shape(V0)}/{np.shape(V1)} vs tracer layers {C.shape[1:]}")
         self._shapes(C, V0, V1, Qx, Qy, Fz)
+        if C.size == 0:
+            return _empty_result(C, flux_face_i)
         flags = self._flags(rk=True)
         vert = bool(np.any(np.asarray(Fz) != 0.0))
         okc = np.broadcast_to(ok, C.shape[1:])

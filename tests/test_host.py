@@ -34,6 +34,22 @@ def test_volume_shape_mismatch_raises_value_error_before_the_gpu():
                        np.ones((3, 4, 5), bool))
 
 
+def test_empty_stack_gives_empty_results():
+    """A stack with no tracers: the reference's loops run over no planes and return empty arrays of the input's
+    shape (the face column (0, N, ny)); the open-edge budget is empty too.  Nothing reaches the device."""
+    adv = _adv()
+    adv.record_open_flux = True
+    C = np.zeros((0, 3, 4, 5))
+    V, Qx, Qy, Fz, ok = (np.ones((3, 4, 5)), np.zeros((3, 4, 6)), np.zeros((3, 5, 5)), np.zeros((4, 4, 5)),
+                         np.ones((3, 4, 5), bool))
+    out, fx = adv.advect_fct(C, V, V, Qx, Qy, ok, flux_face_i=2)
+    assert out.shape == C.shape and fx.shape == (0, 3, 4)
+    assert adv.last_open_flux.shape == (0,)
+    assert adv.advect(C, V, V, Qx, Qy, Fz, ok).shape == C.shape
+    out, fx = adv.advect(C, V, V, Qx, Qy, Fz, ok, flux_face_i=0)
+    assert out.shape == C.shape and fx.shape == (0, 3, 4)
+
+
 @pytest.mark.parametrize("scheme,rk_bits,fct_bits", [("weno5", 0, 0), ("weno5js", 16, 16), ("upwind3", 64, 16),
                                                      ("upwind1", 128, 16)])
 def test_face_flags_follow_the_reference_dispatch(scheme, rk_bits, fct_bits):
