@@ -136,6 +136,20 @@ def test_batch_members_match_the_oracle(gpu, oracle_c):
     b.close()
 
 
+def test_batch_member_flags_outside_the_batch_step_are_refused(gpu):
+    # a batch runs the limited FCT step: a member asking for the unlimited step, or any flag bit beyond the open
+    # edges and WENO5-JS, is refused on the host before any launch
+    _say("batch flags")
+    dom = _domain(70, 60, 3, 1, 5, "f64")
+    for bad in (ops.flags_of(0, unlimited=True), 1 << 6, 1 << 31):
+        dom.flags = bad
+        with pytest.raises(ValueError, match="flags"):
+            batch.Batch([dom])
+    dom.flags = ops.flags_of(15, js=True)
+    b = batch.Batch([dom])
+    b.close()
+
+
 def test_full_64_domain_mixed_batch_conserves_and_bounds_every_member(gpu):
     # the batched configuration itself: 64 regional domains (synth.batch_shapes(64, seed=7), seeds 100 + d, ten
     # tracers), even members FP64 and odd members split, one captured-graph step; every member keeps its content to
