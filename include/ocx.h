@@ -210,11 +210,11 @@ int ocx_content_f64(const double* c, const double* V, int nt, int nz, int ny, in
 
 /* Self-test of the correctly rounded fast divisions the FP64 kernels use in place of the IEEE division (no
  * counterpart in the reference, whose Numba kernels divide with IEEE semantics, sigma_baroclinic.py:212-231): for n
- * device operand pairs (x[e], y[e]), out (5, n) receives div_const(x, 6), div_const(x, 3) (the divisions by the
+ * device operand pairs (x[e], y[e]), out (6, n) receives div_const(x, 6), div_const(x, 3) (the divisions by the
  * WENO and F_H constants, defined for finite x), div_nr(x, y) (reciprocal, two Newton steps and one residual correction; defined for
- * y > 0 with both operands inside 2^-500 .. 2^500 or x = +-0), qdiv(x, y) (div_nr inside that range, IEEE outside)
- * and the device's IEEE quotient x / y.  Each of the first four equals the IEEE quotient bit for bit in its
- * stated range. */
+ * y > 0 with both operands inside 2^-500 .. 2^500 or x = +-0), qdiv(x, y) (div_nr inside that range, IEEE outside),
+ * the device's IEEE quotient x / y, and qdiv_r(x, recip_of(y)) (qdiv with the reciprocal of y refined ahead of x).
+ * Each of the fast forms equals the IEEE quotient bit for bit in its stated range (qdiv and qdiv_r everywhere). */
 int ocx_selftest_div_f64(const double* x, const double* y, long long n, double* out, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------

@@ -169,7 +169,9 @@ __global__ __launch_bounds__(256) void k_thomas_solve_r(const double* u, const d
         }
         double rhs = zz * uu;
         rhs = k == n - 1 ? rhs + dt * tau : rhs;
-        const double v = k == 0 ? rhs / m : (rhs - l * dprev) / m;
+        // the pivot's reciprocal is refined off the recurrence; qdiv_r gives the IEEE quotient (rhs - l dprev) / m
+        const Recip R = recip_of(m);
+        const double v = qdiv_r(k == 0 ? rhs : rhs - l * dprev, R);
         d[k] = v;
         dprev = k < n ? v : dprev;
     }
@@ -199,13 +201,14 @@ __global__ void k_thomas_solve(const double* u, const double* __restrict__ dz, c
                 const TriRow tr = tri_row(dz, nu, r, r_layers, n, M, k, col, dt);
                 const double dzk = dz[(size_t)k * M + col];
                 const double m = k == 0 ? tr.diag : tr.diag - tr.lower * cp[(size_t)(k - 1) * M + col];
+                const Recip R = recip_of(m);            // one refined reciprocal of the pivot for the TC fields
 #pragma unroll
                 for (int q = 0; q < TC; ++q) {
                     if (q < tn) {
                         const size_t x = ((size_t)(t0 + q) * n + k) * M + col;
                         double rhs = dzk * u[x];
                         if (k == n - 1) rhs += dt * tau;
-                        const double d = k == 0 ? rhs / m : (rhs - tr.lower * dprev[q]) / m;
+                        const double d = qdiv_r(k == 0 ? rhs : rhs - tr.lower * dprev[q], R);
                         out[x] = d;
                         dprev[q] = d;
                     }

@@ -44,4 +44,30 @@ __device__ __forceinline__ double qdiv(double x, double y) {
     return x / y;
 }
 
+// qdiv with the divisor's reciprocal refined ahead of the dividend: recip_of(y) holds div_nr's refined reciprocal
+// of y and whether y lies in qdiv's range, and qdiv_r(x, R) is then qdiv(x, y) bit for bit (the same operations on
+// the same operands).  Where y is known before x (a pivot loaded ahead of a recurrence), only the quotient and its
+// residual correction wait on x: three dependent operations instead of the IEEE division's sequence.
+struct Recip {
+    double y, r;
+    bool ok;
+};
+__device__ __forceinline__ Recip recip_of(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double e = __fma_rn(-y, r, 1.0);
+    r = __fma_rn(r, e, r);
+    e = __fma_rn(-y, r, 1.0);
+    r = __fma_rn(r, e, r);
+    return Recip{y, r, y >= 0x1p-500 && y <= 0x1p500};
+}
+__device__ __forceinline__ double qdiv_r(double x, const Recip& R) {
+    const double ax = fabs(x);
+    if (R.ok && ax <= 0x1p500 && (ax >= 0x1p-500 || x == 0.0)) {
+        const double q = x * R.r;
+        const double t = __fma_rn(R.y, q, -x);
+        return __fma_rn(-t, R.r, q);
+    }
+    return x / R.y;
+}
+
 }  // namespace ocx

@@ -784,7 +784,8 @@ __global__ void k_fill(double* __restrict__ dst, double value, size_t count) {
 }
 
 // Self-test of the fast divisions: for n operand pairs (x, y), out[0..n) = div_const(x, 6), [n..2n) = div_const(x, 3),
-// [2n..3n) = div_nr(x, y), [3n..4n) = qdiv(x, y), [4n..5n) = the IEEE quotient x / y.
+// [2n..3n) = div_nr(x, y), [3n..4n) = qdiv(x, y), [4n..5n) = the IEEE quotient x / y, [5n..6n) = qdiv_r(x,
+// recip_of(y)).
 __global__ void k_selftest_div(const double* __restrict__ x, const double* __restrict__ y, size_t n,
                                double* __restrict__ out) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -795,6 +796,7 @@ __global__ void k_selftest_div(const double* __restrict__ x, const double* __res
     out[2 * n + e] = div_nr(a, b);
     out[3 * n + e] = qdiv(a, b);
     out[4 * n + e] = a / b;
+    out[5 * n + e] = qdiv_r(a, recip_of(b));
 }
 
 // Numerical mixing of the advection of c (Burchard & Rennau 2008, eq. 24), with c^2 advected in the same stack:

@@ -50,12 +50,12 @@ def test_fast_divisions_equal_the_ieee_quotient_bit_for_bit(gpu):
     n = x.size
     print(f"[gpu case] selftest div n={n}", flush=True)
     dx, dy = torch.from_numpy(x).to(gpu), torch.from_numpy(y).to(gpu)
-    out = torch.empty(5 * n, dtype=torch.float64, device=gpu)
+    out = torch.empty(6 * n, dtype=torch.float64, device=gpu)
     L = _native.lib()
     _native.check(L.ocx_selftest_div_f64(dx.data_ptr(), dy.data_ptr(), n, out.data_ptr(),
                                          torch.cuda.current_stream().cuda_stream), "ocx_selftest_div_f64")
     torch.cuda.synchronize()
-    o = out.cpu().numpy().reshape(5, n)
+    o = out.cpu().numpy().reshape(6, n)
     with np.errstate(all="ignore"):
         q6, q3, q = x / 6.0, x / 3.0, x / y
     assert _same(o[4], q).all(), "the device's IEEE division differs from the host's"
@@ -74,6 +74,9 @@ def test_fast_divisions_equal_the_ieee_quotient_bit_for_bit(gpu):
     assert not bad.any(), f"div_nr differs at (x, y) = {list(zip(x[bad][:5], y[bad][:5]))}"
     bad = ~_same(o[3], q)
     assert not bad.any(), f"qdiv differs at (x, y) = {list(zip(x[bad][:5], y[bad][:5]))}"
+    # the same with the divisor's reciprocal refined first (the vertical diffusion solve's pivots)
+    bad = ~_same(o[5], q)
+    assert not bad.any(), f"qdiv_r differs at (x, y) = {list(zip(x[bad][:5], y[bad][:5]))}"
 
 
 @pytest.mark.parametrize("scale_exp", [-404, -401, -400, -399, -396, 96, 99, 100, 101, 104, -1000, 700])
