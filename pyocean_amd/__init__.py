@@ -5,6 +5,7 @@ hand-written HIP kernels for the MI355X (gfx950), behind the reference's ``Trace
 * ``pyocean_amd.tracers.TracerAdvection`` - the drop-in component (NumPy in, NumPy out);
 * ``pyocean_amd.ops`` - the same operations on device-resident torch tensors;
 * ``pyocean_amd.synth`` - seeded benchmark workloads;
+* ``pyocean_amd.stack`` - the tracer stack's layout and its open-edge inflow values;
 * ``include/ocx.h`` / ``pyocean_amd/lib/libocx.so`` - the C ABI and its library.
 """
 from .periodic import NONE, TRACER_HALO, Periodic  # noqa: F401

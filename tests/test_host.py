@@ -176,3 +176,52 @@ def test_next_row_mirrors_reject_bad_arguments_before_any_kernel():
         vertical.regrid(None, None, None, cpu, "hybrid", {"dz_top": [1.0]})
     with pytest.raises(ValueError, match="layer densities rho"):
         vertical.regrid(None, None, None, cpu, "hybrid", {"dz_top": [1.0], "rho_target": [1.0]})
+
+
+def test_stack_layout_matches_the_reference_cases():
+    """The positions of [T, (T²), S, (S²), tr…] for the model switches of the reference's own layout tests
+    (test_n2a_salinity.py:188-195, test_n2a_tef.py:247-249; the reference model's mix_diagnostic defaults to on)."""
+    from types import SimpleNamespace as M
+
+    from pyocean_amd.stack import stack_layout
+    assert stack_layout(M(ale=False)) == {"T": 0, "T2": 1, "n": 2}
+    assert stack_layout(M(ale=False, salinity=True)) == {"T": 0, "T2": 1, "S": 2, "n": 3}
+    assert stack_layout(M(ale=True, mix_diagnostic=True, n_tracers=2)) == {"T": 0, "T2": 1, "tr": 2, "n": 4}
+    assert stack_layout(M(ale=True, salinity=True, mix_diagnostic=False, n_tracers=1)) == \
+        {"T": 0, "S": 1, "tr": 2, "n": 3}
+    assert stack_layout(M(ale=True, salinity=True, mix_diagnostic=True, n_tracers=1)) == \
+        {"T": 0, "T2": 1, "S": 2, "tr": 3, "n": 4}
+    assert "S2" in stack_layout(M(ale=False, salinity=True, variance_budget=False, tef_face=(3, "x")))
+    # the passive tracers count only in ALE; the variance budget keeps T² and adds S²
+    assert stack_layout(M(ale=False, n_tracers=3)) == {"T": 0, "T2": 1, "n": 2}
+    assert stack_layout(M(ale=True, salinity=True, variance_budget=True)) == {"T": 0, "T2": 1, "S": 2, "S2": 3, "n": 4}
+
+
+def test_inflow_stack_follows_the_layout_on_host_and_torch_arrays():
+    """OpenBoundary.inflow_stack (open_boundary.py:353-380): the stack itself without open edges or inflow; with a
+    layout the inflow fields in its order (squares as x·x); without one [T, (T²), tr…] by the stack's length."""
+    from types import SimpleNamespace as M
+
+    import torch
+
+    from pyocean_amd.stack import inflow_stack, stack_layout
+    rng = np.random.default_rng(3)
+    T, S = rng.normal(10, 2, (4, 5, 6)), rng.normal(35, 1, (4, 5, 6))
+    tr = [rng.normal(0, 1, (4, 5, 6)) for _ in range(2)]
+    lay = stack_layout(M(ale=True, salinity=True, variance_budget=True, n_tracers=2))
+    c = rng.normal(0, 1, (lay["n"], 4, 5, 6))
+    shut = M(open=(), ale=True, n_tracers=2)
+    assert inflow_stack(shut, c, T, tr, S, lay) is c
+    assert inflow_stack(M(open=("west",)), c, None, tr, S, lay) is c
+    ob = M(open=("west", "south"), ale=True, n_tracers=2)
+    got = inflow_stack(ob, c, T, tr, S, lay)
+    want = np.stack([T, T * T, S, S * S, tr[0], tr[1]])
+    assert got.flags["C_CONTIGUOUS"] and np.array_equal(got, want)
+    tt = inflow_stack(ob, torch.from_numpy(c), torch.from_numpy(T), [torch.from_numpy(a) for a in tr],
+                      torch.from_numpy(S), lay)
+    assert isinstance(tt, torch.Tensor) and np.array_equal(tt.numpy(), want)
+    # no layout: [T, T², tr…] when the stack has room for T², [T, tr…] when it does not
+    assert np.array_equal(inflow_stack(ob, np.zeros((4, 4, 5, 6)), T, tr), np.stack([T, T * T, tr[0], tr[1]]))
+    assert np.array_equal(inflow_stack(ob, np.zeros((3, 4, 5, 6)), T, tr), np.stack([T, tr[0], tr[1]]))
+    assert np.array_equal(inflow_stack(M(open=("north",), ale=False), np.zeros((2, 4, 5, 6)), T, None),
+                          np.stack([T, T * T]))
