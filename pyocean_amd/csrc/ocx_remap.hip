@@ -18,6 +18,9 @@ using namespace ocx::rmp;
 #ifndef REMAP_STAGE
 #define REMAP_STAGE 1
 #endif
+#ifndef REMAP_SPLIT_ROWS
+#define REMAP_SPLIT_ROWS 1
+#endif
 constexpr int REMAP_LDS_LAYERS = 128;   // 64 KB of target thicknesses per workgroup
 
 __global__ void k_set_bad(unsigned long long* p) {
@@ -44,21 +47,32 @@ int ocx_remap_columns_f64(const double* h_src, const double* f, const double* h_
     // one field per thread for a single field; otherwise groups of REMAP_G fields per thread, up to REMAP_FW_MAX
     // groups (waves) of the same 64 columns per workgroup, balanced over the workgroups of a column block; with at
     // most REMAP_LDS_LAYERS target layers the workgroup stages the 64 columns' target thicknesses in LDS once for
-    // its waves.  Workgroups of at most 4 waves (launch bound 256) ran the 10-field remap at 1024^2 x 50 in 8.7 ms;
-    // 5 waves under a launch bound of 512 took 16.4 ms.
+    // its waves.  Workgroups of at most 4 waves (launch bound 256) ran the 10-field remap at 1024^2 x 50 in 8.7 ms
+    // (8.5 with the rows split so that no wave idles); 5 waves under a launch bound of 512 took 16.4 ms.
     const int g = nf == 1 ? 1 : REMAP_G;
     const int groups = (nf + g - 1) / g;
     const int nby = (groups + REMAP_FW_MAX - 1) / REMAP_FW_MAX;
     const int fw = (groups + nby - 1) / nby;
-    const dim3 grid((unsigned)((columns + 63) / 64), (unsigned)nby), block(64, fw);
+    const unsigned ncb = (unsigned)((columns + 63) / 64);
     const bool staged = REMAP_STAGE && n1 <= REMAP_LDS_LAYERS;
     const size_t lds = staged ? (size_t)n1 * 64 * sizeof(double) : 0;
     if (g == 1) {
+        const dim3 grid(ncb, (unsigned)nby), block(64, fw);
         if (staged) k_remap<true><<<grid, block, lds, s>>>(r);
         else k_remap<false><<<grid, block, 0, s>>>(r);
     } else {
-        if (staged) k_remap_g<REMAP_G, true><<<grid, block, lds, s>>>(r);
-        else k_remap_g<REMAP_G, false><<<grid, block, 0, s>>>(r);
+        // the groups as rows of workgroups of fw and fw - 1 waves, so that no wave idles (5 groups: a row of 3 and
+        // a row of 2, each row of a column block one workgroup), each kind its own launch
+        const int full = REMAP_SPLIT_ROWS ? groups - nby * (fw - 1) : nby;    // rows of fw waves
+        const int rest = nby - full;                                           // rows of fw - 1 waves
+        auto launch = [&](int rows, int width, int goff) {
+            if (rows <= 0) return;
+            const dim3 grid(ncb, (unsigned)rows), block(64, width);
+            if (staged) k_remap_g<REMAP_G, true><<<grid, block, lds, s>>>(r, goff);
+            else k_remap_g<REMAP_G, false><<<grid, block, 0, s>>>(r, goff);
+        };
+        launch(full, fw, 0);
+        launch(rest, fw - 1, full * fw);
     }
     return launch_check("remap_columns");
 }

@@ -869,17 +869,17 @@ __device__ __forceinline__ Col stage_targets(double* sht, const RemapArgs& r) {
     return Col{sht + threadIdx.x, 64};
 }
 
-// block (64, FW): x over 64 columns, y over groups of G fields; grid (ceil(M / 64), ceil(groups / FW)).  STAGED:
+// block (64, FW): x over 64 columns, y over groups of G fields from group goff on; grid (ceil(M / 64), rows).  STAGED:
 // the target thicknesses come from LDS (n1 * 512 bytes of dynamic shared memory), shared by the FW waves.
 #ifndef REMAP_FW_MAX
 #define REMAP_FW_MAX 4
 #endif
 
 template <int G, bool STAGED>
-__global__ __launch_bounds__(64 * REMAP_FW_MAX) void k_remap_g(RemapArgs r) {
+__global__ __launch_bounds__(64 * REMAP_FW_MAX) void k_remap_g(RemapArgs r, int goff) {
     extern __shared__ double sht[];
     const long long col = (long long)blockIdx.x * 64 + threadIdx.x;
-    const int q0 = (blockIdx.y * blockDim.y + threadIdx.y) * G;
+    const int q0 = (goff + blockIdx.y * blockDim.y + threadIdx.y) * G;
     Col ht{r.h_tgt + col, r.M};
     if constexpr (STAGED) ht = stage_targets(sht, r);
     if (col >= r.M || q0 >= r.nf) return;
