@@ -47,12 +47,13 @@ def test_remap_matches_the_oracle_bit_for_bit(gpu, scheme, n0, n1):
     _eq(one, cref.remap_columns(hs, f[1], ht, scheme, wet), "single field")
 
 
-@pytest.mark.parametrize("nf,n1", [(2, 20), (10, 20), (17, 20), (10, 100)])
+@pytest.mark.parametrize("nf,n1", [(2, 20), (10, 20), (14, 20), (17, 20), (20, 20), (10, 100)])
 def test_remap_of_many_fields_matches_the_oracle(gpu, nf, n1):
     # the fields go two to a thread and the field groups of a column block to rows of workgroups that no group
     # leaves idle (10 fields: 5 groups, a row of 3 waves and a row of 2, two launches; 17 fields: 9 groups, the last
-    # holding one field, three rows of 3); each workgroup stages its 64 columns' target thicknesses in LDS (with 20
-    # and 100 target layers; the 140-layer targets above take them from HBM)
+    # holding one field, three rows of 3; 14 fields: a row of 4 and a row of 3; 20 fields: a row of 4 and two of 3);
+    # each workgroup stages its 64 columns' target thicknesses in LDS (with 20 and 100 target layers; the 140-layer
+    # targets above take them from HBM)
     _say("remap fields", nf, n1)
     hs, f, ht, wet = cases.remap_case(77 + nf, n0=24, n1=n1, nf=nf)
     out = vertical.remap_columns(_d(hs), _d(f), _d(ht), vertical.PPM_H4, _d(wet))
