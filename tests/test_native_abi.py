@@ -92,3 +92,48 @@ def test_planes_beyond_32_bit_offsets_are_rejected():
     lib = _native.lib()
     rc = _fct(lib, ny=20000, nx=20000)
     assert rc == _native.OCX_E_ARG and "too large" in lib.ocx_last_error().decode()
+
+
+C_HOST = r"""
+#include <stdio.h>
+#include <string.h>
+#include "ocx.h"
+
+/* host-side calls only: the ABI version, workspace sizes, and an argument error with its message */
+int main(void) {
+    double one = 1.0, out = 0.0;
+    unsigned char okv = 1;
+    char ws[16];
+    int rc;
+    printf("version %d\n", ocx_version());
+    printf("fct %zu split %zu rk3 %zu\n", ocx_fct_workspace_bytes(10, 50, 96, 80),
+           ocx_fct_split_workspace_bytes(10, 50, 96, 80), ocx_rk3_workspace_bytes(10, 50, 96, 80));
+    rc = ocx_advect_fct_f64(&one, &one, &one, &one, &one, &okv, NULL, 0, 1, 8, 8, 1.0, OCX_OPEN_W, -1, &out, NULL,
+                            NULL, ws, sizeof ws, NULL);
+    printf("rc %d %s\n", rc, rc == OCX_E_ARG && strstr(ocx_last_error(), "must be positive") ? "ok" : "bad");
+    return 0;
+}
+"""
+
+
+def test_a_c99_host_builds_against_the_header_and_links_the_library(tmp_path):
+    """The boundary is plain C: a C99 program (gcc -std=c99 -pedantic, no HIP or C++ headers) includes
+    include/ocx.h, links libocx.so and gets the same host-side answers as the Python binding."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    lib = _native.lib()
+    src, exe = tmp_path / "host.c", tmp_path / "host"
+    src.write_text(C_HOST)
+    libdir = Path(_native.LIB_PATH).resolve().parent
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", f"-I{HEADER.parent}", str(src),
+                    f"-L{libdir}", f"-l:{Path(_native.LIB_PATH).name}", f"-Wl,-rpath,{libdir}", "-o", str(exe)],
+                   check=True, capture_output=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    assert got[0] == f"version {lib.ocx_version()}"
+    assert got[1] == (f"fct {lib.ocx_fct_workspace_bytes(10, 50, 96, 80)} "
+                      f"split {lib.ocx_fct_split_workspace_bytes(10, 50, 96, 80)} "
+                      f"rk3 {lib.ocx_rk3_workspace_bytes(10, 50, 96, 80)}")
+    assert got[2] == f"rc {_native.OCX_E_ARG} ok"
