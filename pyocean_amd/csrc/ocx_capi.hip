@@ -30,10 +30,28 @@ namespace {
 using namespace ocx_host;
 
 constexpr int TX = 64, TY = 4;
-#ifndef OCX_RK_TY
-#define OCX_RK_TY 4
+// Tiles of the RK3 / pc / RHS layer-marching kernel: 256 cells in 4 wavefronts, RK_TX columns x 256 / RK_TX rows
+// with the WENO faces, RK_TX_UP columns with the UP3 / UP1 / centred faces.  16 x 16 tiles form 3.1 WENO faces per
+// cell (their 16 south and 16 east faces one evaluation) from a 22 x 22 halo tile, against 3.3 from 70 x 10 for
+// 64 x 4: the sigma RK3 step took 36.3 ms against 38.6.  The cheaper faces keep 64 x 4, where the kernel needs
+// fewer registers (the pc step 13.3 ms against 15.5 with 16 x 16 tiles).
+#ifndef OCX_RK_TX
+#define OCX_RK_TX 16
 #endif
-constexpr int RK_TY = OCX_RK_TY;     // tile rows of the RK3 / pc / RHS layer-marching kernel
+#ifndef OCX_RK_TX_UP
+#define OCX_RK_TX_UP 64
+#endif
+constexpr int RK_TX = OCX_RK_TX, RK_TX_UP = OCX_RK_TX_UP;
+template <int SCH>
+constexpr int rk_tx() {
+    return SCH <= ocx::SCH_Z ? RK_TX : RK_TX_UP;
+}
+constexpr bool rk_tile_ok(int tx) { return 256 % tx == 0 && (tx == 64 || tx + 256 / tx <= 64); }
+static_assert(rk_tile_ok(RK_TX) && rk_tile_ok(RK_TX_UP), "RK tile shape");
+long long rk_blocks(int nt, int ny, int nx, int tx) {
+    const int ty = 256 / tx;
+    return (long long)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty) * nt;
+}
 
 int check_shape(int nt, int nz, int ny, int nx) {
     if (nt < 1 || nz < 1 || ny < 1 || nx < 1)
@@ -165,8 +183,9 @@ struct RkArgs {
 // k_rk_stage runs one block per (tile, tracer) on a 1-D grid; the host checks the block count fits it.
 template <int STAGE, int SCH, bool VERT>
 void rk_launch(const RkArgs& a, dim3, hipStream_t s) {
-    const long long nb = (long long)((a.nx + TX - 1) / TX) * ((a.ny + RK_TY - 1) / RK_TY) * a.nt;
-    ocx::k_rk_stage<STAGE, SCH, VERT, TX, RK_TY><<<dim3((unsigned)nb), dim3(TX, RK_TY), 0, s>>>(
+    constexpr int tx = rk_tx<SCH>(), ty = 256 / tx;
+    const long long nb = rk_blocks(a.nt, a.ny, a.nx, tx);
+    ocx::k_rk_stage<STAGE, SCH, VERT, tx, ty><<<dim3((unsigned)nb), dim3(64, 4), 0, s>>>(
         a.cin, a.C, a.ext, a.V0, a.V1, a.Qx, a.Qy, a.Fz, a.ok, a.q, a.cout, a.fxs, a.fi, a.nt, a.nz, a.ny, a.nx, a.dt,
         a.flags);
 }
@@ -187,7 +206,7 @@ void rk_stage(const RkArgs& a, int sch, bool vert, dim3 grid, hipStream_t s) {
 
 // The RK3 / pc / RHS kernel's 1-D grid of (tile, tracer) blocks must fit a launch.
 int check_rk_grid(int nt, int ny, int nx) {
-    const long long nb = (long long)((nx + TX - 1) / TX) * ((ny + RK_TY - 1) / RK_TY) * nt;
+    const long long nb = std::max(rk_blocks(nt, ny, nx, RK_TX), rk_blocks(nt, ny, nx, RK_TX_UP));
     if (nb > 0x7fffffffLL) return fail(OCX_E_ARG, "%lld tiles x tracers exceed one launch", nb);
     return OCX_OK;
 }

@@ -223,6 +223,45 @@ __device__ __forceinline__ double tile_yface(const double* sc, const uint8_t* so
     return val;
 }
 
+// The faces of a narrow tile (TX x TY = 256 cells, TX + TY <= 64; blockDim (64, 4), thread t on cell (t / TX,
+// t % TX)): each thread forms the west face and the north face of its cell; the tile's south faces (TX) and east
+// faces (TY) are one item list for one wavefront, which rotates with the layer, so every wave forms the same number
+// of faces over four layers and the tile's extra faces take one face evaluation instead of two.  In an interior
+// tile the extra items differ only in their LDS line (a column at stride LX for a south face, a row at stride 1 for
+// an east face), so they run as one code path; an edge tile forms them through tile_yface / tile_xface.
+template <int SCH, int TX, int TY, bool INNER>
+__device__ __forceinline__ void tile_faces_narrow(double* sfx, double* sfy, const double* sc, const uint8_t* so,
+                                                  const double* __restrict__ extp, int i0, int j0, int ny, int nx,
+                                                  unsigned flags, int k, const double* q4, bool known) {
+    static_assert(TX * TY == 256 && TX + TY <= 64, "a 256-cell tile whose extra faces fit one wavefront");
+    constexpr int H = 3, LX = TX + 2 * H;
+    const int t = threadIdx.y * 64 + threadIdx.x, r = t / TX, c = t - r * TX;
+    sfx[r * (TX + 1) + c] = tile_xface<SCH, TX, TY, INNER>(sc, so, extp, r, c, i0, j0, ny, nx, flags, q4[0], known);
+    sfy[(r + 1) * TX + c] = tile_yface<SCH, TX, TY, INNER>(sc, so, extp, r + 1, c, i0, j0, ny, nx, flags, q4[1],
+                                                           known);
+    const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
+    if (w != (k & 3) || lane >= TX + TY) return;
+    const bool south = lane < TX;
+    double v;
+    if constexpr (INNER) {
+        const int off = south ? lane + H : (lane - TX + H) * LX, st = south ? LX : 1, cc = south ? H : TX + H;
+        const double q = q4[2];
+        if constexpr (SCH == SCH_C6)
+            v = q * centred_face(sc + off, so + off, st, cc);
+        else if constexpr (SCH >= SCH_UP3)
+            v = q * up_face<SCH == SCH_UP3>(sc + off, so + off, st, cc, q);
+        else
+            v = q != 0.0 ? q * lds_face<SCH == SCH_Z>(sc + off, so + off, st, cc, q, known) : 0.0;
+    } else {
+        v = south ? tile_yface<SCH, TX, TY>(sc, so, extp, 0, lane, i0, j0, ny, nx, flags, q4[2], known)
+                  : tile_xface<SCH, TX, TY>(sc, so, extp, lane - TX, TX, i0, j0, ny, nx, flags, q4[2], known);
+    }
+    if (south)
+        sfy[lane] = v;
+    else
+        sfx[(lane - TX) * (TX + 1) + TX] = v;
+}
+
 // The faces of the tile with one wavefront per tile row (TX = 64, blockDim (64, TY)): wave w forms the west faces
 // of row w and the north faces of row w; the tile's last x-faces (east of each row) and its first y-faces (south of
 // row 0) go to two waves that alternate with the layer parity, so every wave forms the same number of faces over
@@ -232,7 +271,11 @@ template <int SCH, int TX, int TY, bool INNER = false>
 __device__ __forceinline__ void tile_faces_waves(double* sfx, double* sfy, const double* sc, const uint8_t* so,
                                                  const double* __restrict__ extp, int i0, int j0, int ny, int nx,
                                                  unsigned flags, int k, const double* q4, bool known) {
-    static_assert(TX == 64 && TY >= 2, "one wavefront per tile row");
+    if constexpr (TX < 64) {
+        tile_faces_narrow<SCH, TX, TY, INNER>(sfx, sfy, sc, so, extp, i0, j0, ny, nx, flags, k, q4, known);
+        return;
+    }
+    static_assert(TX < 64 || TY >= 2, "one wavefront per tile row");
     const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
     sfx[w * (TX + 1) + lane] =
         tile_xface<SCH, TX, TY, INNER>(sc, so, extp, w, lane, i0, j0, ny, nx, flags, q4[0], known);
@@ -252,6 +295,22 @@ __device__ __forceinline__ void tile_faces_waves(double* sfx, double* sfy, const
 template <int TX, int TY>
 __device__ __forceinline__ void tile_face_q_waves(double* q4, const double* __restrict__ Qxk,
                                                   const double* __restrict__ Qyk, int i0, int j0, int ny, int nx) {
+    if constexpr (TX < 64) {
+        // narrow tile: the thread's cell's west and north faces, and the item of the extra list (lane < TX: south
+        // face of column lane; TX <= lane < TX + TY: east face of row lane - TX)
+        const int t = threadIdx.y * 64 + threadIdx.x, r = t / TX, c = t - r * TX, lane = threadIdx.x;
+        const int j = j0 + r, i = i0 + c;
+        q4[0] = (j < ny && i <= nx) ? Qxk[(size_t)j * (nx + 1) + i] : 0.0;
+        q4[1] = (i < nx && j + 1 <= ny) ? Qyk[(size_t)(j + 1) * nx + i] : 0.0;
+        if (lane < TX)
+            q4[2] = (i0 + lane < nx && j0 <= ny) ? Qyk[(size_t)j0 * nx + i0 + lane] : 0.0;
+        else if (lane < TX + TY)
+            q4[2] = (j0 + lane - TX < ny && i0 + TX <= nx) ? Qxk[(size_t)(j0 + lane - TX) * (nx + 1) + i0 + TX] : 0.0;
+        else
+            q4[2] = 0.0;
+        q4[3] = 0.0;
+        return;
+    }
     const int w = threadIdx.y, lane = threadIdx.x;
     const int j = j0 + w, i = i0 + lane;
     q4[0] = (j < ny && i <= nx) ? Qxk[(size_t)j * (nx + 1) + i] : 0.0;
@@ -425,7 +484,10 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(OCX_RK_W
     // per layer parity: set when a value of that layer's tile lies outside the WENO divisions' fast range
     constexpr bool WENO = SCH <= SCH_Z;
     __shared__ int sbad[2];
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // the thread's cell in the tile: row ty, column tx (one row per wavefront for 64-wide tiles, where the row is
+    // the wave index itself)
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int ty = TX == 64 ? (int)threadIdx.y : tid / TX, tx = TX == 64 ? (int)threadIdx.x : tid - ty * TX;
     const int gx = (nx + TX - 1) / TX, gy = (ny + TY - 1) / TY;
     const long long nb = (long long)gx * gy * nt;
     // tracer fastest: the nt blocks of a tile run together on one XCD and share its per-layer fields through L2
@@ -465,7 +527,6 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(OCX_RK_W
     // the tile of layer k (with its halo) is fetched into registers one layer ahead and stored to LDS at the top of
     // the layer's iteration, so its loads are in flight while the previous layer's faces are formed
     constexpr int NT = TX * TY, NL = (LY * LX + NT - 1) / NT;
-    const int tid = ty * TX + tx;
     double pv[NL];
     uint8_t po[NL];
     bool pfast = true;
@@ -514,7 +575,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(OCX_RK_W
         }
         if constexpr (WENO) {
             check();
-            if (__builtin_amdgcn_ballot_w64(!pfast) != 0 && tx == 0) sbad[k & 1] = 1;
+            if (__builtin_amdgcn_ballot_w64(!pfast) != 0 && threadIdx.x == 0) sbad[k & 1] = 1;
         }
 #pragma unroll
         for (int m = 0; m < 4; ++m) qc4[m] = qn4[m];
