@@ -265,8 +265,9 @@ __device__ __forceinline__ void tile_faces_narrow(double* sfx, double* sfy, cons
 // The faces of the tile with one wavefront per tile row (TX = 64, blockDim (64, TY)): wave w forms the west faces
 // of row w and the north faces of row w; the tile's last x-faces (east of each row) and its first y-faces (south of
 // row 0) go to two waves that alternate with the layer parity, so every wave forms the same number of faces over
-// two layers.  q*: the transports of those items, loaded ahead (tile_face_q_waves).  known: every value of the LDS
-// tile is 0 or of magnitude in [2^-400, 2^100], so the WENO faces skip their per-face range test.
+// two layers.  Narrower tiles go to tile_faces_narrow.  q*: the transports of those items, loaded ahead
+// (tile_face_q_waves).  known: every value of the LDS tile is 0 or of magnitude in [2^-400, 2^100], so the WENO
+// faces skip their per-face range test.
 template <int SCH, int TX, int TY, bool INNER = false>
 __device__ __forceinline__ void tile_faces_waves(double* sfx, double* sfy, const double* sc, const uint8_t* so,
                                                  const double* __restrict__ extp, int i0, int j0, int ny, int nx,
